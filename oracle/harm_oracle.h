@@ -1,0 +1,108 @@
+/*
+ * harm_oracle.h -- CPU oracle for the HARM-COOL per-timestep GRMHD update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (agnieszkajaniuk/HARM_COOL, src/step_ch.c and the files it calls),
+ * used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker.  The product (harm_cool_amd/, libharmcool.so) never links or calls it.
+ *
+ * Conventions
+ *   - Zone indices follow the reference: i in [0, N1+4) with active 2..N1+1,
+ *     j in [0, N2+4) with active 2..N2+1 (2 ghosts each side, decs.h:403,
+ *     defs.h:135).  phi (h) has N3 active planes and no ghost planes: the
+ *     reference's one-plane phi halo is a periodic MPI self-exchange
+ *     (step_ch.c:183-197, mpi.c:20,63), restated here as periodic indexing.
+ *   - The exchange ("canonical") layout is SoA, phi fastest:
+ *       prim[v][i][j][h], v in 0..7 (RHO,UU,U1,U2,U3,B1,B2,B3, decs.h:190-197),
+ *       i in [0,N1+4), j in [0,N2+4), h in [0,N3).
+ */
+#ifndef HARM_ORACLE_H
+#define HARM_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field layout as harm_params in include/harm_cool.h. */
+typedef struct orc_params {
+    int32_t n1, n2, n3;      /* active zones in r (X1), theta (X2), phi (X3)      */
+    int32_t lim;             /* slope limiter: 0 MC, 1 van Leer, 2 minmod          */
+    int32_t font_fix;        /* 1: Font fix fallbacks on inversion failure         */
+    int32_t pad0;
+    double a, gam;           /* BH spin, adiabatic index                          */
+    double rin, rmax;        /* FM torus inner edge and pressure maximum          */
+    double kappa, beta;      /* polytropic constant, plasma beta normalization    */
+    double Rin, Rout;        /* grid radial extent (Rin <= 0: 0.98*r_horizon)     */
+    double hslope, R0, Rbr;  /* MKS theta concentration, r offset, hyperexp break */
+    double cour, dt0, tf, defcon;
+    double hllf, laxf;       /* relative weights of HLL and LLF fluxes            */
+    double pert_amp;         /* u *= 1 + pert_amp*(U[0,1]-0.5)                    */
+    uint32_t seed[4];        /* KISS base seeds                                   */
+} orc_params;
+
+enum {
+    ORC_C2P_CALLS = 0, ORC_NEWTON_ITERS, ORC_NEWTON_KCONV, ORC_FAIL_2D,
+    ORC_FAIL_FONT1, ORC_FAIL_FONT2, ORC_FIXUP_GAMMA, ORC_FIXUP_REPAIRS,
+    ORC_FIXUP_NOSTENCIL, ORC_FLOOR_ZONES, ORC_GAMMA_CAP, ORC_FATAL,
+    ORC_NCOUNTERS
+};
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx *orc_create(const orc_params *p);
+void     orc_destroy(orc_ctx *o);
+void     orc_set_threads(int n);
+int      orc_init_torus(orc_ctx *o);
+int      orc_step(orc_ctx *o);
+int      orc_get_state(const orc_ctx *o, double *prim, int8_t *pflag);
+int      orc_set_state(orc_ctx *o, const double *prim, const int8_t *pflag);
+void     orc_get_time(const orc_ctx *o, double *t, double *dt);
+void     orc_set_time(orc_ctx *o, double t, double dt);
+void     orc_get_counters(const orc_ctx *o, int64_t *c);
+void     orc_get_grid(const orc_ctx *o, double *startx, double *dx);
+double   orc_get_katm(const orc_ctx *o, int i);
+/* Geometry tables in the reference layout: gcov/gcon [SN1][SN2][5][4][4],
+ * gdet [SN1][SN2][5], conn [SN1][SN2][4][4][4], coords [SN1][SN2][4]. */
+void     orc_get_geometry(const orc_ctx *o, double *gcov, double *gcon,
+                          double *gdet, double *conn, double *coords);
+double   orc_divb_max(const orc_ctx *o);
+/* out[0] = mass (sum rho gdet dV), out[1] = sum U[UU] dV, out[2] = mdot_in,
+ * out[3] = mdot_out (diag.c:477-580, diag.c:59-230). */
+void     orc_diag(const orc_ctx *o, double *out);
+
+/* Single-zone primitive solvers; gcov/gcon are row-major 4x4. */
+int  orc_utoprim_2d(const double U[8], const double *gcov, const double *gcon,
+                    double gdet, double prim[8], double gam, int *n_iter, int *k_conv);
+int  orc_utoprim_1dvsq2fix1(const double U[8], const double *gcov, const double *gcon,
+                            double gdet, double prim[8], double K, double gam);
+int  orc_utoprim_1dfix1(const double U[8], const double *gcov, const double *gcon,
+                        double gdet, double prim[8], double K, double gam);
+void orc_primtoU_g(const double prim[8], const double *gcov, const double *gcon,
+                   double gdet, double gam, double U[8]);
+/* HARM's conserved vector (phys.c:100-105) from primitives. */
+void orc_primtoU(const double prim[8], const double *gcov, const double *gcon,
+                 double gdet, double gam, double U[8]);
+/* Geometry of one point in code coordinates X (X[0] ignored). */
+void orc_gcov_func(const orc_params *p, const double X[4], double *gcov);
+void orc_dxdxp_func(const orc_params *p, const double X[4], double *dxdxp);
+void orc_bl_coord_vec(const orc_params *p, const double X[4], double *V);
+void orc_conn_func(const orc_params *p, const double X[4], const double *gcon, double *conn);
+double orc_gdet_func(const double *gcov);
+int  orc_invert_matrix(const double *A, double *Ainv);
+double orc_slope_lim(int lim, double y1, double y2, double y3);
+double orc_kiss_uniform(uint32_t z, uint32_t w, uint32_t jsr, uint32_t jcong);
+/* One face of fluxcalc (step_ch.c:407-487): left/right states -> flux, ctop. */
+void orc_face_flux(const double *p_l, const double *p_r, const double *gcov,
+                   const double *gcon, double gdet, int dir, double gam,
+                   double hllf, double laxf, double *F, double *ctop);
+/* vchar (phys.c:337-476) for one state; returns the fail() code or 0. */
+int  orc_vchar(const double *pr, const double *gcov, const double *gcon, double gdet,
+               int dir, double gam, double *cmax, double *cmin);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

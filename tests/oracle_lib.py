@@ -1,0 +1,210 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and to the compiled
+reference subset (oracle/_ref/libharmref.so).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_int8, c_int64, c_uint32, c_void_p
+
+import numpy as np
+
+from harm_cool_amd.params import HarmParams
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libharmref.so")
+
+NCOUNTERS = 12
+COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_font1",
+                 "fail_font2", "fixup_gamma", "fixup_repairs", "fixup_nostencil",
+                 "floor_zones", "gamma_cap", "fatal"]
+
+_dp = POINTER(c_double)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_create.restype = c_void_p
+        L.orc_create.argtypes = [POINTER(HarmParams)]
+        L.orc_destroy.argtypes = [c_void_p]
+        L.orc_init_torus.argtypes = [c_void_p]
+        L.orc_step.argtypes = [c_void_p]
+        L.orc_get_state.argtypes = [c_void_p, _dp, POINTER(c_int8)]
+        L.orc_set_state.argtypes = [c_void_p, _dp, POINTER(c_int8)]
+        L.orc_get_time.argtypes = [c_void_p, _dp, _dp]
+        L.orc_set_time.argtypes = [c_void_p, c_double, c_double]
+        L.orc_get_counters.argtypes = [c_void_p, POINTER(c_int64)]
+        L.orc_get_grid.argtypes = [c_void_p, _dp, _dp]
+        L.orc_get_katm.restype = c_double
+        L.orc_get_katm.argtypes = [c_void_p, c_int]
+        L.orc_get_geometry.argtypes = [c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_divb_max.restype = c_double
+        L.orc_divb_max.argtypes = [c_void_p]
+        L.orc_diag.argtypes = [c_void_p, _dp]
+        L.orc_set_threads.argtypes = [c_int]
+        L.orc_utoprim_2d.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, POINTER(c_int), POINTER(c_int)]
+        L.orc_utoprim_1dvsq2fix1.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, c_double]
+        L.orc_utoprim_1dfix1.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, c_double]
+        L.orc_primtoU_g.argtypes = [_dp, _dp, _dp, c_double, c_double, _dp]
+        L.orc_primtoU.argtypes = [_dp, _dp, _dp, c_double, c_double, _dp]
+        L.orc_gcov_func.argtypes = [POINTER(HarmParams), _dp, _dp]
+        L.orc_dxdxp_func.argtypes = [POINTER(HarmParams), _dp, _dp]
+        L.orc_bl_coord_vec.argtypes = [POINTER(HarmParams), _dp, _dp]
+        L.orc_conn_func.argtypes = [POINTER(HarmParams), _dp, _dp, _dp]
+        L.orc_gdet_func.restype = c_double
+        L.orc_gdet_func.argtypes = [_dp]
+        L.orc_invert_matrix.argtypes = [_dp, _dp]
+        L.orc_slope_lim.restype = c_double
+        L.orc_slope_lim.argtypes = [c_int, c_double, c_double, c_double]
+        L.orc_kiss_uniform.restype = c_double
+        L.orc_kiss_uniform.argtypes = [c_uint32] * 4
+        L.orc_face_flux.argtypes = [_dp, _dp, _dp, _dp, c_double, c_int, c_double, c_double, c_double, _dp, _dp]
+        L.orc_vchar.argtypes = [_dp, _dp, _dp, c_double, c_int, c_double, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = ctypes.CDLL(REF_SO)
+        R.ref_set_params.argtypes = [c_double] * 5
+        R.ref_utoprim_2d.argtypes = [_dp, _dp, _dp, c_double, _dp]
+        R.ref_utoprim_1dvsq2fix1.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double]
+        R.ref_utoprim_1dfix1.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double]
+        R.ref_gcov_func.argtypes = [_dp, _dp]
+        R.ref_dxdxp_func.argtypes = [_dp, _dp]
+        R.ref_bl_coord_vec.argtypes = [_dp, _dp]
+        R.ref_conn_func.argtypes = [_dp, _dp, _dp]
+        R.ref_gdet_func.restype = c_double
+        R.ref_gdet_func.argtypes = [_dp]
+        R.ref_invert_matrix.argtypes = [_dp, _dp]
+        R.ref_slope_lim.restype = c_double
+        R.ref_slope_lim.argtypes = [c_int, c_double, c_double, c_double]
+        R.ref_kiss_uniform.restype = c_double
+        R.ref_kiss_uniform.argtypes = [c_uint32] * 4
+        R.ref_coord.argtypes = [c_int, c_int, c_int, c_int, _dp, _dp, _dp]
+        _ref = R
+    return _ref
+
+
+class Oracle:
+    """One oracle simulation context (the restated step_ch state)."""
+
+    def __init__(self, params: HarmParams):
+        self.params = params
+        self.L = lib()
+        self.h = self.L.orc_create(ctypes.byref(params))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        self.n1, self.n2, self.n3 = params.n1, params.n2, params.n3
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (8, self.n1 + 4, self.n2 + 4, self.n3)
+
+    def init_torus(self) -> int:
+        return self.L.orc_init_torus(self.h)
+
+    def step(self) -> int:
+        return self.L.orc_step(self.h)
+
+    def prims(self) -> np.ndarray:
+        a = np.empty(self.shape, dtype=np.float64)
+        self.L.orc_get_state(self.h, _ptr(a), None)
+        return a
+
+    def pflag(self) -> np.ndarray:
+        f = np.empty(self.shape[1:], dtype=np.int8)
+        self.L.orc_get_state(self.h, None, f.ctypes.data_as(POINTER(c_int8)))
+        return f
+
+    def set_prims(self, a: np.ndarray, pflag: np.ndarray | None = None) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape
+        fp = None
+        if pflag is not None:
+            pflag = np.ascontiguousarray(pflag, dtype=np.int8)
+            fp = pflag.ctypes.data_as(POINTER(c_int8))
+        self.L.orc_set_state(self.h, _ptr(a), fp)
+
+    def time(self):
+        t, dt = c_double(), c_double()
+        self.L.orc_get_time(self.h, ctypes.byref(t), ctypes.byref(dt))
+        return t.value, dt.value
+
+    def set_time(self, t: float, dt: float) -> None:
+        self.L.orc_set_time(self.h, t, dt)
+
+    def counters(self) -> dict:
+        c = (c_int64 * NCOUNTERS)()
+        self.L.orc_get_counters(self.h, c)
+        return dict(zip(COUNTER_NAMES, list(c)))
+
+    def grid(self):
+        sx = np.zeros(4)
+        dx = np.zeros(4)
+        self.L.orc_get_grid(self.h, _ptr(sx), _ptr(dx))
+        return sx, dx
+
+    def katm(self) -> np.ndarray:
+        return np.array([self.L.orc_get_katm(self.h, i) for i in range(self.n1 + 4)])
+
+    def geometry(self):
+        sn1, sn2 = self.n1 + 4, self.n2 + 4
+        gcov = np.zeros((sn1, sn2, 5, 4, 4))
+        gcon = np.zeros((sn1, sn2, 5, 4, 4))
+        gdet = np.zeros((sn1, sn2, 5))
+        conn = np.zeros((sn1, sn2, 4, 4, 4))
+        coords = np.zeros((sn1, sn2, 4))
+        self.L.orc_get_geometry(self.h, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn), _ptr(coords))
+        return gcov, gcon, gdet, conn, coords
+
+    def divb_max(self) -> float:
+        return self.L.orc_divb_max(self.h)
+
+    def diag(self):
+        out = np.zeros(4)
+        self.L.orc_diag(self.h, _ptr(out))
+        return dict(mass=out[0], energy=out[1], mdot_in=out[2], mdot_out=out[3])
+
+
+def utoprim_2d(U, gcov, gcon, gdet, prim, gam):
+    """Oracle Utoprim_2d on one zone -> (ret, prim_out, n_iter, k_conv)."""
+    L = lib()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    gcov = np.ascontiguousarray(gcov, dtype=np.float64)
+    gcon = np.ascontiguousarray(gcon, dtype=np.float64)
+    p = np.array(prim, dtype=np.float64)
+    ni, kc = c_int(), c_int()
+    r = L.orc_utoprim_2d(_ptr(U), _ptr(gcov), _ptr(gcon), gdet, _ptr(p), gam, ctypes.byref(ni), ctypes.byref(kc))
+    return r, p, ni.value, kc.value
