@@ -1,0 +1,239 @@
+"""harm_cool_amd -- MI355X (gfx950) implementation of HARM-COOL's per-timestep
+GRMHD update behind the reference's step_ch()/Utoprim_2d() interface.
+
+The compute path is libharmcool.so (hand-written HIP kernels, see csrc/); this
+module is the thin Python host over its C ABI (include/harm_cool.h).  There is no
+CPU fallback: if the library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int64, c_size_t, c_uint8, c_void_p
+
+import numpy as np
+
+from .params import HarmParams, config1, config2, reference_default, torus_params  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libharmcool.so")
+
+NCOUNTERS = 12
+COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_font1",
+                 "fail_font2", "fixup_gamma", "fixup_repairs", "fixup_nostencil",
+                 "floor_zones", "gamma_cap", "fatal"]
+KERNEL_NAMES = ["flux1", "flux2", "flux3", "emf", "update", "bound", "fixutoprim", "stepend",
+                "c2p_batch", "mlp"]
+NKERNELS = len(KERNEL_NAMES)
+
+_dp = POINTER(c_double)
+
+
+class HarmError(RuntimeError):
+    pass
+
+
+class U2POpts(ctypes.Structure):
+    _fields_ = [("gam", c_double), ("font_fix", ctypes.c_int32), ("guess_mode", ctypes.c_int32),
+                ("mlp", c_void_p), ("K", c_double)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libharmcool.so (built by harm_cool_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HarmError(f"{LIB_PATH} is missing: run harm_cool_amd/build.py (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.harm_last_error.restype = c_char_p
+    L.harm_create.restype = c_void_p
+    L.harm_create.argtypes = [POINTER(HarmParams), c_int]
+    L.harm_destroy.argtypes = [c_void_p]
+    for name in ("harm_init_torus", "harm_reset_counters", "harm_debug_bound", "harm_debug_fixup_utoprim"):
+        getattr(L, name).argtypes = [c_void_p]
+    L.harm_step.argtypes = [c_void_p, c_int]
+    L.harm_get_prims.argtypes = [c_void_p, _dp, POINTER(c_int8)]
+    L.harm_set_prims.argtypes = [c_void_p, _dp, POINTER(c_int8)]
+    L.harm_get_time.argtypes = [c_void_p, _dp, _dp]
+    L.harm_set_time.argtypes = [c_void_p, c_double, c_double]
+    L.harm_get_counters.argtypes = [c_void_p, POINTER(c_int64)]
+    L.harm_diag.argtypes = [c_void_p, _dp]
+    L.harm_device_state.argtypes = [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p)]
+    L.harm_set_timing.argtypes = [c_void_p, c_int]
+    L.harm_get_timing.argtypes = [c_void_p, _dp, POINTER(c_int64)]
+    L.harm_debug_fluxes.argtypes = [c_void_p, c_int, _dp, _dp, _dp, _dp]
+    L.harm_get_geometry.argtypes = [c_void_p, _dp, _dp, _dp, _dp]
+    L.harm_utoprim2d_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_size_t, POINTER(U2POpts), c_void_p]
+    L.harm_debug_pow.argtypes = [_dp, _dp, _dp, c_size_t]
+    L.Utoprim_2d.argtypes = [_dp, _dp, _dp, c_double, _dp]
+    L.harm_set_gamma.argtypes = [c_double]
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().harm_last_error().decode(errors="replace")
+        raise HarmError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Harm:
+    """One simulation context: the reference's global state (decs.h:310-378)
+    resident on the GPU.  Methods mirror the reference driver: init() is the
+    torus setup of init.c, step_ch() one predictor/corrector step of step_ch.c."""
+
+    def __init__(self, params: HarmParams, device: int = 0):
+        self.params = params
+        self.L = lib()
+        self.h = self.L.harm_create(ctypes.byref(params), device)
+        if not self.h:
+            raise HarmError("harm_create failed: " + self.L.harm_last_error().decode(errors="replace"))
+        self.n1, self.n2, self.n3 = params.n1, params.n2, params.n3
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.harm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def shape(self):
+        return (8, self.n1 + 4, self.n2 + 4, self.n3)
+
+    def init(self) -> None:
+        _check(self.L.harm_init_torus(self.h), "harm_init_torus")
+
+    def step_ch(self, nsteps: int = 1) -> None:
+        _check(self.L.harm_step(self.h, nsteps), "harm_step")
+
+    def prims(self) -> np.ndarray:
+        a = np.empty(self.shape, dtype=np.float64)
+        _check(self.L.harm_get_prims(self.h, _ptr(a), None), "harm_get_prims")
+        return a
+
+    def pflag(self) -> np.ndarray:
+        f = np.empty(self.shape[1:], dtype=np.int8)
+        _check(self.L.harm_get_prims(self.h, None, f.ctypes.data_as(POINTER(c_int8))), "harm_get_prims")
+        return f
+
+    def set_prims(self, a: np.ndarray, pflag: np.ndarray | None = None) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"expected {self.shape}, got {a.shape}")
+        fp = None
+        if pflag is not None:
+            pflag = np.ascontiguousarray(pflag, dtype=np.int8)
+            fp = pflag.ctypes.data_as(POINTER(c_int8))
+        _check(self.L.harm_set_prims(self.h, _ptr(a), fp), "harm_set_prims")
+
+    def time(self):
+        t, dt = c_double(), c_double()
+        _check(self.L.harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "harm_get_time")
+        return t.value, dt.value
+
+    def set_time(self, t: float, dt: float) -> None:
+        _check(self.L.harm_set_time(self.h, t, dt), "harm_set_time")
+
+    def counters(self) -> dict:
+        c = (c_int64 * NCOUNTERS)()
+        _check(self.L.harm_get_counters(self.h, c), "harm_get_counters")
+        return dict(zip(COUNTER_NAMES, list(c)))
+
+    def reset_counters(self) -> None:
+        _check(self.L.harm_reset_counters(self.h), "harm_reset_counters")
+
+    def diag(self) -> dict:
+        out = np.zeros(7)
+        _check(self.L.harm_diag(self.h, _ptr(out)), "harm_diag")
+        return dict(mass=out[0], energy=out[1], mdot_in=out[2], mdot_out=out[3],
+                    mass_in=out[4], mass_out=out[5], divb_max=out[6])
+
+    def set_timing(self, on: bool) -> None:
+        _check(self.L.harm_set_timing(self.h, 1 if on else 0), "harm_set_timing")
+
+    def timing(self) -> dict:
+        ms = np.zeros(NKERNELS)
+        n = (c_int64 * NKERNELS)()
+        _check(self.L.harm_get_timing(self.h, _ptr(ms), n), "harm_get_timing")
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNEL_NAMES)}
+
+    def debug_fluxes(self, fullstep: int = 0):
+        F1 = np.zeros(self.shape)
+        F2 = np.zeros(self.shape)
+        F3 = np.zeros(self.shape)
+        nd = np.zeros(3)
+        _check(self.L.harm_debug_fluxes(self.h, fullstep, _ptr(F1), _ptr(F2), _ptr(F3), _ptr(nd)), "harm_debug_fluxes")
+        return F1, F2, F3, nd
+
+    def debug_bound(self) -> None:
+        _check(self.L.harm_debug_bound(self.h), "harm_debug_bound")
+
+    def debug_fixup_utoprim(self) -> None:
+        _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim")
+
+    def geometry(self):
+        ncol = (self.n1 + 4) * (self.n2 + 4)
+        gcov = np.zeros((5, 16, ncol))
+        gcon = np.zeros((5, 16, ncol))
+        gdet = np.zeros((5, ncol))
+        conn = np.zeros((64, ncol))
+        _check(self.L.harm_get_geometry(self.h, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn)), "harm_get_geometry")
+        sh = (self.n1 + 4, self.n2 + 4)
+        return (gcov.reshape(5, 4, 4, *sh), gcon.reshape(5, 4, 4, *sh), gdet.reshape(5, *sh),
+                conn.reshape(4, 4, 4, *sh))
+
+
+def device_pow(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """The device pow of the Font-fix solvers evaluated on the GPU (test entry)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(x)
+    _check(lib().harm_debug_pow(_ptr(x), _ptr(y), _ptr(out), x.size), "harm_debug_pow")
+    return out
+
+
+def set_gamma(gam: float) -> None:
+    lib().harm_set_gamma(gam)
+
+
+def Utoprim_2d(U, gcov, gcon, gdet: float, prim) -> int:
+    """The reference's Utoprim_2d (utoprim_2d.c:126) for one zone, run on the GPU.
+    prim is updated in place (a float64 numpy array of 8)."""
+    L = lib()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    gcov = np.ascontiguousarray(gcov, dtype=np.float64)
+    gcon = np.ascontiguousarray(gcon, dtype=np.float64)
+    assert prim.dtype == np.float64 and prim.flags.c_contiguous
+    return L.Utoprim_2d(_ptr(U), _ptr(gcov), _ptr(gcon), gdet, _ptr(prim))
+
+
+def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K: float = 0.0, stream=None):
+    """Batched inversion on device tensors (torch, float64, SoA: U[8,n], gcov[16,n],
+    gcon[16,n], gdet[n], prim[8,n] in/out).  Returns (ret int8[n], iters uint8[n],
+    kconv uint8[n]) as device tensors."""
+    import torch
+
+    n = gdet.shape[0]
+    for t, rows in ((U, 8), (gcov, 16), (gcon, 16), (prim, 8)):
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (rows, n):
+            raise ValueError("utoprim2d_batch expects contiguous float64 device tensors of shape (rows, n)")
+    ret = torch.empty(n, dtype=torch.int8, device=U.device)
+    iters = torch.empty(n, dtype=torch.uint8, device=U.device)
+    kconv = torch.empty(n, dtype=torch.uint8, device=U.device)
+    opts = U2POpts(gam=gam, font_fix=font_fix, guess_mode=0, mlp=None, K=K)
+    s = stream if stream is not None else torch.cuda.current_stream(U.device).cuda_stream
+    _check(lib().harm_utoprim2d_batch(U.data_ptr(), gcov.data_ptr(), gcon.data_ptr(), gdet.data_ptr(),
+                                      prim.data_ptr(), ret.data_ptr(), iters.data_ptr(), kconv.data_ptr(), n,
+                                      ctypes.byref(opts), s), "harm_utoprim2d_batch")
+    return ret, iters, kconv

@@ -1,0 +1,1549 @@
+// harm_cool.hip -- gfx950 kernels and the C ABI of libharmcool.so.
+//
+// One step of the reference (step_ch.c:90-140) maps to this launch sequence on
+// the context's stream, with the whole domain resident in HBM:
+//   per half-step:  flux X1, flux X2, [flux X3]   (fluxcalc + fix_flux fused)
+//                   emf                           (flux_ct EMF averages)
+//                   update                        (source + flux divergence +
+//                                                  con2prim + Font fix + fixup)
+//                   bound, fixutoprim scan/serial, bound
+//   end of step:    mdot partials, stepend        (diag_mass, dt control)
+// dt and t live in device memory, so a step needs no host round trip.
+//
+// Layout: SoA, phi fastest.  Zone z = (i*SN2 + j)*N3 + h, variable stride nz.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "harm_host.hpp"
+#include "harm_phys.hpp"
+
+using namespace harm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(const std::string &m, int code = 1)
+{
+    g_err = m;
+    return code;
+}
+
+#define HIPCHK(x)                                                                               \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess)                                                                   \
+            return set_err(std::string(#x) + ": " + hipGetErrorString(e_) + " @" + std::to_string(__LINE__), 2); \
+    } while (0)
+
+constexpr int TPB = 256;
+
+struct DGrid {
+    int N1, N2, N3, SN1, SN2;
+    int is, ie, js, je;
+    int lim, font_fix, polefix;
+    size_t ncol, nz;
+    double dx1, dx2, dx3, gam, cour, hllf, laxf, defcon, tf;
+    const double *geom[5];
+    const double *conn;
+    const double *floors;
+    const double *katm;
+};
+
+__device__ __forceinline__ size_t zidx(const DGrid &g, int i, int j, int h)
+{
+    return ((size_t)i * g.SN2 + j) * g.N3 + h;
+}
+
+// ---- wave-level reductions (64 lanes) ----
+__device__ __forceinline__ int wave_sum_int(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min_dbl(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double w = __shfl_xor(v, o, 64);
+        v = (w < v) ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ void add_counter(unsigned long long *cnt, int idx, int v)
+{
+    int s = wave_sum_int(v);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&cnt[idx], (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------------------
+// fluxcalc (step_ch.c:331-504) + fix_flux (coord.c:264-324), one face per thread.
+// DIR 1,2,3 = X1,X2,X3.  Slopes of the two zones adjacent to the face are formed
+// on the fly from four zones along DIR, as the reference's dq array holds them.
+// ---------------------------------------------------------------------------
+template <int DIR>
+__global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F,
+                                              unsigned long long *__restrict__ ndt_slot, int fullstep,
+                                              unsigned long long *__restrict__ cnt)
+{
+    const int i0 = (DIR == 1) ? g.is : g.is - 1;
+    const int j0 = (DIR == 2) ? g.js : g.js - 1;
+    const int ni = g.ie + 1 - i0 + 1;
+    const int nj = g.je + 1 - j0 + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    const size_t total = (size_t)ni * nj * g.N3;
+    double dtij = 1.e300;
+    int fatal = 0;
+    if (idx < total) {
+        const int h = (int)(idx % g.N3);
+        const size_t rest = idx / g.N3;
+        const int j = j0 + (int)(rest % nj);
+        const int i = i0 + (int)(rest / nj);
+        const size_t z = zidx(g, i, j, h);
+        const size_t nz = g.nz;
+        if (DIR == 2 && (j == g.js || j == g.je + 1)) {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
+        } else {
+            size_t zm2, zm1, zp1;
+            if (DIR == 1) {
+                const size_t s = (size_t)g.SN2 * g.N3;
+                zm2 = z - 2 * s;
+                zm1 = z - s;
+                zp1 = z + s;
+            } else if (DIR == 2) {
+                const size_t s = (size_t)g.N3;
+                zm2 = z - 2 * s;
+                zm1 = z - s;
+                zp1 = z + s;
+            } else {
+                const size_t base = z - h;
+                zm2 = base + (h + 2 * g.N3 - 2) % g.N3;
+                zm1 = base + (h + g.N3 - 1) % g.N3;
+                zp1 = base + (h + 1) % g.N3;
+            }
+            double p_l[NPR], p_r[NPR];
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
+                const double dqm = slope_lim(g.lim, a, b, c);
+                const double dq0 = slope_lim(g.lim, b, c, d);
+                p_l[k] = b + 0.5 * dqm;
+                p_r[k] = c - 0.5 * dq0;
+            }
+            Geom geom;
+            const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
+            load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
+            double Fz[NPR], ctop;
+            face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
+            if (DIR == 1) {
+                // coord.c:310-320: no inflow of mass through the radial boundaries
+                if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
+                if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
+            }
+            // coord.c:290-306: antisymmetric B2 flux in the first polar ghost face
+            const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+#pragma unroll
+            for (int k = 0; k < NPR; k++)
+                if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
+            if (DIR != 2) {
+                if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
+                if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+            }
+            if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+                const double dxd = (DIR == 1) ? g.dx1 : (DIR == 2) ? g.dx2 : g.dx3;
+                const double d = g.cour * dxd / ctop;
+                if (d < 1.e9) dtij = d;
+            }
+        }
+    }
+    if (fullstep) {
+        const double m = wave_min_dbl(dtij);
+        if ((threadIdx.x & 63) == 0 && m < 1.e9) atomicMin(ndt_slot, (unsigned long long)__double_as_longlong(m));
+    }
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
+// flux_ct EMFs (step_ch.c:514-629 for N3 <= 2, step_ch.c:726-798 for N3 > 2).
+__global__ void __launch_bounds__(TPB) k_emf(DGrid g, const double *__restrict__ F1, const double *__restrict__ F2,
+                                             const double *__restrict__ F3, double *__restrict__ e1,
+                                             double *__restrict__ e2, double *__restrict__ e3)
+{
+    const int ni = g.ie + 1 - g.is + 1, nj = g.je + 1 - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx >= (size_t)ni * nj * g.N3) return;
+    const int h = (int)(idx % g.N3);
+    const size_t rest = idx / g.N3;
+    const int j = g.js + (int)(rest % nj);
+    const int i = g.is + (int)(rest / nj);
+    const size_t nz = g.nz;
+    const size_t z = zidx(g, i, j, h);
+    const size_t zjm = zidx(g, i, j - 1, h), zim = zidx(g, i - 1, j, h);
+    if (g.N3 <= 2) {
+        e3[z] = 0.25 * (F1[B2 * nz + z] + F1[B2 * nz + zjm] - F2[B1 * nz + z] - F2[B1 * nz + zim]);
+        return;
+    }
+    const size_t zhm = z - h + (h + g.N3 - 1) % g.N3;
+    e1[z] = 0.25 * ((F2[B3 * nz + z] + F2[B3 * nz + zhm]) - (F3[B2 * nz + z] + F3[B2 * nz + zjm]));
+    e2[z] = 0.25 * ((F3[B1 * nz + z] + F3[B1 * nz + zim]) - (F1[B3 * nz + z] + F1[B3 * nz + zhm]));
+    e3[z] = 0.25 * ((F1[B2 * nz + z] + F1[B2 * nz + zjm]) - (F2[B1 * nz + z] + F2[B1 * nz + zim]));
+}
+
+// The flux_ct-rewritten B-components of F1/F2/F3 at a face (step_ch.c:541-554,
+// step_ch.c:783-797), evaluated from the EMFs where they are consumed.
+__device__ __forceinline__ void ct_f1(const DGrid &g, const double *e1, const double *e2, const double *e3, int i,
+                                      int j, int h, double *fb)
+{
+    const size_t z = zidx(g, i, j, h);
+    fb[0] = 0.;
+    fb[1] = 0.5 * (e3[z] + e3[zidx(g, i, j + 1, h)]);
+    if (g.N3 > 2) fb[2] = -0.5 * (e2[z] + e2[z - h + (h + 1) % g.N3]);
+}
+__device__ __forceinline__ void ct_f2(const DGrid &g, const double *e1, const double *e2, const double *e3, int i,
+                                      int j, int h, double *fb)
+{
+    const size_t z = zidx(g, i, j, h);
+    fb[0] = -0.5 * (e3[z] + e3[zidx(g, i + 1, j, h)]);
+    fb[1] = 0.;
+    if (g.N3 > 2) fb[2] = 0.5 * (e1[z] + e1[z - h + (h + 1) % g.N3]);
+}
+__device__ __forceinline__ void ct_f3(const DGrid &g, const double *e1, const double *e2, int i, int j, int h,
+                                      double *fb)
+{
+    const size_t z = zidx(g, i, j, h);
+    fb[0] = 0.5 * (e2[z] + e2[zidx(g, i + 1, j, h)]);
+    fb[1] = -0.5 * (e1[z] + e1[zidx(g, i, j + 1, h)]);
+    fb[2] = 0.;
+}
+
+// advance() zone loop (step_ch.c:237-306) + fixup() (fixup.c:51-60), one active
+// zone per thread: conserved update from pi with sources at pb, Noble 2D
+// inversion seeded with pi, Font fix on failure, then floors / gamma limit.
+__global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb,
+                                                double *__restrict__ pf, int8_t *__restrict__ pflag,
+                                                const double *__restrict__ F1, const double *__restrict__ F2,
+                                                const double *__restrict__ F3, const double *__restrict__ e1,
+                                                const double *__restrict__ e2, const double *__restrict__ e3,
+                                                const double *__restrict__ tstate, double dtfac,
+                                                unsigned long long *__restrict__ cnt)
+{
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    int c_call = 0, c_iter = 0, c_kconv = 0, c_f2d = 0, c_ff1 = 0, c_ff2 = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
+    if (idx < (size_t)ni * nj * g.N3) {
+        const int h = (int)(idx % g.N3);
+        const size_t rest = idx / g.N3;
+        const int j = g.js + (int)(rest % nj);
+        const int i = g.is + (int)(rest / nj);
+        const size_t z = zidx(g, i, j, h);
+        const size_t nz = g.nz;
+        const size_t col = (size_t)i * g.SN2 + j;
+        const double Dt = dtfac * tstate[1];
+        Geom geom;
+        load_geom(g.geom[CENT], g.ncol, col, geom);
+        double pbz[NPR], piz[NPR], dU[NPR], U[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            pbz[k] = pb[k * nz + z];
+            piz[k] = pi[k * nz + z];
+        }
+        source(pbz, geom, g.conn, g.ncol, col, g.gam, dU, fatal);
+        {
+            State q;
+            get_state(piz, geom, q, fatal);
+            primtoflux(piz, q, 0, geom, g.gam, U);
+        }
+        const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
+        const size_t zh1 = z - h + (h + 1) % g.N3;
+        double f1a[3], f1b[3], f2a[3], f2b[3], f3a[3], f3b[3];
+        ct_f1(g, e1, e2, e3, i, j, h, f1a);
+        ct_f1(g, e1, e2, e3, i + 1, j, h, f1b);
+        ct_f2(g, e1, e2, e3, i, j, h, f2a);
+        ct_f2(g, e1, e2, e3, i, j + 1, h, f2b);
+        if (g.N3 > 2) {
+            ct_f3(g, e1, e2, i, j, h, f3a);
+            ct_f3(g, e1, e2, i, j, (h + 1) % g.N3, f3b);
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            double a1, b1, a2, b2;
+            if (k < B1 || (k == B3 && g.N3 <= 2)) {
+                a1 = F1[k * nz + z];
+                b1 = F1[k * nz + zi1];
+                a2 = F2[k * nz + z];
+                b2 = F2[k * nz + zj1];
+            } else {
+                a1 = f1a[k - B1];
+                b1 = f1b[k - B1];
+                a2 = f2a[k - B1];
+                b2 = f2b[k - B1];
+            }
+            if (g.N3 > 2) {
+                double a3, b3;
+                if (k < B1) {
+                    a3 = F3[k * nz + z];
+                    b3 = F3[k * nz + zh1];
+                } else {
+                    a3 = f3a[k - B1];
+                    b3 = f3b[k - B1];
+                }
+                U[k] += Dt * (-(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2 - (b3 - a3) / g.dx3 + dU[k]);
+            } else {
+                U[k] += Dt * (-(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2 + dU[k]);
+            }
+        }
+        double pr[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pr[k] = piz[k];
+        int niter, kconv;
+        int ret = utoprim_2d(U, geom, g.gam, pr, niter, kconv);
+        c_call = 1;
+        c_iter = niter;
+        c_kconv = kconv;
+        int8_t flag = (int8_t)ret;
+        if (flag) {
+            c_f2d = 1;
+            if (g.font_fix) {
+                const double K = g.katm[i];
+                flag = (int8_t)utoprim_1dvsq2fix1(U, geom, g.gam, K, pr);
+                if (flag) {
+                    c_ff1 = 1;
+                    flag = (int8_t)utoprim_1dfix1(U, geom, g.gam, K, pr);
+                    if (flag) c_ff2 = 1;
+                }
+            }
+        }
+        const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
+        if (fx & FIX_FLOOR) c_flr = 1;
+        if (fx & FIX_GAMMA_CAP) c_cap = 1;
+        if (fx & FIX_GAMMA_FAIL) {
+            flag = -33;
+            c_gam = 1;
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
+        pflag[z] = flag;
+    }
+    add_counter(cnt, HARM_C2P_CALLS, c_call);
+    add_counter(cnt, HARM_NEWTON_ITERS, c_iter);
+    add_counter(cnt, HARM_NEWTON_KCONV, c_kconv);
+    add_counter(cnt, HARM_FAIL_2D, c_f2d);
+    add_counter(cnt, HARM_FAIL_FONT1, c_ff1);
+    add_counter(cnt, HARM_FAIL_FONT2, c_ff2);
+    add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
+    add_counter(cnt, HARM_FLOOR_ZONES, c_flr);
+    add_counter(cnt, HARM_GAMMA_CAP, c_cap);
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
+// fixup() alone over all active zones (used by the initial-data path, init.c:311,420).
+__global__ void __launch_bounds__(TPB) k_fixup(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag)
+{
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx >= (size_t)ni * nj * g.N3) return;
+    const int h = (int)(idx % g.N3);
+    const size_t rest = idx / g.N3;
+    const int j = g.js + (int)(rest % nj);
+    const int i = g.is + (int)(rest / nj);
+    const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
+    Geom geom;
+    load_geom(g.geom[CENT], g.ncol, col, geom);
+    double pr[NPR];
+    for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+    int fatal = 0;
+    const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
+    for (int k = 0; k < NPR; k++) p[k * nz + z] = pr[k];
+    if (fx & FIX_GAMMA_FAIL) pflag[z] = -33;
+}
+
+// ---------------------------------------------------------------------------
+// bound_prim (bounds.c:54-331, polefix bounds.c:381-436): one thread per (i,h)
+// column.  Every value a column thread reads comes from active rows no thread of
+// this launch writes, so the reference's copy -> polefix -> pole ghosts -> sign
+// flip -> inflow sequence is reproduced without inter-thread ordering.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bval(const DGrid &g, const double *p, int hh, int ii, int j, int k)
+{
+    const int c = ii < g.is ? g.is : (ii > g.ie ? g.ie : ii);
+    const size_t nz = g.nz;
+    if (g.polefix && k < B1) {
+        const int jref = g.polefix;
+        if (j < g.js + jref) {
+            const double src = p[k * nz + zidx(g, c, g.js + jref, hh)];
+            return (k == U2) ? ((j - g.js) + 0.5) / (jref + 0.5) * src : src;
+        }
+        if (j > g.je - jref) {
+            const double src = p[k * nz + zidx(g, c, g.je - jref, hh)];
+            return (k == U2) ? ((g.je - j) + 0.5) / (jref + 0.5) * src : src;
+        }
+    }
+    return p[k * nz + zidx(g, c, j, hh)];
+}
+
+__global__ void __launch_bounds__(TPB) k_bound(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag,
+                                               unsigned long long *__restrict__ cnt)
+{
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    int fatal = 0;
+    if (idx < (size_t)g.SN1 * g.N3) {
+        const int h = (int)(idx % g.N3);
+        const int i = (int)(idx / g.N3);
+        const size_t nz = g.nz;
+        const bool ghost = (i < g.is || i > g.ie);
+        const int c = i < g.is ? g.is : (i > g.ie ? g.ie : i);
+        if (ghost) {
+            for (int j = g.js; j <= g.je; j++) {
+                const size_t z = zidx(g, i, j, h);
+                for (int k = 0; k < NPR; k++) p[k * nz + z] = bval(g, p, h, i, j, k);
+                pflag[z] = pflag[zidx(g, c, j, h)];
+            }
+        } else if (g.polefix) {
+            for (int r = 0; r < 2 * g.polefix; r++) {
+                const int j = (r < g.polefix) ? g.js + r : g.je - (r - g.polefix);
+                const size_t z = zidx(g, i, j, h);
+                for (int k = 0; k < B1; k++) p[k * nz + z] = bval(g, p, h, i, j, k);
+            }
+        }
+        const int hs = (g.N3 > 1) ? (h + g.N3 / 2) % g.N3 : h;
+        const int src_row[4] = {g.js, g.js + 1, g.je, g.je - 1};
+        const int dst_row[4] = {g.js - 1, g.js - 2, g.je + 1, g.je + 2};
+        for (int r = 0; r < 4; r++) {
+            const size_t zd = zidx(g, i, dst_row[r], h);
+            for (int k = 0; k < NPR; k++) {
+                double v = bval(g, p, hs, i, src_row[r], k);
+                if (k == U2 || k == B2) v *= -1.;
+                p[k * nz + zd] = v;
+            }
+            pflag[zd] = pflag[zidx(g, c, src_row[r], h)];
+        }
+        if (ghost) {
+            const int type = (i < g.is) ? 0 : 1;
+            for (int j = g.js - 2; j <= g.je + 2; j++) {
+                const size_t z = zidx(g, i, j, h);
+                double pr[NPR];
+                for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+                Geom geom;
+                load_geom(g.geom[CENT], g.ncol, (size_t)i * g.SN2 + j, geom);
+                inflow_check(pr, geom, type, fatal);
+                for (int k = U1; k <= U3; k++) p[k * nz + z] = pr[k];
+            }
+        }
+    }
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
+// ---------------------------------------------------------------------------
+// fixup_utoprim (fixup.c:432-538).  The reference walks zones in (h,i,j) order
+// and its fallback branch reads neighbours repaired earlier in that walk.  A bad
+// zone whose 8 neighbours are all good reads only zones nothing modifies, so it
+// is repaired in parallel; bad zones with a bad neighbour are collected and
+// repaired by one thread in the reference's order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void repair_zone(const DGrid &g, double *p, int8_t *pf_live, int i, int j, int h,
+                                            bool live_flags, const int8_t *pf_in, int &c_rep, int &c_nost,
+                                            int &c_flr, int &c_cap, int &c_gam, int &fatal)
+{
+    const size_t nz = g.nz;
+    const size_t z = zidx(g, i, j, h);
+    auto good = [&](int ii, int jj) -> bool {
+        const size_t zz = zidx(g, ii, jj, h);
+        return live_flags ? (pf_live[zz] != 0) : (pf_in[zz] == 0);
+    };
+    const bool p1 = good(i - 1, j + 1), p2 = good(i, j + 1), p3 = good(i + 1, j + 1), p8 = good(i - 1, j);
+    const bool p4 = good(i + 1, j), p7 = good(i - 1, j - 1), p6 = good(i, j - 1), p5 = good(i + 1, j - 1);
+    const size_t zN = zidx(g, i, j + 1, h), zS = zidx(g, i, j - 1, h), zW = zidx(g, i - 1, j, h), zE = zidx(g, i + 1, j, h);
+    const size_t zNE = zidx(g, i + 1, j + 1, h), zSE = zidx(g, i + 1, j - 1, h), zNW = zidx(g, i - 1, j + 1, h),
+                 zSW = zidx(g, i - 1, j - 1, h);
+    double pr[NPR];
+    for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+    if (p2 && p4 && p6 && p8) {
+        for (int k = 0; k < B1; k++)
+            pr[k] = 0.25 * (p[k * nz + zN] + p[k * nz + zS] + p[k * nz + zW] + p[k * nz + zE]);
+    } else if (p1 && p3 && p5 && p7) {
+        for (int k = 0; k < B1; k++)
+            pr[k] = 0.25 * (p[k * nz + zNE] + p[k * nz + zSE] + p[k * nz + zNW] + p[k * nz + zSW]);
+    } else {
+        for (int k = RHO; k <= UU; k++) {
+            const double a4 = 0.25 * (p[k * nz + zN] + p[k * nz + zS] + p[k * nz + zW] + p[k * nz + zE]);
+            const double a2 = 0.25 * (p[k * nz + zNE] + p[k * nz + zSE] + p[k * nz + zNW] + p[k * nz + zSW]);
+            pr[k] = 0.5 * (a4 + a2);
+        }
+        pr[U1] = pr[U2] = pr[U3] = 0.;
+        c_nost++;
+    }
+    c_rep++;
+    const size_t col = (size_t)i * g.SN2 + j;
+    Geom geom;
+    load_geom(g.geom[CENT], g.ncol, col, geom);
+    const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
+    int8_t fl = 0;
+    if (fx & FIX_FLOOR) c_flr++;
+    if (fx & FIX_GAMMA_CAP) c_cap++;
+    if (fx & FIX_GAMMA_FAIL) {
+        fl = -33;
+        c_gam++;
+    }
+    for (int k = 0; k < NPR; k++) p[k * nz + z] = pr[k];
+    pf_live[z] = fl;
+}
+
+__global__ void __launch_bounds__(TPB) k_fixutoprim_scan(DGrid g, double *__restrict__ p,
+                                                         const int8_t *__restrict__ fin, int8_t *__restrict__ fout,
+                                                         int *__restrict__ list, int *__restrict__ count, int cap,
+                                                         unsigned long long *__restrict__ cnt)
+{
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
+    if (idx < g.nz) {
+        const int h = (int)(idx % g.N3);
+        const size_t col = idx / g.N3;
+        const int j = (int)(col % g.SN2), i = (int)(col / g.SN2);
+        const int8_t f = fin[idx];
+        fout[idx] = (int8_t)(!f);
+        if (f != 0 && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+            bool isolated = true;
+            for (int di = -1; di <= 1; di++)
+                for (int dj = -1; dj <= 1; dj++)
+                    if ((di || dj) && fin[zidx(g, i + di, j + dj, h)] != 0) isolated = false;
+            if (isolated) {
+                repair_zone(g, p, fout, i, j, h, false, fin, c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
+            } else {
+                const int slot = atomicAdd(count, 1);
+                if (slot < cap) list[slot] = (int)idx;
+            }
+        }
+    }
+    add_counter(cnt, HARM_FIXUP_REPAIRS, c_rep);
+    add_counter(cnt, HARM_FIXUP_NOSTENCIL, c_nost);
+    add_counter(cnt, HARM_FLOOR_ZONES, c_flr);
+    add_counter(cnt, HARM_GAMMA_CAP, c_cap);
+    add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
+// Serial repair of clustered bad zones in the reference's (h,i,j) order.
+__global__ void __launch_bounds__(1024) k_fixutoprim_serial(DGrid g, double *__restrict__ p, int8_t *__restrict__ f,
+                                                            int *__restrict__ list, int *__restrict__ count, int cap,
+                                                            unsigned long long *__restrict__ cnt,
+                                                            int *__restrict__ err)
+{
+    const int n_all = *count;
+    if (n_all == 0) return;
+    if (n_all > cap) {
+        if (threadIdx.x == 0) atomicOr(err, 1);
+        return;
+    }
+    const int n = n_all;
+    int *sorted = list + cap;
+    auto key = [&](int z) -> long long {
+        const int h = z % g.N3;
+        const int col = z / g.N3;
+        return (long long)h * g.ncol + col;  // (h, i, j) lexicographic
+    };
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const long long ke = key(list[e]);
+        int rank = 0;
+        for (int q = 0; q < n; q++) rank += (key(list[q]) < ke);
+        sorted[rank] = list[e];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
+        for (int e = 0; e < n; e++) {
+            const int z = sorted[e];
+            const int h = z % g.N3;
+            const int col = z / g.N3;
+            repair_zone(g, p, f, col / g.SN2, col % g.SN2, h, true, nullptr, c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
+        }
+        atomicAdd(&cnt[HARM_FIXUP_REPAIRS], (unsigned long long)c_rep);
+        atomicAdd(&cnt[HARM_FIXUP_NOSTENCIL], (unsigned long long)c_nost);
+        atomicAdd(&cnt[HARM_FLOOR_ZONES], (unsigned long long)c_flr);
+        atomicAdd(&cnt[HARM_GAMMA_CAP], (unsigned long long)c_cap);
+        atomicAdd(&cnt[HARM_FIXUP_GAMMA], (unsigned long long)c_gam);
+        atomicAdd(&cnt[HARM_FATAL], (unsigned long long)fatal);
+        *count = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// diag_mass (diag.c:477-580, MDOTTYPE=NEU_AGA_GHOST1) partial sums of the mass
+// flux through the first ghost columns, one block per phi plane.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_mdot_partials(DGrid g, const double *__restrict__ p, double *__restrict__ part)
+{
+    __shared__ double s_in[TPB], s_out[TPB];
+    const int h = blockIdx.x;
+    double a_in = 0., a_out = 0.;
+    int fatal = 0;
+    for (int j = g.js + threadIdx.x; j <= g.je; j += TPB) {
+        for (int side = 0; side < 2; side++) {
+            const int i = side ? g.ie + 1 : g.is - 1;
+            const size_t z = zidx(g, i, j, h);
+            double pr[NPR], ucon[4];
+            for (int k = 0; k < NPR; k++) pr[k] = p[k * g.nz + z];
+            Geom geom;
+            load_geom(g.geom[CENT], g.ncol, (size_t)i * g.SN2 + j, geom);
+            ucon_calc(pr, geom, ucon, fatal);
+            const double v = pr[RHO] * ucon[1] * geom.g * g.dx2 * g.dx3;
+            if (side) a_out -= v;
+            else a_in += v;
+        }
+    }
+    s_in[threadIdx.x] = a_in;
+    s_out[threadIdx.x] = a_out;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s_in[threadIdx.x] += s_in[threadIdx.x + s];
+            s_out[threadIdx.x] += s_out[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * h] = s_in[0];
+        part[2 * h + 1] = s_out[0];
+    }
+}
+
+// End of step_ch (step_ch.c:119-137): mass budget, dt control.  One thread.
+// tstate: [0] t, [1] dt, [2] mass_in, [3] mass_out, [4] error flags.
+__global__ void k_stepend(DGrid g, double *tstate, unsigned long long *ndt_slot, const double *part, int nparts)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mdot_in = 0., mdot_out = 0.;
+    for (int q = 0; q < nparts; q++) {
+        mdot_in += part[2 * q];
+        mdot_out += part[2 * q + 1];
+    }
+    const double dt = tstate[1];
+    tstate[2] += mdot_in * dt;
+    tstate[3] += mdot_out * dt;
+    const double ndt1 = __longlong_as_double((long long)ndt_slot[0]);
+    const double ndt2 = __longlong_as_double((long long)ndt_slot[1]);
+    const double ndt3 = __longlong_as_double((long long)ndt_slot[2]);
+    double ndt;
+    if (g.N3 > 2)
+        ndt = g.defcon * 1. / (1. / ndt1 + 1. / ndt2 + 1. / ndt3);
+    else
+        ndt = g.defcon * 1. / (1. / ndt1 + 1. / ndt2);
+    if (dt < 1.e-9) {
+        tstate[4] = 11.;
+        return;
+    }
+    double t = tstate[0] + dt;
+    if (ndt > 1.3 * dt) ndt = 1.3 * dt;
+    double ndt_out = ndt;
+    if (t + ndt_out > g.tf) ndt_out = g.tf - t;
+    tstate[0] = t;
+    tstate[1] = ndt_out;
+    const unsigned long long big = (unsigned long long)__double_as_longlong(1.e9);
+    ndt_slot[0] = big;
+    ndt_slot[1] = big;
+    ndt_slot[2] = big;
+}
+
+// ---------------------------------------------------------------------------
+// Initial data (init.c:180-472) on the device; the axisymmetric torus values come
+// from the host, the per-zone KISS perturbation (random.c:44-55) is drawn here.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double kiss_uniform(uint32_t z, uint32_t w, uint32_t jsr, uint32_t jcong)
+{
+    z = 36969u * (z & 65535u) + (z >> 16);
+    w = 18000u * (w & 65535u) + (w >> 16);
+    const uint32_t mwc = (z << 16) + w;
+    jcong = 69069u * jcong + 1234567u;
+    jsr ^= (jsr << 17);
+    jsr ^= (jsr >> 13);
+    jsr ^= (jsr << 5);
+    const uint32_t kiss = (mwc ^ jcong) + jsr;
+    const double u = ((double)kiss) / 4294967295.0;
+    return 0.0 + (1.0 - 0.0) * u;
+}
+
+__global__ void __launch_bounds__(TPB) k_init_expand(DGrid g, double *__restrict__ p, const double *__restrict__ tor,
+                                                     double rhomax, double pert_amp, uint4 seed)
+{
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx >= (size_t)ni * nj * g.N3) return;
+    const int h = (int)(idx % g.N3);
+    const size_t rest = idx / g.N3;
+    const int j = g.js + (int)(rest % nj);
+    const int i = g.is + (int)(rest / nj);
+    const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j, nc = g.ncol;
+    double rho = tor[col], u = tor[nc + col];
+    double u1 = 0., u2 = 0., u3 = 0.;
+    if (tor[5 * nc + col] != 0.) {
+        const unsigned int ii = (unsigned int)(i - 2), jj = (unsigned int)(j - 2);
+        const unsigned int hh = (g.N3 == 1) ? 0u : (unsigned int)h;
+        const unsigned int off = hh * ii * jj + ii * jj + jj;
+        const double rnd = kiss_uniform(seed.x + off, seed.y + off, seed.z + off, seed.w + off);
+        u = u * (1. + pert_amp * (rnd - 0.5));
+        u1 = tor[2 * nc + col];
+        u2 = tor[3 * nc + col];
+        u3 = tor[4 * nc + col];
+    }
+    p[RHO * nz + z] = rho / rhomax;
+    p[UU * nz + z] = u / rhomax;
+    p[U1 * nz + z] = u1;
+    p[U2 * nz + z] = u2;
+    p[U3 * nz + z] = u3;
+    p[B1 * nz + z] = 0.;
+    p[B2 * nz + z] = 0.;
+    p[B3 * nz + z] = 0.;
+}
+
+// Corner vector potential A_phi = max(<rho>/rho_max - 0.2, 0) (init.c:330-365).
+__global__ void __launch_bounds__(TPB) k_init_A(DGrid g, const double *__restrict__ p, double *__restrict__ A)
+{
+    const int ni = g.ie + 1 - g.is + 1, nj = g.je + 1 - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx >= (size_t)ni * nj * g.N3) return;
+    const int h = (int)(idx % g.N3);
+    const size_t rest = idx / g.N3;
+    const int j = g.js + (int)(rest % nj);
+    const int i = g.is + (int)(rest / nj);
+    const double *r = p + RHO * g.nz;
+    const double rho_av = 0.25 * (r[zidx(g, i, j, h)] + r[zidx(g, i - 1, j, h)] + r[zidx(g, i, j - 1, h)] +
+                                  r[zidx(g, i - 1, j - 1, h)]);
+    const double q = rho_av / 1. - 0.2;
+    A[zidx(g, i, j, h)] = (q > 0.) ? q : 0.;
+}
+
+// B = curl A at zone centres and b^2 (init.c:374-386); per-block max of b^2.
+__global__ void __launch_bounds__(TPB) k_init_B(DGrid g, double *__restrict__ p, const double *__restrict__ A,
+                                                double norm, int apply_norm, double *__restrict__ bsq_part)
+{
+    __shared__ double s[TPB];
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    double bsq = 0.;
+    if (idx < (size_t)ni * nj * g.N3) {
+        const int h = (int)(idx % g.N3);
+        const size_t rest = idx / g.N3;
+        const int j = g.js + (int)(rest % nj);
+        const int i = g.is + (int)(rest / nj);
+        const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
+        Geom geom;
+        load_geom(g.geom[CENT], g.ncol, col, geom);
+        double pr[NPR];
+        for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+        if (apply_norm) {
+            pr[B1] *= norm;
+            pr[B2] *= norm;
+        } else {
+            const double a00 = A[zidx(g, i, j, h)], a01 = A[zidx(g, i, j + 1, h)];
+            const double a10 = A[zidx(g, i + 1, j, h)], a11 = A[zidx(g, i + 1, j + 1, h)];
+            pr[B1] = -(a00 - a01 + a10 - a11) / (2. * g.dx2 * geom.g);
+            pr[B2] = (a00 + a01 - a10 - a11) / (2. * g.dx1 * geom.g);
+            pr[B3] = 0.;
+        }
+        int fatal = 0;
+        State q;
+        get_state(pr, geom, q, fatal);
+        bsq = dot4(q.bcon, q.bcov);
+        for (int k = B1; k <= B3; k++) p[k * nz + z] = pr[k];
+    }
+    s[threadIdx.x] = bsq;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) s[threadIdx.x] = (s[threadIdx.x + st] > s[threadIdx.x]) ? s[threadIdx.x + st] : s[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsq_part[blockIdx.x] = s[0];
+}
+
+// Diagnostics (diag.c:115-209): per-block sums of mass and energy, max |divB|.
+__global__ void __launch_bounds__(TPB) k_diag(DGrid g, const double *__restrict__ p, double *__restrict__ part)
+{
+    __shared__ double s0[TPB], s1[TPB], s2[TPB];
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    double mass = 0., e = 0., divb = 0.;
+    if (idx < (size_t)ni * nj * g.N3) {
+        const int h = (int)(idx % g.N3);
+        const size_t rest = idx / g.N3;
+        const int j = g.js + (int)(rest % nj);
+        const int i = g.is + (int)(rest / nj);
+        const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
+        Geom geom;
+        load_geom(g.geom[CENT], g.ncol, col, geom);
+        double pr[NPR], U[NPR];
+        for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+        int fatal = 0;
+        State q;
+        get_state(pr, geom, q, fatal);
+        primtoflux(pr, q, 0, geom, g.gam, U);
+        mass = pr[RHO] * geom.g * g.dx1 * g.dx2 * g.dx3;
+        e = U[UU] * g.dx1 * g.dx2 * g.dx3;
+        const double *GD = g.geom[CENT] + 32 * g.ncol;
+        const double gd00 = GD[col], gdjm = GD[col - 1], gdim = GD[col - g.SN2], gdimjm = GD[col - g.SN2 - 1];
+        auto P = [&](int k, int ii, int jj, int hh) { return p[k * nz + zidx(g, ii, jj, hh)]; };
+        if (g.N3 > 2) {
+            const int hm = (h + g.N3 - 1) % g.N3;
+            divb = fabs(0.25 *
+                            (P(B1, i, j, h) * gd00 + P(B1, i, j, hm) * gd00 + P(B1, i, j - 1, h) * gdjm +
+                             P(B1, i, j - 1, hm) * gdjm - P(B1, i - 1, j, h) * gdim - P(B1, i - 1, j, hm) * gdim -
+                             P(B1, i - 1, j - 1, h) * gdimjm - P(B1, i - 1, j - 1, hm) * gdimjm) /
+                            g.dx1 +
+                        0.25 *
+                            (P(B2, i, j, h) * gd00 + P(B2, i, j, hm) * gd00 + P(B2, i - 1, j, h) * gdim +
+                             P(B2, i - 1, j, hm) * gdim - P(B2, i, j - 1, h) * gdjm - P(B2, i, j - 1, hm) * gdjm -
+                             P(B2, i - 1, j - 1, h) * gdimjm - P(B2, i - 1, j - 1, hm) * gdimjm) /
+                            g.dx2 +
+                        0.25 *
+                            (P(B3, i, j, h) * gd00 + P(B3, i - 1, j, h) * gdim + P(B3, i, j - 1, h) * gdjm +
+                             P(B3, i - 1, j - 1, h) * gdimjm - P(B3, i, j, hm) * gd00 - P(B3, i - 1, j, hm) * gdim -
+                             P(B3, i, j - 1, hm) * gdjm - P(B3, i - 1, j - 1, hm) * gdimjm) /
+                            g.dx3);
+        } else {
+            divb = fabs(0.5 *
+                            (P(B1, i, j, h) * gd00 + P(B1, i, j - 1, h) * gdjm - P(B1, i - 1, j, h) * gdim -
+                             P(B1, i - 1, j - 1, h) * gdimjm) /
+                            g.dx1 +
+                        0.5 *
+                            (P(B2, i, j, h) * gd00 + P(B2, i - 1, j, h) * gdim - P(B2, i, j - 1, h) * gdjm -
+                             P(B2, i - 1, j - 1, h) * gdimjm) /
+                            g.dx2);
+        }
+    }
+    s0[threadIdx.x] = mass;
+    s1[threadIdx.x] = e;
+    s2[threadIdx.x] = divb;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            s0[threadIdx.x] += s0[threadIdx.x + st];
+            s1[threadIdx.x] += s1[threadIdx.x + st];
+            s2[threadIdx.x] = (s2[threadIdx.x + st] > s2[threadIdx.x]) ? s2[threadIdx.x + st] : s2[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0[0];
+        part[3 * blockIdx.x + 1] = s1[0];
+        part[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+
+// Batched Noble 2D inversion over SoA arrays (utoprim_2d.c:126, plus the Font fix).
+__global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U, const double *__restrict__ gcov,
+                                                   const double *__restrict__ gcon, const double *__restrict__ gdet,
+                                                   double *__restrict__ prim, int8_t *__restrict__ ret,
+                                                   uint8_t *__restrict__ iters, uint8_t *__restrict__ kconv, size_t n,
+                                                   double gam, int font_fix, double K, int *__restrict__ ret_full)
+{
+    const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (z >= n) return;
+    Geom geom;
+#pragma unroll
+    for (int a = 0; a < 16; a++) {
+        geom.gcov[a / 4][a % 4] = gcov[(size_t)a * n + z];
+        geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
+    }
+    geom.g = gdet[z];
+    double Uz[NPR], pr[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        Uz[k] = U[(size_t)k * n + z];
+        pr[k] = prim[(size_t)k * n + z];
+    }
+    int niter, kc;
+    int r = utoprim_2d(Uz, geom, gam, pr, niter, kc);
+    int8_t flag = (int8_t)r;
+    if (flag && font_fix) {
+        flag = (int8_t)utoprim_1dvsq2fix1(Uz, geom, gam, K, pr);
+        if (flag) flag = (int8_t)utoprim_1dfix1(Uz, geom, gam, K, pr);
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) prim[(size_t)k * n + z] = pr[k];
+    ret[z] = flag;
+    if (ret_full) ret_full[z] = r;
+    if (iters) iters[z] = (uint8_t)niter;
+    if (kconv) kconv[z] = (uint8_t)kc;
+}
+
+__global__ void __launch_bounds__(TPB) k_pow(const double *__restrict__ x, const double *__restrict__ y,
+                                             double *__restrict__ out, size_t n)
+{
+    const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (z < n) out[z] = cr_pow(x[z], y[z]);
+}
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+}  // namespace
+
+// ===========================================================================
+// Context
+// ===========================================================================
+struct harm_ctx {
+    harm_params P;
+    harm_host::Grid hg;
+    harm_host::GeomTables tab;
+    DGrid g;
+    int device;
+    hipStream_t stream;
+    size_t nz;
+    double *p, *ph, *F1, *F2, *F3, *e1, *e2, *e3;
+    int8_t *pflag, *pflag2;
+    double *d_geom[5], *d_conn, *d_floors, *d_katm;
+    double *tstate;                   // t, dt, mass_in, mass_out, err
+    unsigned long long *ndt_slot;     // 3
+    unsigned long long *cnt;          // HARM_NCOUNTERS
+    double *part;                     // reduction partials
+    size_t npart;
+    int *fix_list, *fix_count, *fix_err;
+    int fix_cap;
+    bool timing;
+    struct Ev {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> ev_live, ev_pool;
+    double ms[HARM_NKERNELS];
+    long long launches[HARM_NKERNELS];
+};
+
+namespace {
+
+int ev_begin(harm_ctx *c, int cls, harm_ctx::Ev &e)
+{
+    if (!c->timing) return 0;
+    if (c->ev_pool.empty()) {
+        harm_ctx::Ev n;
+        HIPCHK(hipEventCreate(&n.a));
+        HIPCHK(hipEventCreate(&n.b));
+        c->ev_pool.push_back(n);
+    }
+    e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    e.cls = cls;
+    HIPCHK(hipEventRecord(e.a, c->stream));
+    return 0;
+}
+int ev_end(harm_ctx *c, harm_ctx::Ev &e)
+{
+    if (!c->timing) return 0;
+    HIPCHK(hipEventRecord(e.b, c->stream));
+    c->ev_live.push_back(e);
+    return 0;
+}
+int ev_collect(harm_ctx *c)
+{
+    if (c->ev_live.empty()) return 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &e : c->ev_live) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+        c->ms[e.cls] += ms;
+        c->launches[e.cls]++;
+        c->ev_pool.push_back(e);
+    }
+    c->ev_live.clear();
+    return 0;
+}
+
+#define LAUNCH(cls, ...)                                       \
+    do {                                                       \
+        harm_ctx::Ev ev_;                                      \
+        int rc_ = ev_begin(c, cls, ev_);                       \
+        if (rc_) return rc_;                                   \
+        __VA_ARGS__;                                           \
+        HIPCHK(hipGetLastError());                             \
+        rc_ = ev_end(c, ev_);                                  \
+        if (rc_) return rc_;                                   \
+    } while (0)
+
+int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
+{
+    const DGrid &g = c->g;
+    const int i0 = (dir == 1) ? g.is : g.is - 1;
+    const int j0 = (dir == 2) ? g.js : g.js - 1;
+    const size_t total = (size_t)(g.ie + 2 - i0) * (g.je + 2 - j0) * g.N3;
+    if (dir == 1)
+        LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL(k_flux<1>, dim3(nblocks(total)), dim3(TPB), 0, c->stream, g, pb, F,
+                                                c->ndt_slot + 0, fullstep, c->cnt));
+    else if (dir == 2)
+        LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL(k_flux<2>, dim3(nblocks(total)), dim3(TPB), 0, c->stream, g, pb, F,
+                                                c->ndt_slot + 1, fullstep, c->cnt));
+    else
+        LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL(k_flux<3>, dim3(nblocks(total)), dim3(TPB), 0, c->stream, g, pb, F,
+                                                c->ndt_slot + 2, fullstep, c->cnt));
+    return 0;
+}
+
+int fluxes_and_emf(harm_ctx *c, const double *pb, int fullstep)
+{
+    const DGrid &g = c->g;
+    int rc;
+    if ((rc = flux_launch(c, 1, pb, c->F1, fullstep))) return rc;
+    if ((rc = flux_launch(c, 2, pb, c->F2, fullstep))) return rc;
+    if (g.N3 > 2 && (rc = flux_launch(c, 3, pb, c->F3, fullstep))) return rc;
+    const size_t ne = (size_t)(g.ie + 2 - g.is) * (g.je + 2 - g.js) * g.N3;
+    LAUNCH(HARM_K_EMF, hipLaunchKernelGGL(k_emf, dim3(nblocks(ne)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
+                                          c->e1, c->e2, c->e3));
+    return 0;
+}
+
+int bound(harm_ctx *c, double *p)
+{
+    const DGrid &g = c->g;
+    LAUNCH(HARM_K_BOUND, hipLaunchKernelGGL(k_bound, dim3(nblocks((size_t)g.SN1 * g.N3)), dim3(TPB), 0, c->stream, g,
+                                            p, c->pflag, c->cnt));
+    return 0;
+}
+
+int fixup_utoprim(harm_ctx *c, double *p)
+{
+    const DGrid &g = c->g;
+    LAUNCH(HARM_K_FIXUTOPRIM,
+           hipLaunchKernelGGL(k_fixutoprim_scan, dim3(nblocks(g.nz)), dim3(TPB), 0, c->stream, g, p, c->pflag,
+                              c->pflag2, c->fix_list, c->fix_count, c->fix_cap, c->cnt);
+           hipLaunchKernelGGL(k_fixutoprim_serial, dim3(1), dim3(1024), 0, c->stream, g, p, c->pflag2, c->fix_list,
+                              c->fix_count, c->fix_cap, c->cnt, c->fix_err));
+    std::swap(c->pflag, c->pflag2);
+    return 0;
+}
+
+// advance() (step_ch.c:153-318) followed by fixup(), bound_prim(),
+// fixup_utoprim(), bound_prim() (step_ch.c:103-117).
+int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int fullstep)
+{
+    const DGrid &g = c->g;
+    int rc;
+    if ((rc = fluxes_and_emf(c, pb, fullstep))) return rc;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL(k_update, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi, pb, pf,
+                                             c->pflag, c->F1, c->F2, c->F3, c->e1, c->e2, c->e3, c->tstate, dtfac,
+                                             c->cnt));
+    if ((rc = bound(c, pf))) return rc;
+    if ((rc = fixup_utoprim(c, pf))) return rc;
+    if ((rc = bound(c, pf))) return rc;
+    return 0;
+}
+
+int one_step(harm_ctx *c)
+{
+    int rc;
+    const DGrid &g = c->g;
+    if ((rc = half_step(c, c->p, c->p, c->ph, 0.5, 0))) return rc;
+    if ((rc = half_step(c, c->p, c->ph, c->p, 1.0, 1))) return rc;
+    LAUNCH(HARM_K_STEPEND,
+           hipLaunchKernelGGL(k_mdot_partials, dim3(g.N3), dim3(TPB), 0, c->stream, g, c->p, c->part);
+           hipLaunchKernelGGL(k_stepend, dim3(1), dim3(1), 0, c->stream, g, c->tstate, c->ndt_slot, c->part, g.N3));
+    return 0;
+}
+
+int check_flags(harm_ctx *c)
+{
+    double ts[5];
+    int fe = 0;
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&fe, c->fix_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ts[4] == 11.) return set_err("timestep too small (dt < 1e-9, step_ch.c:126)", 11);
+    if (fe) return set_err("fixup_utoprim: more clustered bad zones than the serial list holds", 12);
+    return 0;
+}
+
+template <class T>
+int dalloc(T **p, size_t n)
+{
+    HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return 0;
+}
+
+int ctx_alloc(harm_ctx *c)
+{
+    const harm_params &P = c->P;
+    auto &hg = c->hg;
+    hg.N1 = P.n1;
+    hg.N2 = P.n2;
+    hg.N3 = P.n3;
+    hg.SN1 = P.n1 + 4;
+    hg.SN2 = P.n2 + 4;
+    hg.is = 2;
+    hg.ie = P.n1 + 1;
+    hg.js = 2;
+    hg.je = P.n2 + 1;
+    if (harm_host::build_geometry(P, hg, c->tab)) return set_err("metric setup failed (singular metric)", 3);
+    const size_t ncol = (size_t)hg.SN1 * hg.SN2;
+    c->nz = ncol * hg.N3;
+    int rc;
+    for (double **q : {&c->p, &c->ph, &c->F1, &c->F2, &c->F3})
+        if ((rc = dalloc(q, 8 * c->nz))) return rc;
+    if ((rc = dalloc(&c->e3, c->nz))) return rc;
+    if (hg.N3 > 2) {
+        if ((rc = dalloc(&c->e1, c->nz))) return rc;
+        if ((rc = dalloc(&c->e2, c->nz))) return rc;
+    }
+    if ((rc = dalloc(&c->pflag, c->nz))) return rc;
+    if ((rc = dalloc(&c->pflag2, c->nz))) return rc;
+    for (int l = 0; l < 5; l++) {
+        if ((rc = dalloc(&c->d_geom[l], 33 * ncol))) return rc;
+        HIPCHK(hipMemcpy(c->d_geom[l], c->tab.geom[l].data(), 33 * ncol * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if ((rc = dalloc(&c->d_conn, 64 * ncol))) return rc;
+    HIPCHK(hipMemcpy(c->d_conn, c->tab.conn.data(), 64 * ncol * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = dalloc(&c->d_floors, 2 * ncol))) return rc;
+    HIPCHK(hipMemcpy(c->d_floors, c->tab.floors.data(), 2 * ncol * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = dalloc(&c->d_katm, hg.SN1))) return rc;
+    HIPCHK(hipMemcpy(c->d_katm, c->tab.katm.data(), hg.SN1 * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = dalloc(&c->tstate, 8))) return rc;
+    if ((rc = dalloc(&c->ndt_slot, 3))) return rc;
+    if ((rc = dalloc(&c->cnt, HARM_NCOUNTERS))) return rc;
+    c->npart = std::max<size_t>(3 * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
+    if ((rc = dalloc(&c->part, c->npart))) return rc;
+    c->fix_cap = 1 << 16;
+    if ((rc = dalloc(&c->fix_list, 2 * (size_t)c->fix_cap))) return rc;
+    if ((rc = dalloc(&c->fix_count, 1))) return rc;
+    if ((rc = dalloc(&c->fix_err, 1))) return rc;
+    unsigned long long big[3];
+    const double b = 1.e9;
+    for (int k = 0; k < 3; k++) std::memcpy(&big[k], &b, 8);
+    HIPCHK(hipMemcpy(c->ndt_slot, big, sizeof(big), hipMemcpyHostToDevice));
+    double ts[8] = {0., P.dt0, 0., 0., 0., 0., 0., 0.};
+    HIPCHK(hipMemcpy(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice));
+
+    DGrid &g = c->g;
+    g.N1 = hg.N1;
+    g.N2 = hg.N2;
+    g.N3 = hg.N3;
+    g.SN1 = hg.SN1;
+    g.SN2 = hg.SN2;
+    g.is = hg.is;
+    g.ie = hg.ie;
+    g.js = hg.js;
+    g.je = hg.je;
+    g.lim = P.lim;
+    g.font_fix = P.font_fix;
+    g.polefix = (P.n3 > 1 && 2 < P.n2 / 2) ? 2 : 0;  // decs.h:91-95, bounds.c:149
+    g.ncol = ncol;
+    g.nz = c->nz;
+    g.dx1 = hg.dx[1];
+    g.dx2 = hg.dx[2];
+    g.dx3 = hg.dx[3];
+    g.gam = P.gam;
+    g.cour = P.cour;
+    g.hllf = P.hllf;
+    g.laxf = P.laxf;
+    g.defcon = P.defcon;
+    g.tf = P.tf;
+    for (int l = 0; l < 5; l++) g.geom[l] = c->d_geom[l];
+    g.conn = c->d_conn;
+    g.floors = c->d_floors;
+    g.katm = c->d_katm;
+    return 0;
+}
+
+void ctx_free(harm_ctx *c)
+{
+    for (void *q : {(void *)c->p, (void *)c->ph, (void *)c->F1, (void *)c->F2, (void *)c->F3, (void *)c->e1,
+                    (void *)c->e2, (void *)c->e3, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
+                    (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->cnt,
+                    (void *)c->part, (void *)c->fix_list, (void *)c->fix_count, (void *)c->fix_err})
+        if (q) (void)hipFree(q);
+    for (int l = 0; l < 5; l++)
+        if (c->d_geom[l]) (void)hipFree(c->d_geom[l]);
+    for (auto &e : c->ev_pool) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto &e : c->ev_live) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Canonical layout <-> device layout: identical (SoA [v][i][j][h]), so the copies are flat.
+int copy_state(harm_ctx *c, double *host_prim, int8_t *host_flag, bool to_host)
+{
+    const size_t n = 8 * c->nz;
+    if (to_host) {
+        if (host_prim) HIPCHK(hipMemcpyAsync(host_prim, c->p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (host_flag) HIPCHK(hipMemcpyAsync(host_flag, c->pflag, c->nz, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        if (host_prim) HIPCHK(hipMemcpyAsync(c->p, host_prim, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if (host_flag) HIPCHK(hipMemcpyAsync(c->pflag, host_flag, c->nz, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *harm_last_error(void) { return g_err.c_str(); }
+
+harm_ctx *harm_create(const harm_params *p, int device)
+{
+    g_err.clear();
+    if (!p) {
+        set_err("null params");
+        return nullptr;
+    }
+    if (p->n1 < 4 || p->n2 < 8 || p->n3 < 1 || p->n3 == 2 || (p->n3 > 1 && (p->n3 % 2) != 0)) {
+        set_err("unsupported grid: need n1 >= 4, n2 >= 8, n3 == 1 or even n3 >= 4 (mpi.c:376)");
+        return nullptr;
+    }
+    if (p->lim < 0 || p->lim > 2) {
+        set_err("unknown slope limiter");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_err("no HIP device available: libharmcool.so runs only on a gfx950 GPU");
+        return nullptr;
+    }
+    if (device >= 0 && hipSetDevice(device) != hipSuccess) {
+        set_err("hipSetDevice failed");
+        return nullptr;
+    }
+    harm_ctx *c = new harm_ctx();
+    std::memset(c->ms, 0, sizeof(c->ms));
+    std::memset(c->launches, 0, sizeof(c->launches));
+    c->P = *p;
+    if (c->P.Rin <= 0.) c->P.Rin = 0.98 * (1. + sqrt(1. - c->P.a * c->P.a));  // init.c:125
+    (void)hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_err("hipStreamCreate failed");
+        delete c;
+        return nullptr;
+    }
+    if (ctx_alloc(c)) {
+        std::string m = g_err;
+        ctx_free(c);
+        delete c;
+        g_err = m;
+        return nullptr;
+    }
+    return c;
+}
+
+void harm_destroy(harm_ctx *c)
+{
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    ctx_free(c);
+    delete c;
+}
+
+int harm_init_torus(harm_ctx *c)
+{
+    if (!c) return set_err("null context");
+    const DGrid &g = c->g;
+    std::vector<double> tor;
+    double rhomax, umax;
+    harm_host::build_torus2d(c->P, c->hg, c->tab, tor, rhomax, umax);
+    umax /= rhomax;
+    double *d_tor = nullptr;
+    int rc;
+    if ((rc = dalloc(&d_tor, tor.size()))) return rc;
+    HIPCHK(hipMemcpy(d_tor, tor.data(), tor.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(c->p, 0, 8 * c->nz * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->pflag, 0, c->nz, c->stream));
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    uint4 seed = make_uint4(c->P.seed[0], c->P.seed[1], c->P.seed[2], c->P.seed[3]);
+    hipLaunchKernelGGL(k_init_expand, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, d_tor, rhomax,
+                       c->P.pert_amp, seed);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag);
+    if ((rc = bound(c, c->p))) return rc;
+    const size_t nA = (size_t)(g.ie + 2 - g.is) * (g.je + 2 - g.js) * g.N3;
+    HIPCHK(hipMemsetAsync(c->e3, 0, c->nz * sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_init_A, dim3(nblocks(nA)), dim3(TPB), 0, c->stream, g, c->p, c->e3);
+    const unsigned nb = nblocks(na);
+    hipLaunchKernelGGL(k_init_B, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->e3, 1.0, 0, c->part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(nb);
+    HIPCHK(hipMemcpyAsync(part.data(), c->part, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double bsq_max = 0.;
+    for (double v : part)
+        if (v > bsq_max) bsq_max = v;
+    const double beta_act = (c->P.gam - 1.) * umax / (0.5 * bsq_max);  // init.c:391
+    const double norm = sqrt(beta_act / c->P.beta);
+    hipLaunchKernelGGL(k_init_B, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->e3, norm, 1, c->part);
+    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag);
+    HIPCHK(hipGetLastError());
+    if ((rc = bound(c, c->p))) return rc;
+    double ts[8] = {0., c->P.dt0, 0., 0., 0., 0., 0., 0.};
+    HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(d_tor));
+    return 0;
+}
+
+int harm_step(harm_ctx *c, int nsteps)
+{
+    if (!c) return set_err("null context");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = one_step(c);
+        if (rc) return rc;
+    }
+    int rc = ev_collect(c);
+    if (rc) return rc;
+    return check_flags(c);
+}
+
+int harm_get_prims(harm_ctx *c, double *prim, int8_t *pflag) { return c ? copy_state(c, prim, pflag, true) : 1; }
+int harm_set_prims(harm_ctx *c, const double *prim, const int8_t *pflag)
+{
+    return c ? copy_state(c, const_cast<double *>(prim), const_cast<int8_t *>(pflag), false) : 1;
+}
+
+int harm_get_time(harm_ctx *c, double *t, double *dt)
+{
+    double ts[2];
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *t = ts[0];
+    *dt = ts[1];
+    return 0;
+}
+
+int harm_set_time(harm_ctx *c, double t, double dt)
+{
+    double ts[2] = {t, dt};
+    HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_get_counters(harm_ctx *c, int64_t *out)
+{
+    unsigned long long v[HARM_NCOUNTERS];
+    HIPCHK(hipMemcpyAsync(v, c->cnt, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < HARM_NCOUNTERS; k++) out[k] = (int64_t)v[k];
+    return 0;
+}
+
+int harm_reset_counters(harm_ctx *c)
+{
+    HIPCHK(hipMemsetAsync(c->cnt, 0, HARM_NCOUNTERS * sizeof(unsigned long long), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_diag(harm_ctx *c, double *out)
+{
+    const DGrid &g = c->g;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    const unsigned nb = nblocks(na);
+    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(3 * (size_t)nb), md(2 * (size_t)g.N3);
+    HIPCHK(hipMemcpyAsync(part.data(), c->part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_mdot_partials, dim3(g.N3), dim3(TPB), 0, c->stream, g, c->p, c->part);
+    HIPCHK(hipMemcpyAsync(md.data(), c->part, md.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double ts[4];
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double m = 0., e = 0., dv = 0., mi = 0., mo = 0.;
+    for (unsigned b = 0; b < nb; b++) {
+        m += part[3 * b];
+        e += part[3 * b + 1];
+        dv = std::max(dv, part[3 * b + 2]);
+    }
+    for (int h = 0; h < g.N3; h++) {
+        mi += md[2 * h];
+        mo += md[2 * h + 1];
+    }
+    out[0] = m;
+    out[1] = e;
+    out[2] = mi;
+    out[3] = mo;
+    out[4] = ts[2];
+    out[5] = ts[3];
+    out[6] = dv;
+    return 0;
+}
+
+int harm_device_state(harm_ctx *c, double **prim, int8_t **pflag, void **stream)
+{
+    if (prim) *prim = c->p;
+    if (pflag) *pflag = c->pflag;
+    if (stream) *stream = (void *)c->stream;
+    return 0;
+}
+
+int harm_set_timing(harm_ctx *c, int enable)
+{
+    int rc = ev_collect(c);
+    if (rc) return rc;
+    c->timing = enable != 0;
+    std::memset(c->ms, 0, sizeof(c->ms));
+    std::memset(c->launches, 0, sizeof(c->launches));
+    return 0;
+}
+
+int harm_get_timing(harm_ctx *c, double *ms, int64_t *launches)
+{
+    int rc = ev_collect(c);
+    if (rc) return rc;
+    for (int k = 0; k < HARM_NKERNELS; k++) {
+        if (ms) ms[k] = c->ms[k];
+        if (launches) launches[k] = c->launches[k];
+    }
+    return 0;
+}
+
+int harm_debug_fluxes(harm_ctx *c, int fullstep, double *F1, double *F2, double *F3, double *ndt3)
+{
+    int rc;
+    const DGrid &g = c->g;
+    if ((rc = fluxes_and_emf(c, c->p, fullstep))) return rc;
+    // materialize the flux_ct rewrite on the host from the EMFs (step_ch.c:541-554, 783-797)
+    const size_t n = 8 * c->nz;
+    std::vector<double> e1, e2, e3(c->nz);
+    HIPCHK(hipMemcpyAsync(F1, c->F1, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(F2, c->F2, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (g.N3 > 2) HIPCHK(hipMemcpyAsync(F3, c->F3, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(e3.data(), c->e3, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (g.N3 > 2) {
+        e1.resize(c->nz);
+        e2.resize(c->nz);
+        HIPCHK(hipMemcpyAsync(e1.data(), c->e1, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(e2.data(), c->e2, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    unsigned long long nd[3];
+    HIPCHK(hipMemcpyAsync(nd, c->ndt_slot, sizeof(nd), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; k++) std::memcpy(&ndt3[k], &nd[k], 8);
+    unsigned long long big;
+    const double b = 1.e9;
+    std::memcpy(&big, &b, 8);
+    unsigned long long bigs[3] = {big, big, big};
+    HIPCHK(hipMemcpy(c->ndt_slot, bigs, sizeof(bigs), hipMemcpyHostToDevice));
+    auto Z = [&](int i, int j, int h) { return ((size_t)i * g.SN2 + j) * g.N3 + h; };
+    auto hp = [&](int h) { return (h + 1) % g.N3; };
+    const size_t nz = c->nz;
+    for (int h = 0; h < g.N3; h++) {
+        for (int i = g.is; i <= g.ie + 1; i++)
+            for (int j = g.js; j <= g.je; j++) {
+                F1[B1 * nz + Z(i, j, h)] = 0.;
+                F1[B2 * nz + Z(i, j, h)] = 0.5 * (e3[Z(i, j, h)] + e3[Z(i, j + 1, h)]);
+                if (g.N3 > 2) F1[B3 * nz + Z(i, j, h)] = -0.5 * (e2[Z(i, j, h)] + e2[Z(i, j, hp(h))]);
+            }
+        for (int i = g.is; i <= g.ie; i++)
+            for (int j = g.js; j <= g.je + 1; j++) {
+                F2[B1 * nz + Z(i, j, h)] = -0.5 * (e3[Z(i, j, h)] + e3[Z(i + 1, j, h)]);
+                F2[B2 * nz + Z(i, j, h)] = 0.;
+                if (g.N3 > 2) F2[B3 * nz + Z(i, j, h)] = 0.5 * (e1[Z(i, j, h)] + e1[Z(i, j, hp(h))]);
+            }
+        if (g.N3 > 2)
+            for (int i = g.is; i <= g.ie; i++)
+                for (int j = g.js; j <= g.je; j++) {
+                    F3[B1 * nz + Z(i, j, h)] = 0.5 * (e2[Z(i, j, h)] + e2[Z(i + 1, j, h)]);
+                    F3[B2 * nz + Z(i, j, h)] = -0.5 * (e1[Z(i, j, h)] + e1[Z(i, j + 1, h)]);
+                    F3[B3 * nz + Z(i, j, h)] = 0.;
+                }
+    }
+    return 0;
+}
+
+int harm_debug_bound(harm_ctx *c)
+{
+    int rc = bound(c, c->p);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_debug_fixup_utoprim(harm_ctx *c)
+{
+    int rc = fixup_utoprim(c, c->p);
+    if (rc) return rc;
+    if ((rc = bound(c, c->p))) return rc;
+    return check_flags(c);
+}
+
+int harm_get_geometry(harm_ctx *c, double *gcov, double *gcon, double *gdet, double *conn)
+{
+    const size_t ncol = c->g.ncol;
+    for (int l = 0; l < 5; l++) {
+        const std::vector<double> &T = c->tab.geom[l];
+        if (gcov) std::memcpy(gcov + (size_t)l * 16 * ncol, T.data(), 16 * ncol * sizeof(double));
+        if (gcon) std::memcpy(gcon + (size_t)l * 16 * ncol, T.data() + 16 * ncol, 16 * ncol * sizeof(double));
+        if (gdet) std::memcpy(gdet + (size_t)l * ncol, T.data() + 32 * ncol, ncol * sizeof(double));
+    }
+    if (conn) std::memcpy(conn, c->tab.conn.data(), 64 * ncol * sizeof(double));
+    return 0;
+}
+
+int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet, double *prim,
+                         int8_t *ret, uint8_t *iters, uint8_t *kconv, size_t n, const harm_u2p_opts *opts,
+                         void *stream)
+{
+    if (!opts) return set_err("null options");
+    if (n == 0) return 0;
+    if (opts->guess_mode != 0) return set_err("guess_mode 1 needs the MLP entry point (harm_mlp_*)", 4);
+    hipLaunchKernelGGL(k_c2p_batch, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon, gdet, prim,
+                       ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int harm_debug_pow(const double *x, const double *y, double *out, size_t n)
+{
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, 3 * n * sizeof(double)));
+    HIPCHK(hipMemcpy(d, x, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + n, y, n * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_pow, dim3(nblocks(n)), dim3(TPB), 0, 0, d, d + n, d + 2 * n, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d));
+    return 0;
+}
+
+static double g_gamma = 4. / 3.;
+static std::mutex g_u2p_mu;
+
+void harm_set_gamma(double gam) { g_gamma = gam; }
+
+int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8])
+{
+    std::lock_guard<std::mutex> lk(g_u2p_mu);
+    static double *dbuf = nullptr;
+    static int8_t *dret = nullptr;
+    static int *dfull = nullptr;
+    if (!dbuf) {
+        HIPCHK(hipMalloc((void **)&dbuf, 49 * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&dret, 8));
+        HIPCHK(hipMalloc((void **)&dfull, sizeof(int)));
+    }
+    double h[49];
+    std::memcpy(h, U, 8 * sizeof(double));
+    std::memcpy(h + 8, gcov, 16 * sizeof(double));
+    std::memcpy(h + 24, gcon, 16 * sizeof(double));
+    h[40] = gdet;
+    std::memcpy(h + 41, prim, 8 * sizeof(double));
+    HIPCHK(hipMemcpy(dbuf, h, sizeof(h), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_c2p_batch, dim3(1), dim3(TPB), 0, 0, dbuf, dbuf + 8, dbuf + 24, dbuf + 40, dbuf + 41, dret,
+                       (uint8_t *)nullptr, (uint8_t *)nullptr, (size_t)1, g_gamma, 0, 0., dfull);
+    HIPCHK(hipGetLastError());
+    int r;
+    HIPCHK(hipMemcpy(prim, dbuf + 41, 8 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&r, dfull, sizeof(int), hipMemcpyDeviceToHost));
+    return r;
+}
+
+}  // extern "C"

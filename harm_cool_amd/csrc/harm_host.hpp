@@ -1,0 +1,29 @@
+// harm_host.hpp -- host-side grid description and setup tables of libharmcool.so.
+#pragma once
+#include <cstddef>
+#include <vector>
+
+#include "../../include/harm_cool.h"
+
+namespace harm_host {
+
+struct Grid {
+    int N1, N2, N3, SN1, SN2;
+    int is, ie, js, je;
+    double startx[4], dx[4];
+};
+
+// Tables uploaded once per context.  ncol = SN1*SN2, column col = i*SN2 + j.
+struct GeomTables {
+    std::vector<double> geom[5];  // per location (FACE1,FACE2,CORN,CENT,FACE3): [33][ncol]
+    std::vector<double> conn;     // [64][ncol], index (k*4+mu)*4+j = Gamma^k_{mu j}
+    std::vector<double> coords;   // [4][ncol] Boyer-Lindquist (t, r, theta, phi) at CENT
+    std::vector<double> floors;   // [2][ncol] radius-only rho and u floors
+    std::vector<double> katm;     // [SN1] Font-fix isentrope constant per radius
+};
+
+int build_geometry(const harm_params &P, Grid &g, GeomTables &t);
+int build_torus2d(const harm_params &P, const Grid &g, const GeomTables &t, std::vector<double> &tor, double &rhomax,
+                  double &umax);
+
+}  // namespace harm_host

@@ -1,0 +1,992 @@
+// harm_phys.hpp -- per-zone GRMHD physics for the gfx950 kernels (FP64).
+//
+// Every function keeps the reference's operation order (cited file:line) because
+// parity with the CPU path is bitwise: the library is compiled with
+// -ffp-contract=off, IEEE division and correctly rounded sqrt.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace harm {
+
+enum { RHO = 0, UU, U1, U2, U3, B1, B2, B3, NPR };
+enum { FACE1 = 0, FACE2 = 1, CORN = 2, CENT = 3, FACE3 = 4, NPG = 5 };
+
+// decs.h:147-171, u2p_defs.h:79-96
+constexpr double RHOMIN = 1.e-6, UUMIN = 1.e-8, RHOMINLIMIT = 1.e-20, UUMINLIMIT = 1.e-20;
+constexpr double BSQORHOMAX = 50., BSQOUMAX = 2500., UORHOMAX = 50.;
+constexpr double SMALL = 1.e-20, GAMMAMAX = 50.;
+constexpr int MAX_NEWT_ITER = 30, EXTRA_NEWT_ITER = 2;
+constexpr double NEWT_TOL = 1.0e-10, MIN_NEWT_TOL = 1.0e-10, NEWT_TOL2 = 1.0e-15;
+constexpr double W_TOO_BIG = 1.e20, UTSQ_TOO_BIG = 1.e20, FAIL_VAL = 1.e30;
+
+#define HD __device__ __forceinline__
+
+struct Geom {
+    double gcov[4][4];
+    double gcon[4][4];
+    double g;
+};
+
+struct State {
+    double ucon[4], ucov[4], bcon[4], bcov[4];
+};
+
+HD double dot4(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
+
+// ---------------------------------------------------------------------------
+// Correctly rounded pow for x > 0 (used where the reference calls glibc's pow in
+// the Font-fix solvers, utoprim_1dvsq2fix1.c:592, utoprim_1dfix1.c:832,849).
+// glibc's pow is accurate to 0.52 ulp, so a correctly rounded result equals it
+// except when the exact value lies within ~2^-68 of a rounding midpoint.  Method:
+// double-double arithmetic with Dekker/Knuth error-free transformations;
+// log(m) = 2 atanh((m-1)/(m+1)) by its series, exp by Taylor series on r/2^5 and
+// five squarings; total relative error ~2^-100 before the final rounding.
+// ---------------------------------------------------------------------------
+struct dd {
+    double hi, lo;
+};
+HD dd dd_two_sum(double a, double b)
+{
+    double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+HD dd dd_fast(double a, double b)
+{
+    double s = a + b;
+    return {s, b - (s - a)};
+}
+HD dd dd_add(dd a, dd b)
+{
+    dd s = dd_two_sum(a.hi, b.hi);
+    dd t = dd_two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = dd_fast(s.hi, s.lo);
+    s.lo += t.lo;
+    return dd_fast(s.hi, s.lo);
+}
+HD dd dd_mul(dd a, dd b)
+{
+    double p = a.hi * b.hi;
+    double e = fma(a.hi, b.hi, -p);
+    e += a.hi * b.lo + a.lo * b.hi;
+    return dd_fast(p, e);
+}
+HD dd dd_mul_d(dd a, double b)
+{
+    double p = a.hi * b;
+    double e = fma(a.hi, b, -p);
+    e += a.lo * b;
+    return dd_fast(p, e);
+}
+HD dd dd_div(dd a, dd b)
+{
+    double q1 = a.hi / b.hi;
+    dd r = dd_add(a, dd_mul_d(b, -q1));
+    double q2 = r.hi / b.hi;
+    r = dd_add(r, dd_mul_d(b, -q2));
+    double q3 = r.hi / b.hi;
+    dd q = dd_fast(q1, q2);
+    return dd_add(q, dd{q3, 0.});
+}
+HD dd dd_recip_int(int n)  // 1/n to ~2^-106
+{
+    double h = 1.0 / n;
+    double r = fma(-h, (double)n, 1.0);
+    return dd_fast(h, r / n);
+}
+HD dd dd_log(double x)  // x > 0, finite
+{
+    int e;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.70710678118654752) {
+        m *= 2.;
+        e -= 1;
+    }
+    dd num = {m - 1., 0.};  // exact (Sterbenz)
+    dd den = dd_two_sum(m, 1.);
+    dd s = dd_div(num, den);
+    dd s2 = dd_mul(s, s);
+    dd t = dd_recip_int(2 * 23 + 1);
+    for (int k = 22; k >= 0; k--) t = dd_add(dd_recip_int(2 * k + 1), dd_mul(s2, t));
+    dd lm = dd_mul_d(dd_mul(s, t), 2.);
+    const dd ln2 = {6.93147180559945286227e-01, 2.31904681384629955842e-17};
+    return dd_add(dd_mul_d(ln2, (double)e), lm);
+}
+HD double dd_exp_round(dd x)  // exp(x) rounded to double; |x.hi| < 700
+{
+    const dd ln2 = {6.93147180559945286227e-01, 2.31904681384629955842e-17};
+    double k = rint(x.hi / ln2.hi);
+    dd r = dd_add(x, dd_mul_d(ln2, -k));
+    r.hi = ldexp(r.hi, -5);
+    r.lo = ldexp(r.lo, -5);
+    dd t = {1., 0.};
+    for (int n = 13; n >= 1; n--) t = dd_add(dd{1., 0.}, dd_mul(dd_mul(r, t), dd_recip_int(n)));
+    for (int q = 0; q < 5; q++) t = dd_mul(t, t);
+    return ldexp(t.hi + t.lo, (int)k);
+}
+HD double cr_pow(double x, double y)
+{
+    if (!(x > 0.) || !isfinite(x) || !isfinite(y) || y == 0.) return pow(x, y);
+    dd l = dd_mul_d(dd_log(x), y);
+    if (!(fabs(l.hi) < 700.)) return pow(x, y);
+    return dd_exp_round(l);
+}
+HD double dmax(double a, double b) { return (a > b) ? a : b; }
+HD double dmin(double a, double b) { return (a < b) ? a : b; }
+
+// Geometry tables: per location a [33][ncol] block (16 gcov, 16 gcon, gdet), column
+// index col = i*SN2 + j.  All lanes of a wave share one column in 3D runs.
+HD void load_geom(const double *__restrict__ tab, size_t ncol, size_t col, Geom &g)
+{
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            g.gcov[a][b] = tab[(size_t)(a * 4 + b) * ncol + col];
+            g.gcon[a][b] = tab[(size_t)(16 + a * 4 + b) * ncol + col];
+        }
+    g.g = tab[(size_t)32 * ncol + col];
+}
+
+// metric.c:210-229
+HD void lower(const double *ucon, const Geom &geom, double *ucov)
+{
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+        ucov[a] = geom.gcov[a][0] * ucon[0] + geom.gcov[a][1] * ucon[1] + geom.gcov[a][2] * ucon[2] +
+                  geom.gcov[a][3] * ucon[3];
+}
+// metric.c:232-255
+HD void raise(const double *ucov, const Geom &geom, double *ucon)
+{
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+        ucon[a] = geom.gcon[a][0] * ucov[0] + geom.gcon[a][1] * ucov[1] + geom.gcon[a][2] * ucov[2] +
+                  geom.gcon[a][3] * ucov[3];
+}
+
+// phys.c:300-324
+HD int gamma_calc(const double *pr, const Geom &geom, double *gamma)
+{
+    double qsq = geom.gcov[1][1] * pr[U1] * pr[U1] + geom.gcov[2][2] * pr[U2] * pr[U2] +
+                 geom.gcov[3][3] * pr[U3] * pr[U3] +
+                 2. * (geom.gcov[1][2] * pr[U1] * pr[U2] + geom.gcov[1][3] * pr[U1] * pr[U3] +
+                       geom.gcov[2][3] * pr[U2] * pr[U3]);
+    if (qsq < 0.) {
+        if (fabs(qsq) > 1.E-10) {
+            *gamma = 1.;
+            return 1;
+        }
+        qsq = 1.E-10;
+    }
+    *gamma = sqrt(1. + qsq);
+    return 0;
+}
+
+// phys.c:232-252; fatal counts where the reference calls fail()
+HD void ucon_calc(const double *pr, const Geom &geom, double *ucon, int &fatal)
+{
+    double alpha = 1. / sqrt(-geom.gcon[0][0]);
+    double beta[4];
+#pragma unroll
+    for (int j = 1; j < 4; j++) beta[j] = geom.gcon[0][j] * alpha * alpha;
+    double gamma;
+    if (gamma_calc(pr, geom, &gamma)) fatal++;
+    ucon[0] = gamma / alpha;
+#pragma unroll
+    for (int j = 1; j < 4; j++) ucon[j] = pr[U1 + j - 1] - gamma * beta[j] / alpha;
+}
+
+// phys.c:108-117
+HD void bcon_calc(const double *pr, const double *ucon, const double *ucov, double *bcon)
+{
+    bcon[0] = pr[B1] * ucov[1] + pr[B2] * ucov[2] + pr[B3] * ucov[3];
+#pragma unroll
+    for (int j = 1; j < 4; j++) bcon[j] = (pr[B1 - 1 + j] + bcon[0] * ucon[j]) / ucon[0];
+}
+
+// phys.c:219-229
+HD void get_state(const double *pr, const Geom &geom, State &q, int &fatal)
+{
+    ucon_calc(pr, geom, q.ucon, fatal);
+    lower(q.ucon, geom, q.ucov);
+    bcon_calc(pr, q.ucon, q.ucov, q.bcon);
+    lower(q.bcon, geom, q.bcov);
+}
+
+// phys.c:143-161
+HD void mhd_calc(const double *pr, int dir, const State &q, double gam, double *mhd)
+{
+    double r = pr[RHO], u = pr[UU];
+    double P = (gam - 1.) * u;
+    double w = P + r + u;
+    double bsq = dot4(q.bcon, q.bcov);
+    double eta = w + bsq;
+    double ptot = P + 0.5 * bsq;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        mhd[j] = eta * q.ucon[dir] * q.ucov[j] + ptot * ((dir == j) ? 1. : 0.) - q.bcon[dir] * q.bcov[j];
+}
+
+// phys.c:59-96
+HD void primtoflux(const double *pr, const State &q, int dir, const Geom &geom, double gam, double *flux)
+{
+    double mhd[4];
+    flux[RHO] = pr[RHO] * q.ucon[dir];
+    mhd_calc(pr, dir, q, gam, mhd);
+    flux[UU] = mhd[0] + flux[RHO];
+    flux[U1] = mhd[1];
+    flux[U2] = mhd[2];
+    flux[U3] = mhd[3];
+    flux[B1] = q.bcon[1] * q.ucon[dir] - q.bcon[dir] * q.ucon[1];
+    flux[B2] = q.bcon[2] * q.ucon[dir] - q.bcon[dir] * q.ucon[2];
+    flux[B3] = q.bcon[3] * q.ucon[dir] - q.bcon[dir] * q.ucon[3];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) flux[k] *= geom.g;
+}
+
+// phys.c:337-476 (Gamma-law branch); returns how often the reference would fail()
+HD int vchar(const double *pr, const State &q, const Geom &geom, int js, double gam, double *vmax, double *vmin)
+{
+    double Acov[4] = {0., 0., 0., 0.}, Bcov[4] = {1., 0., 0., 0.}, Acon[4], Bcon[4];
+    int nfail = 0;
+    Acov[js] = 1.;
+    raise(Acov, geom, Acon);
+    raise(Bcov, geom, Bcon);
+    double bsq = dot4(q.bcon, q.bcov);
+    double rho = pr[RHO], u = pr[UU];
+    double EF = rho + gam * u;
+    double EE = bsq + EF;
+    double va2 = bsq / EE;
+    double cs2 = gam * (gam - 1.) * u / EF;
+    double cms2 = cs2 + va2 - cs2 * va2;
+    if (cms2 < 0.) {
+        nfail++;
+        cms2 = SMALL;
+    }
+    if (cms2 > 1.) {
+        nfail++;
+        cms2 = 1.;
+    }
+    double Asq = dot4(Acon, Acov), Bsq = dot4(Bcon, Bcov);
+    double Au = dot4(Acov, q.ucon), Bu = dot4(Bcov, q.ucon), AB = dot4(Acon, Bcov);
+    double Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
+    double A = Bu2 - (Bsq + Bu2) * cms2;
+    double B = 2. * (AuBu - (AB + AuBu) * cms2);
+    double C = Au2 - (Asq + Au2) * cms2;
+    double discr = B * B - 4. * A * C;
+    if ((discr < 0.0) && (discr > -1.e-10))
+        discr = 0.0;
+    else if (discr < -1.e-10) {
+        nfail++;
+        discr = 0.;
+    }
+    discr = sqrt(discr);
+    double vp = -(-B + discr) / (2. * A);
+    double vm = -(-B - discr) / (2. * A);
+    if (vp > vm) {
+        *vmax = vp;
+        *vmin = vm;
+    } else {
+        *vmax = vm;
+        *vmin = vp;
+    }
+    return nfail;
+}
+
+// interp.c:49-93
+HD double slope_lim(int lim, double y1, double y2, double y3)
+{
+    double Dqm, Dqp, Dqc, s;
+    if (lim == 0) {
+        Dqm = 2. * (y2 - y1);
+        Dqp = 2. * (y3 - y2);
+        Dqc = 0.5 * (y3 - y1);
+        s = Dqm * Dqp;
+        if (s <= 0.) return 0.;
+        if (fabs(Dqm) < fabs(Dqp) && fabs(Dqm) < fabs(Dqc)) return Dqm;
+        if (fabs(Dqp) < fabs(Dqc)) return Dqp;
+        return Dqc;
+    } else if (lim == 1) {
+        Dqm = (y2 - y1);
+        Dqp = (y3 - y2);
+        s = Dqm * Dqp;
+        if (s <= 0.) return 0.;
+        return 2. * s / (Dqm + Dqp);
+    }
+    Dqm = (y2 - y1);
+    Dqp = (y3 - y2);
+    s = Dqm * Dqp;
+    if (s <= 0.) return 0.;
+    if (fabs(Dqm) < fabs(Dqp)) return Dqm;
+    return Dqp;
+}
+
+// One face of step_ch.c:429-475: reconstructed states -> numerical flux and ctop.
+HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, double gam, double hllf,
+                  double laxf, double *F, double &ctop, int &fatal)
+{
+    State sl, sr;
+    double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+    double cmax_l, cmin_l, cmax_r, cmin_r;
+    get_state(p_l, geom, sl, fatal);
+    get_state(p_r, geom, sr, fatal);
+    primtoflux(p_l, sl, dir, geom, gam, F_l);
+    primtoflux(p_l, sl, 0, geom, gam, U_l);
+    fatal += vchar(p_l, sl, geom, dir, gam, &cmax_l, &cmin_l);
+    primtoflux(p_r, sr, dir, geom, gam, F_r);
+    primtoflux(p_r, sr, 0, geom, gam, U_r);
+    fatal += vchar(p_r, sr, geom, dir, gam, &cmax_r, &cmin_r);
+    double cmax = fabs(dmax(dmax(0., cmax_l), cmax_r));
+    double cmin = fabs(dmax(dmax(0., -cmin_l), -cmin_r));
+    ctop = dmax(cmax, cmin);
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+        F[k] = hllf * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin + SMALL)) +
+               laxf * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+}
+
+// phys.c:165-207: dU = g * sum_jk T^j_k Gamma^k_{mu j}; conn is [64][ncol] (k,mu,j order)
+HD void source(const double *ph, const Geom &geom, const double *__restrict__ conn, size_t ncol, size_t col,
+               double gam, double *dU, int &fatal)
+{
+    State q;
+    double mhd[4][4];
+    get_state(ph, geom, q, fatal);
+    mhd_calc(ph, 0, q, gam, mhd[0]);
+    mhd_calc(ph, 1, q, gam, mhd[1]);
+    mhd_calc(ph, 2, q, gam, mhd[2]);
+    mhd_calc(ph, 3, q, gam, mhd[3]);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) dU[k] = 0.;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            dU[UU] += mhd[j][k] * conn[(size_t)((k * 4 + 0) * 4 + j) * ncol + col];
+            dU[U1] += mhd[j][k] * conn[(size_t)((k * 4 + 1) * 4 + j) * ncol + col];
+            dU[U2] += mhd[j][k] * conn[(size_t)((k * 4 + 2) * 4 + j) * ncol + col];
+            dU[U3] += mhd[j][k] * conn[(size_t)((k * 4 + 3) * 4 + j) * ncol + col];
+        }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) dU[k] *= geom.g;
+}
+
+// ---------------------------------------------------------------------------
+// Conserved -> primitive inversion (utoprim_2d.c, utoprim_1dvsq2fix1.c,
+// utoprim_1dfix1.c, u2p_util.c).  The reference's file-scope solver globals are
+// the per-thread struct U2P.
+// ---------------------------------------------------------------------------
+struct U2P {
+    double Bsq, QdotBsq, Qtsq, Qdotn, D, K_atm, gam;
+    double W_for_gnr2, rho_for_gnr2;
+};
+
+// u2p_util.c:204-256 + utoprim_2d.c:240-266
+HD void u2p_scalars(U2P &s, const double *U, const Geom &geom, double *Bcon, double *Qcon, double *ncon,
+                    double &QdotB)
+{
+    double Bcov[4], Qcov[4], ncov[4];
+    Bcon[0] = 0.;
+#pragma unroll
+    for (int i = 1; i < 4; i++) Bcon[i] = U[B1 + i - 1];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        Bcov[i] = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) Bcov[i] += geom.gcov[i][j] * Bcon[j];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) Qcov[i] = U[1 + i];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        Qcon[i] = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) Qcon[i] += geom.gcon[i][j] * Qcov[j];
+    }
+    s.Bsq = 0.;
+#pragma unroll
+    for (int i = 1; i < 4; i++) s.Bsq += Bcon[i] * Bcov[i];
+    QdotB = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) QdotB += Qcov[i] * Bcon[i];
+    s.QdotBsq = QdotB * QdotB;
+    double lapse = sqrt(-1. / geom.gcon[0][0]);
+    ncov[0] = -lapse;
+    ncov[1] = ncov[2] = ncov[3] = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        ncon[i] = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) ncon[i] += geom.gcon[i][j] * ncov[j];
+    }
+    s.Qdotn = Qcon[0] * ncov[0];
+    double Qsq = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) Qsq += Qcov[i] * Qcon[i];
+    s.Qtsq = Qsq + s.Qdotn * s.Qdotn;
+    s.D = U[RHO];
+}
+
+// utoprim_2d.c:374-383
+HD double vsq_calc(const U2P &s, double W)
+{
+    double Wsq = W * W;
+    double Xsq = (s.Bsq + W) * (s.Bsq + W);
+    return ((Wsq * s.Qtsq + s.QdotBsq * (s.Bsq + 2. * W)) / (Wsq * Xsq));
+}
+
+// utoprim_2d.c:562-624 with the Gamma-law EOS of utoprim_2d.c:645-678
+HD void func_vsq(const U2P &s, double W, double vsq, double &dx0, double &dx1, double &f, double &df)
+{
+    const double GAMMA = s.gam;
+    double Wsq = W * W;
+    double gtmp = 1. - vsq;
+    double p_tmp = (GAMMA - 1.) * (W * gtmp - s.D * sqrt(gtmp)) / GAMMA;
+    double dPdW = (GAMMA - 1.) * (1. - vsq) / GAMMA;
+    double dPdvsq = (GAMMA - 1.) * (0.5 * s.D / sqrt(1. - vsq) - W) / GAMMA;
+    double t2 = -0.5 * s.Bsq + dPdvsq;
+    double t3 = s.Bsq + W;
+    double t4 = t3 * t3;
+    double t9 = 1. / Wsq;
+    double t11 = s.Qtsq - vsq * t4 + s.QdotBsq * (s.Bsq + 2.0 * W) * t9;
+    double t16 = s.QdotBsq * t9;
+    double t18 = -s.Qdotn - 0.5 * s.Bsq * (1.0 + vsq) + 0.5 * t16 - W + p_tmp;
+    double t21 = 1. / t3;
+    double t23 = 1. / W;
+    double t24 = t16 * t23;
+    double t25 = -1.0 + dPdW - t24;
+    double t35 = t25 * t3 + (s.Bsq - 2.0 * dPdvsq) * (s.QdotBsq + vsq * Wsq * W) * t9 * t23;
+    double t36 = 1. / t35;
+    dx0 = -(t2 * t11 + t4 * t18) * t21 * t36;
+    double t40 = (vsq + t24) * t3;
+    dx1 = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+    df = -t11 * t11 - t18 * t18;
+    f = -0.5 * df;
+}
+
+// utoprim_2d.c:442-541 (validate_x: utoprim_2d.c:417-431)
+HD int gnr_2d(const U2P &s, double &x0, double &x1, int &n_iter_out, int &k_conv_out)
+{
+    double f = 1., df = 1., errx = 1.;
+    int n_iter = 0, i_extra = 0, doing_extra = 0, k_conv = -1;
+    bool keep = true;
+    while (keep) {
+        double dx0, dx1;
+        func_vsq(s, x0, x1, dx0, dx1, f, df);
+        double xo0 = x0;
+        x0 += dx0;
+        x1 += dx1;
+        errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
+        errx += (x1 == 0.) ? fabs(dx1) : fabs(dx1 / x1);
+        x0 = fabs(x0);
+        x0 = (x0 > W_TOO_BIG) ? xo0 : x0;
+        x1 = (x1 < 0.) ? 0. : x1;
+        x1 = (x1 > 1.) ? (1. - 1.e-15) : x1;
+        if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) {
+            doing_extra = 1;
+            k_conv = n_iter + 1;
+        }
+        if (doing_extra == 1) i_extra++;
+        if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+            (n_iter >= (MAX_NEWT_ITER - 1)))
+            keep = false;
+        n_iter++;
+    }
+    n_iter_out = n_iter;
+    k_conv_out = (k_conv < 0) ? n_iter : k_conv;
+    if (!isfinite(f) || !isfinite(df)) return 2;
+    if (fabs(errx) > MIN_NEWT_TOL) return 1;
+    return 0;
+}
+
+// Conserved-variable transform of utoprim_2d.c:145-163 (shared by all solvers).
+HD void u2p_transform(const double *U, const Geom &geom, double *U_tmp)
+{
+    double alpha = 1.0 / sqrt(-geom.gcon[0][0]);
+    U_tmp[RHO] = alpha * U[RHO] / geom.g;
+    U_tmp[UU] = alpha * (U[UU] - U[RHO]) / geom.g;
+#pragma unroll
+    for (int i = U1; i <= U3; i++) U_tmp[i] = alpha * U[i] / geom.g;
+#pragma unroll
+    for (int i = B1; i <= B3; i++) U_tmp[i] = alpha * U[i] / geom.g;
+}
+
+// Primitive recovery shared by the solvers (utoprim_2d.c:349-356).
+HD void u2p_recover_vel(const U2P &s, const double *Bcon, const double *Qcon, const double *ncon, double QdotB,
+                        double W, double gamma, double *prim)
+{
+#pragma unroll
+    for (int i = 1; i < 4; i++) {
+        double Qtcon = Qcon[i] + ncon[i] * s.Qdotn;
+        prim[U1 + i - 1] = gamma / (W + s.Bsq) * (Qtcon + QdotB * Bcon[i] / W);
+    }
+}
+
+HD double utsq_of(const Geom &geom, const double *prim)
+{
+    double utsq = 0.;
+#pragma unroll
+    for (int i = 1; i < 4; i++)
+#pragma unroll
+        for (int j = 1; j < 4; j++) utsq += geom.gcov[i][j] * prim[U1 + i - 1] * prim[U1 + j - 1];
+    return utsq;
+}
+
+// Initial W guess of utoprim_2d.c:269-303 from the guess primitives.
+HD bool guess_W(const U2P &s, const Geom &geom, const double *prim, double &W_last)
+{
+    double utsq = utsq_of(geom, prim);
+    if ((utsq < 0.) && (fabs(utsq) < 1.0e-13)) utsq = fabs(utsq);
+    if (utsq < 0. || utsq > UTSQ_TOO_BIG) return false;
+    double gammasq = 1. + utsq;
+    double gamma = sqrt(gammasq);
+    double rho0 = s.D / gamma;
+    double u = prim[UU];
+    double p = (s.gam - 1.) * u;
+    double w = rho0 + u + p;
+    W_last = w * gammasq;
+    return true;
+}
+
+// Raise W until v^2 < 1 can hold (utoprim_2d.c:297-303).
+HD double raise_W(const U2P &s, double W_last)
+{
+    int i_increase = 0;
+    while (((W_last * W_last * W_last * (W_last + 2. * s.Bsq) - s.QdotBsq * (2. * W_last + s.Bsq)) <=
+            W_last * W_last * (s.Qtsq - s.Bsq * s.Bsq)) &&
+           (i_increase < 10)) {
+        W_last *= 10.;
+        i_increase++;
+    }
+    return W_last;
+}
+
+// utoprim_2d.c:306-356 from a starting W: Newton on (W, v^2), then recovery.
+// Returns the reference's code; prim[0..4] written only on success.
+HD int solve_2d_from_W(U2P &s, const double *U_tmp, const double *Bcon, const double *Qcon, const double *ncon,
+                       double QdotB, double W_last, double *prim, int &n_iter, int &k_conv)
+{
+    double x0 = fabs(W_last);
+    double vsq0 = fabs(vsq_calc(s, W_last));
+    double x1 = (vsq0 > 1.) ? (1.0 - 1.e-15) : vsq0;
+    int retval = gnr_2d(s, x0, x1, n_iter, k_conv);
+    double W = x0, vsq = x1;
+    if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
+    if (W <= 0. || W > W_TOO_BIG) return 3;
+    if (vsq >= 1.) return 4;
+    double gtmp = sqrt(1. - vsq);
+    double gamma = 1. / gtmp;
+    double rho0 = s.D * gtmp;
+    double w = W * (1. - vsq);
+    double p = (s.gam - 1.) * (w - rho0) / s.gam;
+    double u = w - (rho0 + p);
+    if ((rho0 <= 0.) || (u <= 0.)) return 5;
+    prim[RHO] = rho0;
+    prim[UU] = u;
+    u2p_recover_vel(s, Bcon, Qcon, ncon, QdotB, W, gamma, prim);
+    return 0;
+}
+
+// utoprim_2d.c:126-176 + 219-362.  U is HARM's conserved vector (x gdet).
+HD int utoprim_2d(const double *U, const Geom &geom, double gam, double *prim, int &n_iter, int &k_conv)
+{
+    n_iter = 0;
+    k_conv = 0;
+#pragma unroll
+    for (int i = B1; i <= B3; i++) prim[i] = U[i] / geom.g;
+    if (U[0] <= 0.) return -100;
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.K_atm = 0.;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    double W_last;
+    if (!guess_W(s, geom, prim, W_last)) return 2;
+    W_last = raise_W(s, W_last);
+    double pt[5];
+    int ret = solve_2d_from_W(s, U_tmp, Bcon, Qcon, ncon, QdotB, W_last, pt, n_iter, k_conv);
+    if (ret == 0) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) prim[i] = pt[i];
+    }
+    return ret;
+}
+
+// ---- Font fix #1 (utoprim_1dvsq2fix1.c) ----
+// utoprim_1dvsq2fix1.c:586-614
+HD double W_of_vsq_iso(const U2P &s, double vsq, double &p, double &rho, double &u)
+{
+    double gtmp = (1. - vsq);
+    rho = s.D * sqrt(gtmp);
+    p = s.K_atm * cr_pow(rho, s.gam);
+    u = p / (s.gam - 1.);
+    return ((rho + u + p) / gtmp);
+}
+
+// utoprim_1dvsq2fix1.c:398-519 with func_1d_gnr (:538-566) and validate_x (:372-384)
+HD int gnr_1dvsq(const U2P &s, double &x)
+{
+    const double GAMMA = s.gam, G_ATM = s.gam;
+    double f = 1., df = 1., errx = 1., W = 0., W_old;
+    int n_iter = 0, i_extra = 0, doing_extra = 0;
+    bool keep = true;
+    while (keep) {
+        double p, rho, u;
+        double vsq = x;
+        double Wv = W_of_vsq_iso(s, vsq, p, rho, u);
+        double Wsq = Wv * Wv;
+        double W3 = Wv * Wsq;
+        double dWdvsq = ((GAMMA * (2. - G_ATM) * p + (GAMMA - 1.) * rho) / (2. * (GAMMA - 1.) * (1. - vsq) * (1. - vsq)));
+        double fact_tmp = (s.Bsq + Wv);
+        double resid = s.Qtsq - vsq * fact_tmp * fact_tmp + s.QdotBsq * (s.Bsq + 2. * Wv) / Wsq;
+        double jac = -fact_tmp * (fact_tmp + 2. * dWdvsq * (vsq + s.QdotBsq / W3));
+        double dx = -resid / jac;
+        f = 0.5 * resid * resid;
+        df = -2. * f;
+        double x_old = x;
+        x += dx;
+        x = (x >= 1.0) ? (0.5 * (x_old + 1.)) : x;
+        x = (x < -1.e-10) ? (0.5 * x_old) : x;
+        x = fabs(x);
+        W_old = W;
+        W = W_of_vsq_iso(s, x, p, rho, u);
+        errx = (W == 0.) ? fabs(W - W_old) : fabs((W - W_old) / W);
+        errx += (x == 0.) ? fabs(x - x_old) : fabs((x - x_old) / x);
+        if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) doing_extra = 1;
+        if (doing_extra == 1) i_extra++;
+        if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+            (n_iter >= (MAX_NEWT_ITER - 1)))
+            keep = false;
+        n_iter++;
+    }
+    if (!isfinite(f) || !isfinite(df)) return 2;
+    if (fabs(errx) > MIN_NEWT_TOL) return 1;
+    return 0;
+}
+
+// utoprim_1dvsq2fix1.c:143-190 + 242-370
+HD int utoprim_1dvsq2fix1(const double *U, const Geom &geom, double gam, double K, double *prim)
+{
+#pragma unroll
+    for (int i = B1; i <= B3; i++) prim[i] = U[i] / geom.g;
+    if (U[0] <= 0.) return -100;
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.K_atm = K;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    double utsq = utsq_of(geom, prim);
+    if ((utsq < 0.) && (fabs(utsq) < 1.0e-13)) utsq = fabs(utsq);
+    if (utsq < 0. || utsq > UTSQ_TOO_BIG) return 2;
+    double gammasq = 1. + utsq;
+    double x = 1. - 1. / gammasq;
+    int retval = gnr_1dvsq(s, x);
+    if (retval != 0) return retval * 100 + 1;
+    double vsq = x;
+    if ((vsq >= 1.) || (vsq < 0.)) return 4;
+    double p, rho0, u;
+    double W = W_of_vsq_iso(s, vsq, p, rho0, u);
+    double gtmp = sqrt(1. - vsq);
+    double gamma = 1. / gtmp;
+    if ((rho0 <= 0.) || (u <= 0.)) return 5;
+    prim[RHO] = rho0;
+    prim[UU] = u;
+    u2p_recover_vel(s, Bcon, Qcon, ncon, QdotB, W, gamma, prim);
+    return 0;
+}
+
+// ---- Font fix #2 (utoprim_1dfix1.c) ----
+// gnr2 (utoprim_1dfix1.c:572-660) on func_gnr2_rho (:814-838); the tolerance
+// branch there never sets doing_extra, so it stops at |drho/rho| <= 1e-15.
+HD int gnr2_rho(const U2P &s, double &x)
+{
+    const double GAMMA = s.gam;
+    double f = 1., df = 1., errx = 1.;
+    int n_iter = 0;
+    bool keep = true;
+    while (keep) {
+        double A = s.D * s.D;
+        double B0 = A * GAMMA * s.K_atm;
+        double B = B0 / (GAMMA - 1.);
+        double rho = x;
+        double W = s.W_for_gnr2;
+        double C = cr_pow(rho, GAMMA - 1.);
+        double resid = rho * W - A - B * C;
+        double jac = W - B0 * C / rho;
+        double dx = -resid / jac;
+        f = 0.5 * resid * resid;
+        df = -2. * f;
+        x += dx;
+        errx = 0.;
+        errx += (x == 0.) ? fabs(dx) : fabs(dx / x);
+        errx /= 1. * 1;
+        x = fabs(x);
+        if ((fabs(errx) <= NEWT_TOL2) || (n_iter >= (MAX_NEWT_ITER - 1))) keep = false;
+        n_iter++;
+    }
+    if (!isfinite(f) || !isfinite(df) || !isfinite(x)) return 2;
+    if (fabs(errx) > MIN_NEWT_TOL) return 1;
+    return 0;
+}
+
+// utoprim_1dfix1.c:695-765 (func_1d_orig1)
+HD void func_1d_orig1(U2P &s, double W, double &dx, double &f, double &df)
+{
+    const double GAMMA = s.gam, Bsq = s.Bsq, QdotBsq = s.QdotBsq, Qtsq = s.Qtsq, K_atm = s.K_atm;
+    s.W_for_gnr2 = W;
+    double rho_g, x_rho;
+    rho_g = x_rho = s.rho_for_gnr2;
+    int ntries = 0, retval;
+    while ((retval = gnr2_rho(s, x_rho)) && (ntries++ < 10)) {
+        rho_g *= 10.;
+        x_rho = rho_g;
+    }
+    double rho = s.rho_for_gnr2 = x_rho;
+    double Dc = s.D;
+    double t1 = Dc * Dc;
+    double t2 = QdotBsq * t1;
+    double t3 = t2 * Bsq;
+    double t5 = Bsq * Bsq;
+    double t8 = t1 * Bsq;
+    double t10 = t1 * W;
+    double t21 = W * W;
+    double t23 = rho * rho;
+    double t26 = 1. / t1;
+    double resid = (t3 + (2.0 * t2 + ((Qtsq - t5) * t1 + (-2.0 * t8 - t10) * W) * W) * W +
+                    (t5 + (2.0 * Bsq + W) * W) * t21 * t23) *
+                   t26 / t21;
+    double t29 = t1 * t1;
+    double t30 = QdotBsq * t29;
+    double t32 = GAMMA * K_atm;
+    double t33 = cr_pow(rho, 1.0 * GAMMA);
+    double t34 = t32 * t33;
+    double t38 = t23 * t33;
+    double t51 = GAMMA * t1 * K_atm * t33;
+    double t67 = t21 * W;
+    double jac = -2.0 *
+                 (t30 * Bsq * t34 +
+                  (t30 * t34 +
+                   ((-t38 * Bsq * t32 + Bsq * GAMMA * t1 * K_atm * t33) * t1 + (-t38 * GAMMA * K_atm + t51) * t1 * W) *
+                       t21) *
+                      W +
+                  ((-t3 + (-t2 + (-t8 - t10) * t21) * W) * W + (-t5 - Bsq * W) * t67 * t23) * t23) *
+                 t26 / (t51 - W * t23) / t67;
+    dx = -resid / jac;
+    f = 0.5 * resid * resid;
+    df = -2. * f;
+}
+
+// utoprim_1dfix1.c:470-560
+HD int gnr_1dfix(U2P &s, double &x)
+{
+    double f = 1., df = 1., errx = 1.;
+    int n_iter = 0, i_extra = 0, doing_extra = 0;
+    bool keep = true;
+    while (keep) {
+        double dx;
+        func_1d_orig1(s, x, dx, f, df);
+        x += dx;
+        int i_increase = 0;
+        while (((x * x * x * (x + 2. * s.Bsq) - s.QdotBsq * (2. * x + s.Bsq)) <= x * x * (s.Qtsq - s.Bsq * s.Bsq)) &&
+               (i_increase < 10)) {
+            x -= (1. * i_increase) * dx / 10.;
+            i_increase++;
+        }
+        errx = (x == 0.) ? fabs(dx) : fabs(dx / x);
+        x = fabs(x);
+        if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) doing_extra = 1;
+        if (doing_extra == 1) i_extra++;
+        if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+            (n_iter >= (MAX_NEWT_ITER - 1)))
+            keep = false;
+        n_iter++;
+    }
+    if (!isfinite(f) || !isfinite(df) || !isfinite(x)) return 2;
+    if (fabs(errx) > MIN_NEWT_TOL) return 1;
+    return 0;
+}
+
+// utoprim_1dfix1.c:137-200 + 254-406
+HD int utoprim_1dfix1(const double *U, const Geom &geom, double gam, double K, double *prim)
+{
+#pragma unroll
+    for (int i = B1; i <= B3; i++) prim[i] = U[i] / geom.g;
+    if (U[0] <= 0.) return -100;
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.K_atm = K;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    double utsq = utsq_of(geom, prim);
+    if ((utsq < 0.) && (fabs(utsq) < 1.0e-13)) utsq = fabs(utsq);
+    if (utsq < 0. || utsq > UTSQ_TOO_BIG) return 2;
+    double gammasq = 1. + utsq;
+    double gamma = sqrt(gammasq);
+    double rho0 = s.D / gamma;
+    double p = s.K_atm * cr_pow(rho0, gam);
+    double u = p / (gam - 1.);
+    double w = rho0 + u + p;
+    double W_last = w * gammasq;
+    W_last = raise_W(s, W_last);
+    s.W_for_gnr2 = W_last;
+    s.rho_for_gnr2 = rho0;
+    double x = W_last;
+    int retval = gnr_1dfix(s, x);
+    double W = x;
+    if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
+    if (W <= 0. || W > W_TOO_BIG) return 3;
+    double vsq = vsq_calc(s, W);
+    if (vsq >= 1.) return 4;
+    double gtmp = sqrt(1. - vsq);
+    gamma = 1. / gtmp;
+    rho0 = s.D * gtmp;
+    p = s.K_atm * cr_pow(rho0, gam);
+    u = p / (gam - 1.);
+    if ((rho0 <= 0.) || (u <= 0.)) return 5;
+    prim[RHO] = rho0;
+    prim[UU] = u;
+    u2p_recover_vel(s, Bcon, Qcon, ncon, QdotB, W, gamma, prim);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Floors and Lorentz-factor limit: fixup.c:142-383 (DRIFT_FLOOR=1).
+// rhoflr0/uuflr0 are the radius-only floors of get_rho_u_floor (fixup.c:117-140),
+// tabulated per (i,j) on the host.  Returns the flags of what happened.
+// ---------------------------------------------------------------------------
+enum { FIX_FLOOR = 1, FIX_GAMMA_FAIL = 2, FIX_GAMMA_CAP = 4 };
+
+// phys.c:255-271
+HD void ucon_to_utcon(const double *ucon, const Geom &geom, double *utcon)
+{
+    double alpha = 1. / sqrt(-geom.gcon[0][0]);
+    double beta[4];
+#pragma unroll
+    for (int j = 1; j < 4; j++) beta[j] = geom.gcon[0][j] * alpha * alpha;
+    double gamma = alpha * ucon[0];
+    utcon[0] = 0;
+#pragma unroll
+    for (int j = 1; j < 4; j++) utcon[j] = ucon[j] + gamma * beta[j] / alpha;
+}
+
+// phys.c:273-297
+HD double ut_calc_3vel(const double *vcon, const Geom &geom)
+{
+    double AA = geom.gcov[0][0];
+    double BB = 2. * (geom.gcov[0][1] * vcon[1] + geom.gcov[0][2] * vcon[2] + geom.gcov[0][3] * vcon[3]);
+    double CC = geom.gcov[1][1] * vcon[1] * vcon[1] + geom.gcov[2][2] * vcon[2] * vcon[2] +
+                geom.gcov[3][3] * vcon[3] * vcon[3] +
+                2. * (geom.gcov[1][2] * vcon[1] * vcon[2] + geom.gcov[1][3] * vcon[1] * vcon[3] +
+                      geom.gcov[2][3] * vcon[2] * vcon[3]);
+    double DD = -1. / (AA + BB + CC);
+    double one_over_alpha_sq = -geom.gcon[0][0];
+    if (DD < one_over_alpha_sq) DD = one_over_alpha_sq;
+    return sqrt(DD);
+}
+
+HD int fixup1zone(double *pv, const Geom &geom, double rhoflr0, double uuflr0, double gam, int &fatal)
+{
+    double pv_prefloor[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) pv_prefloor[k] = pv[k];
+    double rhoflr = rhoflr0, uuflr = uuflr0;
+    State q;
+    get_state(pv, geom, q, fatal);
+    double bsq = dot4(q.bcon, q.bcov);
+    if (rhoflr < bsq / BSQORHOMAX) rhoflr = bsq / BSQORHOMAX;
+    if (uuflr < bsq / BSQOUMAX) uuflr = bsq / BSQOUMAX;
+    if (rhoflr < pv[UU] / UORHOMAX) rhoflr = pv[UU] / UORHOMAX;
+    int dofloor = 0, out = 0;
+    if (pv[RHO] < rhoflr) {
+        pv[RHO] = rhoflr;
+        dofloor = 1;
+    }
+    if (pv[UU] < uuflr) {
+        pv[UU] = uuflr;
+        dofloor = 1;
+    }
+    if (dofloor) out |= FIX_FLOOR;
+    double trans;
+    if (dofloor && (trans = 10. * bsq / dmin(pv[RHO], pv[UU]) - 1.) > 0.) {
+        if (trans > 1.) trans = 1.;
+        double betapar = -q.bcon[0] / ((bsq + SMALL) * q.ucon[0]);
+        double betasq = betapar * betapar * bsq;
+        double betasqmax = 1. - 1. / (GAMMAMAX * GAMMAMAX);
+        if (betasq > betasqmax) betasq = betasqmax;
+        double gamma = 1. / sqrt(1 - betasq);
+        double ucondr[4], Bcon[4], Bcov[4], vcon[4], ucon[4], utcon[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) ucondr[k] = gamma * (q.ucon[k] + betapar * q.bcon[k]);
+        Bcon[0] = 0.;
+#pragma unroll
+        for (int k = 1; k < 4; k++) Bcon[k] = pv[B1 - 1 + k];
+        lower(Bcon, geom, Bcov);
+        double udotB = dot4(q.ucon, Bcov);
+        double Bsq = dot4(Bcon, Bcov);
+        double B = sqrt(Bsq);
+        double wold = pv_prefloor[RHO] + pv_prefloor[UU] * gam;
+        double QdotB = udotB * wold * q.ucon[0];
+        double wnew = pv[RHO] + pv[UU] * gam;
+        double x = 2. * QdotB / (B * wnew * ucondr[0] + SMALL);
+        double vpar = x / (ucondr[0] * (1. + sqrt(1. + x * x)));
+        double one_over_ucondr_t = 1. / ucondr[0];
+        vcon[0] = 1.;
+#pragma unroll
+        for (int k = 1; k < 4; k++) vcon[k] = vpar * Bcon[k] / (B + SMALL) + ucondr[k] * one_over_ucondr_t;
+        double ut = ut_calc_3vel(vcon, geom);
+#pragma unroll
+        for (int k = 0; k < 4; k++) ucon[k] = ut * vcon[k];
+        ucon_to_utcon(ucon, geom, utcon);
+#pragma unroll
+        for (int k = 1; k < 4; k++) pv[k + UU] = utcon[k] * trans + pv_prefloor[k + UU] * (1. - trans);
+    }
+    double gamma;
+    if (gamma_calc(pv, geom, &gamma)) {
+        out |= FIX_GAMMA_FAIL;
+    } else if (gamma > GAMMAMAX) {
+        double f = sqrt((GAMMAMAX * GAMMAMAX - 1.) / (gamma * gamma - 1.));
+        pv[U1] *= f;
+        pv[U2] *= f;
+        pv[U3] *= f;
+        out |= FIX_GAMMA_CAP;
+    }
+    return out;
+}
+
+// bounds.c:333-378: remove inflow through the radial boundaries.
+HD void inflow_check(double *pr, const Geom &geom, int type, int &fatal)
+{
+    double ucon[4];
+    ucon_calc(pr, geom, ucon, fatal);
+    if (((ucon[1] > 0.) && (type == 0)) || ((ucon[1] < 0.) && (type == 1))) {
+        double gamma;
+        if (gamma_calc(pr, geom, &gamma)) fatal++;
+        pr[U1] /= gamma;
+        pr[U2] /= gamma;
+        pr[U3] /= gamma;
+        double alpha = 1. / sqrt(-geom.gcon[0][0]);
+        double beta1 = geom.gcon[0][1] * alpha * alpha;
+        pr[U1] = beta1 / alpha;
+        double vsq = 0.;
+#pragma unroll
+        for (int j = 1; j < 4; j++)
+#pragma unroll
+            for (int k = 1; k < 4; k++) vsq += geom.gcov[j][k] * pr[U1 + j - 1] * pr[U1 + k - 1];
+        if (fabs(vsq) < 1.e-13) vsq = 1.e-13;
+        if (vsq >= 1.) vsq = 1. - 1. / (GAMMAMAX * GAMMAMAX);
+        gamma = 1. / sqrt(1. - vsq);
+        pr[U1] *= gamma;
+        pr[U2] *= gamma;
+        pr[U3] *= gamma;
+    }
+}
+
+}  // namespace harm

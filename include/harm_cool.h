@@ -1,0 +1,151 @@
+/*
+ * harm_cool.h -- C ABI of libharmcool.so, the MI355X (gfx950) implementation of
+ * HARM-COOL's per-timestep GRMHD update.
+ *
+ * The reference (agnieszkajaniuk/HARM_COOL) has no plugin or FFI layer: its hot
+ * path sits behind plain C functions over process-global arrays.  Each entry point
+ * below names the reference interface it replaces.  All functions are extern "C",
+ * take plain pointers and sizes, return 0 on success and a nonzero code on error
+ * (never exit()), and record a message readable with harm_last_error().
+ *
+ * Layouts
+ *   Zone index: i in [0,N1+4) (r, active 2..N1+1), j in [0,N2+4) (theta, active
+ *   2..N2+1), h in [0,N3) (phi, periodic, no ghost planes).
+ *   Host exchange layout ("canonical"): prim[v][i][j][h], v = RHO,UU,U1,U2,U3,B1,B2,B3
+ *   (decs.h:190-197), phi fastest; flags int8 [i][j][h].
+ */
+#ifndef HARM_COOL_H
+#define HARM_COOL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run parameters.  The reference fixes these at compile time (decs.h:59-258,
+ * init.c:90-161, step_ch.c:66-67, u2p_defs.h:79-96). */
+typedef struct harm_params {
+    int32_t n1, n2, n3;      /* active zones in X1 (r), X2 (theta), X3 (phi); n3 = 1 or even >= 4 */
+    int32_t lim;             /* slope limiter: 0 MC, 1 van Leer, 2 minmod (interp.c:49-93) */
+    int32_t font_fix;        /* 1: Font-fix fallbacks on inversion failure (step_ch.c:277-300) */
+    int32_t pad0;
+    double a, gam;           /* BH spin, adiabatic index */
+    double rin, rmax;        /* FM torus inner edge, pressure maximum (init.c:111-112) */
+    double kappa, beta;      /* polytropic constant, plasma-beta normalization */
+    double Rin, Rout;        /* radial extent; Rin <= 0 selects 0.98*(1+sqrt(1-a^2)) */
+    double hslope, R0, Rbr;  /* MKS theta concentration, r offset, hyperexp break (<0: off) */
+    double cour, dt0, tf, defcon;
+    double hllf, laxf;       /* weights of the HLL and LLF fluxes (step_ch.c:66-67) */
+    double pert_amp;         /* u *= 1 + pert_amp*(U[0,1]-0.5) (init.c:273) */
+    uint32_t seed[4];        /* KISS base seeds (init.c:272) */
+} harm_params;
+
+/* Per-context event counters, cumulative over steps (reference: failimage[0..4],
+ * step_ch.c:274-295, fixup.c:96,354,525; Newton iteration counts are new). */
+enum {
+    HARM_C2P_CALLS = 0,      /* Utoprim_2d calls */
+    HARM_NEWTON_ITERS,       /* Newton iterations incl. the 2 extra (utoprim_2d.c:467-520) */
+    HARM_NEWTON_KCONV,       /* iterations until |dx/x| <= 1e-10 first held */
+    HARM_FAIL_2D,            /* Utoprim_2d returned nonzero */
+    HARM_FAIL_FONT1,         /* Utoprim_1dvsq2fix1 also failed */
+    HARM_FAIL_FONT2,         /* Utoprim_1dfix1 also failed */
+    HARM_FIXUP_GAMMA,        /* fixup1zone gamma failure, pflag = -33 */
+    HARM_FIXUP_REPAIRS,      /* zones repaired by fixup_utoprim */
+    HARM_FIXUP_NOSTENCIL,    /* ... of which used the v=0 fallback branch */
+    HARM_FLOOR_ZONES,        /* fixup1zone calls that applied a rho/u floor */
+    HARM_GAMMA_CAP,          /* Lorentz factor capped at GAMMAMAX */
+    HARM_FATAL,              /* events where the reference calls fail() and exits */
+    HARM_NCOUNTERS
+};
+
+/* Kernel classes timed with HIP events when timing is enabled. */
+enum {
+    HARM_K_FLUX1 = 0, HARM_K_FLUX2, HARM_K_FLUX3, HARM_K_EMF, HARM_K_UPDATE,
+    HARM_K_BOUND, HARM_K_FIXUTOPRIM, HARM_K_STEPEND, HARM_K_C2P_BATCH, HARM_K_MLP,
+    HARM_NKERNELS
+};
+
+typedef struct harm_ctx harm_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char *harm_last_error(void);
+
+/* Context = the reference's globals (decs.h:310-378) made explicit: device-resident
+ * SoA state, metric tables, one HIP stream.  Replaces set_arrays()+set_grid()
+ * (main.c:288-508).  device < 0 keeps the current device. */
+harm_ctx *harm_create(const harm_params *p, int device);
+void      harm_destroy(harm_ctx *c);
+
+/* Fishbone-Moncrief torus initial data; replaces init() (init.c:70-472). */
+int harm_init_torus(harm_ctx *c);
+
+/* One predictor/corrector step with floors, bounds, fixups and dt control;
+ * replaces step_ch() (step_ch.c:90-140, decs.h:600).  nsteps >= 1. */
+int harm_step(harm_ctx *c, int nsteps);
+
+/* Host <-> device state in the canonical layout (8*(N1+4)*(N2+4)*N3 doubles). */
+int harm_get_prims(harm_ctx *c, double *prim, int8_t *pflag);
+int harm_set_prims(harm_ctx *c, const double *prim, const int8_t *pflag);
+int harm_get_time(harm_ctx *c, double *t, double *dt);
+int harm_set_time(harm_ctx *c, double t, double dt);
+int harm_get_counters(harm_ctx *c, int64_t *counters /* HARM_NCOUNTERS */);
+int harm_reset_counters(harm_ctx *c);
+
+/* Diagnostics (diag.c:156-209 divB; diag.c:477-580 mass budget):
+ * out[0] mass, out[1] sum U[UU] dV, out[2] mdot_in, out[3] mdot_out,
+ * out[4] accumulated mass_in, out[5] accumulated mass_out, out[6] max |divB|. */
+int harm_diag(harm_ctx *c, double *out /* 7 */);
+
+/* Device pointers of the state (canonical layout, var stride (N1+4)*(N2+4)*N3). */
+int harm_device_state(harm_ctx *c, double **prim, int8_t **pflag, void **stream);
+
+/* Kernel timing with HIP events on the context's stream.  ms[k] = summed elapsed
+ * time of kernel class k since the last reset, launches[k] = number of launches. */
+int harm_set_timing(harm_ctx *c, int enable);
+int harm_get_timing(harm_ctx *c, double *ms, int64_t *launches);
+
+/* Per-stage entry points for parity tests (one half-step pieces of advance(),
+ * step_ch.c:153-318).  Fluxes after flux_ct are returned in the canonical layout
+ * (8 x (N1+4) x (N2+4) x N3 doubles each). */
+int harm_debug_fluxes(harm_ctx *c, int fullstep, double *F1, double *F2, double *F3, double *ndt3);
+int harm_debug_bound(harm_ctx *c);
+/* fixup_utoprim (fixup.c:432-538) then bound_prim on the current state and flags. */
+int harm_debug_fixup_utoprim(harm_ctx *c);
+/* The device pow used by the Font-fix solvers (correctly rounded), on host arrays;
+ * for tests against the host libm pow the reference calls. */
+int harm_debug_pow(const double *x, const double *y, double *out, size_t n);
+
+/* Geometry tables built on the host (main.c:413-508), canonical per-(i,j) layout:
+ * gcov/gcon [5][16][N1+4][N2+4], gdet [5][N1+4][N2+4], conn [64][N1+4][N2+4]. */
+int harm_get_geometry(harm_ctx *c, double *gcov, double *gcon, double *gdet, double *conn);
+
+/* Con2prim options for the batched inversion. */
+typedef struct harm_u2p_opts {
+    double gam;              /* adiabatic index */
+    int32_t font_fix;        /* run the Font-fix fallbacks on failure */
+    int32_t guess_mode;      /* 0: guess from prim_in, 1: MLP guess with prim_in fallback */
+    const void *mlp;         /* harm_mlp handle for guess_mode 1 (may be NULL) */
+    double K;                /* isentrope constant for the Font fix */
+} harm_u2p_opts;
+
+/* Batched Noble 2D inversion on the device; replaces n calls of Utoprim_2d
+ * (utoprim_2d.c:126, decs.h:608-609).  All pointers are device pointers in SoA
+ * form: U[8][n], gcov[16][n], gcon[16][n], gdet[n], prim[8][n] (guess in, result
+ * out; B always overwritten, the rest only where ret == 0), ret[n] (int8, the
+ * reference's char-truncated code), iters[n] (Newton iterations, may be NULL),
+ * kconv[n] (may be NULL).  stream may be NULL (default stream). */
+int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet,
+                         double *prim, int8_t *ret, uint8_t *iters, uint8_t *kconv, size_t n,
+                         const harm_u2p_opts *opts, void *stream);
+
+/* Exact reference signature (utoprim_2d.c:126, decs.h:608-609), evaluated on the
+ * GPU for one zone; gam is taken from harm_set_gamma() (default 4/3, init.c:91). */
+int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8]);
+void harm_set_gamma(double gam);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

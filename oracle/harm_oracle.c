@@ -23,6 +23,7 @@
  * (divB, mass budget, reflection symmetry, stationary torus) and against the
  * reference's measured config-1 Newton-iteration and Font-fix counts.
  */
+#define _GNU_SOURCE
 #include "harm_oracle.h"
 
 #include <math.h>
@@ -86,6 +87,17 @@
 #define MY_MAX(a, b) (((a) > (b)) ? (a) : (b))
 #define dot(a, b) (a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3])
 #define delta(i, j) (((i) == (j)) ? 1. : 0.)
+
+/* The reference is built with g++ -O2, which fuses sin(x) and cos(x) of one
+ * argument in one function into glibc's sincos(); glibc 2.35's sincos differs
+ * from separate cos() by 1 ulp for ~0.1% of arguments.  Where the reference's
+ * compiled code computes both (coord.c:147-154, metric.c:345-347, init.c:188-197)
+ * the restatement calls sincos explicitly so it does not depend on the
+ * compiler's choice. */
+static inline void sincos_ref(double x, double *s, double *c)
+{
+    sincos(x, s, c);
+}
 
 typedef struct {
     double gcon[NDIM][NDIM];
@@ -312,8 +324,7 @@ static void gcov_func(const orc_params *P, const double *X, double gcovp[NDIM][N
     bl_coord_vec(P, X, V);
     r = V[RR];
     th = V[TH];
-    cth = cos(th);
-    sth = sin(th);
+    sincos_ref(th, &sth, &cth);
     s2 = sth * sth;
     rho2 = r * r + a * a * cth * cth;
     dxdxp_func(P, X, dxdxp);
@@ -1938,9 +1949,9 @@ static void bl_gcov_func(double a, double r, double th, double gcov[NDIM][NDIM])
     double sth, cth, s2, a2, r2, DD, mu;
     for (j = 0; j < NDIM; j++)
         for (k = 0; k < NDIM; k++) gcov[j][k] = 0.;
-    sth = fabs(sin(th));
+    sincos_ref(th, &sth, &cth);
+    sth = fabs(sth);
     s2 = sth * sth;
-    cth = cos(th);
     a2 = a * a;
     r2 = r * r;
     DD = 1. - 2. / r + a2 / r2;
@@ -2039,14 +2050,12 @@ int orc_init_torus(orc_ctx *o)
                 bl_coord_vec(P, X, V);
                 r = V[RR];
                 th = V[TH];
-                sth = sin(th);
-                cth = cos(th);
+                sincos_ref(th, &sth, &cth);
                 DD = r * r - 2. * r + a * a;
                 AA = (r * r + a * a) * (r * r + a * a) - DD * a * a * sth * sth;
                 SS = r * r + a * a * cth * cth;
                 thin = M_PI / 2.;
-                sthin = sin(thin);
-                cthin = cos(thin);
+                sincos_ref(thin, &sthin, &cthin);
                 DDin = rin * rin - 2. * rin + a * a;
                 AAin = (rin * rin + a * a) * (rin * rin + a * a) - DDin * a * a * sthin * sthin;
                 SSin = rin * rin + a * a * cthin * cthin;
@@ -2259,6 +2268,46 @@ void orc_diag(const orc_ctx *o, double *out)
 /* ------------------------------------------------------------------------ */
 /* context and exports                                                       */
 /* ------------------------------------------------------------------------ */
+/* Fluxes of one half-step after fix_flux and flux_ct, canonical layout. */
+int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F3, double *ndt)
+{
+    int64_t cnt[ORC_NCOUNTERS];
+    memset(cnt, 0, sizeof(cnt));
+    ndt[0] = fluxcalc(o, o->p, o->F1, 1, fullstep, cnt);
+    ndt[1] = fluxcalc(o, o->p, o->F2, 2, fullstep, cnt);
+    ndt[2] = (o->N3 > 2) ? fluxcalc(o, o->p, o->F3, 3, fullstep, cnt) : 1.e9;
+    fix_flux(o);
+    flux_ct(o);
+    const size_t nh = o->N3, nj = o->SN2, ni = o->SN1;
+    double *dst[3] = {F1, F2, F3};
+    const double *src[3] = {o->F1, o->F2, o->F3};
+    for (int d = 0; d < 3; d++) {
+        if (!dst[d]) continue;
+        for (size_t i = 0; i < ni; i++)
+            for (size_t j = 0; j < nj; j++)
+                for (size_t h = 0; h < nh; h++)
+                    for (int k = 0; k < NPR; k++)
+                        dst[d][(((size_t)k * ni + i) * nj + j) * nh + h] = PV(o, src[d], (int)h, (int)i, (int)j)[k];
+    }
+    return 0;
+}
+
+/* fixup_utoprim + bound_prim on the current state and flags (fixup.c:432-538). */
+int orc_debug_fixup_utoprim(orc_ctx *o)
+{
+    int64_t *cnt = o->cnt;
+    fixup_utoprim(o, o->p, cnt);
+    bound_prim(o, o->p, &cnt[ORC_FATAL]);
+    return 0;
+}
+
+/* bound_prim on the current state (bounds.c:54-331). */
+int orc_debug_bound(orc_ctx *o)
+{
+    bound_prim(o, o->p, &o->cnt[ORC_FATAL]);
+    return 0;
+}
+
 void orc_set_threads(int n)
 {
 #ifdef _OPENMP

@@ -69,6 +69,14 @@ double   orc_get_katm(const orc_ctx *o, int i);
 void     orc_get_geometry(const orc_ctx *o, double *gcov, double *gcon,
                           double *gdet, double *conn, double *coords);
 double   orc_divb_max(const orc_ctx *o);
+/* Fluxes of one half-step on the current state after fix_flux and flux_ct
+ * (canonical layout, 8 x (N1+4) x (N2+4) x N3 each; F3 may be NULL in 2D);
+ * ndt[3] = per-direction CFL limits (1e9 when not fullstep). */
+int      orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F3, double *ndt);
+/* fixup_utoprim() then bound_prim() on the current state and flags. */
+int      orc_debug_fixup_utoprim(orc_ctx *o);
+/* bound_prim() on the current state. */
+int      orc_debug_bound(orc_ctx *o);
 /* out[0] = mass (sum rho gdet dV), out[1] = sum U[UU] dV, out[2] = mdot_in,
  * out[3] = mdot_out (diag.c:477-580, diag.c:59-230). */
 void     orc_diag(const orc_ctx *o, double *out);

@@ -55,6 +55,9 @@ def lib():
         L.orc_get_katm.restype = c_double
         L.orc_get_katm.argtypes = [c_void_p, c_int]
         L.orc_get_geometry.argtypes = [c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_debug_fluxes.argtypes = [c_void_p, c_int, _dp, _dp, _dp, _dp]
+        L.orc_debug_fixup_utoprim.argtypes = [c_void_p]
+        L.orc_debug_bound.argtypes = [c_void_p]
         L.orc_divb_max.restype = c_double
         L.orc_divb_max.argtypes = [c_void_p]
         L.orc_diag.argtypes = [c_void_p, _dp]
@@ -188,6 +191,20 @@ class Oracle:
         coords = np.zeros((sn1, sn2, 4))
         self.L.orc_get_geometry(self.h, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn), _ptr(coords))
         return gcov, gcon, gdet, conn, coords
+
+    def debug_fluxes(self, fullstep: int = 0):
+        F1 = np.zeros(self.shape)
+        F2 = np.zeros(self.shape)
+        F3 = np.zeros(self.shape)
+        nd = np.zeros(3)
+        self.L.orc_debug_fluxes(self.h, fullstep, _ptr(F1), _ptr(F2), _ptr(F3), _ptr(nd))
+        return F1, F2, F3, nd
+
+    def debug_fixup_utoprim(self) -> None:
+        self.L.orc_debug_fixup_utoprim(self.h)
+
+    def debug_bound(self) -> None:
+        self.L.orc_debug_bound(self.h)
 
     def divb_max(self) -> float:
         return self.L.orc_divb_max(self.h)

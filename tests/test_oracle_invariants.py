@@ -1,0 +1,121 @@
+"""Closed forms and invariants that pin the oracle where fixtures leave gaps:
+the inversion round trip P -> U -> P, its special cases, the minimum iteration
+count from an exact guess, the MC limiter definition, divB preservation by
+flux-CT, and equatorial reflection symmetry of the evolution."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from harm_cool_amd.params import config1, torus_params
+from oracle_lib import Oracle, _ptr, lib, utoprim_2d
+
+GAM = 13.0 / 9.0
+
+
+def _flat():
+    g = np.diag([-1.0, 1.0, 1.0, 1.0])
+    return g.copy(), g.copy(), 1.0
+
+
+def _ks_metric(i=40, j=50):
+    o = Oracle(config1())
+    gcov, gcon, gdet, _, _ = o.geometry()
+    return np.ascontiguousarray(gcov[i, j, 3]), np.ascontiguousarray(gcon[i, j, 3]), gdet[i, j, 3]
+
+
+def _primtoU(pr, gcov, gcon, gdet):
+    U = np.zeros(8)
+    lib().orc_primtoU(_ptr(np.asarray(pr, float)), _ptr(gcov), _ptr(gcon), gdet, GAM, _ptr(U))
+    return U
+
+
+@pytest.mark.parametrize("metric", ["flat", "ks"])
+def test_round_trip_prim_to_cons_to_prim(metric):
+    gcov, gcon, gdet = _flat() if metric == "flat" else _ks_metric()
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(400):
+        rho = 10 ** rng.uniform(-5, 0)
+        u = rho * 10 ** rng.uniform(-2, 1)
+        ut = rng.normal(size=3) * 10 ** rng.uniform(-3, 0.5)
+        B = rng.normal(size=3) * np.sqrt(rho) * 10 ** rng.uniform(-2, 0.5)
+        pr = np.array([rho, u, *ut, *B])
+        U = _primtoU(pr, gcov, gcon, gdet)
+        guess = pr * (1 + 1e-2 * rng.normal(size=8))
+        r, out, ni, kc = utoprim_2d(U, gcov, gcon, gdet, guess, GAM)
+        assert r == 0
+        rel = np.abs(out - pr) / np.maximum(np.abs(pr), 1e-300)
+        worst = max(worst, rel[:5].max())
+        assert np.array_equal(out[5:], pr[5:] * 1.0) or np.allclose(out[5:], pr[5:], rtol=1e-15)
+    assert worst < 1e-9
+
+
+def test_static_unmagnetized_flat_space_closed_form():
+    """v = 0, B = 0: U = (rho, -u, 0,0,0) + rho term; the solution is exact."""
+    gcov, gcon, gdet = _flat()
+    rho, u = 0.3, 0.07
+    pr = np.array([rho, u, 0, 0, 0, 0, 0, 0], float)
+    U = _primtoU(pr, gcov, gcon, gdet)
+    # HARM's conserved energy is T^t_t + rho u^t = -(rho+u) + rho = -u
+    assert U[0] == rho and abs(U[1] - (-u)) < 1e-16 and not U[2:].any()
+    r, out, ni, kc = utoprim_2d(U, gcov, gcon, gdet, pr * 1.3, GAM)
+    assert r == 0
+    assert abs(out[0] - rho) < 1e-15 and abs(out[1] - u) < 1e-15 and np.all(np.abs(out[2:5]) < 1e-15)
+
+
+def test_exact_guess_takes_the_minimum_three_iterations():
+    """u2p_defs.h:82: two extra iterations after convergence, so 3 at least."""
+    gcov, gcon, gdet = _ks_metric()
+    pr = np.array([0.5, 0.2, 0.1, -0.02, 0.3, 0.05, 0.01, 0.02])
+    U = _primtoU(pr, gcov, gcon, gdet)
+    r, out, ni, kc = utoprim_2d(U, gcov, gcon, gdet, pr.copy(), GAM)
+    assert r == 0 and ni == 3 and kc == 1
+
+
+def test_negative_density_is_rejected():
+    gcov, gcon, gdet = _ks_metric()
+    U = np.array([-1.0, -0.5, 0, 0, 0, 0.1, 0.2, 0.3])
+    r, out, ni, kc = utoprim_2d(U, gcov, gcon, gdet, np.ones(8), GAM)
+    assert r == -100 and np.array_equal(out[5:], U[5:] / gdet)
+
+
+def test_mc_limiter_is_the_textbook_monotonized_central_slope():
+    L = lib()
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        y1, y2, y3 = rng.integers(-4, 5, 3).astype(float)
+        dm, dp, dc = y2 - y1, y3 - y2, 0.5 * (y3 - y1)
+        want = 0.0 if dm * dp <= 0 else np.sign(dc) * min(2 * abs(dm), 2 * abs(dp), abs(dc))
+        assert L.orc_slope_lim(0, y1, y2, y3) == want
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 1), (24, 16, 8)])
+def test_flux_ct_keeps_divb_at_roundoff(shape):
+    o = Oracle(torus_params(*shape))
+    o.init_torus()
+    d0 = o.divb_max()
+    for _ in range(5):
+        assert o.step() == 0
+    assert d0 < 1e-12 and o.divb_max() < 1e-12
+
+
+def test_equatorial_reflection_symmetry():
+    """Without the random perturbation the torus is symmetric about theta=pi/2:
+    rho, u, u^r, u^phi even; u^theta, B^theta even/odd as a poloidal loop field."""
+    P = config1()
+    P.n1, P.n2 = 48, 48
+    P.pert_amp = 0.0
+    o = Oracle(P)
+    o.init_torus()
+    for _ in range(10):
+        assert o.step() == 0
+    p = o.prims()[:, 2:-2, 2:-2, 0]
+    m = p[:, :, ::-1]
+    # The grid's X2 points mirror only to roundoff, which seeds a fixed asymmetry in
+    # the first step (largest in u^theta, which starts at 0); it must not grow.
+    # An index or sign slip in a theta stencil breaks the symmetry at O(1).
+    bound = {0: 1e-11, 1: 1e-11, 2: 1e-11, 3: 1e-6, 4: 1e-11, 5: 1e-10, 6: 1e-10, 7: 1e-8}
+    for k, sign in [(0, 1), (1, 1), (2, 1), (3, -1), (4, 1), (5, -1), (6, 1), (7, -1)]:
+        scale = np.abs(p[k]).max() + 1e-300
+        assert np.abs(p[k] - sign * m[k]).max() / scale < bound[k], k
