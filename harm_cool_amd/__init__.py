@@ -16,14 +16,14 @@ import numpy as np
 from .params import HarmParams, config1, config2, reference_default, torus_params  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libharmcool.so")
+LIB_PATH = os.environ.get("HARMCOOL_LIB", os.path.join(HERE, "libharmcool.so"))
 
 NCOUNTERS = 12
 COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_font1",
                  "fail_font2", "fixup_gamma", "fixup_repairs", "fixup_nostencil",
                  "floor_zones", "gamma_cap", "fatal"]
 KERNEL_NAMES = ["flux1", "flux2", "flux3", "emf", "update", "bound", "fixutoprim", "stepend",
-                "c2p_batch", "mlp"]
+                "c2p_batch", "mlp", "fontfix"]
 NKERNELS = len(KERNEL_NAMES)
 
 _dp = POINTER(c_double)
@@ -201,6 +201,15 @@ def device_pow(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     out = np.empty_like(x)
     _check(lib().harm_debug_pow(_ptr(x), _ptr(y), _ptr(out), x.size), "harm_debug_pow")
     return out
+
+
+def device_copy(nbytes: int, repeats: int = 10) -> float:
+    """Time a plain device copy of nbytes; returns ms per copy (counter calibration)."""
+    L = lib()
+    L.harm_debug_copy.argtypes = [c_size_t, c_int, _dp]
+    ms = c_double()
+    _check(L.harm_debug_copy(nbytes, repeats, ctypes.byref(ms)), "harm_debug_copy")
+    return ms.value
 
 
 def set_gamma(gam: float) -> None:

@@ -55,6 +55,8 @@ struct DGrid {
     const double *conn;
     const double *floors;
     const double *katm;
+    const double *cblk;     // column-major CENT block, CENT_STRIDE per column
+    const double *fblk[3];  // column-major FACE1/FACE2/FACE3 blocks, FACE_STRIDE per column
 };
 
 __device__ __forceinline__ size_t zidx(const DGrid &g, int i, int j, int h)
@@ -89,7 +91,7 @@ __device__ __forceinline__ void add_counter(unsigned long long *cnt, int idx, in
 // DIR 1,2,3 = X1,X2,X3.  Slopes of the two zones adjacent to the face are formed
 // on the fly from four zones along DIR, as the reference's dq array holds them.
 // ---------------------------------------------------------------------------
-template <int DIR>
+template <int DIR, bool UNI>
 __global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F,
                                               unsigned long long *__restrict__ ndt_slot, int fullstep,
                                               unsigned long long *__restrict__ cnt)
@@ -141,7 +143,10 @@ __global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict_
             }
             Geom geom;
             const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
-            load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
+            if (UNI)
+                load_geom_block(g.fblk[DIR - 1] + uni_col<UNI>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
+            else
+                load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
             double Fz[NPR], ctop;
             face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
             if (DIR == 1) {
@@ -224,20 +229,33 @@ __device__ __forceinline__ void ct_f3(const DGrid &g, const double *e1, const do
     fb[2] = 0.;
 }
 
+// Zones whose Noble 2D inversion failed, handed from k_update to k_fontfix with
+// their conserved vector: the rarely taken Font-fix solvers then do not inflate
+// the register allocation of the per-zone kernel.
+struct FontList {
+    int *count;
+    int *zone;
+    double *U;  // [8][cap]
+    int cap;
+    int *overflow;
+};
+
 // advance() zone loop (step_ch.c:237-306) + fixup() (fixup.c:51-60), one active
 // zone per thread: conserved update from pi with sources at pb, Noble 2D
-// inversion seeded with pi, Font fix on failure, then floors / gamma limit.
+// inversion seeded with pi, then floors / gamma limit.  Zones whose inversion
+// fails go to k_fontfix (step_ch.c:277-300) with the guess left in pf.
+template <bool UNI>
 __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb,
                                                 double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                 const double *__restrict__ F1, const double *__restrict__ F2,
                                                 const double *__restrict__ F3, const double *__restrict__ e1,
                                                 const double *__restrict__ e2, const double *__restrict__ e3,
-                                                const double *__restrict__ tstate, double dtfac,
-                                                unsigned long long *__restrict__ cnt)
+                                                const double *__restrict__ tstate, double dtfac, int pi_is_pb,
+                                                FontList fl, unsigned long long *__restrict__ cnt)
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    int c_call = 0, c_iter = 0, c_kconv = 0, c_f2d = 0, c_ff1 = 0, c_ff2 = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
+    int c_call = 0, c_iter = 0, c_kconv = 0, c_f2d = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
     if (idx < (size_t)ni * nj * g.N3) {
         const int h = (int)(idx % g.N3);
         const size_t rest = idx / g.N3;
@@ -245,20 +263,27 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         const int i = g.is + (int)(rest / nj);
         const size_t z = zidx(g, i, j, h);
         const size_t nz = g.nz;
-        const size_t col = (size_t)i * g.SN2 + j;
+        const size_t col = uni_col<UNI>((size_t)i * g.SN2 + j);
         const double Dt = dtfac * tstate[1];
+        const double *blk = g.cblk + col * CENT_STRIDE;
         Geom geom;
-        load_geom(g.geom[CENT], g.ncol, col, geom);
-        double pbz[NPR], piz[NPR], dU[NPR], U[NPR];
-#pragma unroll
-        for (int k = 0; k < NPR; k++) {
-            pbz[k] = pb[k * nz + z];
-            piz[k] = pi[k * nz + z];
-        }
-        source(pbz, geom, g.conn, g.ncol, col, g.gam, dU, fatal);
+        if (UNI)
+            load_geom_block(blk, geom);
+        else
+            load_geom(g.geom[CENT], g.ncol, col, geom);
+        double piz[NPR], dU[NPR], U[NPR];
         {
+            double pbz[NPR];
+#pragma unroll
+            for (int k = 0; k < NPR; k++) pbz[k] = pb[k * nz + z];
             State q;
-            get_state(piz, geom, q, fatal);
+            if (UNI)
+                source(pbz, geom, blk + CENT_CONN, 1, 0, g.gam, dU, q, fatal);
+            else
+                source(pbz, geom, g.conn, g.ncol, col, g.gam, dU, q, fatal);
+#pragma unroll
+            for (int k = 0; k < NPR; k++) piz[k] = pi_is_pb ? pbz[k] : pi[k * nz + z];
+            if (!pi_is_pb) get_state(piz, geom, q, fatal);
             primtoflux(piz, q, 0, geom, g.gam, U);
         }
         const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
@@ -309,24 +334,31 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         c_iter = niter;
         c_kconv = kconv;
         int8_t flag = (int8_t)ret;
+        bool deferred = false;
         if (flag) {
             c_f2d = 1;
             if (g.font_fix) {
-                const double K = g.katm[i];
-                flag = (int8_t)utoprim_1dvsq2fix1(U, geom, g.gam, K, pr);
-                if (flag) {
-                    c_ff1 = 1;
-                    flag = (int8_t)utoprim_1dfix1(U, geom, g.gam, K, pr);
-                    if (flag) c_ff2 = 1;
+                const int slot = atomicAdd(fl.count, 1);
+                if (slot < fl.cap) {
+                    fl.zone[slot] = (int)z;
+#pragma unroll
+                    for (int k = 0; k < NPR; k++) fl.U[(size_t)k * fl.cap + slot] = U[k];
+                    deferred = true;
+                } else {
+                    atomicOr(fl.overflow, 1);
                 }
             }
         }
-        const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
-        if (fx & FIX_FLOOR) c_flr = 1;
-        if (fx & FIX_GAMMA_CAP) c_cap = 1;
-        if (fx & FIX_GAMMA_FAIL) {
-            flag = -33;
-            c_gam = 1;
+        if (!deferred) {
+            const double rf = UNI ? ldc(blk, 33) : ldc(g.floors, col);
+            const double uf = UNI ? ldc(blk, 34) : ldc(g.floors, g.ncol + col);
+            const int fx = fixup1zone(pr, geom, rf, uf, g.gam, fatal);
+            if (fx & FIX_FLOOR) c_flr = 1;
+            if (fx & FIX_GAMMA_CAP) c_cap = 1;
+            if (fx & FIX_GAMMA_FAIL) {
+                flag = -33;
+                c_gam = 1;
+            }
         }
 #pragma unroll
         for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
@@ -336,6 +368,56 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
     add_counter(cnt, HARM_NEWTON_ITERS, c_iter);
     add_counter(cnt, HARM_NEWTON_KCONV, c_kconv);
     add_counter(cnt, HARM_FAIL_2D, c_f2d);
+    add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
+    add_counter(cnt, HARM_FLOOR_ZONES, c_flr);
+    add_counter(cnt, HARM_GAMMA_CAP, c_cap);
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
+// Font fix (step_ch.c:277-300) + fixup1zone for the zones k_update deferred.
+// Grid-stride over the list; the list count is reset by the next kernel on the
+// stream (k_bound), after every thread here has read it.
+__global__ void __launch_bounds__(TPB) k_fontfix(DGrid g, double *__restrict__ pf, int8_t *__restrict__ pflag,
+                                                 FontList fl, unsigned long long *__restrict__ cnt)
+{
+    const int n = min(*fl.count, fl.cap);
+    int c_ff1 = 0, c_ff2 = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
+    const int stride = gridDim.x * TPB;
+    const int nround = (n + stride - 1) / stride;
+    for (int r = 0; r < nround; r++) {
+        const int e = r * stride + blockIdx.x * TPB + threadIdx.x;
+        if (e < n) {
+            const size_t z = (size_t)fl.zone[e];
+            const size_t nz = g.nz;
+            const size_t col = z / g.N3;
+            const int i = (int)(col / g.SN2);
+            Geom geom;
+            load_geom(g.geom[CENT], g.ncol, col, geom);
+            double U[NPR], pr[NPR];
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                U[k] = fl.U[(size_t)k * fl.cap + e];
+                pr[k] = pf[k * nz + z];
+            }
+            const double K = g.katm[i];
+            int8_t flag = (int8_t)utoprim_1dvsq2fix1(U, geom, g.gam, K, pr);
+            if (flag) {
+                c_ff1++;
+                flag = (int8_t)utoprim_1dfix1(U, geom, g.gam, K, pr);
+                if (flag) c_ff2++;
+            }
+            const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
+            if (fx & FIX_FLOOR) c_flr++;
+            if (fx & FIX_GAMMA_CAP) c_cap++;
+            if (fx & FIX_GAMMA_FAIL) {
+                flag = -33;
+                c_gam++;
+            }
+#pragma unroll
+            for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
+            pflag[z] = flag;
+        }
+    }
     add_counter(cnt, HARM_FAIL_FONT1, c_ff1);
     add_counter(cnt, HARM_FAIL_FONT2, c_ff2);
     add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
@@ -366,10 +448,12 @@ __global__ void __launch_bounds__(TPB) k_fixup(DGrid g, double *__restrict__ p, 
 }
 
 // ---------------------------------------------------------------------------
-// bound_prim (bounds.c:54-331, polefix bounds.c:381-436): one thread per (i,h)
-// column.  Every value a column thread reads comes from active rows no thread of
-// this launch writes, so the reference's copy -> polefix -> pole ghosts -> sign
-// flip -> inflow sequence is reproduced without inter-thread ordering.
+// bound_prim (bounds.c:54-331, polefix bounds.c:381-436): one thread per zone it
+// writes -- the four radial ghost columns (all rows), the four polar ghost rows
+// of the active columns, and the polefix rows.  Every value a thread reads comes
+// from active rows no thread of this launch writes, so the reference's
+// copy -> polefix -> pole ghosts -> sign flip -> inflow sequence is reproduced
+// without inter-thread ordering.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double bval(const DGrid &g, const double *p, int hh, int ii, int j, int k)
 {
@@ -389,53 +473,68 @@ __device__ __forceinline__ double bval(const DGrid &g, const double *p, int hh, 
     return p[k * nz + zidx(g, c, j, hh)];
 }
 
+__device__ __forceinline__ size_t bound_nthreads(const DGrid &g)
+{
+    return ((size_t)4 * g.SN2 + (size_t)g.N1 * (4 + 2 * g.polefix)) * g.N3;
+}
+
 __global__ void __launch_bounds__(TPB) k_bound(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag,
-                                               unsigned long long *__restrict__ cnt)
+                                               int *__restrict__ font_count, unsigned long long *__restrict__ cnt)
 {
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx == 0 && font_count) *font_count = 0;
     int fatal = 0;
-    if (idx < (size_t)g.SN1 * g.N3) {
+    if (idx < bound_nthreads(g)) {
         const int h = (int)(idx % g.N3);
-        const int i = (int)(idx / g.N3);
+        size_t t = idx / g.N3;
+        int i, j;
+        bool poleonly = false;
+        if (t < (size_t)4 * g.SN2) {  // radial ghost columns, every row
+            const int q = (int)(t / g.SN2);
+            i = (q < 2) ? g.is - 2 + q : g.ie + 1 + (q - 2);
+            j = (int)(t % g.SN2);
+        } else {
+            t -= (size_t)4 * g.SN2;
+            const int nr = 4 + 2 * g.polefix;
+            i = g.is + (int)(t / nr);
+            const int r = (int)(t % nr);
+            const int rows[8] = {g.js - 2, g.js - 1, g.je + 1, g.je + 2, g.js, g.js + 1, g.je, g.je - 1};
+            j = rows[r];
+            poleonly = (r >= 4);
+        }
         const size_t nz = g.nz;
         const bool ghost = (i < g.is || i > g.ie);
         const int c = i < g.is ? g.is : (i > g.ie ? g.ie : i);
-        if (ghost) {
-            for (int j = g.js; j <= g.je; j++) {
-                const size_t z = zidx(g, i, j, h);
-                for (int k = 0; k < NPR; k++) p[k * nz + z] = bval(g, p, h, i, j, k);
-                pflag[z] = pflag[zidx(g, c, j, h)];
-            }
-        } else if (g.polefix) {
-            for (int r = 0; r < 2 * g.polefix; r++) {
-                const int j = (r < g.polefix) ? g.js + r : g.je - (r - g.polefix);
-                const size_t z = zidx(g, i, j, h);
-                for (int k = 0; k < B1; k++) p[k * nz + z] = bval(g, p, h, i, j, k);
-            }
-        }
-        const int hs = (g.N3 > 1) ? (h + g.N3 / 2) % g.N3 : h;
-        const int src_row[4] = {g.js, g.js + 1, g.je, g.je - 1};
-        const int dst_row[4] = {g.js - 1, g.js - 2, g.je + 1, g.je + 2};
-        for (int r = 0; r < 4; r++) {
-            const size_t zd = zidx(g, i, dst_row[r], h);
+        const size_t z = zidx(g, i, j, h);
+        double pr[NPR];
+        int8_t fl;
+        if (j >= g.js && j <= g.je) {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) pr[k] = bval(g, p, h, i, j, k);
+            fl = pflag[zidx(g, c, j, h)];
+        } else {
+            const int hs = (g.N3 > 1) ? (h + g.N3 / 2) % g.N3 : h;
+            const int src = (j == g.js - 1) ? g.js : (j == g.js - 2) ? g.js + 1 : (j == g.je + 1) ? g.je : g.je - 1;
+#pragma unroll
             for (int k = 0; k < NPR; k++) {
-                double v = bval(g, p, hs, i, src_row[r], k);
+                double v = bval(g, p, hs, i, src, k);
                 if (k == U2 || k == B2) v *= -1.;
-                p[k * nz + zd] = v;
+                pr[k] = v;
             }
-            pflag[zd] = pflag[zidx(g, c, src_row[r], h)];
+            fl = pflag[zidx(g, c, src, h)];
         }
         if (ghost) {
-            const int type = (i < g.is) ? 0 : 1;
-            for (int j = g.js - 2; j <= g.je + 2; j++) {
-                const size_t z = zidx(g, i, j, h);
-                double pr[NPR];
-                for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
-                Geom geom;
-                load_geom(g.geom[CENT], g.ncol, (size_t)i * g.SN2 + j, geom);
-                inflow_check(pr, geom, type, fatal);
-                for (int k = U1; k <= U3; k++) p[k * nz + z] = pr[k];
-            }
+            Geom geom;
+            load_geom(g.geom[CENT], g.ncol, (size_t)i * g.SN2 + j, geom);
+            inflow_check(pr, geom, (i < g.is) ? 0 : 1, fatal);
+        }
+        if (poleonly) {
+#pragma unroll
+            for (int k = 0; k < B1; k++) p[k * nz + z] = pr[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) p[k * nz + z] = pr[k];
+            pflag[z] = fl;
         }
     }
     add_counter(cnt, HARM_FATAL, fatal);
@@ -539,7 +638,7 @@ __global__ void __launch_bounds__(1024) k_fixutoprim_serial(DGrid g, double *__r
     const int n_all = *count;
     if (n_all == 0) return;
     if (n_all > cap) {
-        if (threadIdx.x == 0) atomicOr(err, 1);
+        if (threadIdx.x == 0) atomicOr(err, 2);
         return;
     }
     const int n = n_all;
@@ -879,6 +978,12 @@ __global__ void __launch_bounds__(TPB) k_pow(const double *__restrict__ x, const
     if (z < n) out[z] = cr_pow(x[z], y[z]);
 }
 
+// Plain device copy of known size: calibrates the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(TPB) k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n)
+{
+    for (size_t z = (size_t)blockIdx.x * TPB + threadIdx.x; z < n; z += (size_t)gridDim.x * TPB) b[z] = a[z];
+}
+
 inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
 }  // namespace
@@ -896,13 +1001,15 @@ struct harm_ctx {
     size_t nz;
     double *p, *ph, *F1, *F2, *F3, *e1, *e2, *e3;
     int8_t *pflag, *pflag2;
-    double *d_geom[5], *d_conn, *d_floors, *d_katm;
+    double *d_geom[5], *d_conn, *d_floors, *d_katm, *d_cblk, *d_fblk;
     double *tstate;                   // t, dt, mass_in, mass_out, err
     unsigned long long *ndt_slot;     // 3
     unsigned long long *cnt;          // HARM_NCOUNTERS
     double *part;                     // reduction partials
     size_t npart;
     int *fix_list, *fix_count, *fix_err;
+    FontList font;
+    bool uniform;  // N3 % 64 == 0: a wave's lanes share one (i,j) column
     int fix_cap;
     bool timing;
     struct Ev {
@@ -970,15 +1077,28 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
     const int i0 = (dir == 1) ? g.is : g.is - 1;
     const int j0 = (dir == 2) ? g.js : g.js - 1;
     const size_t total = (size_t)(g.ie + 2 - i0) * (g.je + 2 - j0) * g.N3;
-    if (dir == 1)
-        LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL(k_flux<1>, dim3(nblocks(total)), dim3(TPB), 0, c->stream, g, pb, F,
-                                                c->ndt_slot + 0, fullstep, c->cnt));
-    else if (dir == 2)
-        LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL(k_flux<2>, dim3(nblocks(total)), dim3(TPB), 0, c->stream, g, pb, F,
-                                                c->ndt_slot + 1, fullstep, c->cnt));
-    else
-        LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL(k_flux<3>, dim3(nblocks(total)), dim3(TPB), 0, c->stream, g, pb, F,
-                                                c->ndt_slot + 2, fullstep, c->cnt));
+    const dim3 grid(nblocks(total)), blk(TPB);
+    if (c->uniform) {
+        if (dir == 1)
+            LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, true>), grid, blk, 0, c->stream, g, pb, F,
+                                                    c->ndt_slot + 0, fullstep, c->cnt));
+        else if (dir == 2)
+            LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL((k_flux<2, true>), grid, blk, 0, c->stream, g, pb, F,
+                                                    c->ndt_slot + 1, fullstep, c->cnt));
+        else
+            LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL((k_flux<3, true>), grid, blk, 0, c->stream, g, pb, F,
+                                                    c->ndt_slot + 2, fullstep, c->cnt));
+    } else {
+        if (dir == 1)
+            LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, false>), grid, blk, 0, c->stream, g, pb, F,
+                                                    c->ndt_slot + 0, fullstep, c->cnt));
+        else if (dir == 2)
+            LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL((k_flux<2, false>), grid, blk, 0, c->stream, g, pb, F,
+                                                    c->ndt_slot + 1, fullstep, c->cnt));
+        else
+            LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL((k_flux<3, false>), grid, blk, 0, c->stream, g, pb, F,
+                                                    c->ndt_slot + 2, fullstep, c->cnt));
+    }
     return 0;
 }
 
@@ -998,8 +1118,9 @@ int fluxes_and_emf(harm_ctx *c, const double *pb, int fullstep)
 int bound(harm_ctx *c, double *p)
 {
     const DGrid &g = c->g;
-    LAUNCH(HARM_K_BOUND, hipLaunchKernelGGL(k_bound, dim3(nblocks((size_t)g.SN1 * g.N3)), dim3(TPB), 0, c->stream, g,
-                                            p, c->pflag, c->cnt));
+    const size_t nb = ((size_t)4 * g.SN2 + (size_t)g.N1 * (4 + 2 * g.polefix)) * g.N3;
+    LAUNCH(HARM_K_BOUND, hipLaunchKernelGGL(k_bound, dim3(nblocks(nb)), dim3(TPB), 0, c->stream, g, p, c->pflag,
+                                            c->font.count, c->cnt));
     return 0;
 }
 
@@ -1023,9 +1144,18 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     int rc;
     if ((rc = fluxes_and_emf(c, pb, fullstep))) return rc;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
-    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL(k_update, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi, pb, pf,
-                                             c->pflag, c->F1, c->F2, c->F3, c->e1, c->e2, c->e3, c->tstate, dtfac,
-                                             c->cnt));
+    const int same = (pi == pb) ? 1 : 0;
+    if (c->uniform)
+        LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<true>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi,
+                                                 pb, pf, c->pflag, c->F1, c->F2, c->F3, c->e1, c->e2, c->e3,
+                                                 c->tstate, dtfac, same, c->font, c->cnt));
+    else
+        LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<false>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g,
+                                                 pi, pb, pf, c->pflag, c->F1, c->F2, c->F3, c->e1, c->e2, c->e3,
+                                                 c->tstate, dtfac, same, c->font, c->cnt));
+    if (g.font_fix)
+        LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(256), dim3(TPB), 0, c->stream, g, pf, c->pflag,
+                                                  c->font, c->cnt));
     if ((rc = bound(c, pf))) return rc;
     if ((rc = fixup_utoprim(c, pf))) return rc;
     if ((rc = bound(c, pf))) return rc;
@@ -1052,7 +1182,8 @@ int check_flags(harm_ctx *c)
     HIPCHK(hipMemcpyAsync(&fe, c->fix_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ts[4] == 11.) return set_err("timestep too small (dt < 1e-9, step_ch.c:126)", 11);
-    if (fe) return set_err("fixup_utoprim: more clustered bad zones than the serial list holds", 12);
+    if (fe & 2) return set_err("fixup_utoprim: more clustered bad zones than the serial list holds", 12);
+    if (fe & 1) return set_err("more failed inversions in one half-step than the Font-fix list holds", 13);
     return 0;
 }
 
@@ -1100,15 +1231,39 @@ int ctx_alloc(harm_ctx *c)
     HIPCHK(hipMemcpy(c->d_floors, c->tab.floors.data(), 2 * ncol * sizeof(double), hipMemcpyHostToDevice));
     if ((rc = dalloc(&c->d_katm, hg.SN1))) return rc;
     HIPCHK(hipMemcpy(c->d_katm, c->tab.katm.data(), hg.SN1 * sizeof(double), hipMemcpyHostToDevice));
+    {
+        // column-major copies for the wave-uniform kernels
+        std::vector<double> cb((size_t)CENT_STRIDE * ncol, 0.), fb((size_t)3 * FACE_STRIDE * ncol, 0.);
+        const int floc[3] = {FACE1, FACE2, FACE3};
+        for (size_t col = 0; col < ncol; col++) {
+            for (int v = 0; v < 33; v++) cb[col * CENT_STRIDE + v] = c->tab.geom[CENT][(size_t)v * ncol + col];
+            cb[col * CENT_STRIDE + 33] = c->tab.floors[col];
+            cb[col * CENT_STRIDE + 34] = c->tab.floors[ncol + col];
+            for (int v = 0; v < 64; v++) cb[col * CENT_STRIDE + CENT_CONN + v] = c->tab.conn[(size_t)v * ncol + col];
+            for (int d = 0; d < 3; d++)
+                for (int v = 0; v < 33; v++)
+                    fb[((size_t)d * ncol + col) * FACE_STRIDE + v] = c->tab.geom[floc[d]][(size_t)v * ncol + col];
+        }
+        if ((rc = dalloc(&c->d_cblk, cb.size()))) return rc;
+        HIPCHK(hipMemcpy(c->d_cblk, cb.data(), cb.size() * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = dalloc(&c->d_fblk, fb.size()))) return rc;
+        HIPCHK(hipMemcpy(c->d_fblk, fb.data(), fb.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     if ((rc = dalloc(&c->tstate, 8))) return rc;
     if ((rc = dalloc(&c->ndt_slot, 3))) return rc;
     if ((rc = dalloc(&c->cnt, HARM_NCOUNTERS))) return rc;
     c->npart = std::max<size_t>(3 * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
     if ((rc = dalloc(&c->part, c->npart))) return rc;
+    c->font.cap = (int)std::min<size_t>(c->nz / 16 + 65536, (size_t)1 << 24);
+    if ((rc = dalloc(&c->font.count, 1))) return rc;
+    if ((rc = dalloc(&c->font.zone, (size_t)c->font.cap))) return rc;
+    if ((rc = dalloc(&c->font.U, 8 * (size_t)c->font.cap))) return rc;
+    c->uniform = (hg.N3 % 64) == 0;
     c->fix_cap = 1 << 16;
     if ((rc = dalloc(&c->fix_list, 2 * (size_t)c->fix_cap))) return rc;
     if ((rc = dalloc(&c->fix_count, 1))) return rc;
     if ((rc = dalloc(&c->fix_err, 1))) return rc;
+    c->font.overflow = c->fix_err;  // bit 1: fixup_utoprim list, bit 0: Font-fix list (atomicOr 1 / 2)
     unsigned long long big[3];
     const double b = 1.e9;
     for (int k = 0; k < 3; k++) std::memcpy(&big[k], &b, 8);
@@ -1144,6 +1299,8 @@ int ctx_alloc(harm_ctx *c)
     g.conn = c->d_conn;
     g.floors = c->d_floors;
     g.katm = c->d_katm;
+    g.cblk = c->d_cblk;
+    for (int d = 0; d < 3; d++) g.fblk[d] = c->d_fblk + (size_t)d * FACE_STRIDE * ncol;
     return 0;
 }
 
@@ -1152,7 +1309,8 @@ void ctx_free(harm_ctx *c)
     for (void *q : {(void *)c->p, (void *)c->ph, (void *)c->F1, (void *)c->F2, (void *)c->F3, (void *)c->e1,
                     (void *)c->e2, (void *)c->e3, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
                     (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->cnt,
-                    (void *)c->part, (void *)c->fix_list, (void *)c->fix_count, (void *)c->fix_err})
+                    (void *)c->part, (void *)c->fix_list, (void *)c->fix_count, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
+                    (void *)c->d_cblk, (void *)c->d_fblk})
         if (q) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
         if (c->d_geom[l]) (void)hipFree(c->d_geom[l]);
@@ -1511,6 +1669,31 @@ int harm_debug_pow(const double *x, const double *y, double *out, size_t n)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d));
+    return 0;
+}
+
+int harm_debug_copy(size_t nbytes, int repeats, double *ms_per_copy)
+{
+    double2 *a = nullptr, *b = nullptr;
+    const size_t n = nbytes / sizeof(double2);
+    HIPCHK(hipMalloc((void **)&a, n * sizeof(double2)));
+    HIPCHK(hipMalloc((void **)&b, n * sizeof(double2)));
+    HIPCHK(hipMemset(a, 0, n * sizeof(double2)));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, 0, a, b, n);
+    HIPCHK(hipEventRecord(e0, 0));
+    for (int r = 0; r < repeats; r++) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, 0, a, b, n);
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_copy) *ms_per_copy = ms / std::max(repeats, 1);
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    HIPCHK(hipFree(a));
+    HIPCHK(hipFree(b));
     return 0;
 }
 

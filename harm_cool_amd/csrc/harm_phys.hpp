@@ -135,6 +135,12 @@ HD double cr_pow(double x, double y)
 HD double dmax(double a, double b) { return (a > b) ? a : b; }
 HD double dmin(double a, double b) { return (a < b) ? a : b; }
 
+// Read-only table access through the constant address space: when the column
+// index is wave-uniform the loads become scalar (s_load) and the values live in
+// SGPRs, so one metric fetch serves the 64 phi-zones of a wave.
+typedef const __attribute__((address_space(4))) double cdouble;
+HD double ldc(const double *p, size_t off) { return ((cdouble *)p)[off]; }
+
 // Geometry tables: per location a [33][ncol] block (16 gcov, 16 gcon, gdet), column
 // index col = i*SN2 + j.  All lanes of a wave share one column in 3D runs.
 HD void load_geom(const double *__restrict__ tab, size_t ncol, size_t col, Geom &g)
@@ -143,10 +149,36 @@ HD void load_geom(const double *__restrict__ tab, size_t ncol, size_t col, Geom 
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            g.gcov[a][b] = tab[(size_t)(a * 4 + b) * ncol + col];
-            g.gcon[a][b] = tab[(size_t)(16 + a * 4 + b) * ncol + col];
+            g.gcov[a][b] = ldc(tab, (size_t)(a * 4 + b) * ncol + col);
+            g.gcon[a][b] = ldc(tab, (size_t)(16 + a * 4 + b) * ncol + col);
         }
-    g.g = tab[(size_t)32 * ncol + col];
+    g.g = ldc(tab, (size_t)32 * ncol + col);
+}
+
+// Column-major copy of the tables for wave-uniform kernels: all values of one
+// (i,j) column are contiguous, so the scalar unit fetches them with wide loads.
+// CENT block (100 doubles): gcov[16] gcon[16] gdet rhoflr uuflr pad conn[64];
+// face blocks (36 doubles): gcov[16] gcon[16] gdet pad[3].
+constexpr int CENT_STRIDE = 100, CENT_CONN = 36, FACE_STRIDE = 36;
+
+HD void load_geom_block(const double *__restrict__ blk, Geom &g)
+{
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            g.gcov[a][b] = ldc(blk, a * 4 + b);
+            g.gcon[a][b] = ldc(blk, 16 + a * 4 + b);
+        }
+    g.g = ldc(blk, 32);
+}
+
+// Wave-uniform column index: N3 % 64 == 0 maps a wave onto one (i,j) column.
+template <bool UNI>
+HD size_t uni_col(size_t col)
+{
+    if (UNI) return (size_t)__builtin_amdgcn_readfirstlane((int)col);
+    return col;
 }
 
 // metric.c:210-229
@@ -348,27 +380,30 @@ HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int di
 }
 
 // phys.c:165-207: dU = g * sum_jk T^j_k Gamma^k_{mu j}; conn is [64][ncol] (k,mu,j order)
+// Each stress-tensor row is formed just before it is contracted; the sums run
+// in the reference's order (j outer, k inner), so the result is unchanged while
+// only one row is live.  The state at ph is returned for reuse.
+// conn[((k*4+mu)*4+j)*cstride + col] = Gamma^k_{mu j}: cstride = ncol for the
+// variable-major table, 1 (with col = 0 and conn at the column block) for the
+// column-major one.
 HD void source(const double *ph, const Geom &geom, const double *__restrict__ conn, size_t ncol, size_t col,
-               double gam, double *dU, int &fatal)
+               double gam, double *dU, State &q, int &fatal)
 {
-    State q;
-    double mhd[4][4];
     get_state(ph, geom, q, fatal);
-    mhd_calc(ph, 0, q, gam, mhd[0]);
-    mhd_calc(ph, 1, q, gam, mhd[1]);
-    mhd_calc(ph, 2, q, gam, mhd[2]);
-    mhd_calc(ph, 3, q, gam, mhd[3]);
 #pragma unroll
     for (int k = 0; k < NPR; k++) dU[k] = 0.;
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < 4; j++) {
+        double mhd[4];
+        mhd_calc(ph, j, q, gam, mhd);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            dU[UU] += mhd[j][k] * conn[(size_t)((k * 4 + 0) * 4 + j) * ncol + col];
-            dU[U1] += mhd[j][k] * conn[(size_t)((k * 4 + 1) * 4 + j) * ncol + col];
-            dU[U2] += mhd[j][k] * conn[(size_t)((k * 4 + 2) * 4 + j) * ncol + col];
-            dU[U3] += mhd[j][k] * conn[(size_t)((k * 4 + 3) * 4 + j) * ncol + col];
+            dU[UU] += mhd[k] * ldc(conn, (size_t)((k * 4 + 0) * 4 + j) * ncol + col);
+            dU[U1] += mhd[k] * ldc(conn, (size_t)((k * 4 + 1) * 4 + j) * ncol + col);
+            dU[U2] += mhd[k] * ldc(conn, (size_t)((k * 4 + 2) * 4 + j) * ncol + col);
+            dU[U3] += mhd[k] * ldc(conn, (size_t)((k * 4 + 3) * 4 + j) * ncol + col);
         }
+    }
 #pragma unroll
     for (int k = 0; k < NPR; k++) dU[k] *= geom.g;
 }

@@ -63,7 +63,7 @@ enum {
 /* Kernel classes timed with HIP events when timing is enabled. */
 enum {
     HARM_K_FLUX1 = 0, HARM_K_FLUX2, HARM_K_FLUX3, HARM_K_EMF, HARM_K_UPDATE,
-    HARM_K_BOUND, HARM_K_FIXUTOPRIM, HARM_K_STEPEND, HARM_K_C2P_BATCH, HARM_K_MLP,
+    HARM_K_BOUND, HARM_K_FIXUTOPRIM, HARM_K_STEPEND, HARM_K_C2P_BATCH, HARM_K_MLP, HARM_K_FONTFIX,
     HARM_NKERNELS
 };
 
@@ -116,6 +116,9 @@ int harm_debug_fixup_utoprim(harm_ctx *c);
 /* The device pow used by the Font-fix solvers (correctly rounded), on host arrays;
  * for tests against the host libm pow the reference calls. */
 int harm_debug_pow(const double *x, const double *y, double *out, size_t n);
+/* A plain device-to-device copy of nbytes (1 + repeats launches): the reference
+ * reading for the HBM traffic counters and the achievable copy bandwidth. */
+int harm_debug_copy(size_t nbytes, int repeats, double *ms_per_copy);
 
 /* Geometry tables built on the host (main.c:413-508), canonical per-(i,j) layout:
  * gcov/gcon [5][16][N1+4][N2+4], gdet [5][N1+4][N2+4], conn [64][N1+4][N2+4]. */
