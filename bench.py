@@ -25,14 +25,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per zone of each kernel class (one launch covers every
-# active zone of one half-step).  flux: read the zone's 8 primitives, write the
-# face's 8 fluxes.  update: read pi and pb (8+8), the non-B fluxes of the zone's
-# six faces (5 x 3 x 2 faces, each face shared by two zones -> 5 x 3 x 1), the
-# three EMFs it owns, write pf (8) and the flag (1 B).
+# active zone of one half-step).
+#   flux: read the zone's 8 primitives, write the face's 8 fluxes.
+#   emf: read 6 B-flux components, write 3 EMFs.
+#   div: read the non-B fluxes of the zone's six faces (5 x 3 x 2 faces, each
+#        face shared by two zones -> 5 x 3 x 1) and the three EMFs it owns,
+#        write the 8 flux divergences.
+#   update: read pb (8) and the divergence (8), in the corrector also pi (8);
+#        write pf (8) and the flag (1 B): 193 B predictor, 257 B corrector,
+#        225 B per launch on average.
 ALG_BYTES = {
     "flux1": 128.0, "flux2": 128.0, "flux3": 128.0,
     "emf": 6 * 8 + 3 * 8,
-    "update": (8 + 8) * 8 + 15 * 8 + 3 * 8 + 8 * 8 + 1,
+    "div": 15 * 8 + 3 * 8 + 8 * 8,
+    "update": ((8 + 8) * 8 + 8 * 8 + 1 + (8 + 8 + 8) * 8 + 8 * 8 + 1) / 2,
 }
 
 
@@ -60,7 +66,7 @@ def cpu_baseline(threads: int) -> dict:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=str, default="288x256x256", help="grid N1xN2xN3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -23,7 +23,7 @@ COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_f
                  "fail_font2", "fixup_gamma", "fixup_repairs", "fixup_nostencil",
                  "floor_zones", "gamma_cap", "fatal"]
 KERNEL_NAMES = ["flux1", "flux2", "flux3", "emf", "update", "bound", "fixutoprim", "stepend",
-                "c2p_batch", "mlp", "fontfix"]
+                "c2p_batch", "mlp", "fontfix", "div"]
 NKERNELS = len(KERNEL_NAMES)
 
 _dp = POINTER(c_double)
@@ -72,6 +72,28 @@ def lib() -> ctypes.CDLL:
     L.harm_debug_pow.argtypes = [_dp, _dp, _dp, c_size_t]
     L.Utoprim_2d.argtypes = [_dp, _dp, _dp, c_double, _dp]
     L.harm_set_gamma.argtypes = [c_double]
+    # learned Newton seed
+    L.harm_mlp_create.restype = c_void_p
+    L.harm_mlp_create.argtypes = [c_int]
+    L.harm_mlp_destroy.argtypes = [c_void_p]
+    L.harm_mlp_init.argtypes = [c_void_p, ctypes.c_uint64]
+    _fp = POINTER(ctypes.c_float)
+    L.harm_mlp_set_params.argtypes = [c_void_p, _fp]
+    L.harm_mlp_get_params.argtypes = [c_void_p, _fp]
+    L.harm_mlp_set_norm.argtypes = [c_void_p, _dp]
+    L.harm_mlp_get_norm.argtypes = [c_void_p, _dp]
+    L.harm_mlp_set_vsq_from_w.argtypes = [c_void_p, c_int]
+    L.harm_mlp_fit_norm.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t]
+    L.harm_mlp_train.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int, c_int, ctypes.c_float,
+                                 ctypes.c_float, ctypes.c_uint64, _fp]
+    L.harm_mlp_train_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_float]
+    L.harm_mlp_grad.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int, _fp, _dp]
+    L.harm_mlp_forward.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int]
+    L.harm_mlp_seed.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
+                                c_void_p, c_void_p]
+    L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
+    L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
+    L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]
     _lib = L
     return L
 
@@ -84,6 +106,14 @@ def _check(rc: int, what: str) -> None:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def _sync_torch() -> None:
+    """The library's streams do not wait for torch's: finish torch's work on
+    the tensors handed over before a call reads them."""
+    import torch
+
+    torch.cuda.synchronize()
 
 
 class Harm:
@@ -182,6 +212,31 @@ class Harm:
     def debug_fixup_utoprim(self) -> None:
         _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim")
 
+    def set_guess(self, mode: int, mlp: "Mlp | None" = None) -> None:
+        """Newton start of the inversion: 0 previous-step primitives (the
+        reference's guess), 1 learned seed with previous-step fallback."""
+        _check(self.L.harm_set_guess(self.h, mode, mlp.h if mlp is not None else None), "harm_set_guess")
+        self._mlp_ref = mlp
+
+    def set_capture(self, feat=None, targ=None, valid=None, stride: int = 1) -> None:
+        """Capture training pairs into device tensors feat[8,cap] (float64),
+        targ[2,cap] (float64), valid[cap] (uint8); feat=None stops capturing."""
+        _sync_torch()
+        if feat is None:
+            _check(self.L.harm_set_capture(self.h, None, None, None, 0, 1), "harm_set_capture")
+            self._cap_ref = None
+            return
+        cap = feat.shape[1]
+        _train_set(feat, targ, valid)
+        _check(self.L.harm_set_capture(self.h, feat.data_ptr(), targ.data_ptr(), valid.data_ptr(), cap, stride),
+               "harm_set_capture")
+        self._cap_ref = (feat, targ, valid)
+
+    def capture_count(self) -> int:
+        n = c_size_t()
+        _check(self.L.harm_get_capture_count(self.h, ctypes.byref(n)), "harm_get_capture_count")
+        return n.value
+
     def geometry(self):
         ncol = (self.n1 + 4) * (self.n2 + 4)
         gcov = np.zeros((5, 16, ncol))
@@ -227,7 +282,147 @@ def Utoprim_2d(U, gcov, gcon, gdet: float, prim) -> int:
     return L.Utoprim_2d(_ptr(U), _ptr(gcov), _ptr(gcon), gdet, _ptr(prim))
 
 
-def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K: float = 0.0, stream=None):
+MLP_NPARAMS = 9026
+MLP_NORM = 20
+
+
+def _need(t, shape, dtype_name: str, what: str) -> None:
+    """The C ABI reads device tensors as dense row-major arrays: reject views."""
+    import torch
+
+    dt = {"f64": torch.float64, "f32": torch.float32, "u8": torch.uint8}[dtype_name]
+    if t.dtype != dt or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what}: expected a contiguous {dtype_name} device tensor of shape {tuple(shape)}, "
+                         f"got {tuple(t.shape)} {t.dtype} contiguous={t.is_contiguous()}")
+
+
+def _train_set(feat, targ, valid):
+    n = feat.shape[1]
+    _need(feat, (8, n), "f64", "feat")
+    _need(targ, (2, n), "f64", "targ")
+    if valid is not None:
+        _need(valid, (n,), "u8", "valid")
+    return n
+
+
+class Mlp:
+    """The learned Newton seed network (harm_mlp_* in include/harm_cool.h):
+    trained on the GPU by the library's own kernels, run in BF16 on MFMA."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self.L = lib()
+        self.h = self.L.harm_mlp_create(device)
+        if not self.h:
+            raise HarmError("harm_mlp_create failed: " + self.L.harm_last_error().decode(errors="replace"))
+        if seed != 1:
+            self.init(seed)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.harm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, seed: int) -> None:
+        _check(self.L.harm_mlp_init(self.h, seed), "harm_mlp_init")
+
+    def params(self) -> np.ndarray:
+        a = np.empty(MLP_NPARAMS, np.float32)
+        _check(self.L.harm_mlp_get_params(self.h, a.ctypes.data_as(POINTER(ctypes.c_float))), "harm_mlp_get_params")
+        return a
+
+    def set_params(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.size == MLP_NPARAMS
+        _check(self.L.harm_mlp_set_params(self.h, a.ctypes.data_as(POINTER(ctypes.c_float))), "harm_mlp_set_params")
+
+    def norm(self) -> np.ndarray:
+        a = np.empty(MLP_NORM)
+        _check(self.L.harm_mlp_get_norm(self.h, _ptr(a)), "harm_mlp_get_norm")
+        return a
+
+    def set_norm(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(self.L.harm_mlp_set_norm(self.h, _ptr(a)), "harm_mlp_set_norm")
+
+    def set_vsq_from_w(self, on: bool) -> None:
+        _check(self.L.harm_mlp_set_vsq_from_w(self.h, int(on)), "harm_mlp_set_vsq_from_w")
+
+    def fit_norm(self, feat, targ, valid) -> None:
+        _sync_torch()
+        n = _train_set(feat, targ, valid)
+        _check(self.L.harm_mlp_fit_norm(self.h, feat.data_ptr(), targ.data_ptr(),
+                                        valid.data_ptr() if valid is not None else None, n),
+               "harm_mlp_fit_norm")
+
+    def train(self, feat, targ, valid, batch: int = 65536, steps: int = 1000, lr0: float = 2e-3,
+              lr1: float = 1e-4, seed: int = 7, trace: bool = False):
+        _sync_torch()
+        n = _train_set(feat, targ, valid)
+        lt = np.zeros(max(steps, 1), np.float32) if trace else None
+        _check(self.L.harm_mlp_train(self.h, feat.data_ptr(), targ.data_ptr(),
+                                     valid.data_ptr() if valid is not None else None, n, batch, steps,
+                                     lr0, lr1, seed,
+                                     lt.ctypes.data_as(POINTER(ctypes.c_float)) if lt is not None else None),
+               "harm_mlp_train")
+        return lt
+
+    def train_batch(self, X, T, mask=None, lr: float = 1e-3) -> None:
+        _sync_torch()
+        n = X.shape[0]
+        _need(X, (n, 8), "f32", "X")
+        _need(T, (n, 2), "f32", "T")
+        if mask is not None:
+            _need(mask, (n,), "u8", "mask")
+        _check(self.L.harm_mlp_train_batch(self.h, X.data_ptr(), T.data_ptr(),
+                                           mask.data_ptr() if mask is not None else None, X.shape[0], lr),
+               "harm_mlp_train_batch")
+
+    def grad(self, X, T, mask=None):
+        _sync_torch()
+        n = X.shape[0]
+        _need(X, (n, 8), "f32", "X")
+        _need(T, (n, 2), "f32", "T")
+        if mask is not None:
+            _need(mask, (n,), "u8", "mask")
+        g = np.empty(MLP_NPARAMS, np.float32)
+        loss = c_double()
+        _check(self.L.harm_mlp_grad(self.h, X.data_ptr(), T.data_ptr(), mask.data_ptr() if mask is not None else None,
+                                    X.shape[0], g.ctypes.data_as(POINTER(ctypes.c_float)), ctypes.byref(loss)),
+               "harm_mlp_grad")
+        return loss.value, g
+
+    def forward(self, X, bf16: bool):
+        _sync_torch()
+        import torch
+
+        _need(X, (X.shape[0], 8), "f32", "X")
+        Y = torch.empty((X.shape[0], 2), dtype=torch.float32, device=X.device)
+        _check(self.L.harm_mlp_forward(self.h, X.data_ptr(), Y.data_ptr(), X.shape[0], int(bf16)), "harm_mlp_forward")
+        return Y
+
+    def seed(self, U, gcov, gcon, gdet, want_seed: bool = True):
+        """Raw features [8,n] and, with want_seed, the network's (W0, vsq0)."""
+        _sync_torch()
+        import torch
+
+        n = gdet.shape[0]
+        for t, rows, nm in ((U, 8, "U"), (gcov, 16, "gcov"), (gcon, 16, "gcon")):
+            _need(t, (rows, n), "f64", nm)
+        _need(gdet, (n,), "f64", "gdet")
+        feat = torch.empty((8, n), dtype=torch.float64, device=U.device)
+        W0 = torch.empty(n, dtype=torch.float64, device=U.device) if want_seed else None
+        v0 = torch.empty(n, dtype=torch.float64, device=U.device) if want_seed else None
+        _check(self.L.harm_mlp_seed(self.h, U.data_ptr(), gcov.data_ptr(), gcon.data_ptr(), gdet.data_ptr(), n,
+                                    feat.data_ptr(), W0.data_ptr() if want_seed else None,
+                                    v0.data_ptr() if want_seed else None, None), "harm_mlp_seed")
+        return feat, W0, v0
+
+
+def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K: float = 0.0, stream=None,
+                    mlp: Mlp | None = None):
     """Batched inversion on device tensors (torch, float64, SoA: U[8,n], gcov[16,n],
     gcon[16,n], gdet[n], prim[8,n] in/out).  Returns (ret int8[n], iters uint8[n],
     kconv uint8[n]) as device tensors."""
@@ -240,7 +435,8 @@ def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K:
     ret = torch.empty(n, dtype=torch.int8, device=U.device)
     iters = torch.empty(n, dtype=torch.uint8, device=U.device)
     kconv = torch.empty(n, dtype=torch.uint8, device=U.device)
-    opts = U2POpts(gam=gam, font_fix=font_fix, guess_mode=0, mlp=None, K=K)
+    opts = U2POpts(gam=gam, font_fix=font_fix, guess_mode=1 if mlp is not None else 0,
+                   mlp=mlp.h if mlp is not None else None, K=K)
     s = stream if stream is not None else torch.cuda.current_stream(U.device).cuda_stream
     _check(lib().harm_utoprim2d_batch(U.data_ptr(), gcov.data_ptr(), gcon.data_ptr(), gdet.data_ptr(),
                                       prim.data_ptr(), ret.data_ptr(), iters.data_ptr(), kconv.data_ptr(), n,

@@ -23,6 +23,7 @@
 
 #include "harm_host.hpp"
 #include "harm_phys.hpp"
+#include "harm_mlp.hpp"
 
 using namespace harm;
 
@@ -240,38 +241,123 @@ struct FontList {
     int *overflow;
 };
 
+// Flux divergence of the zone update (step_ch.c:250-263) with the flux_ct
+// B-fluxes formed from the EMFs, one active zone per thread, into the compact
+// array S[k][idx] (idx = active-zone index):
+//   S = -(F1[i+1] - F1[i])/dx1 - (F2[j+1] - F2[j])/dx2 [- (F3[h+1] - F3[h])/dx3]
+// so that the update's U += Dt*(S + dU) repeats the reference's operation
+// order exactly.  A light, high-occupancy stencil pass: the inversion kernel
+// that follows then reads 8 values per zone instead of ~50 scattered ones.
+__global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__ F1, const double *__restrict__ F2,
+                                             const double *__restrict__ F3, const double *__restrict__ e1,
+                                             const double *__restrict__ e2, const double *__restrict__ e3,
+                                             double *__restrict__ S)
+{
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t na = (size_t)ni * nj * g.N3;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx >= na) return;
+    const int h = (int)(idx % g.N3);
+    const size_t rest = idx / g.N3;
+    const int j = g.js + (int)(rest % nj);
+    const int i = g.is + (int)(rest / nj);
+    const size_t z = zidx(g, i, j, h);
+    const size_t nz = g.nz;
+    const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
+    const size_t zh1 = z - h + (h + 1) % g.N3;
+    double f1a[3], f1b[3], f2a[3], f2b[3], f3a[3], f3b[3];
+    ct_f1(g, e1, e2, e3, i, j, h, f1a);
+    ct_f1(g, e1, e2, e3, i + 1, j, h, f1b);
+    ct_f2(g, e1, e2, e3, i, j, h, f2a);
+    ct_f2(g, e1, e2, e3, i, j + 1, h, f2b);
+    if (g.N3 > 2) {
+        ct_f3(g, e1, e2, i, j, h, f3a);
+        ct_f3(g, e1, e2, i, j, (h + 1) % g.N3, f3b);
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        double a1, b1, a2, b2;
+        if (k < B1 || (k == B3 && g.N3 <= 2)) {
+            a1 = F1[k * nz + z];
+            b1 = F1[k * nz + zi1];
+            a2 = F2[k * nz + z];
+            b2 = F2[k * nz + zj1];
+        } else {
+            a1 = f1a[k - B1];
+            b1 = f1b[k - B1];
+            a2 = f2a[k - B1];
+            b2 = f2b[k - B1];
+        }
+        double d;
+        if (g.N3 > 2) {
+            double a3, b3;
+            if (k < B1) {
+                a3 = F3[k * nz + z];
+                b3 = F3[k * nz + zh1];
+            } else {
+                a3 = f3a[k - B1];
+                b3 = f3b[k - B1];
+            }
+            d = -(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2 - (b3 - a3) / g.dx3;
+        } else {
+            d = -(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2;
+        }
+        S[k * na + idx] = d;
+    }
+}
+
+// Optional parts of k_update: the learned Newton seed and the capture of
+// training pairs (no counterpart in the reference).
+struct UpdateExtra {
+    MlpDev mlp;
+    double *cap_feat;   // [8][cap] raw features
+    double *cap_targ;   // [2][cap] log(W/D), v^2
+    uint8_t *cap_valid; // [cap]
+    size_t cap, cap_base;
+    int stride, phase;
+};
+
+enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2 };
+
 // advance() zone loop (step_ch.c:237-306) + fixup() (fixup.c:51-60), one active
 // zone per thread: conserved update from pi with sources at pb, Noble 2D
 // inversion seeded with pi, then floors / gamma limit.  Zones whose inversion
 // fails go to k_fontfix (step_ch.c:277-300) with the guess left in pf.
-template <bool UNI>
+// MODE UPD_CAPTURE also stores training pairs of sampled zones; UPD_LEARNED
+// seeds the Newton solve from the network (wave-level BF16 MFMA forward) and
+// falls back to the pi guess where the seeded solve fails.
+template <bool UNI, int MODE>
 __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb,
                                                 double *__restrict__ pf, int8_t *__restrict__ pflag,
-                                                const double *__restrict__ F1, const double *__restrict__ F2,
-                                                const double *__restrict__ F3, const double *__restrict__ e1,
-                                                const double *__restrict__ e2, const double *__restrict__ e3,
-                                                const double *__restrict__ tstate, double dtfac, int pi_is_pb,
-                                                FontList fl, unsigned long long *__restrict__ cnt)
+                                                const double *__restrict__ S, const double *__restrict__ tstate,
+                                                double dtfac, int pi_is_pb, FontList fl,
+                                                unsigned long long *__restrict__ cnt, UpdateExtra ex)
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t na = (size_t)ni * nj * g.N3;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    const bool active = idx < na;
     int c_call = 0, c_iter = 0, c_kconv = 0, c_f2d = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
-    if (idx < (size_t)ni * nj * g.N3) {
-        const int h = (int)(idx % g.N3);
+    int i = 0, j = 0, h = 0;
+    size_t z = 0, col = 0;
+    Geom geom;
+    const double *blk = nullptr;
+    double piz[NPR], U[NPR];
+    if (active) {
+        h = (int)(idx % g.N3);
         const size_t rest = idx / g.N3;
-        const int j = g.js + (int)(rest % nj);
-        const int i = g.is + (int)(rest / nj);
-        const size_t z = zidx(g, i, j, h);
+        j = g.js + (int)(rest % nj);
+        i = g.is + (int)(rest / nj);
+        z = zidx(g, i, j, h);
         const size_t nz = g.nz;
-        const size_t col = uni_col<UNI>((size_t)i * g.SN2 + j);
+        col = uni_col<UNI>((size_t)i * g.SN2 + j);
         const double Dt = dtfac * tstate[1];
-        const double *blk = g.cblk + col * CENT_STRIDE;
-        Geom geom;
+        blk = g.cblk + col * CENT_STRIDE;
         if (UNI)
             load_geom_block(blk, geom);
         else
             load_geom(g.geom[CENT], g.ncol, col, geom);
-        double piz[NPR], dU[NPR], U[NPR];
+        double dU[NPR];
         {
             double pbz[NPR];
 #pragma unroll
@@ -286,50 +372,54 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
             if (!pi_is_pb) get_state(piz, geom, q, fatal);
             primtoflux(piz, q, 0, geom, g.gam, U);
         }
-        const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
-        const size_t zh1 = z - h + (h + 1) % g.N3;
-        double f1a[3], f1b[3], f2a[3], f2b[3], f3a[3], f3b[3];
-        ct_f1(g, e1, e2, e3, i, j, h, f1a);
-        ct_f1(g, e1, e2, e3, i + 1, j, h, f1b);
-        ct_f2(g, e1, e2, e3, i, j, h, f2a);
-        ct_f2(g, e1, e2, e3, i, j + 1, h, f2b);
-        if (g.N3 > 2) {
-            ct_f3(g, e1, e2, i, j, h, f3a);
-            ct_f3(g, e1, e2, i, j, (h + 1) % g.N3, f3b);
-        }
 #pragma unroll
-        for (int k = 0; k < NPR; k++) {
-            double a1, b1, a2, b2;
-            if (k < B1 || (k == B3 && g.N3 <= 2)) {
-                a1 = F1[k * nz + z];
-                b1 = F1[k * nz + zi1];
-                a2 = F2[k * nz + z];
-                b2 = F2[k * nz + zj1];
-            } else {
-                a1 = f1a[k - B1];
-                b1 = f1b[k - B1];
-                a2 = f2a[k - B1];
-                b2 = f2b[k - B1];
-            }
-            if (g.N3 > 2) {
-                double a3, b3;
-                if (k < B1) {
-                    a3 = F3[k * nz + z];
-                    b3 = F3[k * nz + zh1];
-                } else {
-                    a3 = f3a[k - B1];
-                    b3 = f3b[k - B1];
+        for (int k = 0; k < NPR; k++) U[k] += Dt * (S[k * na + idx] + dU[k]);
+    }
+    double pr[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) pr[k] = piz[k];
+    int niter = 0, kconv = 0, ret = 0;
+    if constexpr (MODE == UPD_LEARNED) {
+        double f[MLP_NIN];
+        const bool ok = active && mlp_features_U(U, geom, f);
+        float x[MLP_NIN];
+        if (ok)
+            mlp_standardize(ex.mlp, f, x);
+        else
+#pragma unroll
+            for (int k = 0; k < MLP_NIN; k++) x[k] = 0.f;
+        float y0, y1;
+        mlp_forward_wave(ex.mlp, x, y0, y1);
+        if (active) {
+            double W0 = 0., vsq0 = 0.;
+            if (ok) mlp_unstandardize(ex.mlp, y0, y1, exp(f[0]), W0, vsq0);
+            int fb;
+            ret = utoprim_2d_learned(U, geom, g.gam, pr, W0, vsq0, ex.mlp.vsq_from_W, niter, kconv, fb);
+        }
+    } else {
+        if (active) ret = utoprim_2d(U, geom, g.gam, pr, niter, kconv);
+        if constexpr (MODE == UPD_CAPTURE) {
+            if (active && (int)(idx % (size_t)ex.stride) == ex.phase) {
+                const size_t slot = ex.cap_base + idx / (size_t)ex.stride;
+                if (slot < ex.cap) {
+                    double f[MLP_NIN];
+                    const bool ok = mlp_features_U(U, geom, f) && ret == 0;
+                    double t0 = 0., t1 = 0.;
+                    if (ok) {
+                        const double utsq = utsq_of(geom, pr);
+                        t0 = log((pr[RHO] + g.gam * pr[UU]) * (1. + utsq)) - f[0];
+                        t1 = utsq / (1. + utsq);
+                    }
+#pragma unroll
+                    for (int k = 0; k < MLP_NIN; k++) ex.cap_feat[(size_t)k * ex.cap + slot] = f[k];
+                    ex.cap_targ[slot] = t0;
+                    ex.cap_targ[ex.cap + slot] = t1;
+                    ex.cap_valid[slot] = ok ? 1 : 0;
                 }
-                U[k] += Dt * (-(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2 - (b3 - a3) / g.dx3 + dU[k]);
-            } else {
-                U[k] += Dt * (-(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2 + dU[k]);
             }
         }
-        double pr[NPR];
-#pragma unroll
-        for (int k = 0; k < NPR; k++) pr[k] = piz[k];
-        int niter, kconv;
-        int ret = utoprim_2d(U, geom, g.gam, pr, niter, kconv);
+    }
+    if (active) {
         c_call = 1;
         c_iter = niter;
         c_kconv = kconv;
@@ -360,6 +450,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
                 c_gam = 1;
             }
         }
+        const size_t nz = g.nz;
 #pragma unroll
         for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
         pflag[z] = flag;
@@ -935,29 +1026,54 @@ __global__ void __launch_bounds__(TPB) k_diag(DGrid g, const double *__restrict_
 }
 
 // Batched Noble 2D inversion over SoA arrays (utoprim_2d.c:126, plus the Font fix).
+// Batched inversion; LEARNED seeds Newton from the network (all lanes take
+// part in the wave-level forward pass, so there is no early exit).
+template <bool LEARNED>
 __global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U, const double *__restrict__ gcov,
                                                    const double *__restrict__ gcon, const double *__restrict__ gdet,
                                                    double *__restrict__ prim, int8_t *__restrict__ ret,
                                                    uint8_t *__restrict__ iters, uint8_t *__restrict__ kconv, size_t n,
-                                                   double gam, int font_fix, double K, int *__restrict__ ret_full)
+                                                   double gam, int font_fix, double K, int *__restrict__ ret_full,
+                                                   MlpDev mlp)
 {
     const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (z >= n) return;
+    const bool act = z < n;
+    if (!LEARNED && !act) return;
     Geom geom;
-#pragma unroll
-    for (int a = 0; a < 16; a++) {
-        geom.gcov[a / 4][a % 4] = gcov[(size_t)a * n + z];
-        geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
-    }
-    geom.g = gdet[z];
     double Uz[NPR], pr[NPR];
+    if (act) {
 #pragma unroll
-    for (int k = 0; k < NPR; k++) {
-        Uz[k] = U[(size_t)k * n + z];
-        pr[k] = prim[(size_t)k * n + z];
+        for (int a = 0; a < 16; a++) {
+            geom.gcov[a / 4][a % 4] = gcov[(size_t)a * n + z];
+            geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
+        }
+        geom.g = gdet[z];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            Uz[k] = U[(size_t)k * n + z];
+            pr[k] = prim[(size_t)k * n + z];
+        }
     }
-    int niter, kc;
-    int r = utoprim_2d(Uz, geom, gam, pr, niter, kc);
+    int niter = 0, kc = 0, r = 0;
+    if constexpr (LEARNED) {
+        double f[MLP_NIN];
+        const bool ok = act && mlp_features_U(Uz, geom, f);
+        float x[MLP_NIN];
+        if (ok)
+            mlp_standardize(mlp, f, x);
+        else
+#pragma unroll
+            for (int k = 0; k < MLP_NIN; k++) x[k] = 0.f;
+        float y0, y1;
+        mlp_forward_wave(mlp, x, y0, y1);
+        if (!act) return;
+        double W0 = 0., vsq0 = 0.;
+        if (ok) mlp_unstandardize(mlp, y0, y1, exp(f[0]), W0, vsq0);
+        int fb;
+        r = utoprim_2d_learned(Uz, geom, gam, pr, W0, vsq0, mlp.vsq_from_W, niter, kc, fb);
+    } else {
+        r = utoprim_2d(Uz, geom, gam, pr, niter, kc);
+    }
     int8_t flag = (int8_t)r;
     if (flag && font_fix) {
         flag = (int8_t)utoprim_1dvsq2fix1(Uz, geom, gam, K, pr);
@@ -1000,6 +1116,7 @@ struct harm_ctx {
     hipStream_t stream;
     size_t nz;
     double *p, *ph, *F1, *F2, *F3, *e1, *e2, *e3;
+    double *S;  // [8][active zones] flux divergence
     int8_t *pflag, *pflag2;
     double *d_geom[5], *d_conn, *d_floors, *d_katm, *d_cblk, *d_fblk;
     double *tstate;                   // t, dt, mass_in, mass_out, err
@@ -1011,6 +1128,13 @@ struct harm_ctx {
     FontList font;
     bool uniform;  // N3 % 64 == 0: a wave's lanes share one (i,j) column
     int fix_cap;
+    // learned Newton seed and training-pair capture
+    int guess_mode;
+    const harm_mlp *mlp;
+    double *cap_feat, *cap_targ;
+    uint8_t *cap_valid;
+    size_t cap, cap_count;
+    int cap_stride, cap_phase;
     bool timing;
     struct Ev {
         int cls;
@@ -1145,14 +1269,47 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     if ((rc = fluxes_and_emf(c, pb, fullstep))) return rc;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
     const int same = (pi == pb) ? 1 : 0;
-    if (c->uniform)
-        LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<true>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi,
-                                                 pb, pf, c->pflag, c->F1, c->F2, c->F3, c->e1, c->e2, c->e3,
-                                                 c->tstate, dtfac, same, c->font, c->cnt));
-    else
-        LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<false>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g,
-                                                 pi, pb, pf, c->pflag, c->F1, c->F2, c->F3, c->e1, c->e2, c->e3,
-                                                 c->tstate, dtfac, same, c->font, c->cnt));
+    UpdateExtra ex;
+    std::memset(&ex, 0, sizeof(ex));
+    int mode = UPD_PLAIN;
+    if (c->guess_mode == 1 && c->mlp) {
+        mode = UPD_LEARNED;
+        ex.mlp = harm_mlp_view(c->mlp);
+    } else if (c->cap_feat && c->cap_count < c->cap) {
+        mode = UPD_CAPTURE;
+        ex.cap_feat = c->cap_feat;
+        ex.cap_targ = c->cap_targ;
+        ex.cap_valid = c->cap_valid;
+        ex.cap = c->cap;
+        ex.cap_base = c->cap_count;
+        ex.stride = c->cap_stride;
+        ex.phase = c->cap_phase;
+        const size_t ph = (size_t)c->cap_phase;
+        const size_t taken = (na > ph) ? (na - ph + c->cap_stride - 1) / c->cap_stride : 0;
+        c->cap_count = std::min(c->cap, c->cap_count + taken);
+        c->cap_phase = (int)((ph + 7919) % (size_t)c->cap_stride);
+    }
+#define UPD_LAUNCH(U_, M_)                                                                                          \
+    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi, \
+                                             pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cnt, ex))
+    LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
+                                          c->e1, c->e2, c->e3, c->S));
+    if (c->uniform) {
+        if (mode == UPD_PLAIN)
+            UPD_LAUNCH(true, UPD_PLAIN);
+        else if (mode == UPD_CAPTURE)
+            UPD_LAUNCH(true, UPD_CAPTURE);
+        else
+            UPD_LAUNCH(true, UPD_LEARNED);
+    } else {
+        if (mode == UPD_PLAIN)
+            UPD_LAUNCH(false, UPD_PLAIN);
+        else if (mode == UPD_CAPTURE)
+            UPD_LAUNCH(false, UPD_CAPTURE);
+        else
+            UPD_LAUNCH(false, UPD_LEARNED);
+    }
+#undef UPD_LAUNCH
     if (g.font_fix)
         LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(256), dim3(TPB), 0, c->stream, g, pf, c->pflag,
                                                   c->font, c->cnt));
@@ -1215,6 +1372,7 @@ int ctx_alloc(harm_ctx *c)
     for (double **q : {&c->p, &c->ph, &c->F1, &c->F2, &c->F3})
         if ((rc = dalloc(q, 8 * c->nz))) return rc;
     if ((rc = dalloc(&c->e3, c->nz))) return rc;
+    if ((rc = dalloc(&c->S, 8 * (size_t)hg.N1 * hg.N2 * hg.N3))) return rc;
     if (hg.N3 > 2) {
         if ((rc = dalloc(&c->e1, c->nz))) return rc;
         if ((rc = dalloc(&c->e2, c->nz))) return rc;
@@ -1307,7 +1465,7 @@ int ctx_alloc(harm_ctx *c)
 void ctx_free(harm_ctx *c)
 {
     for (void *q : {(void *)c->p, (void *)c->ph, (void *)c->F1, (void *)c->F2, (void *)c->F3, (void *)c->e1,
-                    (void *)c->e2, (void *)c->e3, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
+                    (void *)c->e2, (void *)c->e3, (void *)c->S, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
                     (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->cnt,
                     (void *)c->part, (void *)c->fix_list, (void *)c->fix_count, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk})
@@ -1345,6 +1503,8 @@ int copy_state(harm_ctx *c, double *host_prim, int8_t *host_flag, bool to_host)
 // ===========================================================================
 // C ABI
 // ===========================================================================
+int harm_host::set_error(const std::string &msg, int code) { return set_err(msg, code); }
+
 extern "C" {
 
 const char *harm_last_error(void) { return g_err.c_str(); }
@@ -1646,15 +1806,66 @@ int harm_get_geometry(harm_ctx *c, double *gcov, double *gcon, double *gdet, dou
     return 0;
 }
 
+int harm_set_guess(harm_ctx *c, int mode, const harm_mlp *m)
+{
+    if (!c) return set_err("null context");
+    if (mode != 0 && mode != 1) return set_err("unknown guess mode", 4);
+    if (mode == 1) {
+        if (!m) return set_err("guess mode 1 needs a harm_mlp handle", 4);
+        if (harm_mlp_device(m) != c->device) return set_err("network and context are on different devices", 4);
+    }
+    c->guess_mode = mode;
+    c->mlp = (mode == 1) ? m : nullptr;
+    return 0;
+}
+
+int harm_set_capture(harm_ctx *c, double *feat, double *targ, uint8_t *valid, size_t cap, int stride)
+{
+    if (!c) return set_err("null context");
+    if (!feat) {
+        c->cap_feat = nullptr;
+        c->cap_targ = nullptr;
+        c->cap_valid = nullptr;
+        c->cap = 0;
+        return 0;
+    }
+    if (!targ || !valid || cap == 0 || stride < 1) return set_err("capture needs feat, targ, valid, cap > 0, stride >= 1", 4);
+    c->cap_feat = feat;
+    c->cap_targ = targ;
+    c->cap_valid = valid;
+    c->cap = cap;
+    c->cap_count = 0;
+    c->cap_stride = stride;
+    c->cap_phase = 0;
+    return 0;
+}
+
+int harm_get_capture_count(harm_ctx *c, size_t *n)
+{
+    if (!c || !n) return set_err("null argument");
+    *n = c->cap_count;
+    return 0;
+}
+
 int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet, double *prim,
                          int8_t *ret, uint8_t *iters, uint8_t *kconv, size_t n, const harm_u2p_opts *opts,
                          void *stream)
 {
     if (!opts) return set_err("null options");
     if (n == 0) return 0;
-    if (opts->guess_mode != 0) return set_err("guess_mode 1 needs the MLP entry point (harm_mlp_*)", 4);
-    hipLaunchKernelGGL(k_c2p_batch, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon, gdet, prim,
-                       ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr);
+    MlpDev mv;
+    std::memset(&mv, 0, sizeof(mv));
+    if (opts->guess_mode == 1) {
+        if (!opts->mlp) return set_err("guess_mode 1 needs a harm_mlp handle", 4);
+        mv = harm_mlp_view((const harm_mlp *)opts->mlp);
+        hipLaunchKernelGGL((k_c2p_batch<true>), dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon,
+                           gdet, prim, ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr, mv);
+    } else if (opts->guess_mode == 0) {
+        hipLaunchKernelGGL((k_c2p_batch<false>), dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon,
+                           gdet, prim, ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr, mv);
+    } else {
+        return set_err("unknown guess_mode", 4);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1720,8 +1931,10 @@ int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, d
     h[40] = gdet;
     std::memcpy(h + 41, prim, 8 * sizeof(double));
     HIPCHK(hipMemcpy(dbuf, h, sizeof(h), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_c2p_batch, dim3(1), dim3(TPB), 0, 0, dbuf, dbuf + 8, dbuf + 24, dbuf + 40, dbuf + 41, dret,
-                       (uint8_t *)nullptr, (uint8_t *)nullptr, (size_t)1, g_gamma, 0, 0., dfull);
+    MlpDev mv;
+    std::memset(&mv, 0, sizeof(mv));
+    hipLaunchKernelGGL((k_c2p_batch<false>), dim3(1), dim3(TPB), 0, 0, dbuf, dbuf + 8, dbuf + 24, dbuf + 40,
+                       dbuf + 41, dret, (uint8_t *)nullptr, (uint8_t *)nullptr, (size_t)1, g_gamma, 0, 0., dfull, mv);
     HIPCHK(hipGetLastError());
     int r;
     HIPCHK(hipMemcpy(prim, dbuf + 41, 8 * sizeof(double), hipMemcpyDeviceToHost));

@@ -1,6 +1,7 @@
 // harm_host.hpp -- host-side grid description and setup tables of libharmcool.so.
 #pragma once
 #include <cstddef>
+#include <string>
 #include <vector>
 
 #include "../../include/harm_cool.h"
@@ -23,6 +24,9 @@ struct GeomTables {
 };
 
 int build_geometry(const harm_params &P, Grid &g, GeomTables &t);
+// Sets the message harm_last_error() returns; returns code.
+int set_error(const std::string &msg, int code);
+
 int build_torus2d(const harm_params &P, const Grid &g, const GeomTables &t, std::vector<double> &tor, double &rhomax,
                   double &umax);
 

@@ -600,12 +600,9 @@ HD double raise_W(const U2P &s, double W_last)
 
 // utoprim_2d.c:306-356 from a starting W: Newton on (W, v^2), then recovery.
 // Returns the reference's code; prim[0..4] written only on success.
-HD int solve_2d_from_W(U2P &s, const double *U_tmp, const double *Bcon, const double *Qcon, const double *ncon,
-                       double QdotB, double W_last, double *prim, int &n_iter, int &k_conv)
+HD int solve_2d_from_x(U2P &s, const double *Bcon, const double *Qcon, const double *ncon, double QdotB, double x0,
+                       double x1, double *prim, int &n_iter, int &k_conv)
 {
-    double x0 = fabs(W_last);
-    double vsq0 = fabs(vsq_calc(s, W_last));
-    double x1 = (vsq0 > 1.) ? (1.0 - 1.e-15) : vsq0;
     int retval = gnr_2d(s, x0, x1, n_iter, k_conv);
     double W = x0, vsq = x1;
     if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
@@ -622,6 +619,19 @@ HD int solve_2d_from_W(U2P &s, const double *U_tmp, const double *Bcon, const do
     prim[UU] = u;
     u2p_recover_vel(s, Bcon, Qcon, ncon, QdotB, W, gamma, prim);
     return 0;
+}
+
+// utoprim_2d.c:306-307 (x1_of_x0, utoprim_2d.c:395-406): the Newton start from a W guess.
+HD double x1_of_x0(const U2P &s, double W)
+{
+    double vsq0 = fabs(vsq_calc(s, W));
+    return (vsq0 > 1.) ? (1.0 - 1.e-15) : vsq0;
+}
+
+HD int solve_2d_from_W(U2P &s, const double *U_tmp, const double *Bcon, const double *Qcon, const double *ncon,
+                       double QdotB, double W_last, double *prim, int &n_iter, int &k_conv)
+{
+    return solve_2d_from_x(s, Bcon, Qcon, ncon, QdotB, fabs(W_last), x1_of_x0(s, W_last), prim, n_iter, k_conv);
 }
 
 // utoprim_2d.c:126-176 + 219-362.  U is HARM's conserved vector (x gdet).
@@ -648,6 +658,60 @@ HD int utoprim_2d(const double *U, const Geom &geom, double gam, double *prim, i
 #pragma unroll
         for (int i = 0; i < 5; i++) prim[i] = pt[i];
     }
+    return ret;
+}
+
+// utoprim_2d with the Newton start (W0, vsq0) given instead of the guess of
+// utoprim_2d.c:268-307: x = (|W0|, x1_of_x0(W0)) with vsq_from_W, else
+// (|W0|, vsq0 clipped to [0, 1-1e-15]).  Same codes and writes as utoprim_2d.
+HD int utoprim_2d_seeded(const double *U, const Geom &geom, double gam, double *prim, double W0, double vsq0,
+                         int vsq_from_W, int &n_iter, int &k_conv)
+{
+    n_iter = 0;
+    k_conv = 0;
+#pragma unroll
+    for (int i = B1; i <= B3; i++) prim[i] = U[i] / geom.g;
+    if (U[0] <= 0.) return -100;
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.K_atm = 0.;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    const double x1 = vsq_from_W ? x1_of_x0(s, W0) : ((vsq0 < 0.) ? 0. : ((vsq0 > 1. - 1.e-15) ? (1. - 1.e-15) : vsq0));
+    double pt[5];
+    int ret = solve_2d_from_x(s, Bcon, Qcon, ncon, QdotB, fabs(W0), x1, pt, n_iter, k_conv);
+    if (ret == 0) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) prim[i] = pt[i];
+    }
+    return ret;
+}
+
+// The learned-seed solve: seeded Newton, and on failure utoprim_2d from the
+// guess primitives in prim (the previous-step state).  Counts add up.
+HD int utoprim_2d_learned(const double *U, const Geom &geom, double gam, double *prim, double W0, double vsq0,
+                          int vsq_from_W, int &n_iter, int &k_conv, int &fell_back)
+{
+    double pt[NPR];
+#pragma unroll
+    for (int i = 0; i < NPR; i++) pt[i] = prim[i];
+    int n1, k1;
+    int ret = utoprim_2d_seeded(U, geom, gam, pt, W0, vsq0, vsq_from_W, n1, k1);
+    fell_back = 0;
+    if (ret == 0 || ret == -100) {
+#pragma unroll
+        for (int i = 0; i < NPR; i++) prim[i] = pt[i];
+        n_iter = n1;
+        k_conv = k1;
+        return ret;
+    }
+    fell_back = 1;
+    int n2, k2;
+    ret = utoprim_2d(U, geom, gam, prim, n2, k2);
+    n_iter = n1 + n2;
+    k_conv = n1 + k2;
     return ret;
 }
 

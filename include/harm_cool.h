@@ -64,7 +64,7 @@ enum {
 enum {
     HARM_K_FLUX1 = 0, HARM_K_FLUX2, HARM_K_FLUX3, HARM_K_EMF, HARM_K_UPDATE,
     HARM_K_BOUND, HARM_K_FIXUTOPRIM, HARM_K_STEPEND, HARM_K_C2P_BATCH, HARM_K_MLP, HARM_K_FONTFIX,
-    HARM_NKERNELS
+    HARM_K_DIV, HARM_NKERNELS
 };
 
 typedef struct harm_ctx harm_ctx;
@@ -142,6 +142,67 @@ typedef struct harm_u2p_opts {
 int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet,
                          double *prim, int8_t *ret, uint8_t *iters, uint8_t *kconv, size_t n,
                          const harm_u2p_opts *opts, void *stream);
+
+/* ---- Learned Newton seed (no reference counterpart) -----------------------
+ * A small network predicts the Newton start (W, v^2) of the Noble 2D inversion
+ * in place of the guess block utoprim_2d.c:268-307; the FP64 Newton solve
+ * (utoprim_2d.c:442-541) refines it, and the previous-step guess is the
+ * fallback where the seeded solve fails.  Network 8 -> 64 -> 64 -> 64 -> 2
+ * (tanh), FP32 master parameters (HARM_MLP_NPARAMS floats, row-major W[out][in]:
+ * W1 64x8, b1, W2 64x64, b2, W3 64x64, b3, W4 2x64, b4), trained on the GPU in
+ * FP32; inference in BF16 on MFMA with FP32 accumulation.
+ * Inputs (raw, standardized by harm_mlp_fit_norm): log D, asinh(E/D),
+ * asinh(Qt^2/D^2), asinh(B^2/D), asinh((Q.B)^2/D^3), lapse, asinh((E-D-B^2/2)/D),
+ * asinh(B^2/E) with D, Q, B of utoprim_2d.c:240-266 and E = -Q.n.
+ * Outputs: log(W/D), v^2.  Device buffers are on the handle's device. */
+#define HARM_MLP_NPARAMS 9026
+#define HARM_MLP_NORM 20      /* feature mean[8], 1/std[8], output mean[2], std[2] */
+typedef struct harm_mlp harm_mlp;
+harm_mlp *harm_mlp_create(int device);
+void harm_mlp_destroy(harm_mlp *m);
+/* Random initialization (uniform +-sqrt(3/fan_in), zero biases) from seed. */
+int harm_mlp_init(harm_mlp *m, uint64_t seed);
+int harm_mlp_set_params(harm_mlp *m, const float *params);  /* host array */
+int harm_mlp_get_params(harm_mlp *m, float *params);
+int harm_mlp_set_norm(harm_mlp *m, const double *norm);     /* host array */
+int harm_mlp_get_norm(harm_mlp *m, double *norm);
+/* 1: seed v^2 from the predicted W as utoprim_2d.c:306-307 does (default);
+ * 0: use the network's second output. */
+int harm_mlp_set_vsq_from_w(harm_mlp *m, int on);
+/* Training set (device): feat[8][n] raw FP64 features, targ[2][n] raw FP64
+ * targets (log(W/D), v^2), valid[n] (0 rows carry no loss). */
+int harm_mlp_fit_norm(harm_mlp *m, const double *feat, const double *targ, const uint8_t *valid, size_t n);
+/* steps Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on batches of `batch`
+ * rows drawn with replacement from the set by a seeded hash; learning rate
+ * decays geometrically from lr0 to lr1.  loss_trace (host, may be NULL) gets
+ * the batch loss of every step. */
+int harm_mlp_train(harm_mlp *m, const double *feat, const double *targ, const uint8_t *valid, size_t n,
+                   int batch, int steps, float lr0, float lr1, uint64_t seed, float *loss_trace);
+/* One Adam step on a standardized batch (device X[n][8], T[n][2], mask[n] or NULL). */
+int harm_mlp_train_batch(harm_mlp *m, const float *X, const float *T, const uint8_t *mask, int n, float lr);
+/* Loss 1/(2n) sum mask*|O-T|^2 and its gradient (host array of HARM_MLP_NPARAMS)
+ * on a standardized batch, with the FP32 training kernels. */
+int harm_mlp_grad(harm_mlp *m, const float *X, const float *T, const uint8_t *mask, int n, float *grad,
+                  double *loss);
+/* Forward pass on standardized device inputs X[n][8] -> Y[n][2]: bf16 = 0 runs
+ * the FP32 training kernels, bf16 = 1 the BF16 MFMA inference path. */
+int harm_mlp_forward(harm_mlp *m, const float *X, float *Y, int n, int bf16);
+/* Raw features of n zones (device SoA as harm_utoprim2d_batch: U[8][n],
+ * gcov[16][n], gcon[16][n], gdet[n]) -> feat[8][n]; with W0/vsq0 non-NULL also
+ * the network's seed per zone (BF16 inference). */
+int harm_mlp_seed(harm_mlp *m, const double *U, const double *gcov, const double *gcon, const double *gdet,
+                  size_t n, double *feat, double *W0, double *vsq0, void *stream);
+
+/* Step integration of the learned seed.  mode 0: previous-step guess (the
+ * reference's Utoprim_2d); mode 1: learned seed with previous-step fallback.
+ * The handle must outlive its use by the context. */
+int harm_set_guess(harm_ctx *c, int mode, const harm_mlp *m);
+/* Capture (U, P) training pairs from the plain inversion of later half-steps:
+ * every stride-th active zone (phase rotating per half-step) appends its raw
+ * features, targets and a valid flag to the device buffers feat[8][cap],
+ * targ[2][cap], valid[cap] until cap rows are filled.  feat == NULL stops. */
+int harm_set_capture(harm_ctx *c, double *feat, double *targ, uint8_t *valid, size_t cap, int stride);
+int harm_get_capture_count(harm_ctx *c, size_t *n);
 
 /* Exact reference signature (utoprim_2d.c:126, decs.h:608-609), evaluated on the
  * GPU for one zone; gam is taken from harm_set_gamma() (default 4/3, init.c:91). */

@@ -2547,3 +2547,112 @@ int orc_vchar(const double *pr, const double *gcov, const double *gcon, double g
     get_state(pr, &geom, &q, &fatal);
     return vchar(pr, &q, &geom, dir, gam, cmax, cmin) + (int)fatal;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Learned Newton seed (no reference counterpart; its seed replaces the      */
+/* guess block utoprim_2d.c:268-307 and nothing else).                       */
+/* ------------------------------------------------------------------------ */
+/* The eight raw inputs of the seed network, from the scalars of
+ * utoprim_2d.c:240-266 and the lapse of utoprim_2d.c:145.  All ratios are
+ * taken to D so that the features do not change when U is rescaled. */
+int orc_u2p_features(const double U[8], const double *gcov, const double *gcon, double gdet, double feat[8])
+{
+    double gc[NDIM][NDIM], gn[NDIM][NDIM], U_tmp[NPR], prim_tmp[NPR], prim0[NPR] = {0};
+    double Bcon[NDIM], Qcon[NDIM], ncon[NDIM], QdotB, lapse, D, E;
+    u2p_t s;
+    int i;
+    if (U[0] <= 0.) {
+        for (i = 0; i < 8; i++) feat[i] = 0.;
+        return -100;
+    }
+    memcpy(gc, gcov, sizeof(gc));
+    memcpy(gn, gcon, sizeof(gn));
+    memset(&s, 0, sizeof(s));
+    u2p_transform(U, gn, gdet, prim0, U_tmp, prim_tmp);
+    u2p_scalars(&s, U_tmp, gc, gn, Bcon, Qcon, ncon, &QdotB);
+    lapse = 1.0 / sqrt(-gn[0][0]);
+    D = s.D;
+    E = -s.Qdotn;
+    feat[0] = log(D);
+    feat[1] = asinh(E / D);
+    feat[2] = asinh(s.Qtsq / (D * D));
+    feat[3] = asinh(s.Bsq / D);
+    feat[4] = asinh(s.QdotBsq / (D * D * D));
+    feat[5] = lapse;
+    feat[6] = asinh((E - D - 0.5 * s.Bsq) / D);
+    feat[7] = asinh(s.Bsq / E);
+    return 0;
+}
+
+/* Noble 2D inversion (utoprim_2d.c:126-176,219-362) started from the pair
+ * (W0, vsq0) instead of the guess that utoprim_2d.c:268-307 builds from prim.
+ * x = (|W0|, vsq): with vsq_from_W, vsq = x1_of_x0(W0) as utoprim_2d.c:306-307
+ * sets it from a W guess; otherwise vsq0 clipped to [0, 1-1e-15].  Return codes, prim[B] and
+ * the success-only write of prim[0..4] follow Utoprim_2d. */
+int orc_utoprim_2d_seeded(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8],
+                          double gam, double W0, double vsq0, int vsq_from_W, int *n_iter, int *k_conv)
+{
+    double gc[NDIM][NDIM], gn[NDIM][NDIM], U_tmp[NPR], prim_tmp[NPR], x_2d[2];
+    double Bcon[NDIM], Qcon[NDIM], ncon[NDIM], Qtcon[NDIM], QdotB;
+    double W, vsq, gtmp, gamma, rho0, w, p, u;
+    const double GAMMA = gam;
+    u2p_t s;
+    int i, retval;
+    memcpy(gc, gcov, sizeof(gc));
+    memcpy(gn, gcon, sizeof(gn));
+    memset(&s, 0, sizeof(s));
+    s.gam = gam;
+    if (n_iter) *n_iter = 0;
+    if (k_conv) *k_conv = 0;
+    for (i = B1; i <= B3; i++) prim[i] = U[i] / gdet;
+    if (U[0] <= 0.) return -100;
+    u2p_transform(U, gn, gdet, prim, U_tmp, prim_tmp);
+    u2p_scalars(&s, U_tmp, gc, gn, Bcon, Qcon, ncon, &QdotB);
+    x_2d[0] = fabs(W0);
+    if (vsq_from_W)
+        x_2d[1] = x1_of_x0(&s, W0);
+    else
+        x_2d[1] = (vsq0 < 0.) ? 0. : ((vsq0 > 1. - 1.e-15) ? (1. - 1.e-15) : vsq0);
+    retval = gnr_2d(&s, x_2d, n_iter, k_conv);
+    W = x_2d[0];
+    vsq = x_2d[1];
+    if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
+    if (W <= 0. || W > W_TOO_BIG) return 3;
+    if (vsq >= 1.) return 4;
+    gtmp = sqrt(1. - vsq);
+    gamma = 1. / gtmp;
+    rho0 = s.D * gtmp;
+    w = W * (1. - vsq);
+    p = (GAMMA - 1.) * (w - rho0) / GAMMA;
+    u = w - (rho0 + p);
+    if ((rho0 <= 0.) || (u <= 0.)) return 5;
+    prim[RHO] = rho0;
+    prim[UU] = u;
+    for (i = 1; i < 4; i++) Qtcon[i] = Qcon[i] + ncon[i] * s.Qdotn;
+    for (i = 1; i < 4; i++) prim[U1 + i - 1] = gamma / (W + s.Bsq) * (Qtcon[i] + QdotB * Bcon[i] / W);
+    return 0;
+}
+
+/* The learned-seed solve: seeded Newton, then, when it fails, Utoprim_2d from
+ * the previous-step primitives in prim.  n_iter/k_conv add both attempts. */
+int orc_utoprim_2d_learned(const double U[8], const double *gcov, const double *gcon, double gdet,
+                           double prim[8], double gam, double W0, double vsq0, int vsq_from_W, int *n_iter,
+                           int *k_conv, int *fell_back)
+{
+    double pt[8];
+    int n1 = 0, k1 = 0, n2 = 0, k2 = 0, ret, i;
+    for (i = 0; i < 8; i++) pt[i] = prim[i];
+    ret = orc_utoprim_2d_seeded(U, gcov, gcon, gdet, pt, gam, W0, vsq0, vsq_from_W, &n1, &k1);
+    *fell_back = 0;
+    if (ret == 0 || ret == -100) {
+        for (i = 0; i < 8; i++) prim[i] = pt[i];
+        *n_iter = n1;
+        *k_conv = k1;
+        return ret;
+    }
+    *fell_back = 1;
+    ret = orc_utoprim_2d(U, gcov, gcon, gdet, prim, gam, &n2, &k2);
+    *n_iter = n1 + n2;
+    *k_conv = n1 + k2;
+    return ret;
+}

@@ -93,6 +93,19 @@ void orc_primtoU_g(const double prim[8], const double *gcov, const double *gcon,
 /* HARM's conserved vector (phys.c:100-105) from primitives. */
 void orc_primtoU(const double prim[8], const double *gcov, const double *gcon,
                  double gdet, double gam, double U[8]);
+/* Learned Newton seed (no reference counterpart).  Raw network inputs of one
+ * zone: log D, asinh(E/D), asinh(Qt^2/D^2), asinh(B^2/D), asinh((Q.B)^2/D^3),
+ * lapse, asinh((E-D-B^2/2)/D), asinh(B^2/E) with E = -Q.n; -100 if U[0]<=0. */
+int  orc_u2p_features(const double U[8], const double *gcov, const double *gcon,
+                      double gdet, double feat[8]);
+/* Utoprim_2d started from (W0, vsq0) instead of the guess built from prim. */
+int  orc_utoprim_2d_seeded(const double U[8], const double *gcov, const double *gcon,
+                           double gdet, double prim[8], double gam, double W0, double vsq0,
+                           int vsq_from_W, int *n_iter, int *k_conv);
+/* Seeded solve with fallback to Utoprim_2d from prim; counts add up. */
+int  orc_utoprim_2d_learned(const double U[8], const double *gcov, const double *gcon,
+                            double gdet, double prim[8], double gam, double W0, double vsq0,
+                            int vsq_from_W, int *n_iter, int *k_conv, int *fell_back);
 /* Geometry of one point in code coordinates X (X[0] ignored). */
 void orc_gcov_func(const orc_params *p, const double X[4], double *gcov);
 void orc_dxdxp_func(const orc_params *p, const double X[4], double *dxdxp);
