@@ -87,6 +87,83 @@ __device__ __forceinline__ void add_counter(unsigned long long *cnt, int idx, in
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(&cnt[idx], (unsigned long long)s);
 }
 
+// Per-block totals of N per-thread counts, stored (no atomics) at
+// out[blockIdx.x*N + k]; every thread of the block must call it.  Device-wide
+// atomics from every wave to one address serialize across the XCDs and stall
+// the memory pipeline of the whole kernel; k_sum_partials folds these instead.
+template <int N>
+__device__ __forceinline__ void block_counts(const int (&v)[N], unsigned long long *__restrict__ out)
+{
+    __shared__ int red[TPB / 64][N];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const int s = wave_sum_int(v[k]);
+        if ((threadIdx.x & 63) == 0) red[w][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (int q = 0; q < TPB / 64; q++) t += (unsigned long long)red[q][threadIdx.x];
+        out[(size_t)blockIdx.x * N + threadIdx.x] = t;
+    }
+}
+
+// Per-block minimum of v stored at out[blockIdx.x]; every thread must call it.
+__device__ __forceinline__ void block_min(double v, double *__restrict__ out)
+{
+    __shared__ double red[TPB / 64];
+    const double m = wave_min_dbl(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+#pragma unroll
+        for (int q = 1; q < TPB / 64; q++) t = (red[q] < t) ? red[q] : t;
+        out[blockIdx.x] = t;
+    }
+}
+
+struct CntMap {
+    int n;
+    int idx[8];
+};
+
+// cnt[map.idx[k]] += sum over blocks b of part[b*map.n + k]; block k per counter.
+__global__ void __launch_bounds__(256) k_sum_partials(const unsigned long long *__restrict__ part, size_t nb,
+                                                      CntMap map, unsigned long long *__restrict__ cnt)
+{
+    __shared__ unsigned long long red[256];
+    const int k = blockIdx.x;
+    unsigned long long s = 0;
+    for (size_t b = threadIdx.x; b < nb; b += 256) s += part[b * map.n + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&cnt[map.idx[k]], red[0]);
+}
+
+// slot = min(slot, min over blocks of part[b]) where that minimum is < 1e9
+// (the CFL limit of a direction, step_ch.c:477-484); one block.
+__global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__ part, size_t nb,
+                                                      unsigned long long *__restrict__ slot)
+{
+    __shared__ double red[256];
+    double m = 1.e300;
+    for (size_t b = threadIdx.x; b < nb; b += 256) m = (part[b] < m) ? part[b] : m;
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o && red[threadIdx.x + o] < red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] < 1.e9) atomicMin(slot, (unsigned long long)__double_as_longlong(red[0]));
+}
+
 // ---------------------------------------------------------------------------
 // fluxcalc (step_ch.c:331-504) + fix_flux (coord.c:264-324), one face per thread.
 // DIR 1,2,3 = X1,X2,X3.  Slopes of the two zones adjacent to the face are formed
@@ -94,7 +171,7 @@ __device__ __forceinline__ void add_counter(unsigned long long *cnt, int idx, in
 // ---------------------------------------------------------------------------
 template <int DIR, bool UNI>
 __global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F,
-                                              unsigned long long *__restrict__ ndt_slot, int fullstep,
+                                              double *__restrict__ dtpart, int fullstep,
                                               unsigned long long *__restrict__ cnt)
 {
     const int i0 = (DIR == 1) ? g.is : g.is - 1;
@@ -171,10 +248,7 @@ __global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict_
             }
         }
     }
-    if (fullstep) {
-        const double m = wave_min_dbl(dtij);
-        if ((threadIdx.x & 63) == 0 && m < 1.e9) atomicMin(ndt_slot, (unsigned long long)__double_as_longlong(m));
-    }
+    if (fullstep) block_min(dtij, dtpart);
     add_counter(cnt, HARM_FATAL, fatal);
 }
 
@@ -331,7 +405,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
                                                 double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                 const double *__restrict__ S, const double *__restrict__ tstate,
                                                 double dtfac, int pi_is_pb, FontList fl,
-                                                unsigned long long *__restrict__ cnt, UpdateExtra ex)
+                                                unsigned long long *__restrict__ cpart, UpdateExtra ex)
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
@@ -455,14 +529,8 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
         pflag[z] = flag;
     }
-    add_counter(cnt, HARM_C2P_CALLS, c_call);
-    add_counter(cnt, HARM_NEWTON_ITERS, c_iter);
-    add_counter(cnt, HARM_NEWTON_KCONV, c_kconv);
-    add_counter(cnt, HARM_FAIL_2D, c_f2d);
-    add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
-    add_counter(cnt, HARM_FLOOR_ZONES, c_flr);
-    add_counter(cnt, HARM_GAMMA_CAP, c_cap);
-    add_counter(cnt, HARM_FATAL, fatal);
+    const int v[8] = {c_call, c_iter, c_kconv, c_f2d, c_gam, c_flr, c_cap, fatal};
+    block_counts<8>(v, cpart);
 }
 
 // Font fix (step_ch.c:277-300) + fixup1zone for the zones k_update deferred.
@@ -1121,6 +1189,8 @@ struct harm_ctx {
     double *d_geom[5], *d_conn, *d_floors, *d_katm, *d_cblk, *d_fblk;
     double *tstate;                   // t, dt, mass_in, mass_out, err
     unsigned long long *ndt_slot;     // 3
+    double *dtpart;                   // per-block CFL minima of one flux launch
+    unsigned long long *cpart;        // per-block counter totals of one update launch
     unsigned long long *cnt;          // HARM_NCOUNTERS
     double *part;                     // reduction partials
     size_t npart;
@@ -1205,23 +1275,28 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
     if (c->uniform) {
         if (dir == 1)
             LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, true>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->ndt_slot + 0, fullstep, c->cnt));
+                                                    c->dtpart, fullstep, c->cnt));
         else if (dir == 2)
             LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL((k_flux<2, true>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->ndt_slot + 1, fullstep, c->cnt));
+                                                    c->dtpart, fullstep, c->cnt));
         else
             LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL((k_flux<3, true>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->ndt_slot + 2, fullstep, c->cnt));
+                                                    c->dtpart, fullstep, c->cnt));
     } else {
         if (dir == 1)
             LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, false>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->ndt_slot + 0, fullstep, c->cnt));
+                                                    c->dtpart, fullstep, c->cnt));
         else if (dir == 2)
             LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL((k_flux<2, false>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->ndt_slot + 1, fullstep, c->cnt));
+                                                    c->dtpart, fullstep, c->cnt));
         else
             LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL((k_flux<3, false>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->ndt_slot + 2, fullstep, c->cnt));
+                                                    c->dtpart, fullstep, c->cnt));
+    }
+    if (fullstep) {
+        hipLaunchKernelGGL(k_min_partials, dim3(1), dim3(256), 0, c->stream, c->dtpart, (size_t)grid.x,
+                           c->ndt_slot + (dir - 1));
+        HIPCHK(hipGetLastError());
     }
     return 0;
 }
@@ -1291,7 +1366,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     }
 #define UPD_LAUNCH(U_, M_)                                                                                          \
     LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi, \
-                                             pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cnt, ex))
+                                             pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cpart, ex))
     LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
                                           c->e1, c->e2, c->e3, c->S));
     if (c->uniform) {
@@ -1310,6 +1385,13 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
             UPD_LAUNCH(false, UPD_LEARNED);
     }
 #undef UPD_LAUNCH
+    {
+        CntMap map = {8, {HARM_C2P_CALLS, HARM_NEWTON_ITERS, HARM_NEWTON_KCONV, HARM_FAIL_2D, HARM_FIXUP_GAMMA,
+                          HARM_FLOOR_ZONES, HARM_GAMMA_CAP, HARM_FATAL}};
+        hipLaunchKernelGGL(k_sum_partials, dim3(8), dim3(256), 0, c->stream, c->cpart, (size_t)nblocks(na), map,
+                           c->cnt);
+        HIPCHK(hipGetLastError());
+    }
     if (g.font_fix)
         LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(256), dim3(TPB), 0, c->stream, g, pf, c->pflag,
                                                   c->font, c->cnt));
@@ -1409,6 +1491,8 @@ int ctx_alloc(harm_ctx *c)
     }
     if ((rc = dalloc(&c->tstate, 8))) return rc;
     if ((rc = dalloc(&c->ndt_slot, 3))) return rc;
+    if ((rc = dalloc(&c->dtpart, (size_t)nblocks(c->nz)))) return rc;
+    if ((rc = dalloc(&c->cpart, 8 * (size_t)nblocks(c->nz)))) return rc;
     if ((rc = dalloc(&c->cnt, HARM_NCOUNTERS))) return rc;
     c->npart = std::max<size_t>(3 * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
     if ((rc = dalloc(&c->part, c->npart))) return rc;
@@ -1466,7 +1550,7 @@ void ctx_free(harm_ctx *c)
 {
     for (void *q : {(void *)c->p, (void *)c->ph, (void *)c->F1, (void *)c->F2, (void *)c->F3, (void *)c->e1,
                     (void *)c->e2, (void *)c->e3, (void *)c->S, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
-                    (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->cnt,
+                    (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->dtpart, (void *)c->cpart, (void *)c->cnt,
                     (void *)c->part, (void *)c->fix_list, (void *)c->fix_count, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk})
         if (q) (void)hipFree(q);
