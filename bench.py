@@ -42,6 +42,20 @@ ALG_BYTES = {
 }
 
 
+def measured_traffic(kernel: str, n1: int, n2: int, n3: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this grid (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
+    calibration), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{n1}x{n2}x{n3}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(threads: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from harm_cool_amd.params import torus_params
@@ -63,6 +77,118 @@ def cpu_baseline(threads: int) -> dict:
                       f"after 1 warm-up, {el:.1f} s"}
 
 
+def stress_oracle_check(h: dict, prims: dict, gam: float, nsample: int = 4096) -> dict:
+    """CPU leg: the FP64 oracle's Utoprim_2d on a sample of the stress batch,
+    against the GPU's plain and learned results (max relative error)."""
+    import ctypes
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import _ptr, lib
+
+    L = lib()
+    rng = np.random.default_rng(3)
+    out = {}
+    zs = rng.choice(h["gdet"].shape[0], nsample, replace=False)
+    ref = {}
+    for z in zs:
+        U = np.ascontiguousarray(h["U"][:, z])
+        gc = np.ascontiguousarray(h["gcov"][:, z])
+        gn = np.ascontiguousarray(h["gcon"][:, z])
+        p = np.ascontiguousarray(h["guess"][:, z].copy())
+        ni, kc = ctypes.c_int(), ctypes.c_int()
+        r = L.orc_utoprim_2d(_ptr(U), _ptr(gc), _ptr(gn), float(h["gdet"][z]), _ptr(p), gam, ctypes.byref(ni),
+                             ctypes.byref(kc))
+        ref[z] = (r, p)
+    for name, (ret, prim) in prims.items():
+        errs = []
+        for z in zs:
+            r, p = ref[z]
+            if r == 0 and ret[z] == 0:
+                errs.append(np.max(np.abs(prim[:5, z] - p[:5]) / np.maximum(np.abs(p[:5]), 1e-300)))
+        e = np.array(errs)
+        out[name] = {"zones_compared": int(e.size), "max_rel_err_rho_u_v": float(e.max()) if e.size else None,
+                     "median_rel_err": float(np.median(e)) if e.size else None}
+    return out
+
+
+def learned_torus(g, hc, torch, nz, steps, capture_steps, train_steps, batch):
+    """Capture pairs over capture_steps steps, train the seed network on them,
+    then time `steps` learned steps from the state the plain run started at.
+    Returns (network, report)."""
+    na = nz
+    pairs = int(1e7)
+    stride = max(1, int(2 * capture_steps * na / pairs))
+    cap = min(pairs, 2 * capture_steps * ((na + stride - 1) // stride))
+    feat = torch.zeros((8, cap), dtype=torch.float64, device="cuda:0")
+    targ = torch.zeros((2, cap), dtype=torch.float64, device="cuda:0")
+    valid = torch.zeros(cap, dtype=torch.uint8, device="cuda:0")
+    g.set_capture(feat, targ, valid, stride)
+    t0 = time.perf_counter()
+    g.step_ch(capture_steps)
+    t_cap = time.perf_counter() - t0
+    n = g.capture_count()
+    g.set_capture(None)
+    m = hc.Mlp(seed=11)
+    fd, td, vd = feat[:, :n].contiguous(), targ[:, :n].contiguous(), valid[:n].contiguous()
+    del feat, targ, valid
+    m.fit_norm(fd, td, vd)
+    t0 = time.perf_counter()
+    lt = m.train(fd, td, vd, batch=batch, steps=train_steps, lr0=2e-3, lr1=5e-5, trace=True)
+    t_train = time.perf_counter() - t0
+    del fd, td, vd
+    rep = {"pairs": n, "capture_steps": capture_steps, "capture_stride": stride, "capture_s": round(t_cap, 2),
+           "train_steps": train_steps, "batch": batch, "train_s": round(t_train, 2),
+           "loss_first": float(lt[:10].mean()), "loss_last": float(lt[-50:].mean())}
+    return m, rep
+
+
+def stress(hc, torch, n, m_torus, train_steps, batch):
+    """The inversion stress batch: plain and learned Newton starts."""
+    import numpy as np
+
+    res = {"zones": n}
+    gam = 13. / 9.
+    # a seed network for this distribution, trained on solver-generated pairs of a disjoint batch
+    ntr = min(n, 1 << 22)
+    tr = hc.stress_batch(ntr, hc.stress_spec(0, seed=(777, 888, 999, 1111)))
+    ptr = tr["guess"].clone()
+    rtr, _, _ = hc.utoprim2d_batch(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], ptr, gam)
+    feat, targ, valid = hc.mlp_pairs(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], ptr, rtr, gam)
+    del tr, ptr, rtr
+    m = hc.Mlp(seed=13)
+    m.fit_norm(feat, targ, valid)
+    lt = m.train(feat, targ, valid, batch=batch, steps=train_steps, lr0=2e-3, lr1=5e-5, trace=True)
+    del feat, targ, valid
+    res["stress_net"] = {"train_pairs": ntr, "loss_first": float(lt[:10].mean()), "loss_last": float(lt[-50:].mean())}
+    sample = {}
+    for kind, kname in ((0, "perturbed_1pct"), (1, "cold")):
+        b = hc.stress_batch(n, hc.stress_spec(kind))
+        rows = {}
+        for label, net in (("plain", None), ("learned_stress_net", m), ("learned_torus_net", m_torus)):
+            if label == "learned_torus_net" and net is None:
+                continue
+            prim = b["guess"].clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ret, it, kc = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], prim, gam, mlp=net)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            rows[label] = {"n_iter_per_zone": it.double().mean().item(), "k_conv_per_zone": kc.double().mean().item(),
+                           "fail_rate": (ret != 0).double().mean().item(), "ms": round(el * 1e3, 2)}
+            if kind == 0:
+                keep = slice(0, 1 << 20)
+                sample[label] = (ret[keep].cpu().numpy(), prim[:, keep].cpu().numpy())
+        res[kname] = rows
+        if kind == 0:
+            res["_host"] = {k: v[..., :1 << 20].cpu().numpy() for k, v in b.items()}
+        del b
+    res["_sample"] = sample
+    m.close()
+    return res
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +196,11 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=str, default="288x256x256", help="grid N1xN2xN3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed and stress sections")
+    ap.add_argument("--capture-steps", type=int, default=500, help="steps whose inversions feed the training set")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-batch", type=int, default=65536)
+    ap.add_argument("--stress-n", type=int, default=1 << 26)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -79,19 +210,56 @@ def main() -> None:
     import harm_cool_amd as hc
     from harm_cool_amd.params import torus_params
 
+    import numpy as np
+    import torch
+
     n1, n2, n3 = (int(x) for x in args.n.split("x"))
+    nz = n1 * n2 * n3
     P = torus_params(n1, n2, n3)
     g = hc.Harm(P, device=0)
     g.init()
+    learned_rep, m_torus = None, None
+    if not args.no_learned and args.capture_steps > 0:
+        # steps 0..capture_steps-1 double as the warm-up of the timed run
+        m_torus, learned_rep = learned_torus(g, hc, torch, nz, args.steps, args.capture_steps, args.train_steps,
+                                             args.train_batch)
     g.step_ch(args.warmup)
+    p_start, f_start = g.prims(), g.pflag()
+    t_start = g.time()
     g.reset_counters()
     g.time()  # synchronizes
     t0 = time.perf_counter()
     g.step_ch(args.steps)  # returns after the stream has drained
     el = time.perf_counter() - t0
     c = g.counters()
-    nz = n1 * n2 * n3
     value = nz * args.steps / el
+    if m_torus is not None:
+        p_plain = g.prims()
+        g.set_prims(p_start, f_start)
+        g.set_time(*t_start)
+        g.set_guess(1, m_torus)
+        g.reset_counters()
+        g.time()
+        t0 = time.perf_counter()
+        g.step_ch(args.steps)
+        el_l = time.perf_counter() - t0
+        cl = g.counters()
+        p_l = g.prims()
+        g.set_guess(0)
+        sl = (slice(None), slice(2, n1 + 2), slice(2, n2 + 2), slice(None))
+        dev = [float(np.abs(p_l[sl][v] - p_plain[sl][v]).max() / np.abs(p_plain[sl][v]).max()) for v in range(8)]
+        rho = p_plain[sl][0]
+        rel_rho = float((np.abs(p_l[sl][0] - rho) / rho).max())
+        del p_l, p_plain, rho
+        learned_rep.update({
+            "zone_cycles_per_s": nz * args.steps / el_l, "ms_per_step": el_l / args.steps * 1e3,
+            "n_iter_per_zone": cl["newton_iters"] / max(cl["c2p_calls"], 1),
+            "k_conv_per_zone": cl["newton_kconv"] / max(cl["c2p_calls"], 1),
+            "fail_2d": cl["fail_2d"], "fail_2d_plain": c["fail_2d"],
+            "max_field_normalized_diff_vs_plain": max(dev), "max_rel_diff_rho_vs_plain": rel_rho})
+        g.set_prims(p_start, f_start)  # leave the state as the plain run had it for the timing pass below
+        g.set_time(*t_start)
+    del p_start, f_start
 
     # per-kernel timing with HIP events on the library stream (separate short run)
     g.set_timing(True)
@@ -109,10 +277,23 @@ def main() -> None:
                 "avg_launch_ms": round(per[dom], 4)}
     kernels = {k: {"avg_ms": round(per[k], 4), "launches": int(tim[k][1])} for k in per if tim[k][1]}
     g.close()
+    if roof is not None:
+        roof["traffic"] = measured_traffic(dom, n1, n2, n3)
+
+    st = None
+    if not args.no_learned and args.stress_n > 0:
+        st = stress(hc, torch, args.stress_n, m_torus, args.train_steps, args.train_batch)
+    if m_torus is not None:
+        m_torus.close()
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(min(16, os.cpu_count() or 1))
+        if st is not None:
+            cpu["stress_vs_oracle"] = stress_oracle_check(st["_host"], st["_sample"], 13. / 9.)
+    if st is not None:
+        st.pop("_host", None)
+        st.pop("_sample", None)
 
     out = {
         "metric": "zone-cycles/s at 288x256x256; con2prim Newton iters/zone, MLP-guess vs plain",
@@ -124,6 +305,8 @@ def main() -> None:
                    "n1": n1, "n2": n2, "n3": n3},
         "newton_iters_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1),
         "newton_kconv_per_zone": c["newton_kconv"] / max(c["c2p_calls"], 1),
+        "newton_stopping_rule": "reference rule for both starts: |dW/W|+|dv2/v2| <= 1e-10, then 2 extra iterations",
+        "learned_seed_torus": learned_rep, "stress_batch": st,
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -38,6 +38,29 @@ class U2POpts(ctypes.Structure):
                 ("mlp", c_void_p), ("K", c_double)]
 
 
+class StressSpec(ctypes.Structure):
+    _fields_ = [(n, c_double) for n in ("a", "hslope", "gam", "r_lo", "r_hi", "th_lo", "th_hi", "rho_lo", "rho_hi",
+                                        "urho_lo", "urho_hi", "lorentz_lo", "lorentz_hi", "sigma_lo", "sigma_hi",
+                                        "guess_pert")] + [("seed", ctypes.c_uint32 * 4), ("guess_kind", ctypes.c_int32),
+                                                          ("pad0", ctypes.c_int32)]
+
+
+def stress_spec(guess_kind: int = 0, seed=(12345, 65435, 34221, 43543), **kw) -> StressSpec:
+    """The inversion stress workload: KS/MKS metric a=0.9375 at r log-uniform
+    [1.3, 50], theta uniform (0.01, pi-0.01); rho log-uniform [1e-7, 1], u/rho
+    log-uniform [1e-3, 1e2], Lorentz factor uniform [1, 50], sigma log-uniform
+    [1e-3, 1e2]; guess = truth with a 1% relative perturbation (kind 0) or cold."""
+    d = dict(a=0.9375, hslope=0.3, gam=13. / 9., r_lo=1.3, r_hi=50., th_lo=0.01, th_hi=np.pi - 0.01,
+             rho_lo=1e-7, rho_hi=1., urho_lo=1e-3, urho_hi=1e2, lorentz_lo=1., lorentz_hi=50., sigma_lo=1e-3,
+             sigma_hi=1e2, guess_pert=0.01)
+    d.update(kw)
+    s = StressSpec(**d)
+    for k in range(4):
+        s.seed[k] = seed[k]
+    s.guess_kind = guess_kind
+    return s
+
+
 _lib = None
 
 
@@ -91,6 +114,10 @@ def lib() -> ctypes.CDLL:
     L.harm_mlp_forward.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int]
     L.harm_mlp_seed.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
                                 c_void_p, c_void_p]
+    L.harm_mlp_pairs.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_double,
+                                 c_void_p, c_void_p, c_void_p, c_void_p]
+    L.harm_stress_batch.argtypes = [POINTER(StressSpec), c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]
     L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
     L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
     L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]
@@ -442,3 +469,34 @@ def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K:
                                       prim.data_ptr(), ret.data_ptr(), iters.data_ptr(), kconv.data_ptr(), n,
                                       ctypes.byref(opts), s), "harm_utoprim2d_batch")
     return ret, iters, kconv
+
+
+def stress_batch(n: int, spec: StressSpec, device: str = "cuda:0") -> dict:
+    """Generate the stress batch on the device (harm_stress_batch)."""
+    import torch
+
+    f = lambda *sh: torch.empty(sh, dtype=torch.float64, device=device)
+    out = {"U": f(8, n), "gcov": f(16, n), "gcon": f(16, n), "gdet": f(n), "prim_true": f(8, n), "guess": f(8, n)}
+    _sync_torch()
+    _check(lib().harm_stress_batch(ctypes.byref(spec), n, out["U"].data_ptr(), out["gcov"].data_ptr(),
+                                   out["gcon"].data_ptr(), out["gdet"].data_ptr(), out["prim_true"].data_ptr(),
+                                   out["guess"].data_ptr(), None), "harm_stress_batch")
+    torch.cuda.synchronize()
+    return out
+
+
+def mlp_pairs(U, gcov, gcon, gdet, prim, ret, gam: float):
+    """Training pairs (feat[8,n], targ[2,n], valid[n]) from solved zones."""
+    import torch
+
+    n = gdet.shape[0]
+    for t, rows, nm in ((U, 8, "U"), (gcov, 16, "gcov"), (gcon, 16, "gcon"), (prim, 8, "prim")):
+        _need(t, (rows, n), "f64", nm)
+    feat = torch.empty((8, n), dtype=torch.float64, device=U.device)
+    targ = torch.empty((2, n), dtype=torch.float64, device=U.device)
+    valid = torch.empty(n, dtype=torch.uint8, device=U.device)
+    _sync_torch()
+    _check(lib().harm_mlp_pairs(U.data_ptr(), gcov.data_ptr(), gcon.data_ptr(), gdet.data_ptr(), prim.data_ptr(),
+                                ret.data_ptr(), n, gam, feat.data_ptr(), targ.data_ptr(), valid.data_ptr(), None),
+           "harm_mlp_pairs")
+    return feat, targ, valid

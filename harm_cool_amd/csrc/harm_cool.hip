@@ -183,10 +183,11 @@ __global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict_
     double dtij = 1.e300;
     int fatal = 0;
     if (idx < total) {
-        const int h = (int)(idx % g.N3);
-        const size_t rest = idx / g.N3;
-        const int j = j0 + (int)(rest % nj);
-        const int i = i0 + (int)(rest / nj);
+        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+        const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+        const unsigned ri = rest / (unsigned)nj;
+        const int j = j0 + (int)(rest - ri * (unsigned)nj);
+        const int i = i0 + (int)ri;
         const size_t z = zidx(g, i, j, h);
         const size_t nz = g.nz;
         if (DIR == 2 && (j == g.js || j == g.je + 1)) {
@@ -260,10 +261,11 @@ __global__ void __launch_bounds__(TPB) k_emf(DGrid g, const double *__restrict__
     const int ni = g.ie + 1 - g.is + 1, nj = g.je + 1 - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx >= (size_t)ni * nj * g.N3) return;
-    const int h = (int)(idx % g.N3);
-    const size_t rest = idx / g.N3;
-    const int j = g.js + (int)(rest % nj);
-    const int i = g.is + (int)(rest / nj);
+    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+    const unsigned ri = rest / (unsigned)nj;
+    const int j = g.js + (int)(rest - ri * (unsigned)nj);
+    const int i = g.is + (int)ri;
     const size_t nz = g.nz;
     const size_t z = zidx(g, i, j, h);
     const size_t zjm = zidx(g, i, j - 1, h), zim = zidx(g, i - 1, j, h);
@@ -331,10 +333,11 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__
     const size_t na = (size_t)ni * nj * g.N3;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx >= na) return;
-    const int h = (int)(idx % g.N3);
-    const size_t rest = idx / g.N3;
-    const int j = g.js + (int)(rest % nj);
-    const int i = g.is + (int)(rest / nj);
+    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+    const unsigned ri = rest / (unsigned)nj;
+    const int j = g.js + (int)(rest - ri * (unsigned)nj);
+    const int i = g.is + (int)ri;
     const size_t z = zidx(g, i, j, h);
     const size_t nz = g.nz;
     const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
@@ -418,10 +421,11 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
     const double *blk = nullptr;
     double piz[NPR], U[NPR];
     if (active) {
-        h = (int)(idx % g.N3);
-        const size_t rest = idx / g.N3;
-        j = g.js + (int)(rest % nj);
-        i = g.is + (int)(rest / nj);
+        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+        h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+        const unsigned ri = rest / (unsigned)nj;
+        j = g.js + (int)(rest - ri * (unsigned)nj);
+        i = g.is + (int)ri;
         z = zidx(g, i, j, h);
         const size_t nz = g.nz;
         col = uni_col<UNI>((size_t)i * g.SN2 + j);
@@ -591,10 +595,11 @@ __global__ void __launch_bounds__(TPB) k_fixup(DGrid g, double *__restrict__ p, 
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx >= (size_t)ni * nj * g.N3) return;
-    const int h = (int)(idx % g.N3);
-    const size_t rest = idx / g.N3;
-    const int j = g.js + (int)(rest % nj);
-    const int i = g.is + (int)(rest / nj);
+    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+    const unsigned ri = rest / (unsigned)nj;
+    const int j = g.js + (int)(rest - ri * (unsigned)nj);
+    const int i = g.is + (int)ri;
     const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
     Geom geom;
     load_geom(g.geom[CENT], g.ncol, col, geom);
@@ -762,9 +767,10 @@ __global__ void __launch_bounds__(TPB) k_fixutoprim_scan(DGrid g, double *__rest
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
     if (idx < g.nz) {
-        const int h = (int)(idx % g.N3);
-        const size_t col = idx / g.N3;
-        const int j = (int)(col % g.SN2), i = (int)(col / g.SN2);
+        const unsigned col = (unsigned)idx / (unsigned)g.N3;
+        const int h = (int)((unsigned)idx - col * (unsigned)g.N3);
+        const unsigned ci = col / (unsigned)g.SN2;
+        const int j = (int)(col - ci * (unsigned)g.SN2), i = (int)ci;
         const int8_t f = fin[idx];
         fout[idx] = (int8_t)(!f);
         if (f != 0 && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
@@ -933,10 +939,11 @@ __global__ void __launch_bounds__(TPB) k_init_expand(DGrid g, double *__restrict
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx >= (size_t)ni * nj * g.N3) return;
-    const int h = (int)(idx % g.N3);
-    const size_t rest = idx / g.N3;
-    const int j = g.js + (int)(rest % nj);
-    const int i = g.is + (int)(rest / nj);
+    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+    const unsigned ri = rest / (unsigned)nj;
+    const int j = g.js + (int)(rest - ri * (unsigned)nj);
+    const int i = g.is + (int)ri;
     const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j, nc = g.ncol;
     double rho = tor[col], u = tor[nc + col];
     double u1 = 0., u2 = 0., u3 = 0.;
@@ -966,10 +973,11 @@ __global__ void __launch_bounds__(TPB) k_init_A(DGrid g, const double *__restric
     const int ni = g.ie + 1 - g.is + 1, nj = g.je + 1 - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx >= (size_t)ni * nj * g.N3) return;
-    const int h = (int)(idx % g.N3);
-    const size_t rest = idx / g.N3;
-    const int j = g.js + (int)(rest % nj);
-    const int i = g.is + (int)(rest / nj);
+    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+    const unsigned ri = rest / (unsigned)nj;
+    const int j = g.js + (int)(rest - ri * (unsigned)nj);
+    const int i = g.is + (int)ri;
     const double *r = p + RHO * g.nz;
     const double rho_av = 0.25 * (r[zidx(g, i, j, h)] + r[zidx(g, i - 1, j, h)] + r[zidx(g, i, j - 1, h)] +
                                   r[zidx(g, i - 1, j - 1, h)]);
@@ -986,10 +994,11 @@ __global__ void __launch_bounds__(TPB) k_init_B(DGrid g, double *__restrict__ p,
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     double bsq = 0.;
     if (idx < (size_t)ni * nj * g.N3) {
-        const int h = (int)(idx % g.N3);
-        const size_t rest = idx / g.N3;
-        const int j = g.js + (int)(rest % nj);
-        const int i = g.is + (int)(rest / nj);
+        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+        const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+        const unsigned ri = rest / (unsigned)nj;
+        const int j = g.js + (int)(rest - ri * (unsigned)nj);
+        const int i = g.is + (int)ri;
         const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
         Geom geom;
         load_geom(g.geom[CENT], g.ncol, col, geom);
@@ -1028,10 +1037,11 @@ __global__ void __launch_bounds__(TPB) k_diag(DGrid g, const double *__restrict_
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     double mass = 0., e = 0., divb = 0.;
     if (idx < (size_t)ni * nj * g.N3) {
-        const int h = (int)(idx % g.N3);
-        const size_t rest = idx / g.N3;
-        const int j = g.js + (int)(rest % nj);
-        const int i = g.is + (int)(rest / nj);
+        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+        const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+        const unsigned ri = rest / (unsigned)nj;
+        const int j = g.js + (int)(rest - ri * (unsigned)nj);
+        const int i = g.is + (int)ri;
         const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
         Geom geom;
         load_geom(g.geom[CENT], g.ncol, col, geom);
@@ -1153,6 +1163,165 @@ __global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U,
     if (ret_full) ret_full[z] = r;
     if (iters) iters[z] = (uint8_t)niter;
     if (kconv) kconv[z] = (uint8_t)kc;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic stress batch for the inversion alone (no reference counterpart):
+// zones with a Kerr-Schild metric in MKS code coordinates at random (r, theta)
+// and random primitive states, and the conserved vector U = primtoU(P)
+// (phys.c:100-105).  Each zone draws from its own KISS stream (random.c:30-43)
+// seeded {s0+z, s1+z, s2+z, s3+z}.
+// ---------------------------------------------------------------------------
+struct KissStream {
+    uint32_t z, w, jsr, jcong;
+    __device__ double next()
+    {
+        z = 36969u * (z & 65535u) + (z >> 16);
+        w = 18000u * (w & 65535u) + (w >> 16);
+        const uint32_t mwc = (z << 16) + w;
+        jcong = 69069u * jcong + 1234567u;
+        jsr ^= (jsr << 17);
+        jsr ^= (jsr >> 13);
+        jsr ^= (jsr << 5);
+        return (double)((mwc ^ jcong) + jsr) / 4294967295.0;
+    }
+};
+
+struct StressSpec {
+    double a, hslope, gam;
+    double r_lo, r_hi, th_lo, th_hi;      // r log-uniform, theta uniform
+    double rho_lo, rho_hi;                // log-uniform
+    double urho_lo, urho_hi;              // u/rho log-uniform
+    double lor_lo, lor_hi;                // Lorentz factor uniform
+    double sig_lo, sig_hi;                // sigma = b^2/rho log-uniform
+    double pert;                          // relative perturbation of the guess
+    uint32_t seed[4];
+    int guess_kind;                       // 0: perturbed truth, 1: cold (rho = D, u = rho, v = 0)
+};
+
+// Kerr-Schild metric (coord.c:128-199 before the Jacobian) at (r, th) in MKS
+// coordinates with R0 = 0: Jacobian diag(1, r, dth/dX2, 1) taken analytically,
+// X2 found from th by Newton on th(X2) = pi/2 (1+X2) + (1-h)/2 sin(pi (1+X2)).
+__device__ void ks_mks_geom(double a, double h, double r, double th, Geom &g)
+{
+    double X2 = 2. * th / M_PI - 1.;
+    for (int it = 0; it < 40; it++) {
+        const double f = 0.5 * M_PI * (1. + X2) + 0.5 * (1. - h) * sin(M_PI * (1. + X2)) - th;
+        const double df = 0.5 * M_PI * (1. + (1. - h) * cos(M_PI * (1. + X2)));
+        X2 -= f / df;
+    }
+    const double dth = 0.5 * M_PI * (1. + (1. - h) * cos(M_PI * (1. + X2)));
+    const double sth = sin(th), cth = cos(th), s2 = sth * sth, rho2 = r * r + a * a * cth * cth;
+    double gc[4][4] = {}, gn[4][4] = {};
+    gc[0][0] = -1. + 2. * r / rho2;
+    gc[0][1] = gc[1][0] = 2. * r / rho2;
+    gc[0][3] = gc[3][0] = -2. * a * r * s2 / rho2;
+    gc[1][1] = 1. + 2. * r / rho2;
+    gc[1][3] = gc[3][1] = -a * s2 * (1. + 2. * r / rho2);
+    gc[2][2] = rho2;
+    gc[3][3] = s2 * (rho2 + a * a * s2 * (1. + 2. * r / rho2));
+    gn[0][0] = -(1. + 2. * r / rho2);
+    gn[0][1] = gn[1][0] = 2. * r / rho2;
+    gn[1][1] = (r * r - 2. * r + a * a) / rho2;
+    gn[1][3] = gn[3][1] = a / rho2;
+    gn[2][2] = 1. / rho2;
+    gn[3][3] = 1. / (rho2 * s2);
+    const double J[4] = {1., r, dth, 1.};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            g.gcov[i][j] = gc[i][j] * J[i] * J[j];
+            g.gcon[i][j] = gn[i][j] / (J[i] * J[j]);
+        }
+    g.g = rho2 * fabs(sth) * r * dth;
+}
+
+// x = s L^{-T} n for the Cholesky factor L of the spatial metric gamma_ij =
+// gcov[1..3][1..3]: gamma(x, x) = s^2 for a Euclidean unit vector n.
+__device__ void spatial_scale(const Geom &g, const double *n, double s, double *x)
+{
+    const double l00 = sqrt(g.gcov[1][1]);
+    const double l10 = g.gcov[2][1] / l00, l20 = g.gcov[3][1] / l00;
+    const double l11 = sqrt(g.gcov[2][2] - l10 * l10);
+    const double l21 = (g.gcov[3][2] - l20 * l10) / l11;
+    const double l22 = sqrt(g.gcov[3][3] - l20 * l20 - l21 * l21);
+    x[2] = n[2] / l22;
+    x[1] = (n[1] - l21 * x[2]) / l11;
+    x[0] = (n[0] - l10 * x[1] - l20 * x[2]) / l00;
+    for (int k = 0; k < 3; k++) x[k] *= s;
+}
+
+__device__ void unit3(KissStream &ks, double *n)
+{
+    const double c = 2. * ks.next() - 1., ph = 2. * M_PI * ks.next();
+    const double sn = sqrt(fmax(0., 1. - c * c));
+    n[0] = sn * cos(ph);
+    n[1] = sn * sin(ph);
+    n[2] = c;
+}
+
+__global__ void __launch_bounds__(TPB) k_stress_gen(StressSpec sp, size_t n, double *__restrict__ U,
+                                                    double *__restrict__ gcov, double *__restrict__ gcon,
+                                                    double *__restrict__ gdet, double *__restrict__ ptrue,
+                                                    double *__restrict__ guess)
+{
+    const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (z >= n) return;
+    const uint32_t off = (uint32_t)z;
+    KissStream ks = {sp.seed[0] + off, sp.seed[1] + off, sp.seed[2] + off, sp.seed[3] + off};
+    auto logu = [&](double lo, double hi) { return exp(log(lo) + (log(hi) - log(lo)) * ks.next()); };
+    const double r = logu(sp.r_lo, sp.r_hi);
+    const double th = sp.th_lo + (sp.th_hi - sp.th_lo) * ks.next();
+    Geom geom;
+    ks_mks_geom(sp.a, sp.hslope, r, th, geom);
+    double pr[NPR];
+    pr[RHO] = logu(sp.rho_lo, sp.rho_hi);
+    pr[UU] = pr[RHO] * logu(sp.urho_lo, sp.urho_hi);
+    const double lor = sp.lor_lo + (sp.lor_hi - sp.lor_lo) * ks.next();
+    double nv[3], ut[3], nb[3], bh[3];
+    unit3(ks, nv);
+    spatial_scale(geom, nv, sqrt(lor * lor - 1.), ut);
+    for (int k = 0; k < 3; k++) pr[U1 + k] = ut[k];
+    unit3(ks, nb);
+    spatial_scale(geom, nb, 1., bh);
+    for (int k = 0; k < 3; k++) pr[B1 + k] = bh[k];
+    {
+        // scale B so that b^2 = sigma rho (b^mu is linear in B)
+        State q;
+        int fatal = 0;
+        get_state(pr, geom, q, fatal);
+        const double bsq1 = dot4(q.bcon, q.bcov);
+        const double sig = logu(sp.sig_lo, sp.sig_hi);
+        const double sc = sqrt(sig * pr[RHO] / bsq1);
+        for (int k = 0; k < 3; k++) pr[B1 + k] *= sc;
+    }
+    double Uz[NPR];
+    {
+        State q;
+        int fatal = 0;
+        get_state(pr, geom, q, fatal);
+        primtoflux(pr, q, 0, geom, sp.gam, Uz);
+    }
+    double gs[NPR];
+    if (sp.guess_kind == 0) {
+        for (int k = 0; k < NPR; k++) gs[k] = pr[k];
+        for (int k = RHO; k <= U3; k++) gs[k] = pr[k] * (1. + sp.pert * (2. * ks.next() - 1.));
+    } else {
+        const double alpha = 1. / sqrt(-geom.gcon[0][0]);
+        gs[RHO] = alpha * Uz[RHO] / geom.g;
+        gs[UU] = gs[RHO];
+        gs[U1] = gs[U2] = gs[U3] = 0.;
+        for (int k = B1; k <= B3; k++) gs[k] = pr[k];
+    }
+    for (int k = 0; k < NPR; k++) {
+        U[(size_t)k * n + z] = Uz[k];
+        ptrue[(size_t)k * n + z] = pr[k];
+        guess[(size_t)k * n + z] = gs[k];
+    }
+    for (int a = 0; a < 16; a++) {
+        gcov[(size_t)a * n + z] = geom.gcov[a / 4][a % 4];
+        gcon[(size_t)a * n + z] = geom.gcon[a / 4][a % 4];
+    }
+    gdet[z] = geom.g;
 }
 
 __global__ void __launch_bounds__(TPB) k_pow(const double *__restrict__ x, const double *__restrict__ y,
@@ -1604,6 +1773,10 @@ harm_ctx *harm_create(const harm_params *p, int device)
         set_err("unsupported grid: need n1 >= 4, n2 >= 8, n3 == 1 or even n3 >= 4 (mpi.c:376)");
         return nullptr;
     }
+    if ((double)(p->n1 + 4) * (p->n2 + 4) * p->n3 >= 2147483648.0) {
+        set_err("grid too large: (n1+4)*(n2+4)*n3 must stay below 2^31 zones per device (32-bit zone indices)");
+        return nullptr;
+    }
     if (p->lim < 0 || p->lim > 2) {
         set_err("unknown slope limiter");
         return nullptr;
@@ -1950,6 +2123,41 @@ int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon
     } else {
         return set_err("unknown guess_mode", 4);
     }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int harm_stress_batch(const harm_stress_spec *spec, size_t n, double *U, double *gcov, double *gcon, double *gdet,
+                      double *prim_true, double *guess, void *stream)
+{
+    if (!spec || !U || !gcov || !gcon || !gdet || !prim_true || !guess) return set_err("null argument");
+    if (n == 0) return 0;
+    if (n > ((size_t)1 << 32)) return set_err("stress batch too large (n <= 2^32)", 4);
+    StressSpec sp;
+    sp.a = spec->a;
+    sp.hslope = spec->hslope;
+    sp.gam = spec->gam;
+    sp.r_lo = spec->r_lo;
+    sp.r_hi = spec->r_hi;
+    sp.th_lo = spec->th_lo;
+    sp.th_hi = spec->th_hi;
+    sp.rho_lo = spec->rho_lo;
+    sp.rho_hi = spec->rho_hi;
+    sp.urho_lo = spec->urho_lo;
+    sp.urho_hi = spec->urho_hi;
+    sp.lor_lo = spec->lorentz_lo;
+    sp.lor_hi = spec->lorentz_hi;
+    sp.sig_lo = spec->sigma_lo;
+    sp.sig_hi = spec->sigma_hi;
+    sp.pert = spec->guess_pert;
+    for (int k = 0; k < 4; k++) sp.seed[k] = spec->seed[k];
+    sp.guess_kind = spec->guess_kind;
+    if (!(sp.r_lo > 0.) || !(sp.r_hi > sp.r_lo) || !(sp.th_lo > 0.) ||
+        !(sp.th_hi < M_PI) || !(sp.rho_lo > 0.) || !(sp.urho_lo > 0.) || !(sp.lor_lo >= 1.) ||
+        !(sp.sig_lo > 0.) || sp.guess_kind < 0 || sp.guess_kind > 1)
+        return set_err("bad stress-batch ranges (r > 0, theta in (0, pi), positive scales)", 4);
+    hipLaunchKernelGGL(k_stress_gen, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, sp, n, U, gcov, gcon, gdet,
+                       prim_true, guess);
     HIPCHK(hipGetLastError());
     return 0;
 }

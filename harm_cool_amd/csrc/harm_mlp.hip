@@ -347,6 +347,43 @@ __global__ void __launch_bounds__(256) k_seed(MlpDev m, const double *__restrict
     }
 }
 
+// Training pairs from solved zones: features of U, targets from the converged
+// primitives (W = (rho + gam u) (1 + utsq), v^2 = utsq / (1 + utsq)).
+__global__ void __launch_bounds__(256) k_pairs(const double *__restrict__ U, const double *__restrict__ gcov,
+                                               const double *__restrict__ gcon, const double *__restrict__ gdet,
+                                               const double *__restrict__ prim, const int8_t *__restrict__ ret,
+                                               size_t n, double gam, double *__restrict__ feat,
+                                               double *__restrict__ targ, uint8_t *__restrict__ valid)
+{
+    const size_t z = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (z >= n) return;
+    Geom geom;
+#pragma unroll
+    for (int a = 0; a < 16; a++) {
+        geom.gcov[a / 4][a % 4] = gcov[(size_t)a * n + z];
+        geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
+    }
+    geom.g = gdet[z];
+    double Uz[NPR], pr[NPR], f[MLP_NIN];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        Uz[k] = U[(size_t)k * n + z];
+        pr[k] = prim[(size_t)k * n + z];
+    }
+    const bool ok = mlp_features_U(Uz, geom, f) && ret[z] == 0;
+    double t0 = 0., t1 = 0.;
+    if (ok) {
+        const double utsq = utsq_of(geom, pr);
+        t0 = log((pr[RHO] + gam * pr[UU]) * (1. + utsq)) - f[0];
+        t1 = utsq / (1. + utsq);
+    }
+#pragma unroll
+    for (int k = 0; k < MLP_NIN; k++) feat[(size_t)k * n + z] = f[k];
+    targ[z] = t0;
+    targ[n + z] = t1;
+    valid[z] = ok ? 1 : 0;
+}
+
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 template <class T>
@@ -786,6 +823,20 @@ int harm_mlp_seed(harm_mlp *m, const double *U, const double *gcov, const double
     hipStream_t st = stream ? (hipStream_t)stream : m->stream;
     hipLaunchKernelGGL(k_seed, dim3(cdiv(n, 256)), dim3(256), 0, st, harm_mlp_view(m), U, gcov, gcon, gdet, n, feat,
                        W0, vsq0);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int harm_mlp_pairs(const double *U, const double *gcov, const double *gcon, const double *gdet, const double *prim,
+                   const int8_t *ret, size_t n, double gam, double *feat, double *targ, uint8_t *valid, void *stream)
+{
+    if (!U || !gcov || !gcon || !gdet || !prim || !ret || !feat || !targ || !valid)
+        return harm_host::set_error("null argument", 1);
+    if (n == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_pairs, dim3(cdiv(n, 256)), dim3(256), 0, st, U, gcov, gcon, gdet, prim, ret, n, gam, feat,
+                       targ, valid);
     MCHK(hipGetLastError());
     MCHK(hipStreamSynchronize(st));
     return 0;

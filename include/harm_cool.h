@@ -193,6 +193,12 @@ int harm_mlp_forward(harm_mlp *m, const float *X, float *Y, int n, int bf16);
 int harm_mlp_seed(harm_mlp *m, const double *U, const double *gcov, const double *gcon, const double *gdet,
                   size_t n, double *feat, double *W0, double *vsq0, void *stream);
 
+/* Training pairs from solved zones (device, SoA as harm_utoprim2d_batch):
+ * feat[8][n] raw features of U, targ[2][n] = (log(W/D), v^2) of the converged
+ * primitives prim, valid[n] = (ret[n] == 0). */
+int harm_mlp_pairs(const double *U, const double *gcov, const double *gcon, const double *gdet, const double *prim,
+                   const int8_t *ret, size_t n, double gam, double *feat, double *targ, uint8_t *valid, void *stream);
+
 /* Step integration of the learned seed.  mode 0: previous-step guess (the
  * reference's Utoprim_2d); mode 1: learned seed with previous-step fallback.
  * The handle must outlive its use by the context. */
@@ -203,6 +209,28 @@ int harm_set_guess(harm_ctx *c, int mode, const harm_mlp *m);
  * targ[2][cap], valid[cap] until cap rows are filled.  feat == NULL stops. */
 int harm_set_capture(harm_ctx *c, double *feat, double *targ, uint8_t *valid, size_t cap, int stride);
 int harm_get_capture_count(harm_ctx *c, size_t *n);
+
+/* Synthetic stress batch for the inversion alone (no reference counterpart):
+ * n zones with the Kerr-Schild metric of spin a in MKS coordinates (R0 = 0,
+ * hslope) at r log-uniform in [r_lo, r_hi] and theta uniform in [th_lo, th_hi];
+ * rho log-uniform, u/rho log-uniform, Lorentz factor uniform, isotropic velocity
+ * and field directions, |B| set so that b^2/rho is log-uniform in
+ * [sigma_lo, sigma_hi]; U = primtoU(P) (phys.c:100-105).  Zone z draws from a
+ * KISS stream (random.c:30-43) seeded seed[k] + z.  guess_kind 0: the true P
+ * with rho, u, U1..U3 each scaled by 1 + guess_pert*(2x-1); 1: cold (rho = D,
+ * u = rho, zero velocity).  Device outputs in the harm_utoprim2d_batch layout:
+ * U[8][n], gcov[16][n], gcon[16][n], gdet[n], prim_true[8][n], guess[8][n]. */
+typedef struct harm_stress_spec {
+    double a, hslope, gam;
+    double r_lo, r_hi, th_lo, th_hi;
+    double rho_lo, rho_hi, urho_lo, urho_hi;
+    double lorentz_lo, lorentz_hi, sigma_lo, sigma_hi;
+    double guess_pert;
+    uint32_t seed[4];
+    int32_t guess_kind, pad0;
+} harm_stress_spec;
+int harm_stress_batch(const harm_stress_spec *spec, size_t n, double *U, double *gcov, double *gcon, double *gdet,
+                      double *prim_true, double *guess, void *stream);
 
 /* Exact reference signature (utoprim_2d.c:126, decs.h:608-609), evaluated on the
  * GPU for one zone; gam is taken from harm_set_gamma() (default 4/3, init.c:91). */
