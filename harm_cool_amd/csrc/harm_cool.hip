@@ -65,6 +65,35 @@ __device__ __forceinline__ size_t zidx(const DGrid &g, int i, int j, int h)
     return ((size_t)i * g.SN2 + j) * g.N3 + h;
 }
 
+// Logical block of this workgroup.  Consecutive hardware workgroups are
+// dispatched to the 8 XCDs in turn, and each XCD has its own L2; with this
+// mapping each XCD walks one contiguous range of logical blocks, so the stencil
+// neighbours that adjacent logical blocks read come from the same L2.  Grids
+// using it are launched with a multiple of 8 workgroups.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    return (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+}
+
+// Zone (i, j, h) of logical index idx over an ni x nj x N3 box at (i0, j0).
+// I_FAST orders the columns with i fastest (for stencils along X1), else j.
+template <bool I_FAST>
+__device__ __forceinline__ void box_zone(unsigned idx, unsigned n3, int i0, int j0, unsigned ni, unsigned nj, int &i,
+                                         int &j, int &h)
+{
+    const unsigned rest = idx / n3;
+    h = (int)(idx - rest * n3);
+    if (I_FAST) {
+        const unsigned rj = rest / ni;
+        i = i0 + (int)(rest - rj * ni);
+        j = j0 + (int)rj;
+    } else {
+        const unsigned ri = rest / nj;
+        j = j0 + (int)(rest - ri * nj);
+        i = i0 + (int)ri;
+    }
+}
+
 // ---- wave-level reductions (64 lanes) ----
 __device__ __forceinline__ int wave_sum_int(int v)
 {
@@ -169,25 +198,26 @@ __global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__
 // DIR 1,2,3 = X1,X2,X3.  Slopes of the two zones adjacent to the face are formed
 // on the fly from four zones along DIR, as the reference's dq array holds them.
 // ---------------------------------------------------------------------------
+// Columns are walked with the stencil direction fastest (X1: i, else j) and
+// XCD-contiguous logical blocks, so the four zones each face reads are mostly
+// L2 hits; 4 waves per SIMD (128 VGPRs) measured faster than 3 with a few
+// spilled registers.
 template <int DIR, bool UNI>
-__global__ void __launch_bounds__(TPB) k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F,
-                                              double *__restrict__ dtpart, int fullstep,
-                                              unsigned long long *__restrict__ cnt)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart, int fullstep,
+       unsigned long long *__restrict__ cnt)
 {
     const int i0 = (DIR == 1) ? g.is : g.is - 1;
     const int j0 = (DIR == 2) ? g.js : g.js - 1;
     const int ni = g.ie + 1 - i0 + 1;
     const int nj = g.je + 1 - j0 + 1;
-    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    const size_t total = (size_t)ni * nj * g.N3;
+    const unsigned idx = xcd_block() * TPB + threadIdx.x;
+    const unsigned total = (unsigned)ni * nj * g.N3;
     double dtij = 1.e300;
     int fatal = 0;
     if (idx < total) {
-        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
-        const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
-        const unsigned ri = rest / (unsigned)nj;
-        const int j = j0 + (int)(rest - ri * (unsigned)nj);
-        const int i = i0 + (int)ri;
+        int i, j, h;
+        box_zone<DIR == 1>(idx, g.N3, i0, j0, ni, nj, i, j, h);
         const size_t z = zidx(g, i, j, h);
         const size_t nz = g.nz;
         if (DIR == 2 && (j == g.js || j == g.je + 1)) {
@@ -259,13 +289,10 @@ __global__ void __launch_bounds__(TPB) k_emf(DGrid g, const double *__restrict__
                                              double *__restrict__ e2, double *__restrict__ e3)
 {
     const int ni = g.ie + 1 - g.is + 1, nj = g.je + 1 - g.js + 1;
-    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= (size_t)ni * nj * g.N3) return;
-    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
-    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
-    const unsigned ri = rest / (unsigned)nj;
-    const int j = g.js + (int)(rest - ri * (unsigned)nj);
-    const int i = g.is + (int)ri;
+    const unsigned idx = xcd_block() * TPB + threadIdx.x;
+    if (idx >= (unsigned)ni * nj * g.N3) return;
+    int i, j, h;
+    box_zone<false>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
     const size_t nz = g.nz;
     const size_t z = zidx(g, i, j, h);
     const size_t zjm = zidx(g, i, j - 1, h), zim = zidx(g, i - 1, j, h);
@@ -331,13 +358,10 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
-    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    const unsigned idx = xcd_block() * TPB + threadIdx.x;
     if (idx >= na) return;
-    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
-    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
-    const unsigned ri = rest / (unsigned)nj;
-    const int j = g.js + (int)(rest - ri * (unsigned)nj);
-    const int i = g.is + (int)ri;
+    int i, j, h;
+    box_zone<false>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
     const size_t z = zidx(g, i, j, h);
     const size_t nz = g.nz;
     const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
@@ -1338,6 +1362,8 @@ __global__ void __launch_bounds__(TPB) k_copy(const double2 *__restrict__ a, dou
 }
 
 inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+// Grid for kernels that use xcd_block(): a multiple of 8 workgroups.
+inline unsigned nblocks8(size_t n) { return (nblocks(n) + 7u) & ~7u; }
 
 }  // namespace
 
@@ -1440,7 +1466,7 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
     const int i0 = (dir == 1) ? g.is : g.is - 1;
     const int j0 = (dir == 2) ? g.js : g.js - 1;
     const size_t total = (size_t)(g.ie + 2 - i0) * (g.je + 2 - j0) * g.N3;
-    const dim3 grid(nblocks(total)), blk(TPB);
+    const dim3 grid(nblocks8(total)), blk(TPB);
     if (c->uniform) {
         if (dir == 1)
             LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, true>), grid, blk, 0, c->stream, g, pb, F,
@@ -1478,7 +1504,7 @@ int fluxes_and_emf(harm_ctx *c, const double *pb, int fullstep)
     if ((rc = flux_launch(c, 2, pb, c->F2, fullstep))) return rc;
     if (g.N3 > 2 && (rc = flux_launch(c, 3, pb, c->F3, fullstep))) return rc;
     const size_t ne = (size_t)(g.ie + 2 - g.is) * (g.je + 2 - g.js) * g.N3;
-    LAUNCH(HARM_K_EMF, hipLaunchKernelGGL(k_emf, dim3(nblocks(ne)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
+    LAUNCH(HARM_K_EMF, hipLaunchKernelGGL(k_emf, dim3(nblocks8(ne)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
                                           c->e1, c->e2, c->e3));
     return 0;
 }
@@ -1536,7 +1562,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
 #define UPD_LAUNCH(U_, M_)                                                                                          \
     LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi, \
                                              pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cpart, ex))
-    LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
+    LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
                                           c->e1, c->e2, c->e3, c->S));
     if (c->uniform) {
         if (mode == UPD_PLAIN)
@@ -1660,7 +1686,7 @@ int ctx_alloc(harm_ctx *c)
     }
     if ((rc = dalloc(&c->tstate, 8))) return rc;
     if ((rc = dalloc(&c->ndt_slot, 3))) return rc;
-    if ((rc = dalloc(&c->dtpart, (size_t)nblocks(c->nz)))) return rc;
+    if ((rc = dalloc(&c->dtpart, (size_t)nblocks8(c->nz)))) return rc;
     if ((rc = dalloc(&c->cpart, 8 * (size_t)nblocks(c->nz)))) return rc;
     if ((rc = dalloc(&c->cnt, HARM_NCOUNTERS))) return rc;
     c->npart = std::max<size_t>(3 * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
