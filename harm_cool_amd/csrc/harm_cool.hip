@@ -159,14 +159,18 @@ struct CntMap {
     int idx[8];
 };
 
-// cnt[map.idx[k]] += sum over blocks b of part[b*map.n + k]; block k per counter.
+// cnt[map.idx[k]] += sum over blocks b of part[b*map.n + k].  Grid
+// (RED_BLOCKS, map.n): block (x, k) folds every RED_BLOCKS-th partial of
+// counter k and adds its total with one atomic.
+constexpr int RED_BLOCKS = 64;
 __global__ void __launch_bounds__(256) k_sum_partials(const unsigned long long *__restrict__ part, size_t nb,
                                                       CntMap map, unsigned long long *__restrict__ cnt)
 {
     __shared__ unsigned long long red[256];
-    const int k = blockIdx.x;
+    const int k = blockIdx.y;
     unsigned long long s = 0;
-    for (size_t b = threadIdx.x; b < nb; b += 256) s += part[b * map.n + k];
+    for (size_t b = (size_t)blockIdx.x * 256 + threadIdx.x; b < nb; b += (size_t)RED_BLOCKS * 256)
+        s += part[b * map.n + k];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -177,13 +181,14 @@ __global__ void __launch_bounds__(256) k_sum_partials(const unsigned long long *
 }
 
 // slot = min(slot, min over blocks of part[b]) where that minimum is < 1e9
-// (the CFL limit of a direction, step_ch.c:477-484); one block.
+// (the CFL limit of a direction, step_ch.c:477-484); grid RED_BLOCKS.
 __global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__ part, size_t nb,
                                                       unsigned long long *__restrict__ slot)
 {
     __shared__ double red[256];
     double m = 1.e300;
-    for (size_t b = threadIdx.x; b < nb; b += 256) m = (part[b] < m) ? part[b] : m;
+    for (size_t b = (size_t)blockIdx.x * 256 + threadIdx.x; b < nb; b += (size_t)RED_BLOCKS * 256)
+        m = (part[b] < m) ? part[b] : m;
     red[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -788,25 +793,36 @@ __global__ void __launch_bounds__(TPB) k_fixutoprim_scan(DGrid g, double *__rest
                                                          int *__restrict__ list, int *__restrict__ count, int cap,
                                                          unsigned long long *__restrict__ cnt)
 {
-    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    // 8 zones per thread: the common all-good octet is one 8-byte load and store.
+    const size_t z0 = ((size_t)blockIdx.x * TPB + threadIdx.x) * 8;
     int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
-    if (idx < g.nz) {
-        const unsigned col = (unsigned)idx / (unsigned)g.N3;
-        const int h = (int)((unsigned)idx - col * (unsigned)g.N3);
-        const unsigned ci = col / (unsigned)g.SN2;
-        const int j = (int)(col - ci * (unsigned)g.SN2), i = (int)ci;
-        const int8_t f = fin[idx];
-        fout[idx] = (int8_t)(!f);
-        if (f != 0 && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
-            bool isolated = true;
-            for (int di = -1; di <= 1; di++)
-                for (int dj = -1; dj <= 1; dj++)
-                    if ((di || dj) && fin[zidx(g, i + di, j + dj, h)] != 0) isolated = false;
-            if (isolated) {
-                repair_zone(g, p, fout, i, j, h, false, fin, c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
-            } else {
-                const int slot = atomicAdd(count, 1);
-                if (slot < cap) list[slot] = (int)idx;
+    if (z0 < g.nz) {
+        const bool full = z0 + 8 <= g.nz;
+        const unsigned long long f8 = full ? *reinterpret_cast<const unsigned long long *>(fin + z0) : 1ull;
+        if (full && f8 == 0ull) {
+            *reinterpret_cast<unsigned long long *>(fout + z0) = 0x0101010101010101ull;
+        } else {
+            for (int q = 0; q < 8; q++) {
+                const size_t idx = z0 + q;
+                if (idx >= g.nz) break;
+                const unsigned col = (unsigned)idx / (unsigned)g.N3;
+                const int h = (int)((unsigned)idx - col * (unsigned)g.N3);
+                const unsigned ci = col / (unsigned)g.SN2;
+                const int j = (int)(col - ci * (unsigned)g.SN2), i = (int)ci;
+                const int8_t f = fin[idx];
+                fout[idx] = (int8_t)(!f);
+                if (f != 0 && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+                    bool isolated = true;
+                    for (int di = -1; di <= 1; di++)
+                        for (int dj = -1; dj <= 1; dj++)
+                            if ((di || dj) && fin[zidx(g, i + di, j + dj, h)] != 0) isolated = false;
+                    if (isolated) {
+                        repair_zone(g, p, fout, i, j, h, false, fin, c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
+                    } else {
+                        const int slot = atomicAdd(count, 1);
+                        if (slot < cap) list[slot] = (int)idx;
+                    }
+                }
             }
         }
     }
@@ -1489,7 +1505,7 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
                                                     c->dtpart, fullstep, c->cnt));
     }
     if (fullstep) {
-        hipLaunchKernelGGL(k_min_partials, dim3(1), dim3(256), 0, c->stream, c->dtpart, (size_t)grid.x,
+        hipLaunchKernelGGL(k_min_partials, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->dtpart, (size_t)grid.x,
                            c->ndt_slot + (dir - 1));
         HIPCHK(hipGetLastError());
     }
@@ -1522,7 +1538,7 @@ int fixup_utoprim(harm_ctx *c, double *p)
 {
     const DGrid &g = c->g;
     LAUNCH(HARM_K_FIXUTOPRIM,
-           hipLaunchKernelGGL(k_fixutoprim_scan, dim3(nblocks(g.nz)), dim3(TPB), 0, c->stream, g, p, c->pflag,
+           hipLaunchKernelGGL(k_fixutoprim_scan, dim3(nblocks((g.nz + 7) / 8)), dim3(TPB), 0, c->stream, g, p, c->pflag,
                               c->pflag2, c->fix_list, c->fix_count, c->fix_cap, c->cnt);
            hipLaunchKernelGGL(k_fixutoprim_serial, dim3(1), dim3(1024), 0, c->stream, g, p, c->pflag2, c->fix_list,
                               c->fix_count, c->fix_cap, c->cnt, c->fix_err));
@@ -1583,7 +1599,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     {
         CntMap map = {8, {HARM_C2P_CALLS, HARM_NEWTON_ITERS, HARM_NEWTON_KCONV, HARM_FAIL_2D, HARM_FIXUP_GAMMA,
                           HARM_FLOOR_ZONES, HARM_GAMMA_CAP, HARM_FATAL}};
-        hipLaunchKernelGGL(k_sum_partials, dim3(8), dim3(256), 0, c->stream, c->cpart, (size_t)nblocks(na), map,
+        hipLaunchKernelGGL(k_sum_partials, dim3(RED_BLOCKS, 8), dim3(256), 0, c->stream, c->cpart, (size_t)nblocks(na), map,
                            c->cnt);
         HIPCHK(hipGetLastError());
     }
