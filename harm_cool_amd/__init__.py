@@ -38,6 +38,81 @@ class U2POpts(ctypes.Structure):
                 ("mlp", c_void_p), ("K", c_double)]
 
 
+class EosSpec(ctypes.Structure):
+    _fields_ = [("rho_unit", c_double), ("lrho_lo", c_double), ("lrho_hi", c_double), ("lT_lo", c_double),
+                ("lT_hi", c_double), ("nr", ctypes.c_int32), ("nt", ctypes.c_int32), ("ye", c_double)]
+
+
+class EosTable(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int32), ("nt", ctypes.c_int32), ("rho", c_void_p), ("T", c_void_p), ("e", c_void_p),
+                ("p", c_void_p), ("gam", c_double)]
+
+
+def eos_spec(**kw) -> EosSpec:
+    """The tabulated-EOS workload: density scale 1e11 g/cm^3, log10 rho in
+    [6, 13] g/cm^3 and log10 T in [9, 12] K on 200 x 200 points, Ye = 0.5."""
+    d = dict(rho_unit=1e11, lrho_lo=6., lrho_hi=13., lT_lo=9., lT_hi=12., nr=200, nt=200, ye=0.5)
+    d.update(kw)
+    return EosSpec(**d)
+
+
+def eos_table_build(spec: EosSpec) -> dict:
+    """Host arrays of the build's EOS table (harm_eos_table_build)."""
+    L = lib()
+    rho = np.empty(spec.nr)
+    T = np.empty(spec.nt)
+    e = np.empty(spec.nr * spec.nt)
+    p = np.empty(spec.nr * spec.nt)
+    _check(L.harm_eos_table_build(ctypes.byref(spec), _ptr(rho), _ptr(T), _ptr(e), _ptr(p)), "harm_eos_table_build")
+    return {"rho": rho, "T": T, "e": e.reshape(spec.nr, spec.nt), "p": p.reshape(spec.nr, spec.nt)}
+
+
+class DeviceEos:
+    """A table resident on the device (torch tensors) and its C descriptor."""
+
+    def __init__(self, host: dict, gam: float, device: str = "cuda:0"):
+        import torch
+
+        self.t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in host.items()}
+        torch.cuda.synchronize()
+        self.desc = EosTable(nr=host["rho"].size, nt=host["T"].size, rho=self.t["rho"].data_ptr(),
+                             T=self.t["T"].data_ptr(), e=self.t["e"].data_ptr(), p=self.t["p"].data_ptr(), gam=gam)
+
+
+def eos_eval(tab: DeviceEos, rho, u):
+    """p, dp/drho, dp/du on device tensors (harm_eos_eval)."""
+    import torch
+
+    n = rho.shape[0]
+    _need(rho, (n,), "f64", "rho")
+    _need(u, (n,), "f64", "u")
+    out = [torch.empty(n, dtype=torch.float64, device=rho.device) for _ in range(3)]
+    _sync_torch()
+    _check(lib().harm_eos_eval(ctypes.byref(tab.desc), rho.data_ptr(), u.data_ptr(), out[0].data_ptr(),
+                               out[1].data_ptr(), out[2].data_ptr(), n, None), "harm_eos_eval")
+    torch.cuda.synchronize()
+    return out
+
+
+def utoprim5d_batch(U, gcov, gcon, gdet, prim, gam: float, tab: "DeviceEos | None" = None):
+    """Batched Utoprim_5d on device tensors (layout as utoprim2d_batch); returns
+    (ret int8[n], iters uint8[n])."""
+    import torch
+
+    n = gdet.shape[0]
+    for t, rows, nm in ((U, 8, "U"), (gcov, 16, "gcov"), (gcon, 16, "gcon"), (prim, 8, "prim")):
+        _need(t, (rows, n), "f64", nm)
+    ret = torch.empty(n, dtype=torch.int8, device=U.device)
+    iters = torch.empty(n, dtype=torch.uint8, device=U.device)
+    _sync_torch()
+    _check(lib().harm_utoprim5d_batch(U.data_ptr(), gcov.data_ptr(), gcon.data_ptr(), gdet.data_ptr(),
+                                      prim.data_ptr(), ret.data_ptr(), iters.data_ptr(), n, gam,
+                                      ctypes.byref(tab.desc) if tab is not None else None, None),
+           "harm_utoprim5d_batch")
+    torch.cuda.synchronize()
+    return ret, iters
+
+
 class StressSpec(ctypes.Structure):
     _fields_ = [(n, c_double) for n in ("a", "hslope", "gam", "r_lo", "r_hi", "th_lo", "th_hi", "rho_lo", "rho_hi",
                                         "urho_lo", "urho_hi", "lorentz_lo", "lorentz_hi", "sigma_lo", "sigma_hi",
@@ -71,6 +146,13 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise HarmError(f"{LIB_PATH} is missing: run harm_cool_amd/build.py (hipcc, gfx950)")
+    # torch bundles its own HIP runtime; loading it first makes the library bind
+    # to that same runtime (loaded the other way round, the process would hold
+    # two runtimes and the second one finds no device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     L.harm_last_error.restype = c_char_p
     L.harm_create.restype = c_void_p
@@ -118,6 +200,11 @@ def lib() -> ctypes.CDLL:
                                  c_void_p, c_void_p, c_void_p, c_void_p]
     L.harm_stress_batch.argtypes = [POINTER(StressSpec), c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p]
+    L.harm_eos_table_build.argtypes = [POINTER(EosSpec), _dp, _dp, _dp, _dp]
+    L.harm_eos_eval.argtypes = [POINTER(EosTable), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                c_void_p]
+    L.harm_utoprim5d_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_size_t, c_double, POINTER(EosTable), c_void_p]
     L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
     L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
     L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]

@@ -24,6 +24,7 @@
 #include "harm_host.hpp"
 #include "harm_phys.hpp"
 #include "harm_mlp.hpp"
+#include "harm_eos5d.hpp"
 
 using namespace harm;
 
@@ -1364,6 +1365,51 @@ __global__ void __launch_bounds__(TPB) k_stress_gen(StressSpec sp, size_t n, dou
     gdet[z] = geom.g;
 }
 
+// Batched Utoprim_5d with the Gamma-law (tab.nr == 0) or the tabulated EOS.
+__global__ void __launch_bounds__(TPB) k_c2p5_batch(const double *__restrict__ U, const double *__restrict__ gcov,
+                                                    const double *__restrict__ gcon, const double *__restrict__ gdet,
+                                                    double *__restrict__ prim, int8_t *__restrict__ ret,
+                                                    uint8_t *__restrict__ iters, size_t n, double gam, EosTab tab)
+{
+    const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (z >= n) return;
+    Geom geom;
+#pragma unroll
+    for (int a = 0; a < 16; a++) {
+        geom.gcov[a / 4][a % 4] = gcov[(size_t)a * n + z];
+        geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
+    }
+    geom.g = gdet[z];
+    double Uz[NPR], pr[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        Uz[k] = U[(size_t)k * n + z];
+        pr[k] = prim[(size_t)k * n + z];
+    }
+    Eos5 eos;
+    eos.tab = tab.nr > 0 ? &tab : nullptr;
+    eos.gam = gam;
+    int niter = 0;
+    const int r = utoprim_5d(Uz, geom, eos, pr, niter);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) prim[(size_t)k * n + z] = pr[k];
+    ret[z] = (int8_t)r;
+    if (iters) iters[z] = (uint8_t)niter;
+}
+
+__global__ void __launch_bounds__(TPB) k_eos_eval(EosTab tab, const double *__restrict__ rho,
+                                                  const double *__restrict__ u, double *__restrict__ p,
+                                                  double *__restrict__ dpdrho, double *__restrict__ dpdu, size_t n)
+{
+    const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (z >= n) return;
+    double pp, a, b;
+    eos_table_p(tab, rho[z], u[z], pp, a, b);
+    p[z] = pp;
+    if (dpdrho) dpdrho[z] = a;
+    if (dpdu) dpdu[z] = b;
+}
+
 __global__ void __launch_bounds__(TPB) k_pow(const double *__restrict__ x, const double *__restrict__ y,
                                              double *__restrict__ out, size_t n)
 {
@@ -2200,6 +2246,49 @@ int harm_stress_batch(const harm_stress_spec *spec, size_t n, double *U, double 
         return set_err("bad stress-batch ranges (r > 0, theta in (0, pi), positive scales)", 4);
     hipLaunchKernelGGL(k_stress_gen, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, sp, n, U, gcov, gcon, gdet,
                        prim_true, guess);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int eos_tab_of(const harm_eos_table *t, EosTab &e)
+{
+    std::memset(&e, 0, sizeof(e));
+    if (!t) return 0;
+    if (t->nr < 2 || t->nt < 2 || !t->rho || !t->T || !t->e || !t->p) return set_err("bad EOS table", 4);
+    e.nr = t->nr;
+    e.nt = t->nt;
+    e.rho = t->rho;
+    e.T = t->T;
+    e.e = t->e;
+    e.p = t->p;
+    e.gam = t->gam;
+    return 0;
+}
+
+int harm_eos_eval(const harm_eos_table *tab, const double *rho, const double *u, double *p, double *dpdrho,
+                  double *dpdu, size_t n, void *stream)
+{
+    if (!tab || !rho || !u || !p) return set_err("null argument");
+    if (n == 0) return 0;
+    EosTab e;
+    int rc;
+    if ((rc = eos_tab_of(tab, e))) return rc;
+    hipLaunchKernelGGL(k_eos_eval, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, e, rho, u, p, dpdrho, dpdu,
+                       n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int harm_utoprim5d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet, double *prim,
+                         int8_t *ret, uint8_t *iters, size_t n, double gam, const harm_eos_table *tab, void *stream)
+{
+    if (!U || !gcov || !gcon || !gdet || !prim || !ret) return set_err("null argument");
+    if (n == 0) return 0;
+    EosTab e;
+    int rc;
+    if ((rc = eos_tab_of(tab, e))) return rc;
+    hipLaunchKernelGGL(k_c2p5_batch, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon, gdet, prim,
+                       ret, iters, n, gam, e);
     HIPCHK(hipGetLastError());
     return 0;
 }

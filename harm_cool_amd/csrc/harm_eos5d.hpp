@@ -1,0 +1,496 @@
+// harm_eos5d.hpp -- tabulated equation of state and the 5D primitive
+// solver (utoprim_5d.c) for the gfx950 kernels (FP64).
+//
+// The table holds e(rho, T) and p(rho, T) on an increasing (rho, T) grid in
+// code units.  p(rho, u) is the piecewise-bilinear interpolant in (rho, T),
+// inverted along T at fixed rho by bisection over the cell edges and one exact
+// linear solve inside the cell: no transcendental function, so the CPU oracle
+// evaluates the same numbers bit for bit.  Outside the table the Gamma-law
+// pressure is used, as the reference does outside its table
+// (cooleos.c:342-366).  The reference evaluates p(rho, u) by a T-Newton on an
+// Akima spline of its cached table (cooleos.c:314-444) and keeps the last T
+// per zone as the next start; the direct inversion here needs no such state.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace harm {
+
+struct EosTab {
+    int nr, nt;
+    const double *rho, *T, *e, *p;  // device, rho[nr], T[nt], e/p [nr][nt]
+    double gam;                     // Gamma-law fallback outside the table
+};
+
+__device__ __forceinline__ int eos_cell(const double *x, int n, double v)
+{
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// p(rho, u), dp/drho at fixed u, dp/du at fixed rho; 1 on the Gamma-law branch.
+__device__ inline int eos_table_p(const EosTab &t, double rho, double u, double &p, double &dpdrho, double &dpdu)
+{
+    const int nr = t.nr, nt = t.nt;
+    if (rho >= t.rho[0] && rho <= t.rho[nr - 1]) {
+        const int k = eos_cell(t.rho, nr, rho);
+        const double dr = t.rho[k + 1] - t.rho[k];
+        const double a = (rho - t.rho[k]) / dr;
+        const double *e0 = t.e + (size_t)k * nt, *e1 = e0 + nt;
+        const double *p0 = t.p + (size_t)k * nt, *p1 = p0 + nt;
+        auto EAT = [&](int j) { return e0[j] + a * (e1[j] - e0[j]); };
+        const double Elo = EAT(0), Ehi = EAT(nt - 1);
+        if (u >= Elo && u <= Ehi) {
+            int lo = 0, hi = nt - 1;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (EAT(mid) <= u)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const double E0 = EAT(lo), E1 = EAT(lo + 1);
+            const double P0 = p0[lo] + a * (p1[lo] - p0[lo]);
+            const double P1 = p0[lo + 1] + a * (p1[lo + 1] - p0[lo + 1]);
+            const double dE = E1 - E0, dP = P1 - P0;
+            const double b = (u - E0) / dE;
+            p = P0 + b * dP;
+            dpdu = dP / dE;
+            const double sE0 = (e1[lo] - e0[lo]) / dr, sE1 = (e1[lo + 1] - e0[lo + 1]) / dr;
+            const double sP0 = (p1[lo] - p0[lo]) / dr, sP1 = (p1[lo + 1] - p0[lo + 1]) / dr;
+            dpdrho = sP0 + (-sE0 * dP * dE + (u - E0) * ((sP1 - sP0) * dE - dP * (sE1 - sE0))) / (dE * dE);
+            return 0;
+        }
+    }
+    p = (t.gam - 1.) * u;
+    dpdrho = 0.;
+    dpdu = t.gam - 1.;
+    return 1;
+}
+
+// The 5D solver's view of the EOS: tab == nullptr is the Gamma-law pressure.
+struct Eos5 {
+    const EosTab *tab;
+    double gam;
+    __device__ double p(double rho, double u) const
+    {
+        if (!tab) return (gam - 1.) * u;
+        double pp, a, b;
+        eos_table_p(*tab, rho, u, pp, a, b);
+        return pp;
+    }
+};
+
+// ---- utoprim_5d.c, restated for one zone -----------------------------------
+constexpr int N5 = 5;
+
+// ucon_calc_g / bcon_calc_g / lower_g / ncov_calc (u2p_util.c:204-300)
+__device__ __forceinline__ void ucon_calc_g5(const double *pr, const Geom &g, double *ucon)
+{
+    const double ut[4] = {0., pr[U1], pr[U2], pr[U3]};
+    double utsq = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) utsq += g.gcov[i][j] * ut[i] * ut[j];
+    utsq = fabs(utsq);
+    const double gamma = sqrt(1. + utsq);
+    const double lapse = sqrt(-1. / g.gcon[0][0]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) ucon[i] = ut[i] - lapse * gamma * g.gcon[0][i];
+}
+__device__ __forceinline__ void lower_g5(const double *vcon, const Geom &g, double *vcov)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        vcov[i] = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) vcov[i] += g.gcov[i][j] * vcon[j];
+    }
+}
+__device__ __forceinline__ void bcon_calc_g5(const double *pr, const double *ucon, const double *ucov,
+                                             const double *ncov, double *bcon)
+{
+    double Bcon[4];
+    Bcon[0] = 0.;
+#pragma unroll
+    for (int i = 1; i < 4; i++) Bcon[i] = -ncov[0] * pr[B1 + i - 1];
+    double u_dot_B = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) u_dot_B += ucov[i] * Bcon[i];
+    const double gamma = -ucon[0] * ncov[0];
+#pragma unroll
+    for (int i = 0; i < 4; i++) bcon[i] = (Bcon[i] + ucon[i] * u_dot_B) / gamma;
+}
+
+// primtoU_g (u2p_util.c:84-145) with the solver's EOS.
+__device__ inline void primtoU_g5(const Eos5 &eos, const double *pr, const Geom &g, double *U)
+{
+    double ucon[4], ucov[4], bcon[4], bcov[4], ncov[4];
+    const double alpha = 1.0 / sqrt(-g.gcon[0][0]);
+    ucon_calc_g5(pr, g, ucon);
+    lower_g5(ucon, g, ucov);
+    ncov[0] = -sqrt(-1. / g.gcon[0][0]);
+    ncov[1] = ncov[2] = ncov[3] = 0.;
+    const double gamma = -ncov[0] * ucon[0];
+    bcon_calc_g5(pr, ucon, ucov, ncov, bcon);
+    lower_g5(bcon, g, bcov);
+    double n_dot_b = 0., bsq = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) n_dot_b += ncov[i] * bcon[i];
+#pragma unroll
+    for (int i = 0; i < 4; i++) bsq += bcov[i] * bcon[i];
+    const double rho0 = pr[RHO], u = pr[UU];
+    const double p = eos.p(rho0, u);
+    const double w = rho0 + u + p;
+    U[RHO] = ucon[0] * rho0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        U[1 + i] = gamma * (w + bsq) * ucov[i] - (p + bsq / 2.) * ncov[i] + n_dot_b * bcov[i];
+        U[1 + i] /= alpha;
+    }
+    U[1] = U[1] + U[RHO];
+    U[B1] = pr[B1];
+    U[B2] = pr[B2];
+    U[B3] = pr[B3];
+#pragma unroll
+    for (int i = 0; i < NPR; i++) U[i] *= g.g;
+}
+
+// gamma_calc_g (u2p_util.c:319-341); on failure gamma = sqrt(1 + |utsq|).
+__device__ __forceinline__ int gamma_calc_g5(const double *pr, const Geom &g, double &gamma)
+{
+    double utsq = g.gcov[1][1] * pr[U1] * pr[U1] + g.gcov[2][2] * pr[U2] * pr[U2] + g.gcov[3][3] * pr[U3] * pr[U3] +
+                  2. * (g.gcov[1][2] * pr[U1] * pr[U2] + g.gcov[1][3] * pr[U1] * pr[U3] +
+                        g.gcov[2][3] * pr[U2] * pr[U3]);
+    int bad = 0;
+    if (utsq < 0.0) {
+        if (fabs(utsq) > 1.E-10) bad = 1;
+        utsq = fabs(utsq);
+    }
+    gamma = sqrt(1. + utsq);
+    return bad;
+}
+
+// dudp_calc_g (utoprim_5d.c:409-529) with its helpers (utoprim_5d.c:531-640).
+__device__ inline void dudp_calc_g5(const Eos5 &eos, const double *pr, const Geom &g, double (&Am)[N5][N5])
+{
+    double dutdui[4], dbtdui[4], dbsqdui[4], dbiduj[4][4], dgdvi[4], duidvj[4][4];
+    double dudduu[4][4], dbdiduj[4][4], tempA[N5][N5];
+    double bcon[4], bcov[4], ucov[4], ucon[4], ncov[4], B[4], Bcov[4];
+#pragma unroll
+    for (int j = 0; j < N5; j++)
+#pragma unroll
+        for (int k = 0; k < N5; k++) tempA[j][k] = Am[j][k] = 0.;
+    ucon_calc_g5(pr, g, ucon);
+    lower_g5(ucon, g, ucov);
+    ncov[0] = -sqrt(-1. / g.gcon[0][0]);
+    ncov[1] = ncov[2] = ncov[3] = 0.;
+    bcon_calc_g5(pr, ucon, ucov, ncov, bcon);
+    lower_g5(bcon, g, bcov);
+    const double bsq = bcon[0] * bcov[0] + bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+#pragma unroll
+    for (int j = 1; j < 4; j++) dutdui[j] = -ucov[j] / ucov[0];
+    B[0] = 0.;
+#pragma unroll
+    for (int j = 1; j < 4; j++) B[j] = pr[j + B1 - 1];
+    lower_g5(B, g, Bcov);
+#pragma unroll
+    for (int j = 1; j < 4; j++) dbtdui[j] = Bcov[j] + Bcov[0] * dutdui[j];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) dbiduj[j][k] = 0.;
+#pragma unroll
+    for (int j = 1; j < 4; j++)
+#pragma unroll
+        for (int k = 1; k < 4; k++) dbiduj[j][k] = -bcon[j] * dutdui[k] / ucon[0] + ucon[j] * dbtdui[k] / ucon[0];
+#pragma unroll
+    for (int j = 1; j < 4; j++) dbiduj[j][j] += bcon[0] / ucon[0];
+#pragma unroll
+    for (int j = 1; j < 4; j++) dbiduj[0][j] = dbtdui[j];
+#pragma unroll
+    for (int j = 0; j < 4; j++) dbsqdui[j] = 0.;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) dbsqdui[j] += 2. * bcov[k] * dbiduj[k][j];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 1; k < 4; k++) dudduu[j][k] = g.gcov[j][k] + g.gcov[j][0] * dutdui[k];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 1; k < 4; k++) {
+            dbdiduj[j][k] = 0.0;
+#pragma unroll
+            for (int l = 0; l < 4; l++) dbdiduj[j][k] += g.gcov[j][l] * dbiduj[l][k];
+        }
+    double eta;
+    if (eos.tab) {
+        double p, dpdrho, dpdu;
+        eos_table_p(*eos.tab, pr[RHO], pr[UU], p, dpdrho, dpdu);
+        eta = pr[RHO] + pr[UU] + p + bsq;
+        Am[RHO][RHO] = ucon[0];
+#pragma unroll
+        for (int j = 0; j < 4; j++) Am[UU + j][RHO] = (1.0 + dpdrho) * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * dpdrho;
+        Am[RHO][UU] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) Am[UU + j][UU] = (1.0 + dpdu) * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * dpdu;
+    } else {
+        const double GAMMA = eos.gam;
+        eta = pr[RHO] + GAMMA * pr[UU] + bsq;
+        Am[RHO][RHO] = ucon[0];
+#pragma unroll
+        for (int j = 0; j < 4; j++) Am[UU + j][RHO] = ucon[0] * ucov[j];
+        Am[RHO][UU] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) Am[UU + j][UU] = GAMMA * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * (GAMMA - 1.0);
+    }
+#pragma unroll
+    for (int j = 1; j < 4; j++) Am[RHO][U1 + j - 1] = pr[RHO] * dutdui[j];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 1; k < 4; k++)
+            Am[UU + j][U1 + k - 1] = dbsqdui[k] * ucon[0] * ucov[j] +
+                                    eta * (dutdui[k] * ucov[j] + ucon[0] * dudduu[j][k]) +
+                                    (j == 0 ? 1. : 0.) * 0.5 * dbsqdui[k] -
+                                    (dbtdui[k] * bcov[j] + bcon[0] * dbdiduj[j][k]);
+#pragma unroll
+    for (int k = 0; k < N5; k++) Am[UU][k] += Am[RHO][k];
+    double gamma;
+    gamma_calc_g5(pr, g, gamma);
+#pragma unroll
+    for (int j = 1; j < 4; j++) dgdvi[j] = 0.0;
+#pragma unroll
+    for (int j = 1; j < 4; j++)
+#pragma unroll
+        for (int k = 1; k < 4; k++) dgdvi[j] += 1.0 / gamma * g.gcov[j][k] * pr[U1 + k - 1];
+    const double alpha = 1.0 / sqrt(-g.gcon[0][0]);
+    double betacon[4];
+#pragma unroll
+    for (int j = 1; j < 4; j++) betacon[j] = g.gcon[0][j] * alpha * alpha;
+#pragma unroll
+    for (int j = 1; j < 4; j++) duidvj[0][j] = dgdvi[j] / alpha;
+#pragma unroll
+    for (int j = 1; j < 4; j++)
+#pragma unroll
+        for (int k = 1; k < 4; k++) duidvj[j][k] = (j == k ? 1. : 0.) - betacon[j] / alpha * dgdvi[k];
+#pragma unroll
+    for (int i = 0; i < N5; i++)
+#pragma unroll
+        for (int j = RHO; j <= UU; j++) tempA[i][j] = Am[i][j];
+#pragma unroll
+    for (int i = 0; i < N5; i++)
+#pragma unroll
+        for (int j = 1; j < 4; j++)
+#pragma unroll
+            for (int l = 1; l <= 3; l++) tempA[RHO + i][U1 + j - 1] += Am[RHO + i][U1 + l - 1] * duidvj[l][j];
+#pragma unroll
+    for (int j = 0; j < N5; j++)
+#pragma unroll
+        for (int k = 0; k < N5; k++) Am[j][k] = tempA[j][k] * g.g;
+}
+
+// LU_decompose / LU_substitution (utoprim_5d.c:672-874).
+__device__ inline void lu5_solve(double (&A)[N5][N5], double (&B)[N5])
+{
+    const double absmin = 1e-30;
+    double row_norm[N5];
+    int permute[N5];
+    int max_row = 0;
+    const int n = N5;
+#pragma unroll
+    for (int i = 0; i < n; i++) {
+        double absmax = 0.;
+#pragma unroll
+        for (int j = 0; j < n; j++) {
+            const double m = fabs(A[i][j]);
+            if (m > absmax) absmax = m;
+        }
+        // a zero row: the reference returns before factorizing and then
+        // substitutes with the unfactorized matrix and a zero permutation
+        if (absmax == 0.) {
+#pragma unroll
+            for (int q = 0; q < n; q++) permute[q] = 0;
+            goto subst;
+        }
+        row_norm[i] = 1. / absmax;
+    }
+#pragma unroll
+    for (int j = 0; j < n; j++) {
+#pragma unroll
+        for (int i = 0; i < j; i++)
+#pragma unroll
+            for (int k = 0; k < i; k++) A[i][j] -= A[i][k] * A[k][j];
+        double absmax = 0.0;
+#pragma unroll
+        for (int i = j; i < n; i++) {
+#pragma unroll
+            for (int k = 0; k < j; k++) A[i][j] -= A[i][k] * A[k][j];
+            const double m = fabs(A[i][j]) * row_norm[i];
+            if (m >= absmax) {
+                absmax = m;
+                max_row = i;
+            }
+        }
+        if (max_row != j) {
+            if ((j == (n - 2)) && (A[j][j + 1] == 0.)) {
+                max_row = j;
+            } else {
+#pragma unroll
+                for (int k = 0; k < n; k++) {
+                    // rows j and max_row swapped with register-resident
+                    // arrays: select instead of dynamic indexing
+                    double aj = A[j][k], am = 0.;
+#pragma unroll
+                    for (int r = 0; r < n; r++)
+                        if (r == max_row) am = A[r][k];
+#pragma unroll
+                    for (int r = 0; r < n; r++)
+                        if (r == max_row) A[r][k] = aj;
+                    A[j][k] = am;
+                }
+                double rn = 0.;
+#pragma unroll
+                for (int r = 0; r < n; r++)
+                    if (r == j) rn = row_norm[r];
+#pragma unroll
+                for (int r = 0; r < n; r++)
+                    if (r == max_row) row_norm[r] = rn;
+            }
+        }
+        permute[j] = max_row;
+        if (A[j][j] == 0.) A[j][j] = absmin;
+        if (j != (n - 1)) {
+            const double inv = 1. / A[j][j];
+#pragma unroll
+            for (int i = j + 1; i < n; i++) A[i][j] *= inv;
+        }
+    }
+subst:
+#pragma unroll
+    for (int i = 0; i < n; i++) {
+        double tmpvar = 0., bi = B[i];
+#pragma unroll
+        for (int r = 0; r < n; r++)
+            if (r == permute[i]) tmpvar = B[r];
+#pragma unroll
+        for (int r = 0; r < n; r++)
+            if (r == permute[i]) B[r] = bi;
+#pragma unroll
+        for (int j = i - 1; j >= 0; j--) tmpvar -= A[i][j] * B[j];
+        B[i] = tmpvar;
+    }
+#pragma unroll
+    for (int i = n - 1; i >= 0; i--) {
+#pragma unroll
+        for (int j = i + 1; j < n; j++) B[i] -= A[i][j] * B[j];
+        B[i] /= A[i][i];
+    }
+}
+
+// func_5d (utoprim_5d.c:250-359), NORMMETHOD 1.
+__device__ inline void func_5d(const Eos5 &eos, const double *U_target, const Geom &g, const double *x,
+                               double (&dx)[N5], double &f, double &df)
+{
+    double pr[NPR], U_curr[NPR], Am[N5][N5];
+#pragma unroll
+    for (int i = 0; i < B1; i++) pr[i] = x[i];
+#pragma unroll
+    for (int i = B1; i <= B3; i++) pr[i] = U_target[i] / g.g;
+    primtoU_g5(eos, pr, g, U_curr);
+    dudp_calc_g5(eos, pr, g, Am);
+    double norm = 0.0;
+    int numnormterms = 0;
+#pragma unroll
+    for (int j = 0; j < N5; j++)
+#pragma unroll
+        for (int k = 0; k < N5; k++)
+            if (Am[j][k] > 2.2204460492503131e-16) {
+                norm += fabs(Am[j][k]);
+                numnormterms++;
+            }
+    norm = ((double)(numnormterms)) / (norm);
+#pragma unroll
+    for (int j = 0; j < N5; j++)
+#pragma unroll
+        for (int k = 0; k < N5; k++) Am[j][k] *= (norm);
+    df = 0.;
+#pragma unroll
+    for (int k = 0; k < N5; k++) {
+        const double resid = U_curr[k] - U_target[k];
+        dx[k] = -resid * (norm);
+        double residnorm = 0.5 * (fabs(U_curr[k]) + fabs(U_target[k]));
+        residnorm = (residnorm == 0.) ? 1. : 1. / residnorm;
+        df -= residnorm * resid * resid;
+    }
+    f = -0.5 * df;
+    lu5_solve(Am, dx);
+}
+
+// Utoprim_5d (utoprim_5d.c:179-239) with general_newton_raphson
+// (utoprim_5d.c:890-983) and calc_errx_5d (utoprim_5d.c:378-400).
+__device__ inline int utoprim_5d(const double *U, const Geom &g, const Eos5 &eos, double *prim, int &n_iter_out)
+{
+#pragma unroll
+    for (int k = B1; k <= B3; k++) prim[k] = U[k] / g.g;
+    double x[N5];
+#pragma unroll
+    for (int k = 0; k < N5; k++) x[k] = prim[k];
+    double f = 1., df = 1., errx = 1., W = 0.;
+    int n_iter = 0, i_extra = 0, doing_extra = 0;
+    bool keep = true;
+    while (keep) {
+        double dx[N5];
+        func_5d(eos, U, g, x, dx, f, df);
+#pragma unroll
+        for (int id = 0; id < N5; id++) x[id] += dx[id];
+        const double W_old = W;
+        double vsq = 0.;
+#pragma unroll
+        for (int id = 1; id < 4; id++)
+#pragma unroll
+            for (int jd = 1; jd < 4; jd++) vsq += g.gcov[id][jd] * x[U1 + id - 1] * x[U1 + jd - 1];
+        const double gsqtmp = 1. + vsq;
+        if (eos.tab)
+            W = gsqtmp * (x[UU] + x[RHO] + eos.p(x[RHO], x[UU]));
+        else
+            W = gsqtmp * (eos.gam * x[UU] + x[RHO]);
+        errx = (W == 0.) ? fabs(W - W_old) : fabs((W - W_old) / W);
+        if ((fabs(errx) <= NEWT_TOL) && (f <= 1.e-4) && (doing_extra == 0)) doing_extra = 1;
+        if (doing_extra == 1) i_extra++;
+        if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+            (n_iter >= (MAX_NEWT_ITER - 1)))
+            keep = false;
+        n_iter++;
+    }
+    n_iter_out = n_iter;
+    int ret = 0;
+    if (!isfinite(f) || !isfinite(df))
+        ret = 2;
+    else if ((fabs(errx) > MIN_NEWT_TOL) || (f > 1.e-4))
+        ret = 1;
+    if (ret) return 100 * ret + 1;
+    if ((x[RHO] < 0.0) || (x[UU] < 0.0)) return 5;
+    double pt[NPR];
+#pragma unroll
+    for (int k = 0; k < N5; k++) pt[k] = x[k];
+#pragma unroll
+    for (int k = B1; k <= B3; k++) pt[k] = prim[k];
+    double gamma_tmp;
+    if (gamma_calc_g5(pt, g, gamma_tmp) == 1) return 4;
+#pragma unroll
+    for (int k = 0; k < N5; k++) prim[k] = x[k];
+    return 0;
+}
+
+}  // namespace harm

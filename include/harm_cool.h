@@ -232,6 +232,37 @@ typedef struct harm_stress_spec {
 int harm_stress_batch(const harm_stress_spec *spec, size_t n, double *U, double *gcov, double *gcon, double *gdet,
                       double *prim_true, double *guess, void *stream);
 
+/* ---- Tabulated EOS and the 5D inversion (utoprim_5d.c) --------------------
+ * Table: rho[nr], T[nt] increasing; e[nr][nt], p[nr][nt] in code units (e is
+ * the internal energy density u).  p(rho, u) = the piecewise-bilinear
+ * interpolant in (rho, T), inverted along T at fixed rho; Gamma-law pressure
+ * outside the table (as cooleos.c:342-366).  Replaces
+ * pressure_rho0_u_CoolEOS / dpdrho_calc_CoolEOS / dpdu_calc_CoolEOS
+ * (cooleos.c:314-444, 667-741) and the lazily filled Cache4d tables
+ * (cache4d.c:863-1134). */
+typedef struct harm_eos_spec {
+    double rho_unit;               /* g/cm^3 per code density unit */
+    double lrho_lo, lrho_hi;       /* log10 rho range, g/cm^3 */
+    double lT_lo, lT_hi;           /* log10 T range, K */
+    int32_t nr, nt;
+    double ye;                     /* electron fraction */
+} harm_eos_spec;
+/* Fill a table (host arrays rho[nr], T[nt], e[nr*nt], p[nr*nt]) from the
+ * build's closed-form EOS (degenerate electrons + ideal gas + radiation). */
+int harm_eos_table_build(const harm_eos_spec *spec, double *rho, double *T, double *e, double *p);
+typedef struct harm_eos_table {
+    int32_t nr, nt;
+    const double *rho, *T, *e, *p; /* device pointers */
+    double gam;                    /* Gamma-law fallback outside the table */
+} harm_eos_table;
+/* p, dp/drho (fixed u), dp/du (fixed rho) for n (rho, u) pairs (device arrays). */
+int harm_eos_eval(const harm_eos_table *tab, const double *rho, const double *u, double *p, double *dpdrho,
+                  double *dpdu, size_t n, void *stream);
+/* Batched Utoprim_5d (utoprim_5d.c:179-239; decs.h:610-611): SoA device arrays
+ * as harm_utoprim2d_batch; tab NULL selects the Gamma-law EOS with gam. */
+int harm_utoprim5d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet, double *prim,
+                         int8_t *ret, uint8_t *iters, size_t n, double gam, const harm_eos_table *tab, void *stream);
+
 /* Exact reference signature (utoprim_2d.c:126, decs.h:608-609), evaluated on the
  * GPU for one zone; gam is taken from harm_set_gamma() (default 4/3, init.c:91). */
 int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8]);

@@ -79,6 +79,8 @@
 #define MIN_NEWT_TOL 1.0e-10
 #define EXTRA_NEWT_ITER 2
 #define NEWT_TOL2 1.0e-15
+#define NUMEPS5 (2.2204460492503131e-16) /* NUMEPSILON, u2p_defs.h:96 */
+#define TOL_RESID5 (1.e-4)               /* TOL_RESID, utoprim_5d.c:55 */
 #define W_TOO_BIG 1.e20
 #define UTSQ_TOO_BIG 1.e20
 #define FAIL_VAL 1.e30
@@ -2655,4 +2657,426 @@ int orc_utoprim_2d_learned(const double U[8], const double *gcov, const double *
     *n_iter = n1 + n2;
     *k_conv = n1 + k2;
     return ret;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tabulated equation of state and the 5D inversion (utoprim_5d.c).          */
+/* ------------------------------------------------------------------------ */
+/* The table: rho[nr] and T[nt] increasing (log-spaced in practice), e and p
+ * [nr][nt] in code units (e = internal energy density u).  p(rho, u) is the
+ * piecewise-bilinear interpolant in (rho, T) inverted along T at fixed rho:
+ * E_j(rho) = e[k][j] + a (e[k+1][j] - e[k][j]) with a the rho weight of cell k,
+ * j such that E_j <= u <= E_{j+1}, b = (u - E_j)/(E_{j+1} - E_j),
+ * p = P_j + b (P_{j+1} - P_j).  No transcendental function is involved, so the
+ * device evaluates it with identical arithmetic.  Outside the table the
+ * Gamma-law pressure is used, as the reference does outside its table
+ * (cooleos.c:342-366).  Returns 1 when the Gamma-law branch was taken. */
+static int cell_of(const double *x, int n, double v)
+{
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (x[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+int orc_eos_table_p(const orc_eos *t, double rho, double u, double *p, double *dpdrho, double *dpdu, double *T)
+{
+    const int nr = t->nr, nt = t->nt;
+    int k, lo, hi;
+    double dr, a, Elo, Ehi, E0, E1, P0, P1, b, dE, dP, sE0, sE1, sP0, sP1;
+    if (!(rho >= t->rho[0] && rho <= t->rho[nr - 1])) goto gamma_law;
+    k = cell_of(t->rho, nr, rho);
+    dr = t->rho[k + 1] - t->rho[k];
+    a = (rho - t->rho[k]) / dr;
+#define EAT(j) (t->e[k * nt + (j)] + a * (t->e[(k + 1) * nt + (j)] - t->e[k * nt + (j)]))
+#define PAT(j) (t->p[k * nt + (j)] + a * (t->p[(k + 1) * nt + (j)] - t->p[k * nt + (j)]))
+    Elo = EAT(0);
+    Ehi = EAT(nt - 1);
+    if (!(u >= Elo && u <= Ehi)) goto gamma_law;
+    lo = 0;
+    hi = nt - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (EAT(mid) <= u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    E0 = EAT(lo);
+    E1 = EAT(lo + 1);
+    P0 = PAT(lo);
+    P1 = PAT(lo + 1);
+    dE = E1 - E0;
+    dP = P1 - P0;
+    b = (u - E0) / dE;
+    *p = P0 + b * dP;
+    if (dpdu) *dpdu = dP / dE;
+    if (dpdrho) {
+        sE0 = (t->e[(k + 1) * nt + lo] - t->e[k * nt + lo]) / dr;
+        sE1 = (t->e[(k + 1) * nt + lo + 1] - t->e[k * nt + lo + 1]) / dr;
+        sP0 = (t->p[(k + 1) * nt + lo] - t->p[k * nt + lo]) / dr;
+        sP1 = (t->p[(k + 1) * nt + lo + 1] - t->p[k * nt + lo + 1]) / dr;
+        *dpdrho = sP0 + (-sE0 * dP * dE + (u - E0) * ((sP1 - sP0) * dE - dP * (sE1 - sE0))) / (dE * dE);
+    }
+    if (T) *T = t->T[lo] + b * (t->T[lo + 1] - t->T[lo]);
+#undef EAT
+#undef PAT
+    return 0;
+gamma_law:
+    *p = (t->gam - 1.) * u;
+    if (dpdrho) *dpdrho = 0.;
+    if (dpdu) *dpdu = t->gam - 1.;
+    if (T) *T = 0.;
+    return 1;
+}
+
+/* EOS seen by the 5D solver: t == NULL is the Gamma-law pressure_rho0_u
+ * (utoprim_2d.c:645-650 / u2p_util.c). */
+static double eos5_p(const orc_eos *t, double gam, double rho, double u)
+{
+    double p;
+    if (!t) return (gam - 1.) * u;
+    orc_eos_table_p(t, rho, u, &p, NULL, NULL, NULL);
+    return p;
+}
+
+typedef struct {
+    double U_target[NPR];
+    double gcov[NDIM][NDIM], gcon[NDIM][NDIM], gdet, gam;
+    const orc_eos *eos;
+} u5_t;
+
+/* primtoU_g (u2p_util.c:84-145) with the EOS of the solver. */
+static void primtoU_g5(const u5_t *s, const double prim[NPR], double U[NPR])
+{
+    int i;
+    double ucon[NDIM], ucov[NDIM], bcon[NDIM], bcov[NDIM], ncov[NDIM];
+    double rho0, gamma, n_dot_b, bsq, u, p, w, alpha;
+    double gc[NDIM][NDIM], gn[NDIM][NDIM];
+    memcpy(gc, s->gcov, sizeof(gc));
+    memcpy(gn, s->gcon, sizeof(gn));
+    alpha = 1.0 / sqrt(-gn[0][0]);
+    ucon_calc_g(prim, gc, gn, ucon);
+    lower_g(ucon, gc, ucov);
+    ncov_calc(gn, ncov);
+    gamma = -ncov[0] * ucon[0];
+    bcon_calc_g(prim, ucon, ucov, ncov, bcon);
+    lower_g(bcon, gc, bcov);
+    n_dot_b = 0.;
+    for (i = 0; i < 4; i++) n_dot_b += ncov[i] * bcon[i];
+    bsq = 0.;
+    for (i = 0; i < 4; i++) bsq += bcov[i] * bcon[i];
+    rho0 = prim[RHO];
+    u = prim[UU];
+    p = eos5_p(s->eos, s->gam, rho0, u);
+    w = rho0 + u + p;
+    U[RHO] = ucon[0] * rho0;
+    for (i = 0; i < 4; i++) {
+        U[1 + i] = gamma * (w + bsq) * ucov[i] - (p + bsq / 2.) * ncov[i] + n_dot_b * bcov[i];
+        U[1 + i] /= alpha;
+    }
+    U[1] = U[1] + U[RHO];
+    U[B1] = prim[B1];
+    U[B2] = prim[B2];
+    U[B3] = prim[B3];
+    for (i = 0; i < NPR; i++) U[i] *= s->gdet;
+}
+
+/* gamma_calc_g (u2p_util.c:319-341); the reference leaves gamma unset on
+ * failure, here it is sqrt(1 + |utsq|). */
+static int gamma_calc_g5(const double *pr, double gcov[NDIM][NDIM], double *gamma)
+{
+    double utsq = gcov[1][1] * pr[U1] * pr[U1] + gcov[2][2] * pr[U2] * pr[U2] + gcov[3][3] * pr[U3] * pr[U3] +
+                  2. * (gcov[1][2] * pr[U1] * pr[U2] + gcov[1][3] * pr[U1] * pr[U3] + gcov[2][3] * pr[U2] * pr[U3]);
+    int bad = 0;
+    if (utsq < 0.0) {
+        if (fabs(utsq) > 1.E-10) bad = 1;
+        utsq = fabs(utsq);
+    }
+    *gamma = sqrt(1. + utsq);
+    return bad;
+}
+
+#define N5 5
+/* dudp_calc_g (utoprim_5d.c:409-529) and its helpers (utoprim_5d.c:531-640). */
+static void dudp_calc_g5(const u5_t *s, const double *pr, double Am[N5][N5])
+{
+    double dutdui[NDIM], dbtdui[NDIM], dbsqdui[NDIM], dbiduj[NDIM][NDIM], dgdvi[NDIM], duidvj[NDIM][NDIM];
+    double dudduu[NDIM][NDIM], dbdiduj[NDIM][NDIM], tempA[N5][N5];
+    double bcon[NDIM], bcov[NDIM], ucov[NDIM], ucon[NDIM], ncov[NDIM], B[NDIM], Bcov[NDIM];
+    double eta, bsq, gamma, alpha, betacon[NDIM];
+    double gc[NDIM][NDIM], gn[NDIM][NDIM];
+    int i, j, k, l;
+    memcpy(gc, s->gcov, sizeof(gc));
+    memcpy(gn, s->gcon, sizeof(gn));
+    for (j = 0; j < N5; j++)
+        for (k = 0; k < N5; k++) tempA[j][k] = Am[j][k] = 0.;
+    ucon_calc_g(pr, gc, gn, ucon);
+    lower_g(ucon, gc, ucov);
+    ncov_calc(gn, ncov);
+    bcon_calc_g(pr, ucon, ucov, ncov, bcon);
+    lower_g(bcon, gc, bcov);
+    bsq = bcon[0] * bcov[0] + bcon[1] * bcov[1] + bcon[2] * bcov[2] + bcon[3] * bcov[3];
+    for (j = 1; j < NDIM; j++) dutdui[j] = -ucov[j] / ucov[0];
+    B[0] = 0.;
+    for (j = 1; j < NDIM; j++) B[j] = pr[j + B1 - 1];
+    lower_g(B, gc, Bcov);
+    for (j = 1; j < NDIM; j++) dbtdui[j] = Bcov[j] + Bcov[0] * dutdui[j];
+    for (j = 0; j < NDIM; j++)
+        for (k = 0; k < NDIM; k++) dbiduj[j][k] = 0.;
+    for (j = 1; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++) dbiduj[j][k] = -bcon[j] * dutdui[k] / ucon[0] + ucon[j] * dbtdui[k] / ucon[0];
+    for (j = 1; j < NDIM; j++) dbiduj[j][j] += bcon[0] / ucon[0];
+    for (j = 1; j < NDIM; j++) dbiduj[0][j] = dbtdui[j];
+    for (j = 0; j < NDIM; j++) dbsqdui[j] = 0.;
+    for (j = 0; j < NDIM; j++)
+        for (k = 0; k < NDIM; k++) dbsqdui[j] += 2. * bcov[k] * dbiduj[k][j];
+    for (j = 0; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++) dudduu[j][k] = gc[j][k] + gc[j][0] * dutdui[k];
+    for (j = 0; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++) dbdiduj[j][k] = 0.0;
+    for (j = 0; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++)
+            for (l = 0; l < NDIM; l++) dbdiduj[j][k] += gc[j][l] * dbiduj[l][k];
+    if (s->eos) {
+        double p, dpdrho, dpdu;
+        orc_eos_table_p(s->eos, pr[RHO], pr[UU], &p, &dpdrho, &dpdu, NULL);
+        eta = pr[RHO] + pr[UU] + p + bsq;
+        Am[RHO][RHO] = ucon[0];
+        for (j = 0; j < NDIM; j++) Am[UU + j][RHO] = (1.0 + dpdrho) * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * dpdrho;
+        Am[RHO][UU] = 0.0;
+        for (j = 0; j < NDIM; j++) Am[UU + j][UU] = (1.0 + dpdu) * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * dpdu;
+    } else {
+        const double GAMMA = s->gam;
+        eta = pr[RHO] + GAMMA * pr[UU] + bsq;
+        Am[RHO][RHO] = ucon[0];
+        for (j = 0; j < NDIM; j++) Am[UU + j][RHO] = ucon[0] * ucov[j];
+        Am[RHO][UU] = 0.0;
+        for (j = 0; j < NDIM; j++)
+            Am[UU + j][UU] = GAMMA * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * (GAMMA - 1.0);
+    }
+    for (j = 1; j < NDIM; j++) Am[RHO][U1 + j - 1] = pr[RHO] * dutdui[j];
+    for (j = 0; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++)
+            Am[UU + j][U1 + k - 1] = dbsqdui[k] * ucon[0] * ucov[j] +
+                                    eta * (dutdui[k] * ucov[j] + ucon[0] * dudduu[j][k]) +
+                                    (j == 0 ? 1. : 0.) * 0.5 * dbsqdui[k] -
+                                    (dbtdui[k] * bcov[j] + bcon[0] * dbdiduj[j][k]);
+    for (k = 0; k < N5; k++) Am[UU][k] += Am[RHO][k];
+    gamma_calc_g5(pr, gc, &gamma);
+    for (j = 1; j < NDIM; j++) dgdvi[j] = 0.0;
+    for (j = 1; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++) dgdvi[j] += 1.0 / gamma * gc[j][k] * pr[U1 + k - 1];
+    alpha = 1.0 / sqrt(-gn[0][0]);
+    for (j = 1; j < NDIM; j++) betacon[j] = gn[0][j] * alpha * alpha;
+    for (j = 1; j < NDIM; j++) duidvj[0][j] = dgdvi[j] / alpha;
+    for (j = 1; j < NDIM; j++)
+        for (k = 1; k < NDIM; k++) duidvj[j][k] = (j == k ? 1. : 0.) - betacon[j] / alpha * dgdvi[k];
+    for (i = 0; i < N5; i++)
+        for (j = RHO; j <= UU; j++) tempA[i][j] = Am[i][j];
+    for (i = 0; i < N5; i++)
+        for (j = 1; j < NDIM; j++)
+            for (l = 1; l <= 3; l++) tempA[RHO + i][U1 + j - 1] += Am[RHO + i][U1 + l - 1] * duidvj[l][j];
+    for (i = 0; i < N5; i++)
+        for (j = 0; j < N5; j++) Am[i][j] = tempA[i][j];
+    for (j = 0; j < N5; j++)
+        for (k = 0; k < N5; k++) Am[j][k] *= s->gdet;
+}
+
+/* LU_decompose / LU_substitution (utoprim_5d.c:672-874): Crout with implicit
+ * partial pivoting. */
+static int lu5_decompose(double A[N5][N5], int permute[N5])
+{
+    const double absmin = 1e-30;
+    double row_norm[N5], absmax, maxtemp;
+    int i, j, k, max_row = 0, n = N5;
+    for (i = 0; i < n; i++) {
+        absmax = 0.;
+        for (j = 0; j < n; j++) {
+            maxtemp = fabs(A[i][j]);
+            if (maxtemp > absmax) absmax = maxtemp;
+        }
+        if (absmax == 0.) return 1;
+        row_norm[i] = 1. / absmax;
+    }
+    for (j = 0; j < n; j++) {
+        for (i = 0; i < j; i++)
+            for (k = 0; k < i; k++) A[i][j] -= A[i][k] * A[k][j];
+        absmax = 0.0;
+        for (i = j; i < n; i++) {
+            for (k = 0; k < j; k++) A[i][j] -= A[i][k] * A[k][j];
+            maxtemp = fabs(A[i][j]) * row_norm[i];
+            if (maxtemp >= absmax) {
+                absmax = maxtemp;
+                max_row = i;
+            }
+        }
+        if (max_row != j) {
+            if ((j == (n - 2)) && (A[j][j + 1] == 0.)) {
+                max_row = j;
+            } else {
+                for (k = 0; k < n; k++) {
+                    maxtemp = A[j][k];
+                    A[j][k] = A[max_row][k];
+                    A[max_row][k] = maxtemp;
+                }
+                row_norm[max_row] = row_norm[j];
+            }
+        }
+        permute[j] = max_row;
+        if (A[j][j] == 0.) A[j][j] = absmin;
+        if (j != (n - 1)) {
+            maxtemp = 1. / A[j][j];
+            for (i = (j + 1); i < n; i++) A[i][j] *= maxtemp;
+        }
+    }
+    return 0;
+}
+static void lu5_substitution(double A[N5][N5], double B[N5], const int permute[N5])
+{
+    int i, j, n = N5;
+    double tmpvar;
+    for (i = 0; i < n; i++) {
+        tmpvar = B[permute[i]];
+        B[permute[i]] = B[i];
+        for (j = (i - 1); j >= 0; j--) tmpvar -= A[i][j] * B[j];
+        B[i] = tmpvar;
+    }
+    for (i = (n - 1); i >= 0; i--) {
+        for (j = (i + 1); j < n; j++) B[i] -= A[i][j] * B[j];
+        B[i] /= A[i][i];
+    }
+}
+
+/* func_5d (utoprim_5d.c:250-359), NORMMETHOD 1. */
+static void func_5d(const u5_t *s, const double prguess[], double dxm[], double *f, double *df)
+{
+    double pr[NPR], U_curr[NPR], Am[N5][N5], beta[N5], norm, residnorm, resid;
+    int i, j, k, permute[N5], numnormterms;
+    for (i = 0; i < N5; i++) permute[i] = 0;
+    for (i = 0; i < B1; i++) pr[i] = prguess[i];
+    for (i = B1; i <= B3; i++) pr[i] = s->U_target[i] / s->gdet;
+    primtoU_g5(s, pr, U_curr);
+    dudp_calc_g5(s, pr, Am);
+    norm = 0.0;
+    numnormterms = 0;
+    for (j = 0; j < N5; j++)
+        for (k = 0; k < N5; k++)
+            if (Am[j][k] > NUMEPS5) {
+                norm += fabs(Am[j][k]);
+                numnormterms++;
+            }
+    norm = ((double)(numnormterms)) / (norm);
+    for (j = 0; j < N5; j++)
+        for (k = 0; k < N5; k++) Am[j][k] *= (norm);
+    *df = 0.;
+    for (k = 0; k < N5; k++) {
+        resid = U_curr[k] - s->U_target[k];
+        beta[k] = -resid * (norm);
+        residnorm = 0.5 * (fabs(U_curr[k]) + fabs(s->U_target[k]));
+        residnorm = (residnorm == 0.) ? 1. : 1. / residnorm;
+        *df -= residnorm * resid * resid;
+    }
+    *f = -0.5 * (*df);
+    for (i = 0; i < N5; i++) dxm[i] = beta[i];
+    lu5_decompose(Am, permute);
+    lu5_substitution(Am, dxm, permute);
+}
+
+/* calc_errx_5d (utoprim_5d.c:378-400). */
+static void calc_errx_5d(const u5_t *s, double *W, double *vsq, const double x[])
+{
+    int id, jd;
+    double gsqtmp;
+    *vsq = 0.;
+    for (id = 1; id < 4; id++)
+        for (jd = 1; jd < 4; jd++) *vsq += s->gcov[id][jd] * x[U1 + id - 1] * x[U1 + jd - 1];
+    gsqtmp = 1. + *vsq;
+    *vsq /= gsqtmp;
+    if (s->eos)
+        *W = gsqtmp * (x[UU] + x[RHO] + eos5_p(s->eos, s->gam, x[RHO], x[UU]));
+    else
+        *W = gsqtmp * (s->gam * x[UU] + x[RHO]);
+}
+
+/* general_newton_raphson (utoprim_5d.c:890-983). */
+static int gnr_5d(const u5_t *s, double x[N5], int *n_iter_out)
+{
+    double f = 1., df = 1., dx[N5], errx = 1., W = 0., W_old, vsq = 0.;
+    int n_iter = 0, id, i_extra = 0, doing_extra = 0, keep = 1;
+    while (keep) {
+        func_5d(s, x, dx, &f, &df);
+        for (id = 0; id < N5; id++) x[id] += dx[id];
+        W_old = W;
+        calc_errx_5d(s, &W, &vsq, x);
+        errx = (W == 0.) ? fabs(W - W_old) : fabs((W - W_old) / W);
+        if ((fabs(errx) <= NEWT_TOL) && (f <= TOL_RESID5) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0))
+            doing_extra = 1;
+        if (doing_extra == 1) i_extra++;
+        if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+            (n_iter >= (MAX_NEWT_ITER - 1)))
+            keep = 0;
+        n_iter++;
+    }
+    if (n_iter_out) *n_iter_out = n_iter;
+    if ((isfinite(f) == 0) || (isfinite(df) == 0)) return 2;
+    if ((fabs(errx) > MIN_NEWT_TOL) || (f > TOL_RESID5)) return 1;
+    return 0;
+}
+
+/* Utoprim_5d (utoprim_5d.c:179-239): U is HARM's conserved vector (x gdet),
+ * prim the guess in and the result out (B always, the rest only on success). */
+int orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8], double gam,
+                   const orc_eos *eos, int *n_iter)
+{
+    u5_t s;
+    double prim_tmp[NPR], gamma_tmp;
+    int k, ret;
+    memcpy(s.gcov, gcov, sizeof(s.gcov));
+    memcpy(s.gcon, gcon, sizeof(s.gcon));
+    s.gdet = gdet;
+    s.gam = gam;
+    s.eos = eos;
+    for (k = 0; k < NPR; k++) s.U_target[k] = U[k];
+    for (k = B1; k <= B3; k++) prim[k] = U[k] / gdet;
+    for (k = 0; k < NPR; k++) prim_tmp[k] = prim[k];
+    ret = gnr_5d(&s, prim_tmp, n_iter);
+    if (ret) return 100 * ret + 1;
+    if ((prim_tmp[RHO] < 0.0) || (prim_tmp[UU] < 0.0)) return 5;
+    if (gamma_calc_g5(prim_tmp, s.gcov, &gamma_tmp) == 1) return 4;
+    for (k = 0; k < NPR; k++) prim[k] = prim_tmp[k];
+    return 0;
+}
+
+/* primtoU_g with the solver's EOS (for round trips and tests). */
+void orc_primtoU_eos(const double prim[8], const double *gcov, const double *gcon, double gdet, double gam,
+                     const orc_eos *eos, double U[8])
+{
+    u5_t s;
+    memcpy(s.gcov, gcov, sizeof(s.gcov));
+    memcpy(s.gcon, gcon, sizeof(s.gcon));
+    s.gdet = gdet;
+    s.gam = gam;
+    s.eos = eos;
+    primtoU_g5(&s, prim, U);
+}
+
+/* dudp_calc_g's Jacobian (5x5, row-major) for tests against finite differences. */
+void orc_dudp_5d(const double prim[8], const double *gcov, const double *gcon, double gdet, double gam,
+                 const orc_eos *eos, double *Am)
+{
+    u5_t s;
+    double A[N5][N5];
+    memcpy(s.gcov, gcov, sizeof(s.gcov));
+    memcpy(s.gcon, gcon, sizeof(s.gcon));
+    s.gdet = gdet;
+    s.gam = gam;
+    s.eos = eos;
+    dudp_calc_g5(&s, prim, A);
+    memcpy(Am, A, sizeof(A));
 }

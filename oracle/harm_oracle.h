@@ -106,6 +106,24 @@ int  orc_utoprim_2d_seeded(const double U[8], const double *gcov, const double *
 int  orc_utoprim_2d_learned(const double U[8], const double *gcov, const double *gcon,
                             double gdet, double prim[8], double gam, double W0, double vsq0,
                             int vsq_from_W, int *n_iter, int *k_conv, int *fell_back);
+/* Tabulated EOS: rho[nr], T[nt] increasing; e, p [nr][nt] (code units);
+ * gam is the Gamma-law fallback outside the table. */
+typedef struct orc_eos {
+    int nr, nt;
+    const double *rho, *T, *e, *p;
+    double gam;
+} orc_eos;
+/* p(rho, u) by piecewise-bilinear interpolation inverted along T, its
+ * derivatives at fixed u / fixed rho, and T; returns 1 on the Gamma-law branch. */
+int  orc_eos_table_p(const orc_eos *t, double rho, double u, double *p, double *dpdrho,
+                     double *dpdu, double *T);
+/* Utoprim_5d (utoprim_5d.c:179-239); eos NULL = Gamma-law. */
+int  orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, double gdet,
+                    double prim[8], double gam, const orc_eos *eos, int *n_iter);
+void orc_primtoU_eos(const double prim[8], const double *gcov, const double *gcon, double gdet,
+                     double gam, const orc_eos *eos, double U[8]);
+void orc_dudp_5d(const double prim[8], const double *gcov, const double *gcon, double gdet,
+                 double gam, const orc_eos *eos, double *Am /* 5x5 */);
 /* Geometry of one point in code coordinates X (X[0] ignored). */
 void orc_gcov_func(const orc_params *p, const double X[4], double *gcov);
 void orc_dxdxp_func(const orc_params *p, const double X[4], double *dxdxp);

@@ -32,6 +32,19 @@ def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", ORACLE_DIR, "all"], check=True)
 
 
+class OrcEos(ctypes.Structure):
+    _fields_ = [("nr", c_int), ("nt", c_int), ("rho", _dp), ("T", _dp), ("e", _dp), ("p", _dp), ("gam", c_double)]
+
+
+def orc_eos(tab: dict, gam: float) -> OrcEos:
+    """Oracle view of a host table dict (rho, T, e, p); keeps the arrays alive."""
+    arrs = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in tab.items()}
+    d = OrcEos(nr=arrs["rho"].size, nt=arrs["T"].size, rho=_ptr(arrs["rho"]), T=_ptr(arrs["T"]), e=_ptr(arrs["e"]),
+               p=_ptr(arrs["p"]), gam=gam)
+    d._keep = arrs
+    return d
+
+
 _lib = None
 
 
@@ -80,6 +93,10 @@ def lib():
         L.orc_kiss_uniform.argtypes = [c_uint32] * 4
         L.orc_face_flux.argtypes = [_dp, _dp, _dp, _dp, c_double, c_int, c_double, c_double, c_double, _dp, _dp]
         L.orc_vchar.argtypes = [_dp, _dp, _dp, c_double, c_int, c_double, _dp, _dp]
+        L.orc_eos_table_p.argtypes = [POINTER(OrcEos), c_double, c_double, _dp, _dp, _dp, _dp]
+        L.orc_utoprim_5d.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, POINTER(OrcEos), POINTER(c_int)]
+        L.orc_primtoU_eos.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
+        L.orc_dudp_5d.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
         _lib = L
     return _lib
 
