@@ -205,6 +205,7 @@ def lib() -> ctypes.CDLL:
                                 c_void_p]
     L.harm_utoprim5d_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_size_t, c_double, POINTER(EosTable), c_void_p]
+    L.harm_set_eos.argtypes = [c_void_p, POINTER(EosTable), c_int]
     L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
     L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
     L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]
@@ -325,6 +326,12 @@ class Harm:
 
     def debug_fixup_utoprim(self) -> None:
         _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim")
+
+    def set_eos(self, tab: "DeviceEos | None", ref_derivs: bool = False) -> None:
+        """Tabulated EOS for the step (None: back to the Gamma-law EOS)."""
+        _check(self.L.harm_set_eos(self.h, ctypes.byref(tab.desc) if tab is not None else None, int(ref_derivs)),
+               "harm_set_eos")
+        self._eos_ref = tab
 
     def set_guess(self, mode: int, mlp: "Mlp | None" = None) -> None:
         """Newton start of the inversion: 0 previous-step primitives (the

@@ -59,6 +59,8 @@ struct DGrid {
     const double *katm;
     const double *cblk;     // column-major CENT block, CENT_STRIDE per column
     const double *fblk[3];  // column-major FACE1/FACE2/FACE3 blocks, FACE_STRIDE per column
+    EosTab tab;             // tabulated EOS (used by the TAB kernel variants)
+    int ref_derivs;         // tabulated EOS: zero dp/drho, dp/du as the reference's
 };
 
 __device__ __forceinline__ size_t zidx(const DGrid &g, int i, int j, int h)
@@ -208,7 +210,7 @@ __global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__
 // XCD-contiguous logical blocks, so the four zones each face reads are mostly
 // L2 hits; 4 waves per SIMD (128 VGPRs) measured faster than 3 with a few
 // spilled registers.
-template <int DIR, bool UNI>
+template <int DIR, bool UNI, bool TAB>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart, int fullstep,
        unsigned long long *__restrict__ cnt)
@@ -263,7 +265,10 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
             else
                 load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
             double Fz[NPR], ctop;
-            face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
+            if constexpr (TAB)
+                face_flux(p_l, p_r, geom, DIR, EosTable{g.tab, g.ref_derivs}, g.hllf, g.laxf, Fz, ctop, fatal);
+            else
+                face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
             if (DIR == 1) {
                 // coord.c:310-320: no inflow of mass through the radial boundaries
                 if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
@@ -433,7 +438,7 @@ enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2 };
 // MODE UPD_CAPTURE also stores training pairs of sampled zones; UPD_LEARNED
 // seeds the Newton solve from the network (wave-level BF16 MFMA forward) and
 // falls back to the pi guess where the seeded solve fails.
-template <bool UNI, int MODE>
+template <bool UNI, int MODE, bool TAB>
 __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb,
                                                 double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                 const double *__restrict__ S, const double *__restrict__ tstate,
@@ -471,14 +476,25 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
 #pragma unroll
             for (int k = 0; k < NPR; k++) pbz[k] = pb[k * nz + z];
             State q;
-            if (UNI)
-                source(pbz, geom, blk + CENT_CONN, 1, 0, g.gam, dU, q, fatal);
-            else
-                source(pbz, geom, g.conn, g.ncol, col, g.gam, dU, q, fatal);
+            if constexpr (TAB) {
+                const EosTable eos{g.tab, g.ref_derivs};
+                if (UNI)
+                    source(pbz, geom, blk + CENT_CONN, 1, 0, eos, dU, q, fatal);
+                else
+                    source(pbz, geom, g.conn, g.ncol, col, eos, dU, q, fatal);
+            } else {
+                if (UNI)
+                    source(pbz, geom, blk + CENT_CONN, 1, 0, g.gam, dU, q, fatal);
+                else
+                    source(pbz, geom, g.conn, g.ncol, col, g.gam, dU, q, fatal);
+            }
 #pragma unroll
             for (int k = 0; k < NPR; k++) piz[k] = pi_is_pb ? pbz[k] : pi[k * nz + z];
             if (!pi_is_pb) get_state(piz, geom, q, fatal);
-            primtoflux(piz, q, 0, geom, g.gam, U);
+            if constexpr (TAB)
+                primtoflux(piz, q, 0, geom, EosTable{g.tab, g.ref_derivs}, U);
+            else
+                primtoflux(piz, q, 0, geom, g.gam, U);
         }
 #pragma unroll
         for (int k = 0; k < NPR; k++) U[k] += Dt * (S[k * na + idx] + dU[k]);
@@ -487,7 +503,16 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
 #pragma unroll
     for (int k = 0; k < NPR; k++) pr[k] = piz[k];
     int niter = 0, kconv = 0, ret = 0;
-    if constexpr (MODE == UPD_LEARNED) {
+    if constexpr (TAB) {
+        // advance() with COOL_EOS (step_ch.c:265-268): Utoprim_5d, no Font fix
+        if (active) {
+            Eos5 e5;
+            e5.tab = &g.tab;
+            e5.gam = g.gam;
+            ret = utoprim_5d(U, geom, e5, pr, niter);
+            kconv = niter;
+        }
+    } else if constexpr (MODE == UPD_LEARNED) {
         double f[MLP_NIN];
         const bool ok = active && mlp_features_U(U, geom, f);
         float x[MLP_NIN];
@@ -535,7 +560,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         bool deferred = false;
         if (flag) {
             c_f2d = 1;
-            if (g.font_fix) {
+            if (!TAB && g.font_fix) {
                 const int slot = atomicAdd(fl.count, 1);
                 if (slot < fl.cap) {
                     fl.zone[slot] = (int)z;
@@ -1455,6 +1480,7 @@ struct harm_ctx {
     FontList font;
     bool uniform;  // N3 % 64 == 0: a wave's lanes share one (i,j) column
     int fix_cap;
+    int eos_kind;  // 0 Gamma-law, 1 tabulated (the table's device arrays belong to the caller)
     // learned Newton seed and training-pair capture
     int guess_mode;
     const harm_mlp *mlp;
@@ -1522,34 +1548,43 @@ int ev_collect(harm_ctx *c)
         if (rc_) return rc_;                                   \
     } while (0)
 
+template <int DIR, bool UNI, bool TAB>
+void flux_kernel(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullstep)
+{
+    hipLaunchKernelGGL((k_flux<DIR, UNI, TAB>), grid, dim3(TPB), 0, c->stream, c->g, pb, F, c->dtpart, fullstep,
+                       c->cnt);
+}
+
+template <int DIR>
+void flux_kernel_any(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullstep)
+{
+    const bool tab = c->eos_kind == 1;
+    if (c->uniform) {
+        if (tab)
+            flux_kernel<DIR, true, true>(c, grid, pb, F, fullstep);
+        else
+            flux_kernel<DIR, true, false>(c, grid, pb, F, fullstep);
+    } else {
+        if (tab)
+            flux_kernel<DIR, false, true>(c, grid, pb, F, fullstep);
+        else
+            flux_kernel<DIR, false, false>(c, grid, pb, F, fullstep);
+    }
+}
+
 int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
 {
     const DGrid &g = c->g;
     const int i0 = (dir == 1) ? g.is : g.is - 1;
     const int j0 = (dir == 2) ? g.js : g.js - 1;
     const size_t total = (size_t)(g.ie + 2 - i0) * (g.je + 2 - j0) * g.N3;
-    const dim3 grid(nblocks8(total)), blk(TPB);
-    if (c->uniform) {
-        if (dir == 1)
-            LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, true>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->dtpart, fullstep, c->cnt));
-        else if (dir == 2)
-            LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL((k_flux<2, true>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->dtpart, fullstep, c->cnt));
-        else
-            LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL((k_flux<3, true>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->dtpart, fullstep, c->cnt));
-    } else {
-        if (dir == 1)
-            LAUNCH(HARM_K_FLUX1, hipLaunchKernelGGL((k_flux<1, false>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->dtpart, fullstep, c->cnt));
-        else if (dir == 2)
-            LAUNCH(HARM_K_FLUX2, hipLaunchKernelGGL((k_flux<2, false>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->dtpart, fullstep, c->cnt));
-        else
-            LAUNCH(HARM_K_FLUX3, hipLaunchKernelGGL((k_flux<3, false>), grid, blk, 0, c->stream, g, pb, F,
-                                                    c->dtpart, fullstep, c->cnt));
-    }
+    const dim3 grid(nblocks8(total));
+    if (dir == 1)
+        LAUNCH(HARM_K_FLUX1, flux_kernel_any<1>(c, grid, pb, F, fullstep));
+    else if (dir == 2)
+        LAUNCH(HARM_K_FLUX2, flux_kernel_any<2>(c, grid, pb, F, fullstep));
+    else
+        LAUNCH(HARM_K_FLUX3, flux_kernel_any<3>(c, grid, pb, F, fullstep));
     if (fullstep) {
         hipLaunchKernelGGL(k_min_partials, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->dtpart, (size_t)grid.x,
                            c->ndt_slot + (dir - 1));
@@ -1604,7 +1639,9 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     UpdateExtra ex;
     std::memset(&ex, 0, sizeof(ex));
     int mode = UPD_PLAIN;
-    if (c->guess_mode == 1 && c->mlp) {
+    if (c->eos_kind == 1) {
+        // the tabulated-EOS path has its own (5D) solver: no learned seed, no capture
+    } else if (c->guess_mode == 1 && c->mlp) {
         mode = UPD_LEARNED;
         ex.mlp = harm_mlp_view(c->mlp);
     } else if (c->cap_feat && c->cap_count < c->cap) {
@@ -1622,11 +1659,20 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
         c->cap_phase = (int)((ph + 7919) % (size_t)c->cap_stride);
     }
 #define UPD_LAUNCH(U_, M_)                                                                                          \
-    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, pi, \
-                                             pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cpart, ex))
+    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_, false>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, \
+                                             pi, pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cpart, ex))
     LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
                                           c->e1, c->e2, c->e3, c->S));
-    if (c->uniform) {
+    if (c->eos_kind == 1) {
+        if (c->uniform)
+            LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<true, UPD_PLAIN, true>), dim3(nblocks(na)), dim3(TPB), 0,
+                                                     c->stream, g, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac, same,
+                                                     c->font, c->cpart, ex));
+        else
+            LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<false, UPD_PLAIN, true>), dim3(nblocks(na)), dim3(TPB),
+                                                     0, c->stream, g, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac,
+                                                     same, c->font, c->cpart, ex));
+    } else if (c->uniform) {
         if (mode == UPD_PLAIN)
             UPD_LAUNCH(true, UPD_PLAIN);
         else if (mode == UPD_CAPTURE)
@@ -2148,6 +2194,26 @@ int harm_get_geometry(harm_ctx *c, double *gcov, double *gcon, double *gdet, dou
         if (gdet) std::memcpy(gdet + (size_t)l * ncol, T.data() + 32 * ncol, ncol * sizeof(double));
     }
     if (conn) std::memcpy(conn, c->tab.conn.data(), 64 * ncol * sizeof(double));
+    return 0;
+}
+
+static int eos_tab_of(const harm_eos_table *t, EosTab &e);
+
+int harm_set_eos(harm_ctx *c, const harm_eos_table *tab, int ref_derivs)
+{
+    if (!c) return set_err("null context");
+    if (!tab) {
+        c->eos_kind = 0;
+        std::memset(&c->g.tab, 0, sizeof(c->g.tab));
+        return 0;
+    }
+    EosTab e;
+    int rc;
+    if ((rc = eos_tab_of(tab, e))) return rc;
+    if (!(e.gam > 1.)) return set_err("EOS table: Gamma-law fallback index must be > 1", 4);
+    c->g.tab = e;
+    c->g.ref_derivs = ref_derivs ? 1 : 0;
+    c->eos_kind = 1;
     return 0;
 }
 

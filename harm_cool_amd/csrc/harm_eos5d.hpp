@@ -494,3 +494,38 @@ __device__ inline int utoprim_5d(const double *U, const Geom &g, const Eos5 &eos
 }
 
 }  // namespace harm
+
+namespace harm {
+
+// The tabulated EOS as a policy of the physics templates (harm_phys.hpp):
+// mhd_calc_CoolEOS's pressure (phys.c:122-141) and the tabulated branch of
+// vchar (phys.c:359-378): cs2 = (dp/drho + p/rho^2 dp/de) / h with e = u/rho,
+// dp/de = rho dp/du, h = 1 + e + p/rho.  ref_derivs reproduces the reference's
+// derivatives, whose finite-difference step (u hh / U_UNIT, cooleos.c:667-741)
+// vanishes next to u in FP64 so that both are exactly 0.
+struct EosTable {
+    static constexpr bool TAB = true;
+    EosTab t;
+    int ref_derivs;
+    __device__ double pres(double rho, double u) const
+    {
+        double p, a, b;
+        eos_table_p(t, rho, u, p, a, b);
+        return p;
+    }
+    __device__ double cs2(double rho, double u, double p) const
+    {
+        double pp, dpdrho, dpdu;
+        eos_table_p(t, rho, u, pp, dpdrho, dpdu);
+        if (ref_derivs) {
+            dpdrho = 0.;
+            dpdu = 0.;
+        }
+        const double e = u / rho;
+        const double enthalpy = 1.0 + e + p / rho;
+        const double dpde = dpdu * rho;
+        return (dpdrho + p / rho / rho * dpde) / enthalpy;
+    }
+};
+
+}  // namespace harm

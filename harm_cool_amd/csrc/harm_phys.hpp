@@ -247,11 +247,18 @@ HD void get_state(const double *pr, const Geom &geom, State &q, int &fatal)
     lower(q.bcon, geom, q.bcov);
 }
 
-// phys.c:143-161
-HD void mhd_calc(const double *pr, int dir, const State &q, double gam, double *mhd)
+// The Gamma-law EOS policy of the physics templates below (p = (gam - 1) u,
+// phys.c:150); the tabulated policy lives with the table (harm_eos5d.hpp).
+struct EosGamma {
+    static constexpr bool TAB = false;
+    double gam;
+    HD double pres(double, double u) const { return (gam - 1.) * u; }
+};
+
+// phys.c:122-161: one row of the MHD stress tensor for a given pressure P.
+HD void mhd_calc_P(const double *pr, int dir, const State &q, double P, double *mhd)
 {
     double r = pr[RHO], u = pr[UU];
-    double P = (gam - 1.) * u;
     double w = P + r + u;
     double bsq = dot4(q.bcon, q.bcov);
     double eta = w + bsq;
@@ -261,12 +268,18 @@ HD void mhd_calc(const double *pr, int dir, const State &q, double gam, double *
         mhd[j] = eta * q.ucon[dir] * q.ucov[j] + ptot * ((dir == j) ? 1. : 0.) - q.bcon[dir] * q.bcov[j];
 }
 
-// phys.c:59-96
-HD void primtoflux(const double *pr, const State &q, int dir, const Geom &geom, double gam, double *flux)
+// phys.c:143-161
+HD void mhd_calc(const double *pr, int dir, const State &q, double gam, double *mhd)
+{
+    mhd_calc_P(pr, dir, q, (gam - 1.) * pr[UU], mhd);
+}
+
+// phys.c:59-96 for a given pressure P
+HD void primtoflux_P(const double *pr, const State &q, int dir, const Geom &geom, double P, double *flux)
 {
     double mhd[4];
     flux[RHO] = pr[RHO] * q.ucon[dir];
-    mhd_calc(pr, dir, q, gam, mhd);
+    mhd_calc_P(pr, dir, q, P, mhd);
     flux[UU] = mhd[0] + flux[RHO];
     flux[U1] = mhd[1];
     flux[U2] = mhd[2];
@@ -278,8 +291,23 @@ HD void primtoflux(const double *pr, const State &q, int dir, const Geom &geom, 
     for (int k = 0; k < NPR; k++) flux[k] *= geom.g;
 }
 
-// phys.c:337-476 (Gamma-law branch); returns how often the reference would fail()
-HD int vchar(const double *pr, const State &q, const Geom &geom, int js, double gam, double *vmax, double *vmin)
+// phys.c:59-96
+template <class E>
+HD void primtoflux(const double *pr, const State &q, int dir, const Geom &geom, const E &eos, double *flux)
+{
+    primtoflux_P(pr, q, dir, geom, eos.pres(pr[RHO], pr[UU]), flux);
+}
+HD void primtoflux(const double *pr, const State &q, int dir, const Geom &geom, double gam, double *flux)
+{
+    primtoflux_P(pr, q, dir, geom, (gam - 1.) * pr[UU], flux);
+}
+
+// phys.c:337-476; returns how often the reference would fail().  Gamma-law:
+// cs2 = gam (gam - 1) u / (rho + gam u) and fail() outside [0, 1].  Tabulated
+// (phys.c:359-378, 398-406): cs2 from the EOS derivatives (E::cs2), clipped
+// to [SMALL, 1] without fail().
+template <class E>
+HD int vchar(const double *pr, const State &q, const Geom &geom, int js, const E &eos, double *vmax, double *vmin)
 {
     double Acov[4] = {0., 0., 0., 0.}, Bcov[4] = {1., 0., 0., 0.}, Acon[4], Bcon[4];
     int nfail = 0;
@@ -288,18 +316,31 @@ HD int vchar(const double *pr, const State &q, const Geom &geom, int js, double 
     raise(Bcov, geom, Bcon);
     double bsq = dot4(q.bcon, q.bcov);
     double rho = pr[RHO], u = pr[UU];
-    double EF = rho + gam * u;
-    double EE = bsq + EF;
-    double va2 = bsq / EE;
-    double cs2 = gam * (gam - 1.) * u / EF;
-    double cms2 = cs2 + va2 - cs2 * va2;
-    if (cms2 < 0.) {
-        nfail++;
-        cms2 = SMALL;
-    }
-    if (cms2 > 1.) {
-        nfail++;
-        cms2 = 1.;
+    double cms2;
+    if constexpr (E::TAB) {
+        const double p = eos.pres(rho, u);
+        double EF = rho + u + p;
+        double EE = bsq + EF;
+        double va2 = bsq / EE;
+        double cs2 = eos.cs2(rho, u, p);
+        cms2 = cs2 + va2 - cs2 * va2;
+        if (cms2 < SMALL) cms2 = SMALL;
+        if (cms2 > 1.) cms2 = 1.;
+    } else {
+        const double gam = eos.gam;
+        double EF = rho + gam * u;
+        double EE = bsq + EF;
+        double va2 = bsq / EE;
+        double cs2 = gam * (gam - 1.) * u / EF;
+        cms2 = cs2 + va2 - cs2 * va2;
+        if (cms2 < 0.) {
+            nfail++;
+            cms2 = SMALL;
+        }
+        if (cms2 > 1.) {
+            nfail++;
+            cms2 = 1.;
+        }
     }
     double Asq = dot4(Acon, Acov), Bsq = dot4(Bcon, Bcov);
     double Au = dot4(Acov, q.ucon), Bu = dot4(Bcov, q.ucon), AB = dot4(Acon, Bcov);
@@ -325,6 +366,10 @@ HD int vchar(const double *pr, const State &q, const Geom &geom, int js, double 
         *vmin = vp;
     }
     return nfail;
+}
+HD int vchar(const double *pr, const State &q, const Geom &geom, int js, double gam, double *vmax, double *vmin)
+{
+    return vchar(pr, q, geom, js, EosGamma{gam}, vmax, vmin);
 }
 
 // interp.c:49-93
@@ -356,7 +401,8 @@ HD double slope_lim(int lim, double y1, double y2, double y3)
 }
 
 // One face of step_ch.c:429-475: reconstructed states -> numerical flux and ctop.
-HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, double gam, double hllf,
+template <class E>
+HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, const E &eos, double hllf,
                   double laxf, double *F, double &ctop, int &fatal)
 {
     State sl, sr;
@@ -364,12 +410,13 @@ HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int di
     double cmax_l, cmin_l, cmax_r, cmin_r;
     get_state(p_l, geom, sl, fatal);
     get_state(p_r, geom, sr, fatal);
-    primtoflux(p_l, sl, dir, geom, gam, F_l);
-    primtoflux(p_l, sl, 0, geom, gam, U_l);
-    fatal += vchar(p_l, sl, geom, dir, gam, &cmax_l, &cmin_l);
-    primtoflux(p_r, sr, dir, geom, gam, F_r);
-    primtoflux(p_r, sr, 0, geom, gam, U_r);
-    fatal += vchar(p_r, sr, geom, dir, gam, &cmax_r, &cmin_r);
+    const double P_l = eos.pres(p_l[RHO], p_l[UU]), P_r = eos.pres(p_r[RHO], p_r[UU]);
+    primtoflux_P(p_l, sl, dir, geom, P_l, F_l);
+    primtoflux_P(p_l, sl, 0, geom, P_l, U_l);
+    fatal += vchar(p_l, sl, geom, dir, eos, &cmax_l, &cmin_l);
+    primtoflux_P(p_r, sr, dir, geom, P_r, F_r);
+    primtoflux_P(p_r, sr, 0, geom, P_r, U_r);
+    fatal += vchar(p_r, sr, geom, dir, eos, &cmax_r, &cmin_r);
     double cmax = fabs(dmax(dmax(0., cmax_l), cmax_r));
     double cmin = fabs(dmax(dmax(0., -cmin_l), -cmin_r));
     ctop = dmax(cmax, cmin);
@@ -377,6 +424,11 @@ HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int di
     for (int k = 0; k < NPR; k++)
         F[k] = hllf * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin + SMALL)) +
                laxf * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+}
+HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, double gam, double hllf,
+                  double laxf, double *F, double &ctop, int &fatal)
+{
+    face_flux(p_l, p_r, geom, dir, EosGamma{gam}, hllf, laxf, F, ctop, fatal);
 }
 
 // phys.c:165-207: dU = g * sum_jk T^j_k Gamma^k_{mu j}; conn is [64][ncol] (k,mu,j order)
@@ -386,16 +438,18 @@ HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int di
 // conn[((k*4+mu)*4+j)*cstride + col] = Gamma^k_{mu j}: cstride = ncol for the
 // variable-major table, 1 (with col = 0 and conn at the column block) for the
 // column-major one.
+template <class E>
 HD void source(const double *ph, const Geom &geom, const double *__restrict__ conn, size_t ncol, size_t col,
-               double gam, double *dU, State &q, int &fatal)
+               const E &eos, double *dU, State &q, int &fatal)
 {
     get_state(ph, geom, q, fatal);
+    const double P = eos.pres(ph[RHO], ph[UU]);
 #pragma unroll
     for (int k = 0; k < NPR; k++) dU[k] = 0.;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         double mhd[4];
-        mhd_calc(ph, j, q, gam, mhd);
+        mhd_calc_P(ph, j, q, P, mhd);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             dU[UU] += mhd[k] * ldc(conn, (size_t)((k * 4 + 0) * 4 + j) * ncol + col);
@@ -406,6 +460,11 @@ HD void source(const double *ph, const Geom &geom, const double *__restrict__ co
     }
 #pragma unroll
     for (int k = 0; k < NPR; k++) dU[k] *= geom.g;
+}
+HD void source(const double *ph, const Geom &geom, const double *__restrict__ conn, size_t ncol, size_t col,
+               double gam, double *dU, State &q, int &fatal)
+{
+    source(ph, geom, conn, ncol, col, EosGamma{gam}, dU, q, fatal);
 }
 
 // ---------------------------------------------------------------------------

@@ -258,6 +258,14 @@ typedef struct harm_eos_table {
 /* p, dp/drho (fixed u), dp/du (fixed rho) for n (rho, u) pairs (device arrays). */
 int harm_eos_eval(const harm_eos_table *tab, const double *rho, const double *u, double *p, double *dpdrho,
                   double *dpdu, size_t n, void *stream);
+/* Run the step with the tabulated EOS (the reference's COOL_EOS build,
+ * decs.h:110): pressures in fluxes, sources and U from the table, the
+ * tabulated branch of vchar (phys.c:359-378), Utoprim_5d in place of
+ * Utoprim_2d and no Font fix (step_ch.c:265-300).  ref_derivs = 1 zeroes
+ * dp/drho and dp/du in vchar as the reference's vanishing finite-difference
+ * step does (cooleos.c:667-741).  tab NULL returns to the Gamma-law EOS.  The
+ * table's device arrays must outlive their use by the context. */
+int harm_set_eos(harm_ctx *c, const harm_eos_table *tab, int ref_derivs);
 /* Batched Utoprim_5d (utoprim_5d.c:179-239; decs.h:610-611): SoA device arrays
  * as harm_utoprim2d_batch; tab NULL selects the Gamma-law EOS with gam. */
 int harm_utoprim5d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet, double *prim,

@@ -124,7 +124,24 @@ struct orc_ctx {
     double *Katm;
     int64_t cnt[ORC_NCOUNTERS];
     double mass_in, mass_out;
+    const orc_eos *eos; /* tabulated EOS of the step (NULL: Gamma-law) */
+    int eos_refd;
 };
+
+/* The EOS of the step being computed: set from the context on entry to
+ * orc_step / orc_debug_fluxes and read by the physics functions (one step at
+ * a time per process; OpenMP threads only read it). */
+static const orc_eos *g_step_eos = NULL;
+static int g_step_refd = 0;
+
+/* mhd_calc_CoolEOS's pressure (phys.c:122-141) or the Gamma-law one. */
+static double step_pres(double gam, double r, double u)
+{
+    double p;
+    if (!g_step_eos) return (gam - 1.) * u;
+    orc_eos_table_p(g_step_eos, r, u, &p, NULL, NULL, NULL);
+    return p;
+}
 
 /* ------------------------------------------------------------------------ */
 /* indexing                                                                  */
@@ -555,7 +572,7 @@ static void mhd_calc(const double *pr, int dir, const state_t *q, double gam, do
     double r, u, P, w, bsq, eta, ptot;
     r = pr[RHO];
     u = pr[UU];
-    P = (gam - 1.) * u;
+    P = step_pres(gam, r, u);
     w = P + r + u;
     bsq = dot(q->bcon, q->bcov);
     eta = w + bsq;
@@ -624,18 +641,35 @@ static int vchar(const double *pr, const state_t *q, const geom_t *geom, int js,
     bsq = dot(q->bcon, q->bcov);
     rho = pr[RHO];
     u = pr[UU];
-    EF = rho + gam * u;
-    EE = bsq + EF;
-    va2 = bsq / EE;
-    cs2 = gam * (gam - 1.) * u / EF;
-    cms2 = cs2 + va2 - cs2 * va2;
-    if (cms2 < 0.) {
-        nfail++;
-        cms2 = SMALL;
-    }
-    if (cms2 > 1.) {
-        nfail++;
-        cms2 = 1.;
+    if (g_step_eos) {
+        /* phys.c:359-378, 398-406 */
+        double p = step_pres(gam, rho, u), pp, dpdrho, dpdu, e, enthalpy, dpde;
+        EF = rho + u + p;
+        EE = bsq + EF;
+        va2 = bsq / EE;
+        orc_eos_table_p(g_step_eos, rho, u, &pp, &dpdrho, &dpdu, NULL);
+        if (g_step_refd) dpdrho = dpdu = 0.;
+        e = u / rho;
+        enthalpy = 1.0 + e + p / rho;
+        dpde = dpdu * rho;
+        cs2 = (dpdrho + p / rho / rho * dpde) / enthalpy;
+        cms2 = cs2 + va2 - cs2 * va2;
+        if (cms2 < SMALL) cms2 = SMALL;
+        if (cms2 > 1.) cms2 = 1.;
+    } else {
+        EF = rho + gam * u;
+        EE = bsq + EF;
+        va2 = bsq / EE;
+        cs2 = gam * (gam - 1.) * u / EF;
+        cms2 = cs2 + va2 - cs2 * va2;
+        if (cms2 < 0.) {
+            nfail++;
+            cms2 = SMALL;
+        }
+        if (cms2 > 1.) {
+            nfail++;
+            cms2 = 1.;
+        }
     }
     Asq = dot(Acon, Acov);
     Bsq = dot(Bcon, Bcov);
@@ -1873,6 +1907,18 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
                                   (PV(o, o->F2, h, i, j + 1)[k] - PV(o, o->F2, h, i, j)[k]) / o->dx[TH] + dU[k]);
             }
             double *pz = PV(o, pf, h, i, j);
+            if (g_step_eos) {
+                /* step_ch.c:265-268 with COOL_EOS: Utoprim_5d, no Font fix */
+                ret = orc_utoprim_5d(U, &geom.gcov[0][0], &geom.gcon[0][0], geom.g, pz, o->P.gam, g_step_eos,
+                                     &niter);
+                kconv = niter;
+                lc[ORC_C2P_CALLS]++;
+                lc[ORC_NEWTON_ITERS] += niter;
+                lc[ORC_NEWTON_KCONV] += kconv;
+                if (ret) lc[ORC_FAIL_2D]++;
+                o->pflag[zi(o, h, i, j)] = (int8_t)ret;
+                continue;
+            }
             ret = Utoprim_2d(U, geom.gcov, geom.gcon, geom.g, pz, o->P.gam, &niter, &kconv);
             lc[ORC_C2P_CALLS]++;
             lc[ORC_NEWTON_ITERS] += niter;
@@ -1903,9 +1949,17 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
 }
 
 /* step_ch.c:90-140 */
+void orc_set_eos(orc_ctx *o, const orc_eos *eos, int ref_derivs)
+{
+    o->eos = eos;
+    o->eos_refd = ref_derivs ? 1 : 0;
+}
+
 int orc_step(orc_ctx *o)
 {
     double ndt;
+    g_step_eos = o->eos;
+    g_step_refd = o->eos_refd;
     int64_t fatal0 = o->cnt[ORC_FATAL];
     ndt = advance(o, o->p, o->p, 0.5 * o->dt, o->ph, 0);
     fixup(o, o->ph, o->cnt);
@@ -1923,6 +1977,7 @@ int orc_step(orc_ctx *o)
         o->mass_in += out[2] * o->dt;
         o->mass_out += out[3] * o->dt;
     }
+    g_step_eos = NULL;
     if (o->dt < 1.e-9) return 11;
     o->t += o->dt;
     if (ndt > SAFE * o->dt) ndt = SAFE * o->dt;
@@ -2274,6 +2329,8 @@ void orc_diag(const orc_ctx *o, double *out)
 int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F3, double *ndt)
 {
     int64_t cnt[ORC_NCOUNTERS];
+    g_step_eos = o->eos;
+    g_step_refd = o->eos_refd;
     memset(cnt, 0, sizeof(cnt));
     ndt[0] = fluxcalc(o, o->p, o->F1, 1, fullstep, cnt);
     ndt[1] = fluxcalc(o, o->p, o->F2, 2, fullstep, cnt);
@@ -2291,6 +2348,7 @@ int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F
                     for (int k = 0; k < NPR; k++)
                         dst[d][(((size_t)k * ni + i) * nj + j) * nh + h] = PV(o, src[d], (int)h, (int)i, (int)j)[k];
     }
+    g_step_eos = NULL;
     return 0;
 }
 

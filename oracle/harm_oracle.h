@@ -117,6 +117,8 @@ typedef struct orc_eos {
  * derivatives at fixed u / fixed rho, and T; returns 1 on the Gamma-law branch. */
 int  orc_eos_table_p(const orc_eos *t, double rho, double u, double *p, double *dpdrho,
                      double *dpdu, double *T);
+/* Step with the tabulated EOS (the reference's COOL_EOS build); NULL: Gamma-law. */
+void orc_set_eos(orc_ctx *o, const orc_eos *eos, int ref_derivs);
 /* Utoprim_5d (utoprim_5d.c:179-239); eos NULL = Gamma-law. */
 int  orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, double gdet,
                     double prim[8], double gam, const orc_eos *eos, int *n_iter);

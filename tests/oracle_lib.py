@@ -97,6 +97,7 @@ def lib():
         L.orc_utoprim_5d.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, POINTER(OrcEos), POINTER(c_int)]
         L.orc_primtoU_eos.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
         L.orc_dudp_5d.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
+        L.orc_set_eos.argtypes = [c_void_p, POINTER(OrcEos), c_int]
         _lib = L
     return _lib
 
