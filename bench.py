@@ -62,18 +62,18 @@ def cpu_baseline(threads: int) -> dict:
     from oracle_lib import Oracle, lib
 
     lib().orc_set_threads(threads)
-    P = torus_params(288, 256, 32)
+    P = torus_params(288, 256, 64)
     o = Oracle(P)
     o.init_torus()
     o.step()
-    nsteps = 3
+    nsteps = 10
     t0 = time.perf_counter()
     for _ in range(nsteps):
         o.step()
     el = time.perf_counter() - t0
     return {"value": P.n1 * P.n2 * P.n3 * nsteps / el, "unit": "zone-cycles/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle (C restatement, OpenMP) on a 288x256x32 slab of the same torus, {nsteps} steps "
+            "sample": f"oracle (C restatement, OpenMP) on a 288x256x64 slab of the same torus, {nsteps} steps "
                       f"after 1 warm-up, {el:.1f} s"}
 
 
@@ -189,6 +189,37 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
     return res
 
 
+def tabulated_torus(hc, steps: int, warmup: int) -> dict:
+    """The torus on the tabulated-EOS path (COOL_EOS build): 192x192x128,
+    density scale 1e11 g/cm^3, table over log rho [6, 13] x log T [9, 12] on
+    200 x 200 points, Ye = 0.5."""
+    from harm_cool_amd.params import torus_params
+
+    P = torus_params(192, 192, 128)
+    spec = hc.eos_spec()
+    host = hc.eos_table_build(spec)
+    tab = hc.DeviceEos(host, P.gam)
+    g = hc.Harm(P, device=0)
+    g.set_eos(tab)
+    g.init()
+    g.step_ch(warmup)
+    g.reset_counters()
+    g.time()
+    t0 = time.perf_counter()
+    g.step_ch(steps)
+    el = time.perf_counter() - t0
+    c = g.counters()
+    pr = g.prims()
+    rho = pr[0][2:194, 2:194]
+    inside = float(((rho >= host["rho"][0]) & (rho <= host["rho"][-1])).mean())
+    g.close()
+    nz = 192 * 192 * 128
+    return {"workload": "FM torus 192x192x128, tabulated EOS (rho_unit 1e11 g/cm^3, 200x200 table), Utoprim_5d",
+            "steps": steps, "zone_cycles_per_s": nz * steps / el, "ms_per_step": el / steps * 1e3,
+            "n_iter_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1), "fail_5d": c["fail_2d"],
+            "frac_zones_in_table_rho_range": inside}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +232,7 @@ def main() -> None:
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-batch", type=int, default=65536)
     ap.add_argument("--stress-n", type=int, default=1 << 26)
+    ap.add_argument("--no-tabulated", action="store_true", help="skip the tabulated-EOS torus section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -280,6 +312,10 @@ def main() -> None:
     if roof is not None:
         roof["traffic"] = measured_traffic(dom, n1, n2, n3)
 
+    tabulated = None
+    if not args.no_tabulated:
+        tabulated = tabulated_torus(hc, args.steps, args.warmup)
+
     st = None
     if not args.no_learned and args.stress_n > 0:
         st = stress(hc, torch, args.stress_n, m_torus, args.train_steps, args.train_batch)
@@ -306,7 +342,7 @@ def main() -> None:
         "newton_iters_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1),
         "newton_kconv_per_zone": c["newton_kconv"] / max(c["c2p_calls"], 1),
         "newton_stopping_rule": "reference rule for both starts: |dW/W|+|dv2/v2| <= 1e-10, then 2 extra iterations",
-        "learned_seed_torus": learned_rep, "stress_batch": st,
+        "learned_seed_torus": learned_rep, "stress_batch": st, "tabulated_eos_torus": tabulated,
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
