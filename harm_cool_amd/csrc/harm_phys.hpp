@@ -309,11 +309,7 @@ HD void primtoflux(const double *pr, const State &q, int dir, const Geom &geom, 
 template <class E>
 HD int vchar(const double *pr, const State &q, const Geom &geom, int js, const E &eos, double *vmax, double *vmin)
 {
-    double Acov[4] = {0., 0., 0., 0.}, Bcov[4] = {1., 0., 0., 0.}, Acon[4], Bcon[4];
     int nfail = 0;
-    Acov[js] = 1.;
-    raise(Acov, geom, Acon);
-    raise(Bcov, geom, Bcon);
     double bsq = dot4(q.bcon, q.bcov);
     double rho = pr[RHO], u = pr[UU];
     double cms2;
@@ -342,8 +338,12 @@ HD int vchar(const double *pr, const State &q, const Geom &geom, int js, const E
             cms2 = 1.;
         }
     }
-    double Asq = dot4(Acon, Acov), Bsq = dot4(Bcon, Bcov);
-    double Au = dot4(Acov, q.ucon), Bu = dot4(Bcov, q.ucon), AB = dot4(Acon, Bcov);
+    // The reference raises the unit covectors A = e_js, B = e_0 and contracts
+    // them; for finite metric and velocity those contractions are exactly the
+    // entries picked here (a zero's sign can differ, and every use of vmax and
+    // vmin passes through fabs).
+    double Asq = geom.gcon[js][js], Bsq = geom.gcon[0][0];
+    double Au = q.ucon[js], Bu = q.ucon[0], AB = geom.gcon[0][js];
     double Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
     double A = Bu2 - (Bsq + Bu2) * cms2;
     double B = 2. * (AuBu - (AB + AuBu) * cms2);
