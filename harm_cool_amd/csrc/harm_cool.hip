@@ -261,7 +261,8 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
             Geom geom;
             const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
             if (UNI)
-                load_geom_block(g.fblk[DIR - 1] + uni_col<UNI>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
+                load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<UNI>((size_t)i * g.SN2 + j) * FACE_STRIDE,
+                                              geom);
             else
                 load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
             double Fz[NPR], ctop;
@@ -467,7 +468,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         const double Dt = dtfac * tstate[1];
         blk = g.cblk + col * CENT_STRIDE;
         if (UNI)
-            load_geom_block(blk, geom);
+            load_geom_block<CENT_DERIVED>(blk, geom);
         else
             load_geom(g.geom[CENT], g.ncol, col, geom);
         double dU[NPR];
@@ -1192,6 +1193,7 @@ __global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U,
             geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
         }
         geom.g = gdet[z];
+        geom_derive(geom);
 #pragma unroll
         for (int k = 0; k < NPR; k++) {
             Uz[k] = U[(size_t)k * n + z];
@@ -1299,6 +1301,7 @@ __device__ void ks_mks_geom(double a, double h, double r, double th, Geom &g)
             g.gcon[i][j] = gn[i][j] / (J[i] * J[j]);
         }
     g.g = rho2 * fabs(sth) * r * dth;
+    geom_derive(g);
 }
 
 // x = s L^{-T} n for the Cholesky factor L of the spatial metric gamma_ij =
@@ -1405,6 +1408,7 @@ __global__ void __launch_bounds__(TPB) k_c2p5_batch(const double *__restrict__ U
         geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
     }
     geom.g = gdet[z];
+    geom_derive(geom);
     double Uz[NPR], pr[NPR];
 #pragma unroll
     for (int k = 0; k < NPR; k++) {
@@ -1764,9 +1768,20 @@ int ctx_alloc(harm_ctx *c)
     }
     if ((rc = dalloc(&c->pflag, c->nz))) return rc;
     if ((rc = dalloc(&c->pflag2, c->nz))) return rc;
+    // device tables: the host's GEO_BASE rows per location plus the derived rows
+    std::vector<double> gx[5];
     for (int l = 0; l < 5; l++) {
-        if ((rc = dalloc(&c->d_geom[l], 33 * ncol))) return rc;
-        HIPCHK(hipMemcpy(c->d_geom[l], c->tab.geom[l].data(), 33 * ncol * sizeof(double), hipMemcpyHostToDevice));
+        const std::vector<double> &T = c->tab.geom[l];
+        gx[l].assign((size_t)GEO_ROWS * ncol, 0.);
+        std::copy(T.begin(), T.begin() + (size_t)GEO_BASE * ncol, gx[l].begin());
+        for (size_t col = 0; col < ncol; col++) {
+            double d[GEO_DERIVED];
+            geom_derived(T[16 * ncol + col], T[17 * ncol + col], T[18 * ncol + col], T[19 * ncol + col],
+                         T[32 * ncol + col], d);
+            for (int k = 0; k < GEO_DERIVED; k++) gx[l][(size_t)(GEO_BASE + k) * ncol + col] = d[k];
+        }
+        if ((rc = dalloc(&c->d_geom[l], (size_t)GEO_ROWS * ncol))) return rc;
+        HIPCHK(hipMemcpy(c->d_geom[l], gx[l].data(), (size_t)GEO_ROWS * ncol * sizeof(double), hipMemcpyHostToDevice));
     }
     if ((rc = dalloc(&c->d_conn, 64 * ncol))) return rc;
     HIPCHK(hipMemcpy(c->d_conn, c->tab.conn.data(), 64 * ncol * sizeof(double), hipMemcpyHostToDevice));
@@ -1779,13 +1794,15 @@ int ctx_alloc(harm_ctx *c)
         std::vector<double> cb((size_t)CENT_STRIDE * ncol, 0.), fb((size_t)3 * FACE_STRIDE * ncol, 0.);
         const int floc[3] = {FACE1, FACE2, FACE3};
         for (size_t col = 0; col < ncol; col++) {
-            for (int v = 0; v < 33; v++) cb[col * CENT_STRIDE + v] = c->tab.geom[CENT][(size_t)v * ncol + col];
+            for (int v = 0; v < GEO_BASE; v++) cb[col * CENT_STRIDE + v] = gx[CENT][(size_t)v * ncol + col];
+            for (int v = 0; v < GEO_DERIVED; v++)
+                cb[col * CENT_STRIDE + CENT_DERIVED + v] = gx[CENT][(size_t)(GEO_BASE + v) * ncol + col];
             cb[col * CENT_STRIDE + 33] = c->tab.floors[col];
             cb[col * CENT_STRIDE + 34] = c->tab.floors[ncol + col];
             for (int v = 0; v < 64; v++) cb[col * CENT_STRIDE + CENT_CONN + v] = c->tab.conn[(size_t)v * ncol + col];
             for (int d = 0; d < 3; d++)
-                for (int v = 0; v < 33; v++)
-                    fb[((size_t)d * ncol + col) * FACE_STRIDE + v] = c->tab.geom[floc[d]][(size_t)v * ncol + col];
+                for (int v = 0; v < GEO_ROWS; v++)
+                    fb[((size_t)d * ncol + col) * FACE_STRIDE + v] = gx[floc[d]][(size_t)v * ncol + col];
         }
         if ((rc = dalloc(&c->d_cblk, cb.size()))) return rc;
         HIPCHK(hipMemcpy(c->d_cblk, cb.data(), cb.size() * sizeof(double), hipMemcpyHostToDevice));

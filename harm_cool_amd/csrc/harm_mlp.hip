@@ -321,6 +321,7 @@ __global__ void __launch_bounds__(256) k_seed(MlpDev m, const double *__restrict
             geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
         }
         geom.g = gdet[z];
+        geom_derive(geom);
         double Uz[NPR];
 #pragma unroll
         for (int k = 0; k < NPR; k++) Uz[k] = U[(size_t)k * n + z];
@@ -364,6 +365,7 @@ __global__ void __launch_bounds__(256) k_pairs(const double *__restrict__ U, con
         geom.gcon[a / 4][a % 4] = gcon[(size_t)a * n + z];
     }
     geom.g = gdet[z];
+    geom_derive(geom);
     double Uz[NPR], pr[NPR], f[MLP_NIN];
 #pragma unroll
     for (int k = 0; k < NPR; k++) {

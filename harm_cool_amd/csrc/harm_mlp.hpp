@@ -87,7 +87,7 @@ HD bool mlp_features_U(const double *U, const Geom &geom, double *f)
     s.gam = 0.;
     s.K_atm = 0.;
     u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
-    mlp_features(s, 1.0 / sqrt(-geom.gcon[0][0]), f);
+    mlp_features(s, geom.alpha, f);
     return true;
 }
 

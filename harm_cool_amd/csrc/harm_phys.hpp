@@ -26,7 +26,70 @@ struct Geom {
     double gcov[4][4];
     double gcon[4][4];
     double g;
+    // Factors of the metric alone that the physics divides by or takes roots of,
+    // formed once per location by geom_derived() with the expressions of
+    // ucon_calc (phys.c:232-252) and utoprim_2d.c:145,176.
+    double alpha;    // 1 / sqrt(-g^00)
+    double ralpha;   // 1 / alpha, correctly rounded
+    double lapse;    // sqrt(-1 / g^00)
+    double rg;       // 1 / gdet, correctly rounded
+    double beta[3];  // g^0j alpha alpha, j = 1..3
 };
+
+// Geometry tables hold GEO_BASE rows per location (gcov[16], gcon[16], gdet)
+// followed by the GEO_DERIVED rows of geom_derived().
+constexpr int GEO_BASE = 33, GEO_DERIVED = 7, GEO_ROWS = GEO_BASE + GEO_DERIVED;
+
+// Host and device evaluate these with the same IEEE operations (correctly
+// rounded sqrt and division), so a table built on the host equals what a kernel
+// would compute for the same metric.
+__host__ __device__ inline void geom_derived(double gcon00, double gcon01, double gcon02, double gcon03, double gdet,
+                                             double *d)
+{
+    const double alpha = 1. / sqrt(-gcon00);
+    d[0] = alpha;
+    d[1] = 1. / alpha;
+    d[2] = sqrt(-1. / gcon00);
+    d[3] = 1. / gdet;
+    d[4] = gcon01 * alpha * alpha;
+    d[5] = gcon02 * alpha * alpha;
+    d[6] = gcon03 * alpha * alpha;
+}
+
+HD void geom_set_derived(Geom &g, const double *d)
+{
+    g.alpha = d[0];
+    g.ralpha = d[1];
+    g.lapse = d[2];
+    g.rg = d[3];
+    g.beta[0] = d[4];
+    g.beta[1] = d[5];
+    g.beta[2] = d[6];
+}
+
+// For metrics formed inside a kernel (batch entry points, generated states).
+HD void geom_derive(Geom &g)
+{
+    double d[GEO_DERIVED];
+    geom_derived(g.gcon[0][0], g.gcon[0][1], g.gcon[0][2], g.gcon[0][3], g.g, d);
+    geom_set_derived(g, d);
+}
+
+// a / b, correctly rounded, given rb = 1 / b correctly rounded.  Markstein's
+// theorem: if rb is within half an ulp of 1/b and q within one ulp of a/b, the
+// FMA residual r = a - b q is exact and RN(q + r rb) = RN(a/b).  q0 = RN(a rb)
+// can be 1.5 ulp off, so one correction makes it faithful and a second rounds
+// it.  Holds for finite a, b with residuals clear of underflow (|a| > 2^-960, or
+// a = 0); the sign of a zero quotient is that of a rb.  Six operations in place
+// of a full division where one denominator serves several numerators or its
+// reciprocal is tabulated.
+__host__ __device__ inline double div_r(double a, double b, double rb)
+{
+    const double q0 = a * rb;
+    const double q1 = fma(fma(-b, q0, a), rb, q0);
+    const double q2 = fma(fma(-b, q1, a), rb, q1);
+    return copysign(q2, q0);
+}
 
 struct State {
     double ucon[4], ucov[4], bcon[4], bcov[4];
@@ -153,14 +216,19 @@ HD void load_geom(const double *__restrict__ tab, size_t ncol, size_t col, Geom 
             g.gcon[a][b] = ldc(tab, (size_t)(16 + a * 4 + b) * ncol + col);
         }
     g.g = ldc(tab, (size_t)32 * ncol + col);
+    double d[GEO_DERIVED];
+#pragma unroll
+    for (int k = 0; k < GEO_DERIVED; k++) d[k] = ldc(tab, (size_t)(GEO_BASE + k) * ncol + col);
+    geom_set_derived(g, d);
 }
 
 // Column-major copy of the tables for wave-uniform kernels: all values of one
 // (i,j) column are contiguous, so the scalar unit fetches them with wide loads.
-// CENT block (100 doubles): gcov[16] gcon[16] gdet rhoflr uuflr pad conn[64];
-// face blocks (36 doubles): gcov[16] gcon[16] gdet pad[3].
-constexpr int CENT_STRIDE = 100, CENT_CONN = 36, FACE_STRIDE = 36;
+// CENT block (108 doubles): gcov[16] gcon[16] gdet rhoflr uuflr pad conn[64]
+// derived[7] pad; face blocks (40 doubles): gcov[16] gcon[16] gdet derived[7].
+constexpr int CENT_STRIDE = 108, CENT_CONN = 36, CENT_DERIVED = 100, FACE_STRIDE = 40, FACE_DERIVED = GEO_BASE;
 
+template <int DERIVED>
 HD void load_geom_block(const double *__restrict__ blk, Geom &g)
 {
 #pragma unroll
@@ -171,6 +239,10 @@ HD void load_geom_block(const double *__restrict__ blk, Geom &g)
             g.gcon[a][b] = ldc(blk, 16 + a * 4 + b);
         }
     g.g = ldc(blk, 32);
+    double d[GEO_DERIVED];
+#pragma unroll
+    for (int k = 0; k < GEO_DERIVED; k++) d[k] = ldc(blk, DERIVED + k);
+    geom_set_derived(g, d);
 }
 
 // Wave-uniform column index: N3 % 64 == 0 maps a wave onto one (i,j) column.
@@ -217,25 +289,24 @@ HD int gamma_calc(const double *pr, const Geom &geom, double *gamma)
 }
 
 // phys.c:232-252; fatal counts where the reference calls fail()
+// alpha, beta and 1/alpha come from the geometry table (same expressions).
 HD void ucon_calc(const double *pr, const Geom &geom, double *ucon, int &fatal)
 {
-    double alpha = 1. / sqrt(-geom.gcon[0][0]);
-    double beta[4];
-#pragma unroll
-    for (int j = 1; j < 4; j++) beta[j] = geom.gcon[0][j] * alpha * alpha;
+    const double alpha = geom.alpha, ralpha = geom.ralpha;
     double gamma;
     if (gamma_calc(pr, geom, &gamma)) fatal++;
-    ucon[0] = gamma / alpha;
+    ucon[0] = div_r(gamma, alpha, ralpha);
 #pragma unroll
-    for (int j = 1; j < 4; j++) ucon[j] = pr[U1 + j - 1] - gamma * beta[j] / alpha;
+    for (int j = 1; j < 4; j++) ucon[j] = pr[U1 + j - 1] - div_r(gamma * geom.beta[j - 1], alpha, ralpha);
 }
 
 // phys.c:108-117
 HD void bcon_calc(const double *pr, const double *ucon, const double *ucov, double *bcon)
 {
     bcon[0] = pr[B1] * ucov[1] + pr[B2] * ucov[2] + pr[B3] * ucov[3];
+    const double r0 = 1. / ucon[0];
 #pragma unroll
-    for (int j = 1; j < 4; j++) bcon[j] = (pr[B1 - 1 + j] + bcon[0] * ucon[j]) / ucon[0];
+    for (int j = 1; j < 4; j++) bcon[j] = div_r(pr[B1 - 1 + j] + bcon[0] * ucon[j], ucon[0], r0);
 }
 
 // phys.c:219-229
@@ -420,9 +491,10 @@ HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int di
     double cmax = fabs(dmax(dmax(0., cmax_l), cmax_r));
     double cmin = fabs(dmax(dmax(0., -cmin_l), -cmin_r));
     ctop = dmax(cmax, cmin);
+    const double den = cmax + cmin + SMALL, rden = 1. / den;
 #pragma unroll
     for (int k = 0; k < NPR; k++)
-        F[k] = hllf * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin + SMALL)) +
+        F[k] = hllf * div_r(cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k]), den, rden) +
                laxf * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
 }
 HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, double gam, double hllf,
@@ -474,6 +546,7 @@ HD void source(const double *ph, const Geom &geom, const double *__restrict__ co
 // ---------------------------------------------------------------------------
 struct U2P {
     double Bsq, QdotBsq, Qtsq, Qdotn, D, K_atm, gam;
+    double rgam;  // 1 / gam, correctly rounded (for div_r in the 2D solver)
     double W_for_gnr2, rho_for_gnr2;
 };
 
@@ -506,7 +579,7 @@ HD void u2p_scalars(U2P &s, const double *U, const Geom &geom, double *Bcon, dou
 #pragma unroll
     for (int i = 0; i < 4; i++) QdotB += Qcov[i] * Bcon[i];
     s.QdotBsq = QdotB * QdotB;
-    double lapse = sqrt(-1. / geom.gcon[0][0]);
+    double lapse = geom.lapse;
     ncov[0] = -lapse;
     ncov[1] = ncov[2] = ncov[3] = 0.;
 #pragma unroll
@@ -537,9 +610,9 @@ HD void func_vsq(const U2P &s, double W, double vsq, double &dx0, double &dx1, d
     const double GAMMA = s.gam;
     double Wsq = W * W;
     double gtmp = 1. - vsq;
-    double p_tmp = (GAMMA - 1.) * (W * gtmp - s.D * sqrt(gtmp)) / GAMMA;
-    double dPdW = (GAMMA - 1.) * (1. - vsq) / GAMMA;
-    double dPdvsq = (GAMMA - 1.) * (0.5 * s.D / sqrt(1. - vsq) - W) / GAMMA;
+    double p_tmp = div_r((GAMMA - 1.) * (W * gtmp - s.D * sqrt(gtmp)), GAMMA, s.rgam);
+    double dPdW = div_r((GAMMA - 1.) * (1. - vsq), GAMMA, s.rgam);
+    double dPdvsq = div_r((GAMMA - 1.) * (0.5 * s.D / sqrt(1. - vsq) - W), GAMMA, s.rgam);
     double t2 = -0.5 * s.Bsq + dPdvsq;
     double t3 = s.Bsq + W;
     double t4 = t3 * t3;
@@ -598,13 +671,13 @@ HD int gnr_2d(const U2P &s, double &x0, double &x1, int &n_iter_out, int &k_conv
 // Conserved-variable transform of utoprim_2d.c:145-163 (shared by all solvers).
 HD void u2p_transform(const double *U, const Geom &geom, double *U_tmp)
 {
-    double alpha = 1.0 / sqrt(-geom.gcon[0][0]);
-    U_tmp[RHO] = alpha * U[RHO] / geom.g;
-    U_tmp[UU] = alpha * (U[UU] - U[RHO]) / geom.g;
+    const double alpha = geom.alpha, g = geom.g, rg = geom.rg;
+    U_tmp[RHO] = div_r(alpha * U[RHO], g, rg);
+    U_tmp[UU] = div_r(alpha * (U[UU] - U[RHO]), g, rg);
 #pragma unroll
-    for (int i = U1; i <= U3; i++) U_tmp[i] = alpha * U[i] / geom.g;
+    for (int i = U1; i <= U3; i++) U_tmp[i] = div_r(alpha * U[i], g, rg);
 #pragma unroll
-    for (int i = B1; i <= B3; i++) U_tmp[i] = alpha * U[i] / geom.g;
+    for (int i = B1; i <= B3; i++) U_tmp[i] = div_r(alpha * U[i], g, rg);
 }
 
 // Primitive recovery shared by the solvers (utoprim_2d.c:349-356).
@@ -671,7 +744,7 @@ HD int solve_2d_from_x(U2P &s, const double *Bcon, const double *Qcon, const dou
     double gamma = 1. / gtmp;
     double rho0 = s.D * gtmp;
     double w = W * (1. - vsq);
-    double p = (s.gam - 1.) * (w - rho0) / s.gam;
+    double p = div_r((s.gam - 1.) * (w - rho0), s.gam, s.rgam);
     double u = w - (rho0 + p);
     if ((rho0 <= 0.) || (u <= 0.)) return 5;
     prim[RHO] = rho0;
@@ -699,12 +772,13 @@ HD int utoprim_2d(const double *U, const Geom &geom, double gam, double *prim, i
     n_iter = 0;
     k_conv = 0;
 #pragma unroll
-    for (int i = B1; i <= B3; i++) prim[i] = U[i] / geom.g;
+    for (int i = B1; i <= B3; i++) prim[i] = div_r(U[i], geom.g, geom.rg);
     if (U[0] <= 0.) return -100;
     double U_tmp[NPR];
     u2p_transform(U, geom, U_tmp);
     U2P s;
     s.gam = gam;
+    s.rgam = 1. / gam;
     s.K_atm = 0.;
     double Bcon[4], Qcon[4], ncon[4], QdotB;
     u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
@@ -729,12 +803,13 @@ HD int utoprim_2d_seeded(const double *U, const Geom &geom, double gam, double *
     n_iter = 0;
     k_conv = 0;
 #pragma unroll
-    for (int i = B1; i <= B3; i++) prim[i] = U[i] / geom.g;
+    for (int i = B1; i <= B3; i++) prim[i] = div_r(U[i], geom.g, geom.rg);
     if (U[0] <= 0.) return -100;
     double U_tmp[NPR];
     u2p_transform(U, geom, U_tmp);
     U2P s;
     s.gam = gam;
+    s.rgam = 1. / gam;
     s.K_atm = 0.;
     double Bcon[4], Qcon[4], ncon[4], QdotB;
     u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
