@@ -518,7 +518,9 @@ HD void source(const double *ph, const Geom &geom, const double *__restrict__ co
     const double P = eos.pres(ph[RHO], ph[UU]);
 #pragma unroll
     for (int k = 0; k < NPR; k++) dU[k] = 0.;
-#pragma unroll
+    // Not unrolled: the 16 connection coefficients of row j are fetched in
+    // iteration j, so only one row occupies scalar registers at a time.
+#pragma unroll 1
     for (int j = 0; j < 4; j++) {
         double mhd[4];
         mhd_calc_P(ph, j, q, P, mhd);
