@@ -48,6 +48,82 @@ class EosTable(ctypes.Structure):
                 ("p", c_void_p), ("gam", c_double)]
 
 
+class RestartHdr(ctypes.Structure):
+    """Binary image of ``harm_restart_hdr``: the restart header in file order
+    (restart.c:105-138)."""
+
+    _fields_ = [
+        ("N1", ctypes.c_int32), ("N2", ctypes.c_int32), ("N3", ctypes.c_int32), ("nstep", ctypes.c_int32),
+        ("t", c_double), ("tf", c_double), ("a", c_double), ("gam", c_double), ("cour", c_double),
+        ("DTd", c_double), ("DTl", c_double), ("DTi", c_double),
+        ("DTr", ctypes.c_int32), ("dump_cnt", ctypes.c_int32), ("image_cnt", ctypes.c_int32),
+        ("rdump_cnt", ctypes.c_int32),
+        ("dt", c_double),
+        ("lim", ctypes.c_int32), ("failed", ctypes.c_int32),
+        ("Rin", c_double), ("Rout", c_double), ("hslope", c_double), ("R0", c_double),
+        ("mass_start", c_double), ("mass_in", c_double), ("mass_out", c_double),
+        ("tdump", c_double), ("timage", c_double), ("tlog", c_double), ("defcon", c_double),
+        ("nfailed", ctypes.c_int32), ("ener_size", ctypes.c_int32),
+        ("Rbr", c_double), ("cyl_R0", c_double), ("cyl_Th0", c_double),
+    ]
+
+    @classmethod
+    def of(cls, d: "dict | None" = None) -> "RestartHdr":
+        h = cls()
+        h.ener_size = -1
+        for k, v in (d or {}).items():
+            setattr(h, k, v)
+        return h
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def restart_peek(path: str) -> dict:
+    """Header of a restart file."""
+    h = RestartHdr()
+    _check(lib().harm_restart_peek(os.fsencode(path), ctypes.byref(h)), "harm_restart_peek")
+    return h.as_dict()
+
+
+def restart_write_host(path: str, hdr: dict, prim: np.ndarray, temp: "np.ndarray | None" = None) -> None:
+    """Write a restart file from host arrays in the canonical layout."""
+    h = RestartHdr.of(hdr)
+    shape = (8, h.N1 + 4, h.N2 + 4, h.N3)
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    if prim.shape != shape:
+        raise ValueError(f"expected prim {shape}, got {prim.shape}")
+    tp = None
+    if temp is not None:
+        temp = np.ascontiguousarray(temp, dtype=np.float64)
+        if temp.shape != shape[1:]:
+            raise ValueError(f"expected temp {shape[1:]}, got {temp.shape}")
+        tp = _ptr(temp)
+    _check(lib().harm_restart_write_host(os.fsencode(path), ctypes.byref(h), _ptr(prim), tp), "harm_restart_write_host")
+
+
+def restart_read_host(path: str, with_temp: bool = False):
+    """(header dict, prim[8,N1+4,N2+4,N3], temp or None) of a restart file."""
+    hd = restart_peek(path)
+    shape = (8, hd["N1"] + 4, hd["N2"] + 4, hd["N3"])
+    prim = np.empty(shape, dtype=np.float64)
+    temp = np.empty(shape[1:], dtype=np.float64) if with_temp else None
+    h, has = RestartHdr(), c_int(0)
+    _check(lib().harm_restart_read_host(os.fsencode(path), ctypes.byref(h), _ptr(prim),
+                                        _ptr(temp) if temp is not None else None, ctypes.byref(has)),
+           "harm_restart_read_host")
+    return h.as_dict(), prim, (temp if has.value else None)
+
+
+def params_from_restart(hdr: dict, **kw) -> HarmParams:
+    """Torus run parameters carrying a restart header's grid, metric and run
+    controls (other fields from ``torus_params`` defaults or ``kw``)."""
+    keys = ("a", "gam", "Rin", "Rout", "hslope", "R0", "Rbr", "cour", "lim", "tf", "defcon")
+    args = {k: hdr[k] for k in keys}
+    args.update(kw)
+    return torus_params(hdr["N1"], hdr["N2"], hdr["N3"], **args)
+
+
 def eos_spec(**kw) -> EosSpec:
     """The tabulated-EOS workload: density scale 1e11 g/cm^3, log10 rho in
     [6, 13] g/cm^3 and log10 T in [9, 12] K on 200 x 200 points, Ye = 0.5."""
@@ -206,6 +282,11 @@ def lib() -> ctypes.CDLL:
     L.harm_utoprim5d_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_size_t, c_double, POINTER(EosTable), c_void_p]
     L.harm_set_eos.argtypes = [c_void_p, POINTER(EosTable), c_int]
+    L.harm_restart_write.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
+    L.harm_restart_read.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
+    L.harm_restart_peek.argtypes = [c_char_p, POINTER(RestartHdr)]
+    L.harm_restart_write_host.argtypes = [c_char_p, POINTER(RestartHdr), _dp, _dp]
+    L.harm_restart_read_host.argtypes = [c_char_p, POINTER(RestartHdr), _dp, _dp, POINTER(c_int)]
     L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
     L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
     L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]
@@ -326,6 +407,21 @@ class Harm:
 
     def debug_fixup_utoprim(self) -> None:
         _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim")
+
+    def restart_write(self, path: str, **driver) -> None:
+        """restart_write() (restart.c:73-184); ``driver`` sets the header fields
+        the run driver owns (nstep, dump counters, clocks, ener_size, ...)."""
+        h = RestartHdr.of(driver)
+        _check(self.L.harm_restart_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_write")
+
+    def restart_read(self, path: str) -> dict:
+        """restart_init()/restart_read() (restart.c:195-376) into this context;
+        returns the header."""
+        h = RestartHdr()
+        _check(self.L.harm_restart_read(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_read")
+        for k in ("cour", "lim", "tf", "defcon"):
+            setattr(self.params, k, getattr(h, k))
+        return h.as_dict()
 
     def set_eos(self, tab: "DeviceEos | None", ref_derivs: bool = False) -> None:
         """Tabulated EOS for the step (None: back to the Gamma-law EOS)."""

@@ -1446,6 +1446,17 @@ __global__ void __launch_bounds__(TPB) k_pow(const double *__restrict__ x, const
     if (z < n) out[z] = cr_pow(x[z], y[z]);
 }
 
+// Zone temperatures of the tabulated EOS for the restart file (all zones).
+__global__ void __launch_bounds__(TPB) k_zone_temp(EosTab tab, const double *__restrict__ p, double *__restrict__ T,
+                                                   size_t nz)
+{
+    const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (z >= nz) return;
+    double pp, a, b, tz;
+    eos_table_p(tab, p[z], p[nz + z], pp, a, b, &tz);
+    T[z] = tz;
+}
+
 // Plain device copy of known size: calibrates the FETCH_SIZE / WRITE_SIZE counters.
 __global__ void __launch_bounds__(TPB) k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n)
 {
@@ -2231,6 +2242,88 @@ int harm_set_eos(harm_ctx *c, const harm_eos_table *tab, int ref_derivs)
     c->g.tab = e;
     c->g.ref_derivs = ref_derivs ? 1 : 0;
     c->eos_kind = 1;
+    return 0;
+}
+
+int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr)
+{
+    if (!c || !path) return set_err("null argument");
+    harm_restart_hdr h;
+    if (hdr) {
+        h = *hdr;
+    } else {
+        std::memset(&h, 0, sizeof(h));
+        h.ener_size = -1;
+    }
+    const harm_params &P = c->P;
+    h.N1 = P.n1;
+    h.N2 = P.n2;
+    h.N3 = P.n3;
+    h.tf = P.tf;
+    h.a = P.a;
+    h.gam = P.gam;
+    h.cour = P.cour;
+    h.lim = P.lim;
+    h.Rin = P.Rin;
+    h.Rout = P.Rout;
+    h.hslope = P.hslope;
+    h.R0 = P.R0;
+    h.defcon = P.defcon;
+    h.Rbr = P.Rbr;
+    double ts[4];
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> prim(8 * c->nz), temp;
+    HIPCHK(hipMemcpyAsync(prim.data(), c->p, prim.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (c->eos_kind == 1) {
+        // the temperatures go through the divergence buffer (8*N1*N2*N3 >= nz doubles)
+        temp.resize(c->nz);
+        hipLaunchKernelGGL(k_zone_temp, dim3(nblocks(c->nz)), dim3(TPB), 0, c->stream, c->g.tab, c->p, c->S, c->nz);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(temp.data(), c->S, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    h.t = ts[0];
+    h.dt = ts[1];
+    h.mass_in = ts[2];
+    h.mass_out = ts[3];
+    return harm_host::restart_write_file(path, h, prim.data(), temp.empty() ? nullptr : temp.data());
+}
+
+int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
+{
+    if (!c || !path) return set_err("null argument");
+    harm_restart_hdr h;
+    int rc;
+    if ((rc = harm_host::restart_read_file(path, h, nullptr, nullptr, nullptr))) return rc;
+    const harm_params &P = c->P;
+    if (h.N1 != P.n1 || h.N2 != P.n2 || h.N3 != P.n3) return set_err("restart file: grid size differs from the context's", 3);
+    const struct {
+        const char *name;
+        double file, ctx;
+    } same[] = {{"a", h.a, P.a},         {"gam", h.gam, P.gam},          {"Rin", h.Rin, P.Rin},
+                {"Rout", h.Rout, P.Rout}, {"hslope", h.hslope, P.hslope}, {"R0", h.R0, P.R0},
+                {"Rbr", h.Rbr, P.Rbr}};
+    for (const auto &f : same)
+        if (!(f.file == f.ctx))
+            return set_err(std::string("restart file: ") + f.name +
+                               " differs from the context's (create the context from harm_restart_peek's header)",
+                           3);
+    if (h.lim < 0 || h.lim > 2) return set_err("restart file: unknown slope limiter", 3);
+    std::vector<double> prim(8 * c->nz);
+    if ((rc = harm_host::restart_read_file(path, h, prim.data(), nullptr, nullptr))) return rc;
+    // cour, lim, tf, defcon come from the file (restart.c:335-367)
+    c->P.cour = c->g.cour = h.cour;
+    c->P.lim = c->g.lim = h.lim;
+    c->P.tf = c->g.tf = h.tf;
+    c->P.defcon = c->g.defcon = h.defcon;
+    HIPCHK(hipMemcpyAsync(c->p, prim.data(), prim.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->pflag, 0, c->nz, c->stream));
+    const double ts[4] = {h.t, h.dt, h.mass_in, h.mass_out};
+    HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((rc = bound(c, c->p))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (hdr) *hdr = h;
     return 0;
 }
 

@@ -35,7 +35,10 @@ __device__ __forceinline__ int eos_cell(const double *x, int n, double v)
 }
 
 // p(rho, u), dp/drho at fixed u, dp/du at fixed rho; 1 on the Gamma-law branch.
-__device__ inline int eos_table_p(const EosTab &t, double rho, double u, double &p, double &dpdrho, double &dpdu)
+// T (optional) receives the temperature along the same interpolation, 0 on the
+// Gamma-law branch.
+__device__ inline int eos_table_p(const EosTab &t, double rho, double u, double &p, double &dpdrho, double &dpdu,
+                                  double *T = nullptr)
 {
     const int nr = t.nr, nt = t.nt;
     if (rho >= t.rho[0] && rho <= t.rho[nr - 1]) {
@@ -65,12 +68,14 @@ __device__ inline int eos_table_p(const EosTab &t, double rho, double u, double 
             const double sE0 = (e1[lo] - e0[lo]) / dr, sE1 = (e1[lo + 1] - e0[lo + 1]) / dr;
             const double sP0 = (p1[lo] - p0[lo]) / dr, sP1 = (p1[lo + 1] - p0[lo + 1]) / dr;
             dpdrho = sP0 + (-sE0 * dP * dE + (u - E0) * ((sP1 - sP0) * dE - dP * (sE1 - sE0))) / (dE * dE);
+            if (T) *T = t.T[lo] + b * (t.T[lo + 1] - t.T[lo]);
             return 0;
         }
     }
     p = (t.gam - 1.) * u;
     dpdrho = 0.;
     dpdu = t.gam - 1.;
+    if (T) *T = 0.;
     return 1;
 }
 

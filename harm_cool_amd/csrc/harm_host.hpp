@@ -27,6 +27,10 @@ int build_geometry(const harm_params &P, Grid &g, GeomTables &t);
 // Sets the message harm_last_error() returns; returns code.
 int set_error(const std::string &msg, int code);
 
+// Restart file text (harm_restart.cpp); prim/temp in the canonical layout.
+int restart_write_file(const char *path, const harm_restart_hdr &h, const double *prim, const double *temp);
+int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp);
+
 int build_torus2d(const harm_params &P, const Grid &g, const GeomTables &t, std::vector<double> &tor, double &rhomax,
                   double &umax);
 

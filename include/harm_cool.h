@@ -271,6 +271,44 @@ int harm_set_eos(harm_ctx *c, const harm_eos_table *tab, int ref_derivs);
 int harm_utoprim5d_batch(const double *U, const double *gcov, const double *gcon, const double *gdet, double *prim,
                          int8_t *ret, uint8_t *iters, size_t n, double gam, const harm_eos_table *tab, void *stream);
 
+/* Restart files (restart.c:73-383): the reference's ASCII layout, one header
+ * line ("%10d" / "%32.22e", restart.c:105-138), one line of the 8 primitives
+ * per zone in (h, i, j) order over all zones including ghosts, the ener.out
+ * size when ener_size >= 0 (rank 0 in the reference, restart.c:150-157), and
+ * with a tabulated EOS one temperature per zone (cooleos.c:996-1030).
+ * Header fields in file order; the context owns N1..N3, t, tf, a, gam, cour,
+ * dt, lim, Rin, Rout, hslope, R0, defcon, Rbr, mass_in and mass_out, the
+ * caller (the run driver) the dump/image counters and clocks. */
+typedef struct harm_restart_hdr {
+    int32_t N1, N2, N3, nstep;
+    double t, tf, a, gam, cour, DTd, DTl, DTi;
+    int32_t DTr, dump_cnt, image_cnt, rdump_cnt;
+    double dt;
+    int32_t lim, failed;
+    double Rin, Rout, hslope, R0, mass_start, mass_in, mass_out, tdump, timage, tlog, defcon;
+    int32_t nfailed, ener_size; /* ener_size < 0: no ener.out size in the file */
+    double Rbr, cyl_R0, cyl_Th0;
+} harm_restart_hdr;
+/* restart_write(), restart.c:73-184: context-owned fields of hdr are replaced by
+ * the context's values; hdr may be NULL (driver fields zero, no ener.out size).
+ * With a tabulated EOS set, the zone temperatures of the table inversion follow
+ * (0 where a zone lies outside the table). */
+int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr);
+/* restart_init()/restart_read(), restart.c:195-376: grid size, a, gam, Rin,
+ * Rout, hslope, R0 and Rbr must equal the context's (create the context from
+ * harm_restart_peek's header); cour, lim, tf and defcon are taken from the
+ * file as the reference does.  Loads p, pflag = 0, t, dt, mass_in, mass_out,
+ * then bound_prim (restart.c:278-279).  hdr (may be NULL) receives the header. */
+int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr);
+/* The header alone (ener_size, which follows the body, reads as -1). */
+int harm_restart_peek(const char *path, harm_restart_hdr *hdr);
+/* Host-array forms (no device): prim in the canonical layout of
+ * harm_get_prims (8 x (N1+4) x (N2+4) x N3), temp (may be NULL) (N1+4) x (N2+4)
+ * x N3.  Reading fills prim (NULL: header only) and, when temp is not NULL and
+ * the file holds the block, temp; *has_temp (may be NULL) says which. */
+int harm_restart_write_host(const char *path, const harm_restart_hdr *hdr, const double *prim, const double *temp);
+int harm_restart_read_host(const char *path, harm_restart_hdr *hdr, double *prim, double *temp, int *has_temp);
+
 /* Exact reference signature (utoprim_2d.c:126, decs.h:608-609), evaluated on the
  * GPU for one zone; gam is taken from harm_set_gamma() (default 4/3, init.c:91). */
 int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8]);

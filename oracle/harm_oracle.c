@@ -3138,3 +3138,141 @@ void orc_dudp_5d(const double prim[8], const double *gcov, const double *gcon, d
     dudp_calc_g5(&s, prim, A);
     memcpy(Am, A, sizeof(A));
 }
+
+/* ------------------------------------------------------------------------ */
+/* Restart file, restart.c:73-383 (and cooleos.c:996-1030).                 */
+/* ------------------------------------------------------------------------ */
+#define ORC_FMT_DBL_OUT "%32.22e" /* decs.h:175 */
+#define ORC_FMT_INT_OUT "%10d"    /* decs.h:176 */
+
+int orc_restart_write(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp)
+{
+    const int sn1 = h->N1 + 4, sn2 = h->N2 + 4, sn3 = h->N3;
+    FILE *fp = fopen(path, "wt");
+    int hh, i, j, k;
+    if (!fp) return 2;
+    /* restart.c:105-138 */
+    fprintf(fp, ORC_FMT_INT_OUT, h->N1);
+    fprintf(fp, ORC_FMT_INT_OUT, h->N2);
+    fprintf(fp, ORC_FMT_INT_OUT, h->N3);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->t);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->tf);
+    fprintf(fp, ORC_FMT_INT_OUT, h->nstep);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->a);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->gam);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->cour);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->DTd);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->DTl);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->DTi);
+    fprintf(fp, ORC_FMT_INT_OUT, h->DTr);
+    fprintf(fp, ORC_FMT_INT_OUT, h->dump_cnt);
+    fprintf(fp, ORC_FMT_INT_OUT, h->image_cnt);
+    fprintf(fp, ORC_FMT_INT_OUT, h->rdump_cnt);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->dt);
+    fprintf(fp, ORC_FMT_INT_OUT, h->lim);
+    fprintf(fp, ORC_FMT_INT_OUT, h->failed);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->Rin);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->Rout);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->hslope);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->R0);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->mass_start);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->mass_in);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->mass_out);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->tdump);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->timage);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->tlog);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->defcon);
+    fprintf(fp, ORC_FMT_INT_OUT, h->nfailed);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->Rbr);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->cyl_R0);
+    fprintf(fp, ORC_FMT_DBL_OUT, h->cyl_Th0);
+    fprintf(fp, "\n");
+    /* restart.c:144-147: p[h][i][j][k] */
+    for (hh = 0; hh < sn3; ++hh)
+        for (i = 0; i < sn1; ++i)
+            for (j = 0; j < sn2; ++j) {
+                for (k = 0; k < 8; k++) fprintf(fp, ORC_FMT_DBL_OUT, prim[(((size_t)k * sn1 + i) * sn2 + j) * sn3 + hh]);
+                fprintf(fp, "\n");
+            }
+    if (h->ener_size >= 0) fprintf(fp, ORC_FMT_INT_OUT, h->ener_size); /* restart.c:150-157 */
+    if (temp)                                                          /* cooleos.c:1002-1005 */
+        for (hh = 0; hh < sn3; ++hh)
+            for (i = 0; i < sn1; ++i)
+                for (j = 0; j < sn2; ++j) fprintf(fp, ORC_FMT_DBL_OUT, temp[((size_t)i * sn2 + j) * sn3 + hh]);
+    return fclose(fp) == 0 ? 0 : 2;
+}
+
+int orc_restart_read(const char *path, orc_restart_hdr *h, double *prim, double *temp, int read_ener)
+{
+    FILE *fp = fopen(path, "r");
+    int hh, i, j, k, n = 0, sn1, sn2, sn3;
+    if (!fp) return 2;
+    memset(h, 0, sizeof(*h));
+    h->ener_size = -1;
+    /* restart.c:317-367 */
+    n += fscanf(fp, "%d", &h->N1);
+    n += fscanf(fp, "%d", &h->N2);
+    n += fscanf(fp, "%d", &h->N3);
+    n += fscanf(fp, "%lf", &h->t);
+    n += fscanf(fp, "%lf", &h->tf);
+    n += fscanf(fp, "%d", &h->nstep);
+    n += fscanf(fp, "%lf", &h->a);
+    n += fscanf(fp, "%lf", &h->gam);
+    n += fscanf(fp, "%lf", &h->cour);
+    n += fscanf(fp, "%lf", &h->DTd);
+    n += fscanf(fp, "%lf", &h->DTl);
+    n += fscanf(fp, "%lf", &h->DTi);
+    n += fscanf(fp, "%d", &h->DTr);
+    n += fscanf(fp, "%d", &h->dump_cnt);
+    n += fscanf(fp, "%d", &h->image_cnt);
+    n += fscanf(fp, "%d", &h->rdump_cnt);
+    n += fscanf(fp, "%lf", &h->dt);
+    n += fscanf(fp, "%d", &h->lim);
+    n += fscanf(fp, "%d", &h->failed);
+    n += fscanf(fp, "%lf", &h->Rin);
+    n += fscanf(fp, "%lf", &h->Rout);
+    n += fscanf(fp, "%lf", &h->hslope);
+    n += fscanf(fp, "%lf", &h->R0);
+    n += fscanf(fp, "%lf", &h->mass_start);
+    n += fscanf(fp, "%lf", &h->mass_in);
+    n += fscanf(fp, "%lf", &h->mass_out);
+    n += fscanf(fp, "%lf", &h->tdump);
+    n += fscanf(fp, "%lf", &h->timage);
+    n += fscanf(fp, "%lf", &h->tlog);
+    n += fscanf(fp, "%lf", &h->defcon);
+    n += fscanf(fp, "%d", &h->nfailed);
+    n += fscanf(fp, "%lf", &h->Rbr);
+    n += fscanf(fp, "%lf", &h->cyl_R0);
+    n += fscanf(fp, "%lf", &h->cyl_Th0);
+    if (n != 34) {
+        fclose(fp);
+        return 3;
+    }
+    sn1 = h->N1 + 4;
+    sn2 = h->N2 + 4;
+    sn3 = h->N3;
+    if (prim) { /* restart.c:372 */
+        for (hh = 0; hh < sn3; ++hh)
+            for (i = 0; i < sn1; ++i)
+                for (j = 0; j < sn2; ++j)
+                    for (k = 0; k < 8; k++)
+                        if (fscanf(fp, "%lf", &prim[(((size_t)k * sn1 + i) * sn2 + j) * sn3 + hh]) != 1) {
+                            fclose(fp);
+                            return 3;
+                        }
+        if (read_ener && fscanf(fp, "%d", &h->ener_size) != 1) { /* restart.c:375-380 */
+            fclose(fp);
+            return 3;
+        }
+        if (temp) /* cooleos.c:1024-1027 */
+            for (hh = 0; hh < sn3; ++hh)
+                for (i = 0; i < sn1; ++i)
+                    for (j = 0; j < sn2; ++j)
+                        if (fscanf(fp, "%lf", &temp[((size_t)i * sn2 + j) * sn3 + hh]) != 1) {
+                            fclose(fp);
+                            return 3;
+                        }
+    }
+    fclose(fp);
+    return 0;
+}

@@ -143,6 +143,26 @@ void orc_face_flux(const double *p_l, const double *p_r, const double *gcov,
 int  orc_vchar(const double *pr, const double *gcov, const double *gcon, double gdet,
                int dir, double gam, double *cmax, double *cmin);
 
+/* Restart file (restart.c:73-383).  Same field layout as harm_restart_hdr in
+ * include/harm_cool.h; prim in the canonical layout, temp [N1+4][N2+4][N3]. */
+typedef struct orc_restart_hdr {
+    int32_t N1, N2, N3, nstep;
+    double t, tf, a, gam, cour, DTd, DTl, DTi;
+    int32_t DTr, dump_cnt, image_cnt, rdump_cnt;
+    double dt;
+    int32_t lim, failed;
+    double Rin, Rout, hslope, R0, mass_start, mass_in, mass_out, tdump, timage, tlog, defcon;
+    int32_t nfailed, ener_size;
+    double Rbr, cyl_R0, cyl_Th0;
+} orc_restart_hdr;
+/* restart_write(): the header (restart.c:105-138), the body (restart.c:144-147),
+ * the ener.out size when ener_size >= 0 (restart.c:150-157), the COOL_EOS
+ * temperatures when temp != NULL (cooleos.c:996-1014). */
+int orc_restart_write(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp);
+/* restart_read() (restart.c:317-376) with fscanf; read_ener as rank 0 does,
+ * temp (may be NULL) as restart_read_CoolEOS (cooleos.c:1018-1030). */
+int orc_restart_read(const char *path, orc_restart_hdr *h, double *prim, double *temp, int read_ener);
+
 #ifdef __cplusplus
 }
 #endif

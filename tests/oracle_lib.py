@@ -48,6 +48,22 @@ def orc_eos(tab: dict, gam: float) -> OrcEos:
 _lib = None
 
 
+class OrcRestartHdr(ctypes.Structure):
+    """orc_restart_hdr (oracle/harm_oracle.h), restart.c:105-138 in file order."""
+    _fields_ = [
+        ("N1", ctypes.c_int32), ("N2", ctypes.c_int32), ("N3", ctypes.c_int32), ("nstep", ctypes.c_int32),
+        ("t", c_double), ("tf", c_double), ("a", c_double), ("gam", c_double), ("cour", c_double),
+        ("DTd", c_double), ("DTl", c_double), ("DTi", c_double),
+        ("DTr", ctypes.c_int32), ("dump_cnt", ctypes.c_int32), ("image_cnt", ctypes.c_int32),
+        ("rdump_cnt", ctypes.c_int32), ("dt", c_double), ("lim", ctypes.c_int32), ("failed", ctypes.c_int32),
+        ("Rin", c_double), ("Rout", c_double), ("hslope", c_double), ("R0", c_double),
+        ("mass_start", c_double), ("mass_in", c_double), ("mass_out", c_double),
+        ("tdump", c_double), ("timage", c_double), ("tlog", c_double), ("defcon", c_double),
+        ("nfailed", ctypes.c_int32), ("ener_size", ctypes.c_int32),
+        ("Rbr", c_double), ("cyl_R0", c_double), ("cyl_Th0", c_double),
+    ]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -98,6 +114,8 @@ def lib():
         L.orc_primtoU_eos.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
         L.orc_dudp_5d.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
         L.orc_set_eos.argtypes = [c_void_p, POINTER(OrcEos), c_int]
+        L.orc_restart_write.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp]
+        L.orc_restart_read.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_int]
         _lib = L
     return _lib
 
