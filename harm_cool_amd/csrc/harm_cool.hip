@@ -97,6 +97,25 @@ __device__ __forceinline__ void box_zone(unsigned idx, unsigned n3, int i0, int 
     }
 }
 
+// Same box, columns walked in tiles of TI rows of i with i fastest inside a
+// tile: the i+-1 and j+-1 neighbours of a column are visited at most TI columns
+// away, so 2D stencil kernels find them in L2 (the last tile may be shorter).
+template <unsigned TI>
+__device__ __forceinline__ void box_zone_tiled(unsigned idx, unsigned n3, int i0, int j0, unsigned ni, unsigned nj,
+                                               int &i, int &j, int &h)
+{
+    const unsigned rest = idx / n3;
+    h = (int)(idx - rest * n3);
+    const unsigned tile = rest / (TI * nj);
+    const unsigned r = rest - tile * (TI * nj);
+    const unsigned ti0 = tile * TI;
+    const unsigned th = min(TI, ni - ti0);
+    const unsigned jj = r / th;
+    i = i0 + (int)(ti0 + (r - jj * th));
+    j = j0 + (int)jj;
+}
+constexpr unsigned STENCIL_TILE = 8;
+
 // ---- wave-level reductions (64 lanes) ----
 __device__ __forceinline__ int wave_sum_int(int v)
 {
@@ -304,7 +323,7 @@ __global__ void __launch_bounds__(TPB) k_emf(DGrid g, const double *__restrict__
     const unsigned idx = xcd_block() * TPB + threadIdx.x;
     if (idx >= (unsigned)ni * nj * g.N3) return;
     int i, j, h;
-    box_zone<false>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
+    box_zone_tiled<STENCIL_TILE>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
     const size_t nz = g.nz;
     const size_t z = zidx(g, i, j, h);
     const size_t zjm = zidx(g, i, j - 1, h), zim = zidx(g, i - 1, j, h);
@@ -373,7 +392,9 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__
     const unsigned idx = xcd_block() * TPB + threadIdx.x;
     if (idx >= na) return;
     int i, j, h;
-    box_zone<false>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
+    box_zone_tiled<STENCIL_TILE>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
+    // S is stored in the update kernel's order (i, j, h over the active box)
+    const size_t sidx = ((size_t)(i - g.is) * nj + (j - g.js)) * g.N3 + h;
     const size_t z = zidx(g, i, j, h);
     const size_t nz = g.nz;
     const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
@@ -415,7 +436,7 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__
         } else {
             d = -(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2;
         }
-        S[k * na + idx] = d;
+        S[k * na + sidx] = d;
     }
 }
 

@@ -476,18 +476,25 @@ template <class E>
 HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, const E &eos, double hllf,
                   double laxf, double *F, double &ctop, int &fatal)
 {
-    State sl, sr;
     double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
     double cmax_l, cmin_l, cmax_r, cmin_r;
-    get_state(p_l, geom, sl, fatal);
-    get_state(p_r, geom, sr, fatal);
-    const double P_l = eos.pres(p_l[RHO], p_l[UU]), P_r = eos.pres(p_r[RHO], p_r[UU]);
-    primtoflux_P(p_l, sl, dir, geom, P_l, F_l);
-    primtoflux_P(p_l, sl, 0, geom, P_l, U_l);
-    fatal += vchar(p_l, sl, geom, dir, eos, &cmax_l, &cmin_l);
-    primtoflux_P(p_r, sr, dir, geom, P_r, F_r);
-    primtoflux_P(p_r, sr, 0, geom, P_r, U_r);
-    fatal += vchar(p_r, sr, geom, dir, eos, &cmax_r, &cmin_r);
+    {
+        // the left state is finished before the right one is formed
+        State sl;
+        get_state(p_l, geom, sl, fatal);
+        const double P_l = eos.pres(p_l[RHO], p_l[UU]);
+        primtoflux_P(p_l, sl, dir, geom, P_l, F_l);
+        primtoflux_P(p_l, sl, 0, geom, P_l, U_l);
+        fatal += vchar(p_l, sl, geom, dir, eos, &cmax_l, &cmin_l);
+    }
+    {
+        State sr;
+        get_state(p_r, geom, sr, fatal);
+        const double P_r = eos.pres(p_r[RHO], p_r[UU]);
+        primtoflux_P(p_r, sr, dir, geom, P_r, F_r);
+        primtoflux_P(p_r, sr, 0, geom, P_r, U_r);
+        fatal += vchar(p_r, sr, geom, dir, eos, &cmax_r, &cmin_r);
+    }
     double cmax = fabs(dmax(dmax(0., cmax_l), cmax_r));
     double cmin = fabs(dmax(dmax(0., -cmin_l), -cmin_r));
     ctop = dmax(cmax, cmin);
