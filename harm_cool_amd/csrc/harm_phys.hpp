@@ -525,12 +525,19 @@ HD void source(const double *ph, const Geom &geom, const double *__restrict__ co
     const double P = eos.pres(ph[RHO], ph[UU]);
 #pragma unroll
     for (int k = 0; k < NPR; k++) dU[k] = 0.;
+    // mhd_calc_P's row j with u^j, b^j picked by selects (an index into the
+    // state would move it to scratch memory in the loop below)
+    const double bsq = dot4(q.bcon, q.bcov);
+    const double eta = (P + ph[RHO] + ph[UU]) + bsq, ptot = P + 0.5 * bsq;
     // Not unrolled: the 16 connection coefficients of row j are fetched in
     // iteration j, so only one row occupies scalar registers at a time.
 #pragma unroll 1
     for (int j = 0; j < 4; j++) {
+        const double uj = (j == 0) ? q.ucon[0] : (j == 1) ? q.ucon[1] : (j == 2) ? q.ucon[2] : q.ucon[3];
+        const double bj = (j == 0) ? q.bcon[0] : (j == 1) ? q.bcon[1] : (j == 2) ? q.bcon[2] : q.bcon[3];
         double mhd[4];
-        mhd_calc_P(ph, j, q, P, mhd);
+#pragma unroll
+        for (int k = 0; k < 4; k++) mhd[k] = eta * uj * q.ucov[k] + ptot * ((j == k) ? 1. : 0.) - bj * q.bcov[k];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             dU[UU] += mhd[k] * ldc(conn, (size_t)((k * 4 + 0) * 4 + j) * ncol + col);
