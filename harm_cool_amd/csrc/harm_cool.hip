@@ -229,7 +229,9 @@ __global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__
 // XCD-contiguous logical blocks, so the four zones each face reads are mostly
 // L2 hits; 4 waves per SIMD (128 VGPRs) measured faster than 3 with a few
 // spilled registers.
-template <int DIR, bool UNI, bool TAB>
+// LIM >= 0 fixes the slope limiter at compile time (the hot MC path); -1 reads
+// it from the grid.
+template <int DIR, bool UNI, bool TAB, int LIM>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart, int fullstep,
        unsigned long long *__restrict__ cnt)
@@ -272,8 +274,9 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
 #pragma unroll
             for (int k = 0; k < NPR; k++) {
                 const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
-                const double dqm = slope_lim(g.lim, a, b, c);
-                const double dq0 = slope_lim(g.lim, b, c, d);
+                const int lim = (LIM >= 0) ? LIM : g.lim;
+                const double dqm = slope_lim(lim, a, b, c);
+                const double dq0 = slope_lim(lim, b, c, d);
                 p_l[k] = b + 0.5 * dqm;
                 p_r[k] = c - 0.5 * dq0;
             }
@@ -1584,11 +1587,11 @@ int ev_collect(harm_ctx *c)
         if (rc_) return rc_;                                   \
     } while (0)
 
-template <int DIR, bool UNI, bool TAB>
+template <int DIR, bool UNI, bool TAB, int LIM = -1>
 void flux_kernel(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullstep)
 {
-    hipLaunchKernelGGL((k_flux<DIR, UNI, TAB>), grid, dim3(TPB), 0, c->stream, c->g, pb, F, c->dtpart, fullstep,
-                       c->cnt);
+    hipLaunchKernelGGL((k_flux<DIR, UNI, TAB, LIM>), grid, dim3(TPB), 0, c->stream, c->g, pb, F, c->dtpart,
+                       fullstep, c->cnt);
 }
 
 template <int DIR>
@@ -1598,6 +1601,8 @@ void flux_kernel_any(harm_ctx *c, dim3 grid, const double *pb, double *F, int fu
     if (c->uniform) {
         if (tab)
             flux_kernel<DIR, true, true>(c, grid, pb, F, fullstep);
+        else if (c->g.lim == 0)
+            flux_kernel<DIR, true, false, 0>(c, grid, pb, F, fullstep);
         else
             flux_kernel<DIR, true, false>(c, grid, pb, F, fullstep);
     } else {
