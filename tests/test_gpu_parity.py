@@ -22,6 +22,10 @@ CASES = {
     "3d_32x32x16": lambda: torus_params(32, 32, 16),
     "2d_llf_vanleer": lambda: torus_params(64, 64, 1, lim=1, hllf=0.0, laxf=1.0),
     "3d_minmod_gamma43": lambda: torus_params(24, 16, 8, lim=2, gam=4.0 / 3.0, a=0.6, rin=4.0, rmax=11.8),
+    # N3 a multiple of 64: the wave-uniform kernels of the measured workload
+    # (compile-time MC limiter with HLL; run-time limiter with van Leer + LLF)
+    "3d_uniform_32x32x64": lambda: torus_params(32, 32, 64),
+    "3d_uniform_vanleer_llf": lambda: torus_params(16, 16, 128, lim=1, hllf=0.0, laxf=1.0),
 }
 
 
@@ -76,7 +80,8 @@ def test_initial_data_and_fluxes_bitwise(case):
 
 
 @pytest.mark.parametrize("case,nsteps", [("2d_128", 10), ("3d_32x32x16", 10), ("2d_llf_vanleer", 10),
-                                         ("3d_minmod_gamma43", 10)])
+                                         ("3d_minmod_gamma43", 10), ("3d_uniform_32x32x64", 6),
+                                         ("3d_uniform_vanleer_llf", 6)])
 def test_steps_match_oracle(case, nsteps):
     g, o = _pair(CASES[case]())
     g.init()
