@@ -53,6 +53,7 @@ struct DGrid {
     int lim, font_fix, polefix;
     size_t ncol, nz;
     double dx1, dx2, dx3, gam, cour, hllf, laxf, defcon, tf;
+    double rdx1, rdx2, rdx3;  // 1/dx, correctly rounded (div_r in the flux divergence)
     const double *geom[5];
     const double *conn;
     const double *floors;
@@ -378,17 +379,63 @@ struct FontList {
     int *overflow;
 };
 
-// Flux divergence of the zone update (step_ch.c:250-263) with the flux_ct
-// B-fluxes formed from the EMFs, one active zone per thread, into the compact
-// array S[k][idx] (idx = active-zone index):
-//   S = -(F1[i+1] - F1[i])/dx1 - (F2[j+1] - F2[j])/dx2 [- (F3[h+1] - F3[h])/dx3]
-// so that the update's U += Dt*(S + dU) repeats the reference's operation
-// order exactly.  A light, high-occupancy stencil pass: the inversion kernel
-// that follows then reads 8 values per zone instead of ~50 scattered ones.
-__global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__ F1, const double *__restrict__ F2,
-                                             const double *__restrict__ F3, const double *__restrict__ e1,
-                                             const double *__restrict__ e2, const double *__restrict__ e3,
-                                             double *__restrict__ S)
+// Flux divergence of the zone update (step_ch.c:250-263) at active zone
+// (i, j, h) with the flux_ct B-fluxes formed from the EMFs:
+//   d = -(F1[i+1] - F1[i])/dx1 - (F2[j+1] - F2[j])/dx2 [- (F3[h+1] - F3[h])/dx3]
+// so that U += Dt*(d + dU) repeats the reference's operation order exactly.
+struct DivArgs {
+    const double *F1, *F2, *F3, *e1, *e2, *e3;
+};
+
+__device__ __forceinline__ void flux_div(const DGrid &g, const DivArgs &a, int i, int j, int h, double *d)
+{
+    const size_t z = zidx(g, i, j, h);
+    const size_t nz = g.nz;
+    const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
+    const size_t zh1 = z - h + (h + 1) % g.N3;
+    double f1a[3], f1b[3], f2a[3], f2b[3], f3a[3], f3b[3];
+    ct_f1(g, a.e1, a.e2, a.e3, i, j, h, f1a);
+    ct_f1(g, a.e1, a.e2, a.e3, i + 1, j, h, f1b);
+    ct_f2(g, a.e1, a.e2, a.e3, i, j, h, f2a);
+    ct_f2(g, a.e1, a.e2, a.e3, i, j + 1, h, f2b);
+    if (g.N3 > 2) {
+        ct_f3(g, a.e1, a.e2, i, j, h, f3a);
+        ct_f3(g, a.e1, a.e2, i, j, (h + 1) % g.N3, f3b);
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        double a1, b1, a2, b2;
+        if (k < B1 || (k == B3 && g.N3 <= 2)) {
+            a1 = a.F1[k * nz + z];
+            b1 = a.F1[k * nz + zi1];
+            a2 = a.F2[k * nz + z];
+            b2 = a.F2[k * nz + zj1];
+        } else {
+            a1 = f1a[k - B1];
+            b1 = f1b[k - B1];
+            a2 = f2a[k - B1];
+            b2 = f2b[k - B1];
+        }
+        if (g.N3 > 2) {
+            double a3, b3;
+            if (k < B1) {
+                a3 = a.F3[k * nz + z];
+                b3 = a.F3[k * nz + zh1];
+            } else {
+                a3 = f3a[k - B1];
+                b3 = f3b[k - B1];
+            }
+            d[k] = -div_r(b1 - a1, g.dx1, g.rdx1) - div_r(b2 - a2, g.dx2, g.rdx2) - div_r(b3 - a3, g.dx3, g.rdx3);
+        } else {
+            d[k] = -div_r(b1 - a1, g.dx1, g.rdx1) - div_r(b2 - a2, g.dx2, g.rdx2);
+        }
+    }
+}
+
+// The divergence as its own light, high-occupancy stencil pass into the compact
+// array S[k][idx] (idx = active-zone index in the update kernel's order), for
+// the update variants that read it instead of forming it.
+__global__ void __launch_bounds__(TPB) k_div(DGrid g, DivArgs a, double *__restrict__ S)
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
@@ -396,51 +443,11 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, const double *__restrict__
     if (idx >= na) return;
     int i, j, h;
     box_zone_tiled<STENCIL_TILE>(idx, g.N3, g.is, g.js, ni, nj, i, j, h);
-    // S is stored in the update kernel's order (i, j, h over the active box)
     const size_t sidx = ((size_t)(i - g.is) * nj + (j - g.js)) * g.N3 + h;
-    const size_t z = zidx(g, i, j, h);
-    const size_t nz = g.nz;
-    const size_t zi1 = zidx(g, i + 1, j, h), zj1 = zidx(g, i, j + 1, h);
-    const size_t zh1 = z - h + (h + 1) % g.N3;
-    double f1a[3], f1b[3], f2a[3], f2b[3], f3a[3], f3b[3];
-    ct_f1(g, e1, e2, e3, i, j, h, f1a);
-    ct_f1(g, e1, e2, e3, i + 1, j, h, f1b);
-    ct_f2(g, e1, e2, e3, i, j, h, f2a);
-    ct_f2(g, e1, e2, e3, i, j + 1, h, f2b);
-    if (g.N3 > 2) {
-        ct_f3(g, e1, e2, i, j, h, f3a);
-        ct_f3(g, e1, e2, i, j, (h + 1) % g.N3, f3b);
-    }
+    double d[NPR];
+    flux_div(g, a, i, j, h, d);
 #pragma unroll
-    for (int k = 0; k < NPR; k++) {
-        double a1, b1, a2, b2;
-        if (k < B1 || (k == B3 && g.N3 <= 2)) {
-            a1 = F1[k * nz + z];
-            b1 = F1[k * nz + zi1];
-            a2 = F2[k * nz + z];
-            b2 = F2[k * nz + zj1];
-        } else {
-            a1 = f1a[k - B1];
-            b1 = f1b[k - B1];
-            a2 = f2a[k - B1];
-            b2 = f2b[k - B1];
-        }
-        double d;
-        if (g.N3 > 2) {
-            double a3, b3;
-            if (k < B1) {
-                a3 = F3[k * nz + z];
-                b3 = F3[k * nz + zh1];
-            } else {
-                a3 = f3a[k - B1];
-                b3 = f3b[k - B1];
-            }
-            d = -(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2 - (b3 - a3) / g.dx3;
-        } else {
-            d = -(b1 - a1) / g.dx1 - (b2 - a2) / g.dx2;
-        }
-        S[k * na + sidx] = d;
-    }
+    for (int k = 0; k < NPR; k++) S[k * na + sidx] = d[k];
 }
 
 // Optional parts of k_update: the learned Newton seed and the capture of
@@ -463,12 +470,13 @@ enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2 };
 // MODE UPD_CAPTURE also stores training pairs of sampled zones; UPD_LEARNED
 // seeds the Newton solve from the network (wave-level BF16 MFMA forward) and
 // falls back to the pi guess where the seeded solve fails.
-template <bool UNI, int MODE, bool TAB>
+// FUSED forms the flux divergence in place (flux_div) instead of reading S.
+template <bool UNI, int MODE, bool TAB, bool FUSED>
 __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb,
                                                 double *__restrict__ pf, int8_t *__restrict__ pflag,
-                                                const double *__restrict__ S, const double *__restrict__ tstate,
-                                                double dtfac, int pi_is_pb, FontList fl,
-                                                unsigned long long *__restrict__ cpart, UpdateExtra ex)
+                                                const double *__restrict__ S, DivArgs dv,
+                                                const double *__restrict__ tstate, double dtfac, int pi_is_pb,
+                                                FontList fl, unsigned long long *__restrict__ cpart, UpdateExtra ex)
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
@@ -521,8 +529,15 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
             else
                 primtoflux(piz, q, 0, geom, g.gam, U);
         }
+        if constexpr (FUSED) {
+            double d[NPR];
+            flux_div(g, dv, i, j, h, d);
 #pragma unroll
-        for (int k = 0; k < NPR; k++) U[k] += Dt * (S[k * na + idx] + dU[k]);
+            for (int k = 0; k < NPR; k++) U[k] += Dt * (d[k] + dU[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) U[k] += Dt * (S[k * na + idx] + dU[k]);
+        }
     }
     double pr[NPR];
 #pragma unroll
@@ -1699,34 +1714,34 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
         c->cap_count = std::min(c->cap, c->cap_count + taken);
         c->cap_phase = (int)((ph + 7919) % (size_t)c->cap_stride);
     }
-#define UPD_LAUNCH(U_, M_)                                                                                          \
-    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_, false>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, \
-                                             pi, pb, pf, c->pflag, c->S, c->tstate, dtfac, same, c->font, c->cpart, ex))
-    LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
-                                          c->e1, c->e2, c->e3, c->S));
+    const DivArgs dv = {c->F1, c->F2, c->F3, c->e1, c->e2, c->e3};
+    // the plain wave-uniform update forms the divergence itself; the others read S
+    const bool fused = c->eos_kind != 1 && c->uniform && mode == UPD_PLAIN;
+    if (!fused)
+        LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, dv, c->S));
+#define UPD_LAUNCH(U_, M_, T_, F_)                                                                                  \
+    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_, T_, F_>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, \
+                                             g, pi, pb, pf, c->pflag, c->S, dv, c->tstate, dtfac, same, c->font,   \
+                                             c->cpart, ex))
     if (c->eos_kind == 1) {
         if (c->uniform)
-            LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<true, UPD_PLAIN, true>), dim3(nblocks(na)), dim3(TPB), 0,
-                                                     c->stream, g, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac, same,
-                                                     c->font, c->cpart, ex));
+            UPD_LAUNCH(true, UPD_PLAIN, true, false);
         else
-            LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<false, UPD_PLAIN, true>), dim3(nblocks(na)), dim3(TPB),
-                                                     0, c->stream, g, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac,
-                                                     same, c->font, c->cpart, ex));
+            UPD_LAUNCH(false, UPD_PLAIN, true, false);
     } else if (c->uniform) {
         if (mode == UPD_PLAIN)
-            UPD_LAUNCH(true, UPD_PLAIN);
+            UPD_LAUNCH(true, UPD_PLAIN, false, true);
         else if (mode == UPD_CAPTURE)
-            UPD_LAUNCH(true, UPD_CAPTURE);
+            UPD_LAUNCH(true, UPD_CAPTURE, false, false);
         else
-            UPD_LAUNCH(true, UPD_LEARNED);
+            UPD_LAUNCH(true, UPD_LEARNED, false, false);
     } else {
         if (mode == UPD_PLAIN)
-            UPD_LAUNCH(false, UPD_PLAIN);
+            UPD_LAUNCH(false, UPD_PLAIN, false, false);
         else if (mode == UPD_CAPTURE)
-            UPD_LAUNCH(false, UPD_CAPTURE);
+            UPD_LAUNCH(false, UPD_CAPTURE, false, false);
         else
-            UPD_LAUNCH(false, UPD_LEARNED);
+            UPD_LAUNCH(false, UPD_LEARNED, false, false);
     }
 #undef UPD_LAUNCH
     {
@@ -1888,6 +1903,9 @@ int ctx_alloc(harm_ctx *c)
     g.dx1 = hg.dx[1];
     g.dx2 = hg.dx[2];
     g.dx3 = hg.dx[3];
+    g.rdx1 = 1. / g.dx1;
+    g.rdx2 = 1. / g.dx2;
+    g.rdx3 = 1. / g.dx3;
     g.gam = P.gam;
     g.cour = P.cour;
     g.hllf = P.hllf;
