@@ -28,17 +28,19 @@ HBM_PEAK_GBS = 8000.0
 # active zone of one half-step).
 #   flux: read the zone's 8 primitives, write the face's 8 fluxes.
 #   emf: read 6 B-flux components, write 3 EMFs.
-#   div: read the non-B fluxes of the zone's six faces (5 x 3 x 2 faces, each
-#        face shared by two zones -> 5 x 3 x 1) and the three EMFs it owns,
-#        write the 8 flux divergences.
-#   update: read pb (8) and the divergence (8), in the corrector also pi (8);
-#        write pf (8) and the flag (1 B): 193 B predictor, 257 B corrector,
-#        225 B per launch on average.
+#   div (the separate pass of the learned/capture/tabulated variants): read the
+#        non-B fluxes of the zone's six faces (5 x 3 x 2 faces, each face shared
+#        by two zones -> 5 x 3 x 1) and the three EMFs it owns, write the 8 flux
+#        divergences.
+#   update (the plain wave-uniform kernel, which forms the divergence itself):
+#        read pb (8), the 15 non-B fluxes and 3 EMFs of the divergence, in the
+#        corrector also pi (8); write pf (8) and the flag (1 B): 273 B predictor,
+#        337 B corrector, 305 B per launch on average.
 ALG_BYTES = {
     "flux1": 128.0, "flux2": 128.0, "flux3": 128.0,
     "emf": 6 * 8 + 3 * 8,
     "div": 15 * 8 + 3 * 8 + 8 * 8,
-    "update": ((8 + 8) * 8 + 8 * 8 + 1 + (8 + 8 + 8) * 8 + 8 * 8 + 1) / 2,
+    "update": ((8 + 18) * 8 + 8 * 8 + 1 + (8 + 8 + 18) * 8 + 8 * 8 + 1) / 2,
 }
 
 
