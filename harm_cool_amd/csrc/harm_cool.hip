@@ -231,87 +231,126 @@ __global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__
 // L2 hits; 4 waves per SIMD (128 VGPRs) measured faster than 3 with a few
 // spilled registers.
 // LIM >= 0 fixes the slope limiter at compile time (the hot MC path); -1 reads
-// it from the grid.
-template <int DIR, bool UNI, bool TAB, int LIM>
+// it from the grid.  ROWS (DIR 3 with N3 dividing TPB): the block holds whole
+// phi rows, so each thread loads only its own zone and forms only its own
+// zone's slopes; the neighbour's values come through LDS.
+template <int DIR, bool UNI, bool TAB, int LIM, bool ROWS = false>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart, int fullstep,
        unsigned long long *__restrict__ cnt)
 {
+    static_assert(!ROWS || DIR == 3, "row-local reconstruction is along phi only");
     const int i0 = (DIR == 1) ? g.is : g.is - 1;
     const int j0 = (DIR == 2) ? g.js : g.js - 1;
     const int ni = g.ie + 1 - i0 + 1;
     const int nj = g.je + 1 - j0 + 1;
     const unsigned idx = xcd_block() * TPB + threadIdx.x;
     const unsigned total = (unsigned)ni * nj * g.N3;
+    const bool act = idx < total;
+    const int lim = (LIM >= 0) ? LIM : g.lim;
+    const size_t nz = g.nz;
     double dtij = 1.e300;
     int fatal = 0;
-    if (idx < total) {
-        int i, j, h;
+    int i = 0, j = 0, h = 0;
+    size_t z = 0;
+    if (act) {
         box_zone<DIR == 1>(idx, g.N3, i0, j0, ni, nj, i, j, h);
-        const size_t z = zidx(g, i, j, h);
-        const size_t nz = g.nz;
-        if (DIR == 2 && (j == g.js || j == g.je + 1)) {
-#pragma unroll
-            for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
-        } else {
-            size_t zm2, zm1, zp1;
-            if (DIR == 1) {
-                const size_t s = (size_t)g.SN2 * g.N3;
-                zm2 = z - 2 * s;
-                zm1 = z - s;
-                zp1 = z + s;
-            } else if (DIR == 2) {
-                const size_t s = (size_t)g.N3;
-                zm2 = z - 2 * s;
-                zm1 = z - s;
-                zp1 = z + s;
-            } else {
-                const size_t base = z - h;
-                zm2 = base + (h + 2 * g.N3 - 2) % g.N3;
-                zm1 = base + (h + g.N3 - 1) % g.N3;
-                zp1 = base + (h + 1) % g.N3;
-            }
-            double p_l[NPR], p_r[NPR];
+        z = zidx(g, i, j, h);
+    }
+    const bool zero_face = act && DIR == 2 && (j == g.js || j == g.je + 1);
+    double p_l[NPR], p_r[NPR];
+    if constexpr (ROWS) {
+        // whole rows per block (a row is all active or all inactive); every
+        // thread reaches both barriers
+        __shared__ double sp[NPR][TPB], sdq[NPR][TPB];
+        const unsigned t = threadIdx.x;
+        double pc[NPR], pm[NPR], dq[NPR];
+        if (act) {
 #pragma unroll
             for (int k = 0; k < NPR; k++) {
-                const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
-                const int lim = (LIM >= 0) ? LIM : g.lim;
-                const double dqm = slope_lim(lim, a, b, c);
-                const double dq0 = slope_lim(lim, b, c, d);
-                p_l[k] = b + 0.5 * dqm;
-                p_r[k] = c - 0.5 * dq0;
+                pc[k] = pb[k * nz + z];
+                sp[k][t] = pc[k];
             }
-            Geom geom;
-            const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
-            if (UNI)
-                load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<UNI>((size_t)i * g.SN2 + j) * FACE_STRIDE,
-                                              geom);
-            else
-                load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
-            double Fz[NPR], ctop;
-            if constexpr (TAB)
-                face_flux(p_l, p_r, geom, DIR, EosTable{g.tab, g.ref_derivs}, g.hllf, g.laxf, Fz, ctop, fatal);
-            else
-                face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
-            if (DIR == 1) {
-                // coord.c:310-320: no inflow of mass through the radial boundaries
-                if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
-                if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
-            }
-            // coord.c:290-306: antisymmetric B2 flux in the first polar ghost face
-            const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+        }
+        __syncthreads();
+        const unsigned rs = t - (unsigned)h;
+        const unsigned tm1 = rs + (unsigned)((h + g.N3 - 1) % g.N3), tp1 = rs + (unsigned)((h + 1) % g.N3);
+        if (act) {
 #pragma unroll
-            for (int k = 0; k < NPR; k++)
-                if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
-            if (DIR != 2) {
-                if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
-                if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+            for (int k = 0; k < NPR; k++) {
+                pm[k] = sp[k][tm1];
+                dq[k] = slope_lim(lim, pm[k], pc[k], sp[k][tp1]);
+                sdq[k][t] = dq[k];
             }
-            if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
-                const double dxd = (DIR == 1) ? g.dx1 : (DIR == 2) ? g.dx2 : g.dx3;
-                const double d = g.cour * dxd / ctop;
-                if (d < 1.e9) dtij = d;
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                p_l[k] = pm[k] + 0.5 * sdq[k][tm1];
+                p_r[k] = pc[k] - 0.5 * dq[k];
             }
+        }
+    } else if (act && !zero_face) {
+        size_t zm2, zm1, zp1;
+        if (DIR == 1) {
+            const size_t s = (size_t)g.SN2 * g.N3;
+            zm2 = z - 2 * s;
+            zm1 = z - s;
+            zp1 = z + s;
+        } else if (DIR == 2) {
+            const size_t s = (size_t)g.N3;
+            zm2 = z - 2 * s;
+            zm1 = z - s;
+            zp1 = z + s;
+        } else {
+            const size_t base = z - h;
+            zm2 = base + (h + 2 * g.N3 - 2) % g.N3;
+            zm1 = base + (h + g.N3 - 1) % g.N3;
+            zp1 = base + (h + 1) % g.N3;
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
+            const double dqm = slope_lim(lim, a, b, c);
+            const double dq0 = slope_lim(lim, b, c, d);
+            p_l[k] = b + 0.5 * dqm;
+            p_r[k] = c - 0.5 * dq0;
+        }
+    }
+    if (zero_face) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
+    } else if (act) {
+        Geom geom;
+        const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
+        if (UNI)
+            load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<UNI>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
+        else
+            load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
+        double Fz[NPR], ctop;
+        if constexpr (TAB)
+            face_flux(p_l, p_r, geom, DIR, EosTable{g.tab, g.ref_derivs}, g.hllf, g.laxf, Fz, ctop, fatal);
+        else
+            face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
+        if (DIR == 1) {
+            // coord.c:310-320: no inflow of mass through the radial boundaries
+            if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
+            if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
+        }
+        // coord.c:290-306: antisymmetric B2 flux in the first polar ghost face
+        const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+#pragma unroll
+        for (int k = 0; k < NPR; k++)
+            if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
+        if (DIR != 2) {
+            if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
+            if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+        }
+        if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+            const double dxd = (DIR == 1) ? g.dx1 : (DIR == 2) ? g.dx2 : g.dx3;
+            const double d = g.cour * dxd / ctop;
+            if (d < 1.e9) dtij = d;
         }
     }
     if (fullstep) block_min(dtij, dtpart);
@@ -1602,10 +1641,10 @@ int ev_collect(harm_ctx *c)
         if (rc_) return rc_;                                   \
     } while (0)
 
-template <int DIR, bool UNI, bool TAB, int LIM = -1>
+template <int DIR, bool UNI, bool TAB, int LIM = -1, bool ROWS = false>
 void flux_kernel(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullstep)
 {
-    hipLaunchKernelGGL((k_flux<DIR, UNI, TAB, LIM>), grid, dim3(TPB), 0, c->stream, c->g, pb, F, c->dtpart,
+    hipLaunchKernelGGL((k_flux<DIR, UNI, TAB, LIM, ROWS>), grid, dim3(TPB), 0, c->stream, c->g, pb, F, c->dtpart,
                        fullstep, c->cnt);
 }
 
@@ -1616,6 +1655,8 @@ void flux_kernel_any(harm_ctx *c, dim3 grid, const double *pb, double *F, int fu
     if (c->uniform) {
         if (tab)
             flux_kernel<DIR, true, true>(c, grid, pb, F, fullstep);
+        else if (c->g.lim == 0 && DIR == 3 && TPB % c->g.N3 == 0)
+            flux_kernel<DIR, true, false, 0, DIR == 3>(c, grid, pb, F, fullstep);
         else if (c->g.lim == 0)
             flux_kernel<DIR, true, false, 0>(c, grid, pb, F, fullstep);
         else
