@@ -357,6 +357,137 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
     add_counter(cnt, HARM_FATAL, fatal);
 }
 
+// The X1 / X2 fluxes of the wave-uniform Gamma-law MC path with the
+// reconstruction shared along the stencil direction.  A block is 4 waves: wave
+// w computes face f0 + w (f0 = first face of the block's strip) for 64 phi
+// zones of one (i, j) column.  The block loads the 7 zone rows f0-2 .. f0+4
+// once (1.75 rows per face instead of 4) and forms the slopes of rows
+// f0-1 .. f0+3 once (1.25 per face instead of 2); the reconstructed states
+// p[r] +- dq[r]/2 are exchanged through LDS.  Same arithmetic as k_flux.
+constexpr int STRIP = TPB / 64;  // faces per block along the stencil direction
+
+template <int DIR>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart,
+             int fullstep, unsigned long long *__restrict__ cnt)
+{
+    static_assert(DIR == 1 || DIR == 2, "strips run along X1 or X2");
+    // faces along DIR: [fs, fe]; the other index o over [os, oe]
+    const int fs = (DIR == 1) ? g.is : g.js, fe = (DIR == 1) ? g.ie + 1 : g.je + 1;
+    const int os = (DIR == 1) ? g.js - 1 : g.is - 1, oe = (DIR == 1) ? g.je + 1 : g.ie + 1;
+    const unsigned nstrip = (unsigned)(fe - fs + STRIP) / STRIP, no = (unsigned)(oe - os + 1);
+    const unsigned nseg = (unsigned)g.N3 / 64;
+    const unsigned b = xcd_block();
+    const bool blk_act = b < nstrip * no * nseg;
+    const unsigned s = b % nstrip, rest = b / nstrip;
+    const int o = os + (int)(rest % no);
+    const unsigned seg = rest / no;
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int h = (int)seg * 64 + lane;
+    const int f0 = fs + (int)s * STRIP;
+    const int sn = (DIR == 1) ? g.SN1 : g.SN2;
+    const size_t nz = g.nz;
+    const size_t dstride = (DIR == 1) ? (size_t)g.SN2 * g.N3 : (size_t)g.N3;
+    // zone of row r (r = 0..STRIP+2 is zone index f0 - 2 + r along DIR) in this lane's column
+    auto zone = [&](int r) -> size_t {
+        const int zd = f0 - 2 + r;
+        return (DIR == 1) ? zidx(g, zd, o, h) : zidx(g, o, zd, h);
+    };
+    // LDS: B[8 rows][8 vars][64 lanes]; first the primitives of rows 0..6, then
+    // left states p+dq/2 of rows 1..4 in B[0..3] and right states p-dq/2 of rows
+    // 2..5 in B[4..7]
+    __shared__ double B[STRIP + 4][NPR][64];
+    // phase 1: rows w+1 (every wave) and 0, 5, 6 (waves 0, 1, 2)
+    const int r1 = w + 1, r2 = (w == 0) ? 0 : (w == 1) ? STRIP + 1 : (w == 2) ? STRIP + 2 : -1;
+    double pc[NPR];
+    if (blk_act && f0 - 2 + r1 < sn) {
+        const size_t zz = zone(r1);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            pc[k] = pb[k * nz + zz];
+            B[r1][k][lane] = pc[k];
+        }
+    }
+    if (blk_act && r2 >= 0 && f0 - 2 + r2 < sn) {
+        const size_t zz = zone(r2);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) B[r2][k][lane] = pb[k * nz + zz];
+    }
+    __syncthreads();
+    // phase 2: slopes of row w+1, and of row 5 by wave 3
+    double dq[NPR], dq5[NPR], p5[NPR];
+    const bool row1_ok = blk_act && f0 - 1 + r1 < sn;  // rows r1-1 .. r1+1 loaded
+    const bool row5_ok = blk_act && w == STRIP - 1 && f0 + STRIP + 1 <= sn;
+    if (row1_ok) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) dq[k] = slope_lim(0, B[r1 - 1][k][lane], pc[k], B[r1 + 1][k][lane]);
+    }
+    if (row5_ok) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            p5[k] = B[STRIP + 1][k][lane];
+            dq5[k] = slope_lim(0, B[STRIP][k][lane], p5[k], B[STRIP + 2][k][lane]);
+        }
+    }
+    __syncthreads();
+    // phase 3: left state of row w+1 (kept), right states of rows 2..5 to LDS
+    double p_l[NPR];
+    if (row1_ok) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            p_l[k] = pc[k] + 0.5 * dq[k];
+            if (r1 >= 2) B[r1 + 2][k][lane] = pc[k] - 0.5 * dq[k];
+        }
+    }
+    if (row5_ok) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) B[STRIP + 3][k][lane] = p5[k] - 0.5 * dq5[k];
+    }
+    __syncthreads();
+    // phase 4: face f0 + w between rows w+1 and w+2
+    const int f = f0 + w;
+    double dtij = 1.e300;
+    int fatal = 0;
+    if (blk_act && f <= fe) {
+        double p_r[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) p_r[k] = B[w + 4][k][lane];
+        const int i = (DIR == 1) ? f : o, j = (DIR == 1) ? o : f;
+        const size_t z = zidx(g, i, j, h);
+        if (DIR == 2 && (j == g.js || j == g.je + 1)) {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
+        } else {
+            Geom geom;
+            load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<true>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
+            double Fz[NPR], ctop;
+            face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
+            if (DIR == 1) {
+                // coord.c:310-320: no inflow of mass through the radial boundaries
+                if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
+                if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
+            }
+            // coord.c:290-306: antisymmetric B2 flux in the first polar ghost face
+            const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+#pragma unroll
+            for (int k = 0; k < NPR; k++)
+                if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
+            if (DIR != 2) {
+                if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
+                if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+            }
+            if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+                const double dxd = (DIR == 1) ? g.dx1 : g.dx2;
+                const double d = g.cour * dxd / ctop;
+                if (d < 1.e9) dtij = d;
+            }
+        }
+    }
+    (void)dstride;
+    if (fullstep) block_min(dtij, dtpart);
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
 // flux_ct EMFs (step_ch.c:514-629 for N3 <= 2, step_ch.c:726-798 for N3 > 2).
 __global__ void __launch_bounds__(TPB) k_emf(DGrid g, const double *__restrict__ F1, const double *__restrict__ F2,
                                              const double *__restrict__ F3, double *__restrict__ e1,
@@ -1649,13 +1780,22 @@ void flux_kernel(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullst
 }
 
 template <int DIR>
-void flux_kernel_any(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullstep)
+void flux_kernel_any(harm_ctx *c, dim3 &grid, const double *pb, double *F, int fullstep)
 {
     const bool tab = c->eos_kind == 1;
     if (c->uniform) {
         if (tab)
             flux_kernel<DIR, true, true>(c, grid, pb, F, fullstep);
-        else if (c->g.lim == 0 && DIR == 3 && TPB % c->g.N3 == 0)
+        else if (c->g.lim == 0 && DIR != 3) {
+            // strips of STRIP faces along DIR x one other index x 64 phi zones
+            const DGrid &g = c->g;
+            const size_t nf = (DIR == 1) ? (size_t)(g.ie + 2 - g.is) : (size_t)(g.je + 2 - g.js);
+            const size_t no = (DIR == 1) ? (size_t)(g.je + 3 - g.js) : (size_t)(g.ie + 3 - g.is);
+            const size_t nb = (nf + STRIP - 1) / STRIP * no * (size_t)(g.N3 / 64);
+            grid = dim3((unsigned)((nb + 7) / 8 * 8));
+            hipLaunchKernelGGL((k_flux_strip<DIR == 3 ? 1 : DIR>), grid, dim3(TPB), 0, c->stream, c->g, pb, F,
+                               c->dtpart, fullstep, c->cnt);
+        } else if (c->g.lim == 0 && DIR == 3 && TPB % c->g.N3 == 0)
             flux_kernel<DIR, true, false, 0, DIR == 3>(c, grid, pb, F, fullstep);
         else if (c->g.lim == 0)
             flux_kernel<DIR, true, false, 0>(c, grid, pb, F, fullstep);
@@ -1675,7 +1815,7 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
     const int i0 = (dir == 1) ? g.is : g.is - 1;
     const int j0 = (dir == 2) ? g.js : g.js - 1;
     const size_t total = (size_t)(g.ie + 2 - i0) * (g.je + 2 - j0) * g.N3;
-    const dim3 grid(nblocks8(total));
+    dim3 grid(nblocks8(total));  // a variant may launch another grid and report it here
     if (dir == 1)
         LAUNCH(HARM_K_FLUX1, flux_kernel_any<1>(c, grid, pb, F, fullstep));
     else if (dir == 2)
