@@ -162,17 +162,19 @@ __device__ __forceinline__ void block_counts(const int (&v)[N], unsigned long lo
     }
 }
 
-// Per-block minimum of v stored at out[blockIdx.x]; every thread must call it.
+// Per-block minimum of v stored at out[blockIdx.x]; every thread of the NT-thread
+// block must call it.
+template <int NT = TPB>
 __device__ __forceinline__ void block_min(double v, double *__restrict__ out)
 {
-    __shared__ double red[TPB / 64];
+    __shared__ double red[NT / 64];
     const double m = wave_min_dbl(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = red[0];
 #pragma unroll
-        for (int q = 1; q < TPB / 64; q++) t = (red[q] < t) ? red[q] : t;
+        for (int q = 1; q < NT / 64; q++) t = (red[q] < t) ? red[q] : t;
         out[blockIdx.x] = t;
     }
 }
@@ -358,16 +360,18 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
 }
 
 // The X1 / X2 fluxes of the wave-uniform Gamma-law MC path with the
-// reconstruction shared along the stencil direction.  A block is 4 waves: wave
-// w computes face f0 + w (f0 = first face of the block's strip) for 64 phi
-// zones of one (i, j) column.  The block loads the 7 zone rows f0-2 .. f0+4
-// once (1.75 rows per face instead of 4) and forms the slopes of rows
-// f0-1 .. f0+3 once (1.25 per face instead of 2); the reconstructed states
-// p[r] +- dq[r]/2 are exchanged through LDS.  Same arithmetic as k_flux.
-constexpr int STRIP = TPB / 64;  // faces per block along the stencil direction
+// reconstruction shared along the stencil direction.  A block is STRIP waves:
+// wave w computes face f0 + w (f0 = first face of the block's strip) for 64 phi
+// zones of one (i, j) column.  The block loads the STRIP+3 zone rows
+// f0-2 .. f0+STRIP once and forms the slopes of rows f0-1 .. f0+STRIP-1 once
+// (per face (STRIP+3)/STRIP loads instead of 4 and (STRIP+1)/STRIP slopes
+// instead of 2); the reconstructed states p[r] +- dq[r]/2 are exchanged through
+// LDS.  Same arithmetic as k_flux.
+constexpr int STRIP = 4;                // faces per block (8 and 16 measured 3% and 11% slower)
+constexpr int STRIP_TPB = 64 * STRIP;
 
 template <int DIR>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(STRIP_TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart,
              int fullstep, unsigned long long *__restrict__ cnt)
 {
@@ -387,17 +391,16 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
     const int f0 = fs + (int)s * STRIP;
     const int sn = (DIR == 1) ? g.SN1 : g.SN2;
     const size_t nz = g.nz;
-    const size_t dstride = (DIR == 1) ? (size_t)g.SN2 * g.N3 : (size_t)g.N3;
     // zone of row r (r = 0..STRIP+2 is zone index f0 - 2 + r along DIR) in this lane's column
     auto zone = [&](int r) -> size_t {
         const int zd = f0 - 2 + r;
         return (DIR == 1) ? zidx(g, zd, o, h) : zidx(g, o, zd, h);
     };
-    // LDS: B[8 rows][8 vars][64 lanes]; first the primitives of rows 0..6, then
-    // left states p+dq/2 of rows 1..4 in B[0..3] and right states p-dq/2 of rows
-    // 2..5 in B[4..7]
+    // LDS: B[STRIP+4 rows][8 vars][64 lanes]; first the primitives of rows
+    // 0..STRIP+2, then the right states p-dq/2 of rows 2..STRIP+1 in
+    // B[4..STRIP+3] (left states stay in registers)
     __shared__ double B[STRIP + 4][NPR][64];
-    // phase 1: rows w+1 (every wave) and 0, 5, 6 (waves 0, 1, 2)
+    // phase 1: row w+1 by every wave, rows 0, STRIP+1, STRIP+2 by waves 0, 1, 2
     const int r1 = w + 1, r2 = (w == 0) ? 0 : (w == 1) ? STRIP + 1 : (w == 2) ? STRIP + 2 : -1;
     double pc[NPR];
     if (blk_act && f0 - 2 + r1 < sn) {
@@ -414,7 +417,7 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
         for (int k = 0; k < NPR; k++) B[r2][k][lane] = pb[k * nz + zz];
     }
     __syncthreads();
-    // phase 2: slopes of row w+1, and of row 5 by wave 3
+    // phase 2: slopes of row w+1, and of row STRIP+1 by the last wave
     double dq[NPR], dq5[NPR], p5[NPR];
     const bool row1_ok = blk_act && f0 - 1 + r1 < sn;  // rows r1-1 .. r1+1 loaded
     const bool row5_ok = blk_act && w == STRIP - 1 && f0 + STRIP + 1 <= sn;
@@ -430,7 +433,7 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
         }
     }
     __syncthreads();
-    // phase 3: left state of row w+1 (kept), right states of rows 2..5 to LDS
+    // phase 3: left state of row w+1 (kept), right states of rows 2..STRIP+1 to LDS
     double p_l[NPR];
     if (row1_ok) {
 #pragma unroll
@@ -483,8 +486,7 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
             }
         }
     }
-    (void)dstride;
-    if (fullstep) block_min(dtij, dtpart);
+    if (fullstep) block_min<STRIP_TPB>(dtij, dtpart);
     add_counter(cnt, HARM_FATAL, fatal);
 }
 
@@ -1793,8 +1795,8 @@ void flux_kernel_any(harm_ctx *c, dim3 &grid, const double *pb, double *F, int f
             const size_t no = (DIR == 1) ? (size_t)(g.je + 3 - g.js) : (size_t)(g.ie + 3 - g.is);
             const size_t nb = (nf + STRIP - 1) / STRIP * no * (size_t)(g.N3 / 64);
             grid = dim3((unsigned)((nb + 7) / 8 * 8));
-            hipLaunchKernelGGL((k_flux_strip<DIR == 3 ? 1 : DIR>), grid, dim3(TPB), 0, c->stream, c->g, pb, F,
-                               c->dtpart, fullstep, c->cnt);
+            hipLaunchKernelGGL((k_flux_strip<DIR == 3 ? 1 : DIR>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb,
+                               F, c->dtpart, fullstep, c->cnt);
         } else if (c->g.lim == 0 && DIR == 3 && TPB % c->g.N3 == 0)
             flux_kernel<DIR, true, false, 0, DIR == 3>(c, grid, pb, F, fullstep);
         else if (c->g.lim == 0)
