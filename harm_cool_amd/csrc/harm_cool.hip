@@ -359,8 +359,8 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
     add_counter(cnt, HARM_FATAL, fatal);
 }
 
-// The X1 / X2 fluxes of the wave-uniform Gamma-law MC path with the
-// reconstruction shared along the stencil direction.  A block is STRIP waves:
+// The X1 / X2 fluxes of the wave-uniform path with the reconstruction shared
+// along the stencil direction (TAB, LIM as k_flux).  A block is STRIP waves:
 // wave w computes face f0 + w (f0 = first face of the block's strip) for 64 phi
 // zones of one (i, j) column.  The block loads the STRIP+3 zone rows
 // f0-2 .. f0+STRIP once and forms the slopes of rows f0-1 .. f0+STRIP-1 once
@@ -370,7 +370,7 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
 constexpr int STRIP = 4;                // faces per block (8 and 16 measured 3% and 11% slower)
 constexpr int STRIP_TPB = 64 * STRIP;
 
-template <int DIR>
+template <int DIR, bool TAB, int LIM>
 __global__ void __launch_bounds__(STRIP_TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart,
              int fullstep, unsigned long long *__restrict__ cnt)
@@ -390,6 +390,7 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
     const int h = (int)seg * 64 + lane;
     const int f0 = fs + (int)s * STRIP;
     const int sn = (DIR == 1) ? g.SN1 : g.SN2;
+    const int lim = (LIM >= 0) ? LIM : g.lim;
     const size_t nz = g.nz;
     // zone of row r (r = 0..STRIP+2 is zone index f0 - 2 + r along DIR) in this lane's column
     auto zone = [&](int r) -> size_t {
@@ -423,13 +424,13 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
     const bool row5_ok = blk_act && w == STRIP - 1 && f0 + STRIP + 1 <= sn;
     if (row1_ok) {
 #pragma unroll
-        for (int k = 0; k < NPR; k++) dq[k] = slope_lim(0, B[r1 - 1][k][lane], pc[k], B[r1 + 1][k][lane]);
+        for (int k = 0; k < NPR; k++) dq[k] = slope_lim(lim, B[r1 - 1][k][lane], pc[k], B[r1 + 1][k][lane]);
     }
     if (row5_ok) {
 #pragma unroll
         for (int k = 0; k < NPR; k++) {
             p5[k] = B[STRIP + 1][k][lane];
-            dq5[k] = slope_lim(0, B[STRIP][k][lane], p5[k], B[STRIP + 2][k][lane]);
+            dq5[k] = slope_lim(lim, B[STRIP][k][lane], p5[k], B[STRIP + 2][k][lane]);
         }
     }
     __syncthreads();
@@ -464,7 +465,10 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
             Geom geom;
             load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<true>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
             double Fz[NPR], ctop;
-            face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
+            if constexpr (TAB)
+                face_flux(p_l, p_r, geom, DIR, EosTable{g.tab, g.ref_derivs}, g.hllf, g.laxf, Fz, ctop, fatal);
+            else
+                face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
             if (DIR == 1) {
                 // coord.c:310-320: no inflow of mass through the radial boundaries
                 if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
@@ -1784,22 +1788,36 @@ void flux_kernel(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullst
 template <int DIR>
 void flux_kernel_any(harm_ctx *c, dim3 &grid, const double *pb, double *F, int fullstep)
 {
-    const bool tab = c->eos_kind == 1;
-    if (c->uniform) {
+    const bool tab = c->eos_kind == 1, mc = c->g.lim == 0;
+    if (c->uniform && DIR != 3) {
+        // strips of STRIP faces along DIR x one other index x 64 phi zones
+        const DGrid &g = c->g;
+        const size_t nf = (DIR == 1) ? (size_t)(g.ie + 2 - g.is) : (size_t)(g.je + 2 - g.js);
+        const size_t no = (DIR == 1) ? (size_t)(g.je + 3 - g.js) : (size_t)(g.ie + 3 - g.is);
+        const size_t nb = (nf + STRIP - 1) / STRIP * no * (size_t)(g.N3 / 64);
+        grid = dim3((unsigned)((nb + 7) / 8 * 8));
+        constexpr int D = (DIR == 3) ? 1 : DIR;
+        if (tab)
+            hipLaunchKernelGGL((k_flux_strip<D, true, -1>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb, F,
+                               c->dtpart, fullstep, c->cnt);
+        else if (mc)
+            hipLaunchKernelGGL((k_flux_strip<D, false, 0>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb, F,
+                               c->dtpart, fullstep, c->cnt);
+        else
+            hipLaunchKernelGGL((k_flux_strip<D, false, -1>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb, F,
+                               c->dtpart, fullstep, c->cnt);
+    } else if (c->uniform && TPB % c->g.N3 == 0) {
+        // DIR 3, whole phi rows per block
+        if (tab)
+            flux_kernel<DIR, true, true, -1, DIR == 3>(c, grid, pb, F, fullstep);
+        else if (mc)
+            flux_kernel<DIR, true, false, 0, DIR == 3>(c, grid, pb, F, fullstep);
+        else
+            flux_kernel<DIR, true, false, -1, DIR == 3>(c, grid, pb, F, fullstep);
+    } else if (c->uniform) {
         if (tab)
             flux_kernel<DIR, true, true>(c, grid, pb, F, fullstep);
-        else if (c->g.lim == 0 && DIR != 3) {
-            // strips of STRIP faces along DIR x one other index x 64 phi zones
-            const DGrid &g = c->g;
-            const size_t nf = (DIR == 1) ? (size_t)(g.ie + 2 - g.is) : (size_t)(g.je + 2 - g.js);
-            const size_t no = (DIR == 1) ? (size_t)(g.je + 3 - g.js) : (size_t)(g.ie + 3 - g.is);
-            const size_t nb = (nf + STRIP - 1) / STRIP * no * (size_t)(g.N3 / 64);
-            grid = dim3((unsigned)((nb + 7) / 8 * 8));
-            hipLaunchKernelGGL((k_flux_strip<DIR == 3 ? 1 : DIR>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb,
-                               F, c->dtpart, fullstep, c->cnt);
-        } else if (c->g.lim == 0 && DIR == 3 && TPB % c->g.N3 == 0)
-            flux_kernel<DIR, true, false, 0, DIR == 3>(c, grid, pb, F, fullstep);
-        else if (c->g.lim == 0)
+        else if (mc)
             flux_kernel<DIR, true, false, 0>(c, grid, pb, F, fullstep);
         else
             flux_kernel<DIR, true, false>(c, grid, pb, F, fullstep);
