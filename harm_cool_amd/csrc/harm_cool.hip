@@ -69,6 +69,21 @@ __device__ __forceinline__ size_t zidx(const DGrid &g, int i, int j, int h)
     return ((size_t)i * g.SN2 + j) * g.N3 + h;
 }
 
+// Element zz of an array whose base is wave-uniform, addressed as base plus a
+// 32-bit byte offset (the SGPR-base form of global memory instructions: one
+// offset register serves the same zone of every variable).  Valid while
+// 8 * nz < 2^32 (OFF32_MAX_ZONES), which the host checks before it selects
+// the kernels that use it.
+constexpr size_t OFF32_MAX_ZONES = (size_t)1 << 29;
+__device__ __forceinline__ const double &at32(const double *base, unsigned zz)
+{
+    return *(const double *)((const char *)base + (size_t)(zz * 8u));
+}
+__device__ __forceinline__ double &at32(double *base, unsigned zz)
+{
+    return *(double *)((char *)base + (size_t)(zz * 8u));
+}
+
 // Logical block of this workgroup.  Consecutive hardware workgroups are
 // dispatched to the 8 XCDs in turn, and each XCD has its own L2; with this
 // mapping each XCD walks one contiguous range of logical blocks, so the stencil
@@ -608,6 +623,51 @@ __device__ __forceinline__ void flux_div(const DGrid &g, const DivArgs &a, int i
     }
 }
 
+// flux_div with 32-bit zone offsets (nz < OFF32_MAX_ZONES, N3 > 2); the same
+// operations in the same order.
+__device__ __forceinline__ void flux_div32(const DGrid &g, const DivArgs &a, int i, int j, int h, double *d)
+{
+    const unsigned sI = (unsigned)g.SN2 * (unsigned)g.N3, sJ = (unsigned)g.N3;
+    const unsigned z = (unsigned)zidx(g, i, j, h);
+    const unsigned zh1 = z - (unsigned)h + (unsigned)((h + 1) % g.N3);
+    const unsigned zi1 = z + sI, zj1 = z + sJ, zi1h1 = zh1 + sI, zj1h1 = zh1 + sJ, zi1j1 = zi1 + sJ;
+    const size_t nz = g.nz;
+    // ct_f1 at (i,j,h) and (i+1,j,h), ct_f2 at (i,j,h) and (i,j+1,h), ct_f3 at (i,j,h) and (i,j,h+1)
+    const double f1a1 = 0.5 * (at32(a.e3, z) + at32(a.e3, zj1)), f1a2 = -0.5 * (at32(a.e2, z) + at32(a.e2, zh1));
+    const double f1b1 = 0.5 * (at32(a.e3, zi1) + at32(a.e3, zi1j1)),
+                 f1b2 = -0.5 * (at32(a.e2, zi1) + at32(a.e2, zi1h1));
+    const double f2a0 = -0.5 * (at32(a.e3, z) + at32(a.e3, zi1)), f2a2 = 0.5 * (at32(a.e1, z) + at32(a.e1, zh1));
+    const double f2b0 = -0.5 * (at32(a.e3, zj1) + at32(a.e3, zi1j1)),
+                 f2b2 = 0.5 * (at32(a.e1, zj1) + at32(a.e1, zj1h1));
+    const double f3a0 = 0.5 * (at32(a.e2, z) + at32(a.e2, zi1)), f3a1 = -0.5 * (at32(a.e1, z) + at32(a.e1, zj1));
+    const double f3b0 = 0.5 * (at32(a.e2, zh1) + at32(a.e2, zi1h1)),
+                 f3b1 = -0.5 * (at32(a.e1, zh1) + at32(a.e1, zj1h1));
+    const double f1a[3] = {0., f1a1, f1a2}, f1b[3] = {0., f1b1, f1b2};
+    const double f2a[3] = {f2a0, 0., f2a2}, f2b[3] = {f2b0, 0., f2b2};
+    const double f3a[3] = {f3a0, f3a1, 0.}, f3b[3] = {f3b0, f3b1, 0.};
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        double a1, b1, a2, b2, a3, b3;
+        if (k < B1) {
+            const double *F1 = a.F1 + k * nz, *F2 = a.F2 + k * nz, *F3 = a.F3 + k * nz;
+            a1 = at32(F1, z);
+            b1 = at32(F1, zi1);
+            a2 = at32(F2, z);
+            b2 = at32(F2, zj1);
+            a3 = at32(F3, z);
+            b3 = at32(F3, zh1);
+        } else {
+            a1 = f1a[k - B1];
+            b1 = f1b[k - B1];
+            a2 = f2a[k - B1];
+            b2 = f2b[k - B1];
+            a3 = f3a[k - B1];
+            b3 = f3b[k - B1];
+        }
+        d[k] = -div_r(b1 - a1, g.dx1, g.rdx1) - div_r(b2 - a2, g.dx2, g.rdx2) - div_r(b3 - a3, g.dx3, g.rdx3);
+    }
+}
+
 // The divergence as its own light, high-occupancy stencil pass into the compact
 // array S[k][idx] (idx = active-zone index in the update kernel's order), for
 // the update variants that read it instead of forming it.
@@ -683,7 +743,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         {
             double pbz[NPR];
 #pragma unroll
-            for (int k = 0; k < NPR; k++) pbz[k] = pb[k * nz + z];
+            for (int k = 0; k < NPR; k++) pbz[k] = FUSED ? at32(pb + k * nz, (unsigned)z) : pb[k * nz + z];
             State q;
             if constexpr (TAB) {
                 const EosTable eos{g.tab, g.ref_derivs};
@@ -698,7 +758,8 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
                     source(pbz, geom, g.conn, g.ncol, col, g.gam, dU, q, fatal);
             }
 #pragma unroll
-            for (int k = 0; k < NPR; k++) piz[k] = pi_is_pb ? pbz[k] : pi[k * nz + z];
+            for (int k = 0; k < NPR; k++)
+                piz[k] = pi_is_pb ? pbz[k] : (FUSED ? at32(pi + k * nz, (unsigned)z) : pi[k * nz + z]);
             if (!pi_is_pb) get_state(piz, geom, q, fatal);
             if constexpr (TAB)
                 primtoflux(piz, q, 0, geom, EosTable{g.tab, g.ref_derivs}, U);
@@ -707,7 +768,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         }
         if constexpr (FUSED) {
             double d[NPR];
-            flux_div(g, dv, i, j, h, d);
+            flux_div32(g, dv, i, j, h, d);
 #pragma unroll
             for (int k = 0; k < NPR; k++) U[k] += Dt * (d[k] + dU[k]);
         } else {
@@ -801,7 +862,12 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         }
         const size_t nz = g.nz;
 #pragma unroll
-        for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
+        for (int k = 0; k < NPR; k++) {
+            if (FUSED)
+                at32(pf + k * nz, (unsigned)z) = pr[k];
+            else
+                pf[k * nz + z] = pr[k];
+        }
         pflag[z] = flag;
     }
     const int v[8] = {c_call, c_iter, c_kconv, c_f2d, c_gam, c_flr, c_cap, fatal};
@@ -1917,7 +1983,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     }
     const DivArgs dv = {c->F1, c->F2, c->F3, c->e1, c->e2, c->e3};
     // the plain wave-uniform update forms the divergence itself; the others read S
-    const bool fused = c->eos_kind != 1 && c->uniform && mode == UPD_PLAIN;
+    const bool fused = c->eos_kind != 1 && c->uniform && mode == UPD_PLAIN && c->nz < OFF32_MAX_ZONES;
     if (!fused)
         LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, dv, c->S));
 #define UPD_LAUNCH(U_, M_, T_, F_)                                                                                  \
