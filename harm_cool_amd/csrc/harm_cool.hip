@@ -877,6 +877,9 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
 // Font fix (step_ch.c:277-300) + fixup1zone for the zones k_update deferred.
 // Grid-stride over the list; the list count is reset by the next kernel on the
 // stream (k_bound), after every thread here has read it.
+// A small grid: the list is usually empty or short, and a full-size launch of
+// this register-heavy kernel costs tens of microseconds even then.
+constexpr int FONTFIX_BLOCKS = 16;
 __global__ void __launch_bounds__(TPB) k_fontfix(DGrid g, double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                  FontList fl, unsigned long long *__restrict__ cnt)
 {
@@ -2023,7 +2026,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
         HIPCHK(hipGetLastError());
     }
     if (g.font_fix)
-        LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(256), dim3(TPB), 0, c->stream, g, pf, c->pflag,
+        LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(FONTFIX_BLOCKS), dim3(TPB), 0, c->stream, g, pf, c->pflag,
                                                   c->font, c->cnt));
     if ((rc = bound(c, pf))) return rc;
     if ((rc = fixup_utoprim(c, pf))) return rc;
