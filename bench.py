@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+C5_BYTES = 64 + 128 + 128 + 8 + 64 + 64 + 3  # batched inversion, per zone (see stress())
 
 # Algorithmic HBM bytes per zone of each kernel class (one launch covers every
 # active zone of one half-step).
@@ -177,8 +178,12 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
             ret, it, kc = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], prim, gam, mlp=net)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
+            # algorithmic bytes per zone: U 64, gcov 128, gcon 128, gdet 8, guess in 64 and
+            # result out 64, ret/iters/k_conv 3
+            gbps = n * C5_BYTES / el / 1e9
             rows[label] = {"n_iter_per_zone": it.double().mean().item(), "k_conv_per_zone": kc.double().mean().item(),
-                           "fail_rate": (ret != 0).double().mean().item(), "ms": round(el * 1e3, 2)}
+                           "fail_rate": (ret != 0).double().mean().item(), "ms": round(el * 1e3, 2),
+                           "alg_GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
             if kind == 0:
                 keep = slice(0, 1 << 20)
                 sample[label] = (ret[keep].cpu().numpy(), prim[:, keep].cpu().numpy())
