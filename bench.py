@@ -137,6 +137,7 @@ def learned_torus(g, hc, torch, nz, steps, capture_steps, train_steps, batch):
     fd, td, vd = feat[:, :n].contiguous(), targ[:, :n].contiguous(), valid[:n].contiguous()
     del feat, targ, valid
     m.fit_norm(fd, td, vd)
+    m.set_loss_weights(1.0, 0.0)  # the seeded solve takes v^2 from W: train the W output only
     t0 = time.perf_counter()
     lt = m.train(fd, td, vd, batch=batch, steps=train_steps, lr0=2e-3, lr1=5e-5, trace=True)
     t_train = time.perf_counter() - t0
@@ -162,6 +163,7 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
     del tr, ptr, rtr
     m = hc.Mlp(seed=13)
     m.fit_norm(feat, targ, valid)
+    m.set_loss_weights(1.0, 0.0)
     lt = m.train(feat, targ, valid, batch=batch, steps=train_steps, lr0=2e-3, lr1=5e-5, trace=True)
     del feat, targ, valid
     res["stress_net"] = {"train_pairs": ntr, "loss_first": float(lt[:10].mean()), "loss_last": float(lt[-50:].mean())}

@@ -264,6 +264,7 @@ def lib() -> ctypes.CDLL:
     L.harm_mlp_set_norm.argtypes = [c_void_p, _dp]
     L.harm_mlp_get_norm.argtypes = [c_void_p, _dp]
     L.harm_mlp_set_vsq_from_w.argtypes = [c_void_p, c_int]
+    L.harm_mlp_set_loss_weights.argtypes = [c_void_p, ctypes.c_float, ctypes.c_float]
     L.harm_mlp_fit_norm.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t]
     L.harm_mlp_train.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int, c_int, ctypes.c_float,
                                  ctypes.c_float, ctypes.c_uint64, _fp]
@@ -566,6 +567,9 @@ class Mlp:
 
     def set_vsq_from_w(self, on: bool) -> None:
         _check(self.L.harm_mlp_set_vsq_from_w(self.h, int(on)), "harm_mlp_set_vsq_from_w")
+
+    def set_loss_weights(self, w_logw: float, w_vsq: float) -> None:
+        _check(self.L.harm_mlp_set_loss_weights(self.h, w_logw, w_vsq), "harm_mlp_set_loss_weights")
 
     def fit_norm(self, feat, targ, valid) -> None:
         _sync_torch()

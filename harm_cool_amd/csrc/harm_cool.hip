@@ -691,7 +691,7 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, DivArgs a, double *__restr
 struct UpdateExtra {
     MlpDev mlp;
     double *cap_feat;   // [8][cap] raw features
-    double *cap_targ;   // [2][cap] log(W/D), v^2
+    double *cap_targ;   // [2][cap] log(W/Wref), v^2
     uint8_t *cap_valid; // [cap]
     size_t cap, cap_base;
     int stride, phase;
@@ -802,7 +802,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
         mlp_forward_wave(ex.mlp, x, y0, y1);
         if (active) {
             double W0 = 0., vsq0 = 0.;
-            if (ok) mlp_unstandardize(ex.mlp, y0, y1, exp(f[0]), W0, vsq0);
+            if (ok) mlp_unstandardize(ex.mlp, y0, y1, mlp_wref(f), W0, vsq0);
             int fb;
             ret = utoprim_2d_learned(U, geom, g.gam, pr, W0, vsq0, ex.mlp.vsq_from_W, niter, kconv, fb);
         }
@@ -813,11 +813,11 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
                 const size_t slot = ex.cap_base + idx / (size_t)ex.stride;
                 if (slot < ex.cap) {
                     double f[MLP_NIN];
-                    const bool ok = mlp_features_U(U, geom, f) && ret == 0;
+                    const bool ok = mlp_features_U(U, geom, f) && ret == 0 && mlp_wref(f) > 0.;
                     double t0 = 0., t1 = 0.;
                     if (ok) {
                         const double utsq = utsq_of(geom, pr);
-                        t0 = log((pr[RHO] + g.gam * pr[UU]) * (1. + utsq)) - f[0];
+                        t0 = log((pr[RHO] + g.gam * pr[UU]) * (1. + utsq)) - log(mlp_wref(f));
                         t1 = utsq / (1. + utsq);
                     }
 #pragma unroll
@@ -1498,7 +1498,7 @@ __global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U,
         mlp_forward_wave(mlp, x, y0, y1);
         if (!act) return;
         double W0 = 0., vsq0 = 0.;
-        if (ok) mlp_unstandardize(mlp, y0, y1, exp(f[0]), W0, vsq0);
+        if (ok) mlp_unstandardize(mlp, y0, y1, mlp_wref(f), W0, vsq0);
         int fb;
         r = utoprim_2d_learned(Uz, geom, gam, pr, W0, vsq0, mlp.vsq_from_W, niter, kc, fb);
     } else {

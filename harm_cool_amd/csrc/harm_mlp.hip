@@ -32,6 +32,7 @@ struct harm_mlp {
     float *bias;
     double *norm;
     int vsq_from_W;
+    float loss_w[2];  // weights of the two outputs in the training loss
     long long adam_t;
     // training workspace, sized for wsB rows
     int wsB;
@@ -123,8 +124,8 @@ __global__ void k_fill(float *__restrict__ p, float v, int n)
 
 // dO = mask (O - T) / B; per-block partial sums of mask |O - T|^2.
 __global__ void __launch_bounds__(256) k_loss(const float *__restrict__ O, const float *__restrict__ T,
-                                              const uint8_t *__restrict__ M, int B, float *__restrict__ dO,
-                                              float *__restrict__ lossp)
+                                              const uint8_t *__restrict__ M, int B, float lw0, float lw1,
+                                              float *__restrict__ dO, float *__restrict__ lossp)
 {
     __shared__ float red[256];
     const int b = blockIdx.x * 256 + threadIdx.x;
@@ -132,9 +133,9 @@ __global__ void __launch_bounds__(256) k_loss(const float *__restrict__ O, const
     if (b < B) {
         const float w = M ? (float)M[b] : 1.f;
         const float e0 = O[2 * b] - T[2 * b], e1 = O[2 * b + 1] - T[2 * b + 1];
-        dO[2 * b] = w * e0 / (float)B;
-        dO[2 * b + 1] = w * e1 / (float)B;
-        l = w * (e0 * e0 + e1 * e1);
+        dO[2 * b] = w * lw0 * e0 / (float)B;
+        dO[2 * b + 1] = w * lw1 * e1 / (float)B;
+        l = w * (lw0 * e0 * e0 + lw1 * e1 * e1);
     }
     red[threadIdx.x] = l;
     __syncthreads();
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(256) k_seed(MlpDev m, const double *__restrict
 #pragma unroll
         for (int k = 0; k < NPR; k++) Uz[k] = U[(size_t)k * n + z];
         ok = mlp_features_U(Uz, geom, f);
-        if (ok) D = exp(f[0]);
+        if (ok) D = mlp_wref(f);
         if (feat)
 #pragma unroll
             for (int k = 0; k < MLP_NIN; k++) feat[(size_t)k * n + z] = f[k];
@@ -372,11 +373,11 @@ __global__ void __launch_bounds__(256) k_pairs(const double *__restrict__ U, con
         Uz[k] = U[(size_t)k * n + z];
         pr[k] = prim[(size_t)k * n + z];
     }
-    const bool ok = mlp_features_U(Uz, geom, f) && ret[z] == 0;
+    const bool ok = mlp_features_U(Uz, geom, f) && ret[z] == 0 && mlp_wref(f) > 0.;
     double t0 = 0., t1 = 0.;
     if (ok) {
         const double utsq = utsq_of(geom, pr);
-        t0 = log((pr[RHO] + gam * pr[UU]) * (1. + utsq)) - f[0];
+        t0 = log((pr[RHO] + gam * pr[UU]) * (1. + utsq)) - log(mlp_wref(f));
         t1 = utsq / (1. + utsq);
     }
 #pragma unroll
@@ -501,7 +502,8 @@ int loss_grad(harm_mlp *m, const float *X, const float *T, const uint8_t *M, int
 {
     int rc;
     if ((rc = forward_f32(m, X, B))) return rc;
-    hipLaunchKernelGGL(k_loss, dim3(cdiv(B, 256)), dim3(256), 0, m->stream, m->O, T, M, B, m->dO, m->lossp);
+    hipLaunchKernelGGL(k_loss, dim3(cdiv(B, 256)), dim3(256), 0, m->stream, m->O, T, M, B, m->loss_w[0],
+                       m->loss_w[1], m->dO, m->lossp);
     MCHK(hipGetLastError());
     const float *P = m->params;
     float *G = m->grad;
@@ -617,6 +619,7 @@ harm_mlp *harm_mlp_create(int device)
     std::memset(m, 0, sizeof(*m));
     (void)hipGetDevice(&m->device);
     m->vsq_from_W = 1;
+    m->loss_w[0] = m->loss_w[1] = 1.f;
     bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && !dalloc(&m->params, MLP_NPARAMS) && !dalloc(&m->adam_m, MLP_NPARAMS) &&
          !dalloc(&m->adam_v, MLP_NPARAMS) && !dalloc(&m->grad, MLP_NPARAMS) && !dalloc(&m->wpack, MLP_CHUNKS * 64) &&
@@ -719,6 +722,15 @@ int harm_mlp_set_vsq_from_w(harm_mlp *m, int on)
 {
     if (!m) return harm_host::set_error("null network", 1);
     m->vsq_from_W = on ? 1 : 0;
+    return 0;
+}
+
+int harm_mlp_set_loss_weights(harm_mlp *m, float w_logw, float w_vsq)
+{
+    if (!m) return harm_host::set_error("null network", 1);
+    if (!(w_logw >= 0.f) || !(w_vsq >= 0.f)) return harm_host::set_error("loss weights must be >= 0", 4);
+    m->loss_w[0] = w_logw;
+    m->loss_w[1] = w_vsq;
     return 0;
 }
 

@@ -7,8 +7,8 @@
 // fallback when the seeded solve fails.
 //
 // Network: 8 inputs -> 64 tanh -> 64 tanh -> 64 tanh -> 2 outputs
-// (log(W/D), v^2), all standardized by per-feature mean/std fitted on the
-// training pairs.  FP32 master parameters, row-major W[out][in]:
+// (log(W/Wref), v^2; Wref = E - B^2/2, mlp_wref), all standardized by
+// per-feature mean/std fitted on the training pairs.  FP32 master parameters, row-major W[out][in]:
 //   W1 64x8 | b1 64 | W2 64x64 | b2 64 | W3 64x64 | b3 64 | W4 2x64 | b4 2.
 //
 // Inference runs one wavefront over 64 zones as four 16-zone tiles with
@@ -72,6 +72,12 @@ HD void mlp_features(const U2P &s, double lapse, double *f)
     f[6] = asinh((E - D - 0.5 * s.Bsq) / D);
     f[7] = asinh(s.Bsq / E);
 }
+
+// Reference scale of the first output, Wref = E - B^2/2 = D (sinh f1 - sinh f3 / 2):
+// W = Wref + p - B^2 v^2/2 + (Q.B)^2/(2W^2), so log(W/Wref) spans a far
+// narrower range than log(W/D) (standard deviations 0.32 and 1.7 on the
+// inversion stress distribution).  No seed where it is not positive.
+HD double mlp_wref(const double *f) { return exp(f[0]) * (sinh(f[1]) - 0.5 * sinh(f[3])); }
 
 // Raw features of HARM's conserved vector U (x gdet); false when U[0] <= 0.
 HD bool mlp_features_U(const double *U, const Geom &geom, double *f)
@@ -197,11 +203,11 @@ __device__ __forceinline__ void mlp_standardize(const MlpDev &m, const double *f
     for (int k = 0; k < MLP_NIN; k++) x[k] = (float)((f[k] - m.norm[k]) * m.norm[8 + k]);
 }
 
-// Seed (W0, vsq0) from the standardized outputs; D = U_tmp[RHO].
-__device__ __forceinline__ void mlp_unstandardize(const MlpDev &m, float y0, float y1, double D, double &W0,
+// Seed (W0, vsq0) from the standardized outputs; wref = mlp_wref(features).
+__device__ __forceinline__ void mlp_unstandardize(const MlpDev &m, float y0, float y1, double wref, double &W0,
                                                   double &vsq0)
 {
-    W0 = D * exp((double)y0 * m.norm[18] + m.norm[16]);
+    W0 = (wref > 0.) ? wref * exp((double)y0 * m.norm[18] + m.norm[16]) : 0.;
     vsq0 = (double)y1 * m.norm[19] + m.norm[17];
 }
 

@@ -154,7 +154,7 @@ int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon
  * Inputs (raw, standardized by harm_mlp_fit_norm): log D, asinh(E/D),
  * asinh(Qt^2/D^2), asinh(B^2/D), asinh((Q.B)^2/D^3), lapse, asinh((E-D-B^2/2)/D),
  * asinh(B^2/E) with D, Q, B of utoprim_2d.c:240-266 and E = -Q.n.
- * Outputs: log(W/D), v^2.  Device buffers are on the handle's device. */
+ * Outputs: log(W/Wref) (Wref = E - B^2/2 from the features), v^2.  Device buffers are on the handle's device. */
 #define HARM_MLP_NPARAMS 9026
 #define HARM_MLP_NORM 20      /* feature mean[8], 1/std[8], output mean[2], std[2] */
 typedef struct harm_mlp harm_mlp;
@@ -169,8 +169,11 @@ int harm_mlp_get_norm(harm_mlp *m, double *norm);
 /* 1: seed v^2 from the predicted W as utoprim_2d.c:306-307 does (default);
  * 0: use the network's second output. */
 int harm_mlp_set_vsq_from_w(harm_mlp *m, int on);
+/* Weights of the two outputs in the training loss (default 1, 1); with v^2 taken
+ * from W the second output is unused and (1, 0) spends the network on W. */
+int harm_mlp_set_loss_weights(harm_mlp *m, float w_logw, float w_vsq);
 /* Training set (device): feat[8][n] raw FP64 features, targ[2][n] raw FP64
- * targets (log(W/D), v^2), valid[n] (0 rows carry no loss). */
+ * targets (log(W/Wref), v^2), valid[n] (0 rows carry no loss). */
 int harm_mlp_fit_norm(harm_mlp *m, const double *feat, const double *targ, const uint8_t *valid, size_t n);
 /* steps Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on batches of `batch`
  * rows drawn with replacement from the set by a seeded hash; learning rate
@@ -194,7 +197,7 @@ int harm_mlp_seed(harm_mlp *m, const double *U, const double *gcov, const double
                   size_t n, double *feat, double *W0, double *vsq0, void *stream);
 
 /* Training pairs from solved zones (device, SoA as harm_utoprim2d_batch):
- * feat[8][n] raw features of U, targ[2][n] = (log(W/D), v^2) of the converged
+ * feat[8][n] raw features of U, targ[2][n] = (log(W/Wref), v^2) of the converged
  * primitives prim, valid[n] = (ret[n] == 0). */
 int harm_mlp_pairs(const double *U, const double *gcov, const double *gcon, const double *gdet, const double *prim,
                    const int8_t *ret, size_t n, double gam, double *feat, double *targ, uint8_t *valid, void *stream);

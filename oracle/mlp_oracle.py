@@ -44,13 +44,14 @@ def forward(params: np.ndarray, X: np.ndarray):
     return O, [X, H1, H2, H3]
 
 
-def loss_grad(params: np.ndarray, X: np.ndarray, T: np.ndarray, mask=None):
-    """Loss and gradient (FP64) of 1/(2B) sum mask |O - T|^2."""
+def loss_grad(params: np.ndarray, X: np.ndarray, T: np.ndarray, mask=None, out_w=(1.0, 1.0)):
+    """Loss and gradient (FP64) of 1/(2B) sum mask sum_j out_w[j] (O_j - T_j)^2."""
     B = X.shape[0]
     w = np.ones(B) if mask is None else np.asarray(mask, np.float64)
     O, (X, H1, H2, H3) = forward(params, X)
-    E = O - np.asarray(T, np.float64)
+    E = (O - np.asarray(T, np.float64)) * np.sqrt(np.asarray(out_w, np.float64))[None, :]
     loss = 0.5 * np.sum(w[:, None] * E * E) / B
+    E = E * np.sqrt(np.asarray(out_w, np.float64))[None, :]
     W1, b1, W2, b2, W3, b3, W4, b4 = [np.asarray(a, np.float64) for a in split(params)]
     dO = w[:, None] * E / B
     gW4 = dO.T @ H3

@@ -69,6 +69,13 @@ def test_fp32_forward_and_gradient_match_oracle(n):
     l_ref, g_ref = mo.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64), mask)
     assert abs(loss - l_ref) <= 1e-5 * l_ref
     _per_tensor_close(g, g_ref, 1e-4)
+    # weighted outputs (the W-only training of the seed uses (1, 0))
+    for ow in ((1.0, 0.0), (0.25, 2.0)):
+        m.set_loss_weights(*ow)
+        loss, g = m.grad(Xd, Td, Md)
+        l_ref, g_ref = mo.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64), mask, ow)
+        assert abs(loss - l_ref) <= 1e-5 * l_ref
+        _per_tensor_close(g, g_ref, 1e-4)
     m.close()
     torch.cuda.synchronize()
 
@@ -156,7 +163,9 @@ def _trained_on_golden(vsq_from_w):
                            _ptr(np.ascontiguousarray(g["gcon"][ok][z])), float(g["gdet"][ok][z]), _ptr(feat[z]))
     ut = P[:, 2:5]
     utsq = np.einsum("ni,nij,nj->n", ut, gc[:, 1:, 1:], ut)
-    targ = np.stack([np.log((P[:, 0] + gam * P[:, 1]) * (1 + utsq)) - feat[:, 0], utsq / (1 + utsq)])
+    wref = np.exp(feat[:, 0]) * (np.sinh(feat[:, 1]) - 0.5 * np.sinh(feat[:, 3]))  # E - B^2/2
+    assert (wref > 0).all()
+    targ = np.stack([np.log((P[:, 0] + gam * P[:, 1]) * (1 + utsq)) - np.log(wref), utsq / (1 + utsq)])
     m = hc.Mlp(seed=9)
     fd, td = _dev(feat.T), _dev(targ)
     vd = _dev(np.ones(len(U), np.uint8))
@@ -235,7 +244,8 @@ def test_capture_is_passive_and_consistent():
     v = valid[:want].cpu().numpy().astype(bool)
     assert v.mean() > 0.99
     assert np.isfinite(f[:, v]).all()
-    assert (t[0, v] >= 0).all() and (t[1, v] >= 0).all() and (t[1, v] < 1).all()
+    # log(W/Wref) lies within a few units of 0; v^2 in [0, 1)
+    assert (np.abs(t[0, v]) < 10).all() and (t[1, v] >= 0).all() and (t[1, v] < 1).all()
     a.close()
     b.close()
 

@@ -46,6 +46,30 @@ def test_backward_matches_finite_differences():
         assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(g[k]), (k, fd, g[k])
 
 
+def test_output_weights():
+    # a zero weight removes the output from loss and gradient; the weighted loss
+    # equals the per-output losses combined; its gradient matches finite differences
+    p = mo.init_params(5).astype(np.float64)
+    p += 0.05 * np.random.default_rng(7).standard_normal(p.size)
+    X, T = _batch(12, 8)
+    l0, g0 = mo.loss_grad(p, X, T, None, (1.0, 0.0))
+    T2 = T.copy()
+    T2[:, 1] += 50.0
+    l0b, g0b = mo.loss_grad(p, X, T2, None, (1.0, 0.0))
+    assert l0 == l0b and np.array_equal(g0, g0b)
+    la, _ = mo.loss_grad(p, X, T, None, (1.0, 0.0))
+    lb, _ = mo.loss_grad(p, X, T, None, (0.0, 1.0))
+    lw, gw = mo.loss_grad(p, X, T, None, (0.3, 2.0))
+    assert abs(lw - (0.3 * la + 2.0 * lb)) <= 1e-12 * lw
+    h = 1e-6
+    for k in np.random.default_rng(9).choice(mo.NPARAMS, 100, replace=False):
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (mo.loss_grad(pp, X, T, None, (0.3, 2.0))[0] - mo.loss_grad(pm, X, T, None, (0.3, 2.0))[0]) / (2 * h)
+        assert abs(fd - gw[k]) <= 1e-7 + 1e-5 * abs(gw[k]), (k, fd, gw[k])
+
+
 def test_masked_rows_carry_no_gradient():
     p = mo.init_params(1).astype(np.float64)
     X, T = _batch(8, 2)
