@@ -1986,7 +1986,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     }
     const DivArgs dv = {c->F1, c->F2, c->F3, c->e1, c->e2, c->e3};
     // the plain wave-uniform update forms the divergence itself; the others read S
-    const bool fused = c->uniform && (mode == UPD_PLAIN || c->eos_kind == 1) && c->nz < OFF32_MAX_ZONES;
+    const bool fused = c->uniform && (mode != UPD_CAPTURE || c->eos_kind == 1) && c->nz < OFF32_MAX_ZONES;
     if (!fused)
         LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, dv, c->S));
 #define UPD_LAUNCH(U_, M_, T_, F_)                                                                                  \
@@ -2001,12 +2001,14 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
         else
             UPD_LAUNCH(false, UPD_PLAIN, true, false);
     } else if (c->uniform) {
-        if (fused)
+        if (mode == UPD_PLAIN && fused)
             UPD_LAUNCH(true, UPD_PLAIN, false, true);
         else if (mode == UPD_PLAIN)
             UPD_LAUNCH(true, UPD_PLAIN, false, false);
         else if (mode == UPD_CAPTURE)
             UPD_LAUNCH(true, UPD_CAPTURE, false, false);
+        else if (fused)
+            UPD_LAUNCH(true, UPD_LEARNED, false, true);
         else
             UPD_LAUNCH(true, UPD_LEARNED, false, false);
     } else {
