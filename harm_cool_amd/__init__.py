@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int64, c_size_t, c_uint8, c_void_p
 
 import numpy as np
@@ -77,6 +78,20 @@ class RestartHdr(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DiagRec(ctypes.Structure):
+    """harm_diag_rec: the ener.out record of diag() (diag.c:59-326)."""
+    _fields_ = [(n, c_double) for n in (
+        "t", "rmed", "pp", "e", "mdot", "edot", "ldot", "lnu", "lnu_rho", "lnu_rho_2", "lnu2",
+        "mdot_in", "mdot_out", "mass_start", "mass_curr", "mass_in", "mass_out", "divbmax")] + [
+        ("imax", ctypes.c_int32 * 3), ("flux_valid", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["imax"] = tuple(self.imax)
+        d["flux_valid"] = bool(self.flux_valid)
+        return d
 
 
 def restart_peek(path: str) -> dict:
@@ -243,6 +258,9 @@ def lib() -> ctypes.CDLL:
     L.harm_set_time.argtypes = [c_void_p, c_double, c_double]
     L.harm_get_counters.argtypes = [c_void_p, POINTER(c_int64)]
     L.harm_diag.argtypes = [c_void_p, _dp]
+    L.harm_diag_log.argtypes = [c_void_p, POINTER(DiagRec)]
+    L.harm_ener_line.argtypes = [POINTER(DiagRec), c_char_p, c_size_t]
+    L.harm_set_mass_budget.argtypes = [c_void_p, c_double, c_double, c_double]
     L.harm_device_state.argtypes = [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p)]
     L.harm_set_timing.argtypes = [c_void_p, c_int]
     L.harm_get_timing.argtypes = [c_void_p, _dp, POINTER(c_int64)]
@@ -385,6 +403,35 @@ class Harm:
         _check(self.L.harm_diag(self.h, _ptr(out)), "harm_diag")
         return dict(mass=out[0], energy=out[1], mdot_in=out[2], mdot_out=out[3],
                     mass_in=out[4], mass_out=out[5], divb_max=out[6])
+
+    def diag_log(self) -> DiagRec:
+        """diag(LOG_OUT)'s record (diag.c:59-326): conserved totals, |div B| max
+        and its zone, the inner-face fluxes of the last full step, the mass
+        budget."""
+        r = DiagRec()
+        _check(self.L.harm_diag_log(self.h, ctypes.byref(r)), "harm_diag_log")
+        return r
+
+    def ener_line(self, rec: "DiagRec | None" = None) -> str:
+        """One ener.out line (diag.c:317-326) of rec, or of a fresh record."""
+        r = rec if rec is not None else self.diag_log()
+        buf = ctypes.create_string_buffer(1024)
+        n = self.L.harm_ener_line(ctypes.byref(r), buf, len(buf))
+        if n < 0:
+            raise HarmError("harm_ener_line: line longer than its buffer")
+        return buf.value.decode()
+
+    def log_out(self, ener_path: str) -> DiagRec:
+        """diag(LOG_OUT): append the ener.out line and print the LOG line
+        (diag.c:310-327) to stderr."""
+        r = self.diag_log()
+        with open(ener_path, "a") as f:
+            f.write(self.ener_line(r))
+        sys.stderr.write("LOG      t=%g \t divbmax: %d %d %d %g\n" % (r.t, r.imax[0], r.imax[1], r.imax[2], r.divbmax))
+        return r
+
+    def set_mass_budget(self, mass_start: float, mass_in: float = 0.0, mass_out: float = 0.0) -> None:
+        _check(self.L.harm_set_mass_budget(self.h, mass_start, mass_in, mass_out), "harm_set_mass_budget")
 
     def set_timing(self, on: bool) -> None:
         _check(self.L.harm_set_timing(self.h, 1 if on else 0), "harm_set_timing")

@@ -1380,13 +1380,22 @@ __global__ void __launch_bounds__(TPB) k_init_B(DGrid g, double *__restrict__ p,
     if (threadIdx.x == 0) bsq_part[blockIdx.x] = s[0];
 }
 
-// Diagnostics (diag.c:115-209): per-block sums of mass and energy, max |divB|.
+// Diagnostics of diag() (diag.c:115-209) and diag_mass (diag.c:485-488): per
+// block, the sums of U[RHO] dV, U[U3] dV, U[UU] dV, rho gdet dx1 dx2 dx3 and
+// gdet rho (the COOL_EOS luminosity weight, diag.c:203), the largest |div B|
+// and the ZLOOP rank ((h ni + i - is) nj + j - js) of the first zone that
+// attains it (diag.c:190-195 keeps the first strict maximum).
+constexpr int DIAG_NPART = 7;
+constexpr double DIAG_NO_KEY = 9007199254740992.;  // 2^53, above every rank
+
+template <bool TAB>
 __global__ void __launch_bounds__(TPB) k_diag(DGrid g, const double *__restrict__ p, double *__restrict__ part)
 {
-    __shared__ double s0[TPB], s1[TPB], s2[TPB];
+    __shared__ double s[DIAG_NPART][TPB];
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    double mass = 0., e = 0., divb = 0.;
+    const double dV = g.dx1 * g.dx2 * g.dx3;  // main.c:427
+    double acc[5] = {0., 0., 0., 0., 0.}, divb = 0., key = DIAG_NO_KEY;
     if (idx < (size_t)ni * nj * g.N3) {
         const unsigned rest = (unsigned)idx / (unsigned)g.N3;
         const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
@@ -1401,9 +1410,15 @@ __global__ void __launch_bounds__(TPB) k_diag(DGrid g, const double *__restrict_
         int fatal = 0;
         State q;
         get_state(pr, geom, q, fatal);
-        primtoflux(pr, q, 0, geom, g.gam, U);
-        mass = pr[RHO] * geom.g * g.dx1 * g.dx2 * g.dx3;
-        e = U[UU] * g.dx1 * g.dx2 * g.dx3;
+        if (TAB)
+            primtoflux(pr, q, 0, geom, EosTable{g.tab, g.ref_derivs}, U);
+        else
+            primtoflux(pr, q, 0, geom, g.gam, U);
+        acc[0] = U[RHO] * dV;
+        acc[1] = U[U3] * dV;
+        acc[2] = U[UU] * dV;
+        acc[3] = pr[RHO] * geom.g * g.dx1 * g.dx2 * g.dx3;
+        acc[4] = geom.g * pr[RHO];
         const double *GD = g.geom[CENT] + 32 * g.ncol;
         const double gd00 = GD[col], gdjm = GD[col - 1], gdim = GD[col - g.SN2], gdimjm = GD[col - g.SN2 - 1];
         auto P = [&](int k, int ii, int jj, int hh) { return p[k * nz + zidx(g, ii, jj, hh)]; };
@@ -1434,24 +1449,83 @@ __global__ void __launch_bounds__(TPB) k_diag(DGrid g, const double *__restrict_
                              P(B2, i - 1, j - 1, h) * gdimjm) /
                             g.dx2);
         }
+        if (divb > 0.) key = (double)(((size_t)h * ni + ri) * nj + (size_t)(j - g.js));
+        else divb = 0.;
     }
-    s0[threadIdx.x] = mass;
-    s1[threadIdx.x] = e;
-    s2[threadIdx.x] = divb;
+    for (int k = 0; k < 5; k++) s[k][threadIdx.x] = acc[k];
+    s[5][threadIdx.x] = divb;
+    s[6][threadIdx.x] = key;
     __syncthreads();
     for (int st = TPB / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
-            s0[threadIdx.x] += s0[threadIdx.x + st];
-            s1[threadIdx.x] += s1[threadIdx.x + st];
-            s2[threadIdx.x] = (s2[threadIdx.x + st] > s2[threadIdx.x]) ? s2[threadIdx.x + st] : s2[threadIdx.x];
+            const int o = threadIdx.x + st;
+            for (int k = 0; k < 5; k++) s[k][threadIdx.x] += s[k][o];
+            const double d0 = s[5][threadIdx.x], d1 = s[5][o];
+            if (d1 > d0 || (d1 == d0 && s[6][o] < s[6][threadIdx.x])) {
+                s[5][threadIdx.x] = d1;
+                s[6][threadIdx.x] = s[6][o];
+            }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = s0[0];
-        part[3 * blockIdx.x + 1] = s1[0];
-        part[3 * blockIdx.x + 2] = s2[0];
+    if ((int)threadIdx.x < DIAG_NPART) part[DIAG_NPART * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// diag_flux (diag.c:437-465) from the inner-face F1 of the last full step, and
+// the ghost-face mass fluxes of diag_mass (diag.c:498-512), each summed in the
+// reference's (h, j) loop order: the block forms the terms of a chunk, thread 0
+// adds them up.  out: mdot, edot, ldot, mdot_in, mdot_out.
+constexpr int DIAG_SEQ_CHUNK = 1024;
+__global__ void __launch_bounds__(TPB) k_diag_seq(DGrid g, const double *__restrict__ p,
+                                                  const double *__restrict__ F1, int flux_valid, double *out)
+{
+    __shared__ double t[5][DIAG_SEQ_CHUNK];
+    const int nj = g.je - g.js + 1, n = g.N3 * nj;
+    const size_t nz = g.nz;
+    double acc[5] = {0., 0., 0., 0., 0.};
+    for (int base = 0; base < n; base += DIAG_SEQ_CHUNK) {
+        const int m = min(DIAG_SEQ_CHUNK, n - base);
+        for (int q = threadIdx.x; q < m; q += TPB) {
+            const int r = base + q, h = r / nj, j = g.js + (r - h * nj);
+            double fm = 0., fe = 0., fl = 0.;
+            if (flux_valid) {
+                const size_t z = zidx(g, g.is, j, h);
+                const double FR = F1[RHO * nz + z], FU = F1[UU * nz + z], FL = F1[U3 * nz + z];
+                fm = FR * g.dx2 * g.dx3;
+                fe = (FU - FR) * g.dx2 * g.dx3;
+                fl = FL * g.dx2 * g.dx3;
+            }
+            double v[2];
+            int fatal = 0;
+            for (int side = 0; side < 2; side++) {
+                const int i = side ? g.ie + 1 : g.is - 1;
+                const size_t z = zidx(g, i, j, h);
+                double pr[NPR], ucon[4];
+                for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+                Geom geom;
+                load_geom(g.geom[CENT], g.ncol, (size_t)i * g.SN2 + j, geom);
+                ucon_calc(pr, geom, ucon, fatal);
+                v[side] = pr[RHO] * ucon[1] * geom.g * g.dx2 * g.dx3;
+            }
+            t[0][q] = fm;
+            t[1][q] = fe;
+            t[2][q] = fl;
+            t[3][q] = v[0];
+            t[4][q] = v[1];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int q = 0; q < m; q++) {
+                acc[0] += t[0][q];
+                acc[1] -= t[1][q];
+                acc[2] += t[2][q];
+                acc[3] += t[3][q];
+                acc[4] -= t[4][q];
+            }
+        __syncthreads();
     }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; k++) out[k] = acc[k];
 }
 
 // Batched Noble 2D inversion over SoA arrays (utoprim_2d.c:126, plus the Font fix).
@@ -1795,6 +1869,8 @@ struct harm_ctx {
     std::vector<Ev> ev_live, ev_pool;
     double ms[HARM_NKERNELS];
     long long launches[HARM_NKERNELS];
+    double mass_start;  // main.c:173-176
+    bool flux_valid;    // F1 holds the fluxes of a completed full step (diag_flux)
 };
 
 namespace {
@@ -2042,6 +2118,7 @@ int one_step(harm_ctx *c)
     const DGrid &g = c->g;
     if ((rc = half_step(c, c->p, c->p, c->ph, 0.5, 0))) return rc;
     if ((rc = half_step(c, c->p, c->ph, c->p, 1.0, 1))) return rc;
+    c->flux_valid = true;
     LAUNCH(HARM_K_STEPEND,
            hipLaunchKernelGGL(k_mdot_partials, dim3(g.N3), dim3(TPB), 0, c->stream, g, c->p, c->part);
            hipLaunchKernelGGL(k_stepend, dim3(1), dim3(1), 0, c->stream, g, c->tstate, c->ndt_slot, c->part, g.N3));
@@ -2142,7 +2219,7 @@ int ctx_alloc(harm_ctx *c)
     if ((rc = dalloc(&c->dtpart, (size_t)nblocks8(c->nz)))) return rc;
     if ((rc = dalloc(&c->cpart, 8 * (size_t)nblocks(c->nz)))) return rc;
     if ((rc = dalloc(&c->cnt, HARM_NCOUNTERS))) return rc;
-    c->npart = std::max<size_t>(3 * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
+    c->npart = std::max<size_t>(DIAG_NPART * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
     if ((rc = dalloc(&c->part, c->npart))) return rc;
     c->font.cap = (int)std::min<size_t>(c->nz / 16 + 65536, (size_t)1 << 24);
     if ((rc = dalloc(&c->font.count, 1))) return rc;
@@ -2301,6 +2378,10 @@ void harm_destroy(harm_ctx *c)
     delete c;
 }
 
+namespace {
+int diag_volume(harm_ctx *c, double *sums, double &divbmax, double &key);
+}
+
 int harm_init_torus(harm_ctx *c)
 {
     if (!c) return set_err("null context");
@@ -2344,6 +2425,11 @@ int harm_init_torus(harm_ctx *c)
     HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipFree(d_tor));
+    // main.c:173-176
+    double sums[5], dv, key;
+    if ((rc = diag_volume(c, sums, dv, key))) return rc;
+    c->mass_start = sums[3];
+    c->flux_valid = false;
     return 0;
 }
 
@@ -2362,7 +2448,9 @@ int harm_step(harm_ctx *c, int nsteps)
 int harm_get_prims(harm_ctx *c, double *prim, int8_t *pflag) { return c ? copy_state(c, prim, pflag, true) : 1; }
 int harm_set_prims(harm_ctx *c, const double *prim, const int8_t *pflag)
 {
-    return c ? copy_state(c, const_cast<double *>(prim), const_cast<int8_t *>(pflag), false) : 1;
+    if (!c) return 1;
+    c->flux_valid = false;
+    return copy_state(c, const_cast<double *>(prim), const_cast<int8_t *>(pflag), false);
 }
 
 int harm_get_time(harm_ctx *c, double *t, double *dt)
@@ -2399,37 +2487,135 @@ int harm_reset_counters(harm_ctx *c)
     return 0;
 }
 
-int harm_diag(harm_ctx *c, double *out)
+namespace {
+
+// k_diag over the active zones, its block partials finished on the host in
+// block order: sums[5] as k_diag documents, the largest |div B| and its rank.
+int diag_volume(harm_ctx *c, double *sums, double &divbmax, double &key)
 {
     const DGrid &g = c->g;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
     const unsigned nb = nblocks(na);
-    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->part);
+    if (c->eos_kind == 1)
+        hipLaunchKernelGGL(k_diag<true>, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->part);
+    else
+        hipLaunchKernelGGL(k_diag<false>, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->part);
     HIPCHK(hipGetLastError());
-    std::vector<double> part(3 * (size_t)nb), md(2 * (size_t)g.N3);
+    std::vector<double> part(DIAG_NPART * (size_t)nb);
     HIPCHK(hipMemcpyAsync(part.data(), c->part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 5; k++) sums[k] = 0.;
+    divbmax = 0.;
+    key = DIAG_NO_KEY;
+    for (unsigned b = 0; b < nb; b++) {
+        const double *q = &part[DIAG_NPART * (size_t)b];
+        for (int k = 0; k < 5; k++) sums[k] += q[k];
+        if (q[5] > divbmax || (q[5] == divbmax && q[6] < key)) {
+            divbmax = q[5];
+            key = q[6];
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+int harm_diag(harm_ctx *c, double *out)
+{
+    const DGrid &g = c->g;
+    double sums[5], dv, key;
+    int rc;
+    if ((rc = diag_volume(c, sums, dv, key))) return rc;
+    std::vector<double> md(2 * (size_t)g.N3);
     hipLaunchKernelGGL(k_mdot_partials, dim3(g.N3), dim3(TPB), 0, c->stream, g, c->p, c->part);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(md.data(), c->part, md.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     double ts[4];
     HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    double m = 0., e = 0., dv = 0., mi = 0., mo = 0.;
-    for (unsigned b = 0; b < nb; b++) {
-        m += part[3 * b];
-        e += part[3 * b + 1];
-        dv = std::max(dv, part[3 * b + 2]);
-    }
+    double mi = 0., mo = 0.;
     for (int h = 0; h < g.N3; h++) {
         mi += md[2 * h];
         mo += md[2 * h + 1];
     }
-    out[0] = m;
-    out[1] = e;
+    out[0] = sums[3];
+    out[1] = sums[2];
     out[2] = mi;
     out[3] = mo;
     out[4] = ts[2];
     out[5] = ts[3];
     out[6] = dv;
+    return 0;
+}
+
+int harm_diag_log(harm_ctx *c, harm_diag_rec *r)
+{
+    if (!c || !r) return set_err("null argument");
+    const DGrid &g = c->g;
+    double sums[5], dv, key;
+    int rc;
+    if ((rc = diag_volume(c, sums, dv, key))) return rc;
+    hipLaunchKernelGGL(k_diag_seq, dim3(1), dim3(TPB), 0, c->stream, g, c->p, c->F1, c->flux_valid ? 1 : 0, c->part);
+    HIPCHK(hipGetLastError());
+    double seq[5], ts[4];
+    HIPCHK(hipMemcpyAsync(seq, c->part, sizeof(seq), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memset(r, 0, sizeof(*r));
+    r->t = ts[0];
+    r->rmed = sums[0];
+    r->pp = sums[1];
+    r->e = sums[2];
+    r->mdot = seq[0];
+    r->edot = seq[1];
+    r->ldot = seq[2];
+    // diag.c:198-205, 263-265: without the cooling table's Lambda, Qnu and H_d
+    // only the density weight of the COOL_EOS build is known.
+    r->lnu_rho_2 = (c->eos_kind == 1) ? sums[4] : 0.;
+    r->mdot_in = seq[3];
+    r->mdot_out = seq[4];
+    r->mass_start = c->mass_start;
+    r->mass_curr = sums[3];
+    r->mass_in = ts[2];
+    r->mass_out = ts[3];
+    r->divbmax = dv;
+    r->imax[0] = r->imax[1] = r->imax[2] = -1;
+    if (key < DIAG_NO_KEY) {
+        const size_t ni = (size_t)(g.ie - g.is + 1), nj = (size_t)(g.je - g.js + 1), k = (size_t)key;
+        r->imax[1] = (int32_t)(k % nj);
+        r->imax[0] = (int32_t)((k / nj) % ni);
+        r->imax[2] = (int32_t)(k / nj / ni);
+    }
+    r->flux_valid = c->flux_valid ? 1 : 0;
+    return 0;
+}
+
+int harm_ener_line(const harm_diag_rec *r, char *buf, size_t n)
+{
+    if (!r || !buf) return -1;
+    // diag.c:317-326
+    const int len =
+        std::snprintf(buf, n,
+                      "%10.5g %10.5g %10.5g %10.5g %15.8g %15.8g "
+                      "%15.8g %15.8g %15.8g %15.8g %15.8g %15.8g %15.8g"
+                      " -0 %15.8g %15.8g %15.8g %15.8g %15.8g %15.8g %15.8g\n",
+                      r->t, r->rmed, r->pp, r->e, 0.0, 0.0, r->mdot, r->edot, r->ldot, r->lnu, r->lnu_rho,
+                      r->lnu_rho_2, r->lnu2, r->mdot_in, r->mdot_out, r->mass_start, r->mass_curr, r->mass_in,
+                      r->mass_out, (r->mass_curr - r->mass_in - r->mass_out) / r->mass_start * 100.0);
+    return (len < 0 || (size_t)len >= n) ? -1 : len;
+}
+
+int harm_set_mass_budget(harm_ctx *c, double mass_start, double mass_in, double mass_out)
+{
+    if (!c) return set_err("null argument");
+    c->mass_start = mass_start;
+    double ts[4];
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ts[2] = mass_in;
+    ts[3] = mass_out;
+    HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -2466,6 +2652,7 @@ int harm_debug_fluxes(harm_ctx *c, int fullstep, double *F1, double *F2, double 
 {
     int rc;
     const DGrid &g = c->g;
+    c->flux_valid = false;
     if ((rc = fluxes_and_emf(c, c->p, fullstep))) return rc;
     // materialize the flux_ct rewrite on the host from the EMFs (step_ch.c:541-554, 783-797)
     const size_t n = 8 * c->nz;
@@ -2606,6 +2793,7 @@ int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hd
     h.dt = ts[1];
     h.mass_in = ts[2];
     h.mass_out = ts[3];
+    h.mass_start = c->mass_start;
     return harm_host::restart_write_file(path, h, prim.data(), temp.empty() ? nullptr : temp.data());
 }
 
@@ -2641,6 +2829,8 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
     const double ts[4] = {h.t, h.dt, h.mass_in, h.mass_out};
     HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->mass_start = h.mass_start;
+    c->flux_valid = false;
     if ((rc = bound(c, c->p))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     if (hdr) *hdr = h;

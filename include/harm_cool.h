@@ -98,6 +98,32 @@ int harm_reset_counters(harm_ctx *c);
  * out[4] accumulated mass_in, out[5] accumulated mass_out, out[6] max |divB|. */
 int harm_diag(harm_ctx *c, double *out /* 7 */);
 
+/* The record diag() writes to ener.out (diag.c:59-326), replacing diag(LOG_OUT)
+ * with the diag_flux (diag.c:437-465) of the last full step and the diag_mass
+ * (diag.c:477-580) of the current state.  mdot/edot/ldot and mdot_in/mdot_out
+ * are summed in the reference's (h, j) order; the volume sums in a fixed tree
+ * order.  The neutrino luminosities need the cooling table's Lambda, Qnu and
+ * H_d, which the tabulated EOS here does not carry: lnu, lnu_rho and lnu2 are
+ * 0; lnu_rho_2 = sum gdet rho with a tabulated EOS (the COOL_EOS build's weight),
+ * else 0. */
+typedef struct harm_diag_rec {
+    double t, rmed, pp, e;                 /* t, sum U[RHO] dV, sum U[U3] dV, sum U[UU] dV */
+    double mdot, edot, ldot;               /* inner face, last full step (0 before one) */
+    double lnu, lnu_rho, lnu_rho_2, lnu2;
+    double mdot_in, mdot_out;              /* first ghost zones (MDOTTYPE NEU_AGA_GHOST1) */
+    double mass_start, mass_curr, mass_in, mass_out;
+    double divbmax;
+    int32_t imax[3];    /* zero-based (i, j, h) of the first zone with divbmax; -1 if 0 */
+    int32_t flux_valid; /* mdot/edot/ldot come from a completed full step */
+} harm_diag_rec;
+int harm_diag_log(harm_ctx *c, harm_diag_rec *r);
+/* One ener.out line in the format of diag.c:317-326, newline included;
+ * returns its length, -1 if it does not fit in n bytes. */
+int harm_ener_line(const harm_diag_rec *r, char *buf, size_t n);
+/* mass_start (main.c:173-176; set by harm_init_torus and harm_restart_read) and
+ * the mass_in / mass_out that diag_mass accumulates (diag.c:577-578). */
+int harm_set_mass_budget(harm_ctx *c, double mass_start, double mass_in, double mass_out);
+
 /* Device pointers of the state (canonical layout, var stride (N1+4)*(N2+4)*N3). */
 int harm_device_state(harm_ctx *c, double **prim, int8_t **pflag, void **stream);
 
@@ -280,8 +306,8 @@ int harm_utoprim5d_batch(const double *U, const double *gcov, const double *gcon
  * size when ener_size >= 0 (rank 0 in the reference, restart.c:150-157), and
  * with a tabulated EOS one temperature per zone (cooleos.c:996-1030).
  * Header fields in file order; the context owns N1..N3, t, tf, a, gam, cour,
- * dt, lim, Rin, Rout, hslope, R0, defcon, Rbr, mass_in and mass_out, the
- * caller (the run driver) the dump/image counters and clocks. */
+ * dt, lim, Rin, Rout, hslope, R0, defcon, Rbr, mass_start, mass_in and
+ * mass_out, the caller (the run driver) the dump/image counters and clocks. */
 typedef struct harm_restart_hdr {
     int32_t N1, N2, N3, nstep;
     double t, tf, a, gam, cour, DTd, DTl, DTi;
@@ -300,8 +326,8 @@ int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hd
 /* restart_init()/restart_read(), restart.c:195-376: grid size, a, gam, Rin,
  * Rout, hslope, R0 and Rbr must equal the context's (create the context from
  * harm_restart_peek's header); cour, lim, tf and defcon are taken from the
- * file as the reference does.  Loads p, pflag = 0, t, dt, mass_in, mass_out,
- * then bound_prim (restart.c:278-279).  hdr (may be NULL) receives the header. */
+ * file as the reference does.  Loads p, pflag = 0, t, dt, mass_start, mass_in,
+ * mass_out, then bound_prim (restart.c:278-279).  hdr (may be NULL) receives the header. */
 int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr);
 /* The header alone (ener_size, which follows the body, reads as -1). */
 int harm_restart_peek(const char *path, harm_restart_hdr *hdr);

@@ -123,7 +123,8 @@ struct orc_ctx {
     double *gcov, *gcon, *gdet, *conn, *coords;
     double *Katm;
     int64_t cnt[ORC_NCOUNTERS];
-    double mass_in, mass_out;
+    double mass_in, mass_out, mass_start;
+    int flux_valid; /* F1 holds the fluxes of a completed full step (diag_flux) */
     const orc_eos *eos; /* tabulated EOS of the step (NULL: Gamma-law) */
     int eos_refd;
 };
@@ -1940,6 +1941,7 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
         }
     }
     for (int k = 0; k < ORC_NCOUNTERS; k++) cnt[k] += lc[k];
+    o->flux_valid = fullstep ? 1 : 0;
 
     if (N3 > 2)
         ndt = o->P.defcon * 1. / (1. / ndt1 + 1. / ndt2 + 1. / ndt3);
@@ -2232,62 +2234,77 @@ int orc_init_torus(orc_ctx *o)
             for (j = js; j <= je; j++)
                 if (isnan(PV(o, o->p, h, i, j)[UU])) return 2;
     o->mass_in = o->mass_out = 0.;
+    /* main.c:173-176 */
+    o->mass_start = 0.;
+    for (h = 0; h < N3; h++)
+        for (i = is; i <= ie; i++)
+            for (j = js; j <= je; j++)
+                o->mass_start += PV(o, o->p, h, i, j)[RHO] * GDET(o, i, j, CENT) * o->dx[RR] * o->dx[TH] * o->dx[PH];
+    o->flux_valid = 0;
     return fatal ? 1 : 0;
 }
 
 /* ------------------------------------------------------------------------ */
 /* diagnostics: diag.c:156-209, diag.c:477-580                               */
 /* ------------------------------------------------------------------------ */
-double orc_divb_max(const orc_ctx *o)
+/* |div B| of the flux-CT corner at zone (h, i, j) (diag.c:160-209). */
+static double divb_at(const orc_ctx *o, int h, int i, int j)
 {
-    double divbmax = 0.;
     const int N3 = o->N3;
 #define PB(h, i, j, k) PV(o, o->p, h, i, j)[k]
 #define GD(i, j) GDET(o, i, j, CENT)
-    for (int h = 0; h < N3; h++)
-        for (int i = o->istart; i <= o->istop; i++)
-            for (int j = o->jstart; j <= o->jstop; j++) {
-                double divb;
-                if (N3 > 2) {
-                    const int hm = hwrap(o, h - 1);
-                    divb = fabs(0.25 *
-                                    (PB(h, i, j, B1) * GD(i, j) + PB(hm, i, j, B1) * GD(i, j) +
-                                     PB(h, i, j - 1, B1) * GD(i, j - 1) + PB(hm, i, j - 1, B1) * GD(i, j - 1) -
-                                     PB(h, i - 1, j, B1) * GD(i - 1, j) - PB(hm, i - 1, j, B1) * GD(i - 1, j) -
-                                     PB(h, i - 1, j - 1, B1) * GD(i - 1, j - 1) - PB(hm, i - 1, j - 1, B1) * GD(i - 1, j - 1)) /
-                                    o->dx[RR] +
-                                0.25 *
-                                    (PB(h, i, j, B2) * GD(i, j) + PB(hm, i, j, B2) * GD(i, j) +
-                                     PB(h, i - 1, j, B2) * GD(i - 1, j) + PB(hm, i - 1, j, B2) * GD(i - 1, j) -
-                                     PB(h, i, j - 1, B2) * GD(i, j - 1) - PB(hm, i, j - 1, B2) * GD(i, j - 1) -
-                                     PB(h, i - 1, j - 1, B2) * GD(i - 1, j - 1) - PB(hm, i - 1, j - 1, B2) * GD(i - 1, j - 1)) /
-                                    o->dx[TH] +
-                                0.25 *
-                                    (PB(h, i, j, B3) * GD(i, j) + PB(h, i - 1, j, B3) * GD(i - 1, j) +
-                                     PB(h, i, j - 1, B3) * GD(i, j - 1) + PB(h, i - 1, j - 1, B3) * GD(i - 1, j - 1) -
-                                     PB(hm, i, j, B3) * GD(i, j) - PB(hm, i - 1, j, B3) * GD(i - 1, j) -
-                                     PB(hm, i, j - 1, B3) * GD(i, j - 1) - PB(hm, i - 1, j - 1, B3) * GD(i - 1, j - 1)) /
-                                    o->dx[PH]);
-                } else {
-                    divb = fabs(0.5 *
-                                    (PB(h, i, j, B1) * GD(i, j) + PB(h, i, j - 1, B1) * GD(i, j - 1) -
-                                     PB(h, i - 1, j, B1) * GD(i - 1, j) - PB(h, i - 1, j - 1, B1) * GD(i - 1, j - 1)) /
-                                    o->dx[RR] +
-                                0.5 *
-                                    (PB(h, i, j, B2) * GD(i, j) + PB(h, i - 1, j, B2) * GD(i - 1, j) -
-                                     PB(h, i, j - 1, B2) * GD(i, j - 1) - PB(h, i - 1, j - 1, B2) * GD(i - 1, j - 1)) /
-                                    o->dx[TH]);
-                }
-                if (divb > divbmax) divbmax = divb;
-            }
+    double divb;
+    if (N3 > 2) {
+        const int hm = hwrap(o, h - 1);
+        divb = fabs(0.25 *
+                        (PB(h, i, j, B1) * GD(i, j) + PB(hm, i, j, B1) * GD(i, j) +
+                         PB(h, i, j - 1, B1) * GD(i, j - 1) + PB(hm, i, j - 1, B1) * GD(i, j - 1) -
+                         PB(h, i - 1, j, B1) * GD(i - 1, j) - PB(hm, i - 1, j, B1) * GD(i - 1, j) -
+                         PB(h, i - 1, j - 1, B1) * GD(i - 1, j - 1) - PB(hm, i - 1, j - 1, B1) * GD(i - 1, j - 1)) /
+                        o->dx[RR] +
+                    0.25 *
+                        (PB(h, i, j, B2) * GD(i, j) + PB(hm, i, j, B2) * GD(i, j) +
+                         PB(h, i - 1, j, B2) * GD(i - 1, j) + PB(hm, i - 1, j, B2) * GD(i - 1, j) -
+                         PB(h, i, j - 1, B2) * GD(i, j - 1) - PB(hm, i, j - 1, B2) * GD(i, j - 1) -
+                         PB(h, i - 1, j - 1, B2) * GD(i - 1, j - 1) - PB(hm, i - 1, j - 1, B2) * GD(i - 1, j - 1)) /
+                        o->dx[TH] +
+                    0.25 *
+                        (PB(h, i, j, B3) * GD(i, j) + PB(h, i - 1, j, B3) * GD(i - 1, j) +
+                         PB(h, i, j - 1, B3) * GD(i, j - 1) + PB(h, i - 1, j - 1, B3) * GD(i - 1, j - 1) -
+                         PB(hm, i, j, B3) * GD(i, j) - PB(hm, i - 1, j, B3) * GD(i - 1, j) -
+                         PB(hm, i, j - 1, B3) * GD(i, j - 1) - PB(hm, i - 1, j - 1, B3) * GD(i - 1, j - 1)) /
+                        o->dx[PH]);
+    } else {
+        divb = fabs(0.5 *
+                        (PB(h, i, j, B1) * GD(i, j) + PB(h, i, j - 1, B1) * GD(i, j - 1) -
+                         PB(h, i - 1, j, B1) * GD(i - 1, j) - PB(h, i - 1, j - 1, B1) * GD(i - 1, j - 1)) /
+                        o->dx[RR] +
+                    0.5 *
+                        (PB(h, i, j, B2) * GD(i, j) + PB(h, i - 1, j, B2) * GD(i - 1, j) -
+                         PB(h, i, j - 1, B2) * GD(i, j - 1) - PB(h, i - 1, j - 1, B2) * GD(i - 1, j - 1)) /
+                        o->dx[TH]);
+    }
+    return divb;
 #undef PB
 #undef GD
+}
+
+double orc_divb_max(const orc_ctx *o)
+{
+    double divbmax = 0.;
+    for (int h = 0; h < o->N3; h++)
+        for (int i = o->istart; i <= o->istop; i++)
+            for (int j = o->jstart; j <= o->jstop; j++) {
+                const double divb = divb_at(o, h, i, j);
+                if (divb > divbmax) divbmax = divb;
+            }
     return divbmax;
 }
 
 void orc_diag(const orc_ctx *o, double *out)
 {
     const int N3 = o->N3;
+    const double dV = o->dx[RR] * o->dx[TH] * o->dx[PH]; /* main.c:427 */
     double mass = 0., e = 0., mdot_in = 0., mdot_out = 0.;
     int64_t fatal = 0;
     for (int h = 0; h < N3; h++)
@@ -2301,7 +2318,7 @@ void orc_diag(const orc_ctx *o, double *out)
                 get_geometry(o, i, j, CENT, &geom);
                 get_state(pz, &geom, &q, &fatal);
                 primtoflux(pz, &q, 0, &geom, o->P.gam, U);
-                e += U[UU] * o->dx[RR] * o->dx[TH] * o->dx[PH];
+                e += U[UU] * dV;
             }
     for (int h = 0; h < N3; h++)
         for (int j = o->jstart; j <= o->jstop; j++) {
@@ -2326,6 +2343,74 @@ void orc_diag(const orc_ctx *o, double *out)
 /* context and exports                                                       */
 /* ------------------------------------------------------------------------ */
 /* Fluxes of one half-step after fix_flux and flux_ct, canonical layout. */
+/* diag() log record (diag.c:59-326): conserved totals and |div B| with its
+ * zone (ZLOOP order, first maximum), diag_flux of the last full step
+ * (diag.c:437-465) and diag_mass of the current state (diag.c:477-580), each
+ * summed in the reference's loop order.  The neutrino luminosities need the
+ * cooling table's Qnu, Lambda and H_d, which the tabulated EOS here does not
+ * carry: lnu, lnu_rho and lnu2 are 0, and with a tabulated EOS lnu_rho_2 holds
+ * the COOL_EOS build's density weight sum gdet rho (diag.c:203). */
+void orc_diag_log(orc_ctx *o, orc_diag_rec *r)
+{
+    const int N3 = o->N3, is = o->istart, ie = o->istop, js = o->jstart, je = o->jstop;
+    const double dV = o->dx[RR] * o->dx[TH] * o->dx[PH]; /* main.c:427 */
+    int64_t fatal = 0;
+    memset(r, 0, sizeof(*r));
+    r->imax[0] = r->imax[1] = r->imax[2] = -1;
+    g_step_eos = o->eos; /* primtoU with the step's EOS (phys.c:122-141) */
+    g_step_refd = o->eos_refd;
+    for (int h = 0; h < N3; h++)
+        for (int i = is; i <= ie; i++)
+            for (int j = js; j <= je; j++) {
+                geom_t geom;
+                state_t q;
+                double U[NPR];
+                const double *pz = PV(o, o->p, h, i, j);
+                get_geometry(o, i, j, CENT, &geom);
+                get_state(pz, &geom, &q, &fatal);
+                primtoflux(pz, &q, 0, &geom, o->P.gam, U);
+                r->rmed += U[RHO] * dV;
+                r->pp += U[U3] * dV;
+                r->e += U[UU] * dV;
+                const double divb = divb_at(o, h, i, j);
+                if (divb > r->divbmax) {
+                    r->imax[0] = i - is;
+                    r->imax[1] = j - js;
+                    r->imax[2] = h;
+                    r->divbmax = divb;
+                }
+                r->mass_curr += pz[RHO] * GDET(o, i, j, CENT) * o->dx[RR] * o->dx[TH] * o->dx[PH];
+                if (o->eos) r->lnu_rho_2 += GDET(o, i, j, CENT) * pz[RHO]; /* diag.c:203 */
+            }
+    g_step_eos = NULL;
+    if (o->flux_valid)
+        for (int h = 0; h < N3; h++)
+            for (int j = js; j <= je; j++) {
+                const double *F = PV(o, o->F1, h, is, j);
+                r->mdot += F[RHO] * o->dx[TH] * o->dx[PH];
+                r->edot -= (F[UU] - F[RHO]) * o->dx[TH] * o->dx[PH];
+                r->ldot += F[U3] * o->dx[TH] * o->dx[PH];
+            }
+    for (int h = 0; h < N3; h++)
+        for (int j = js; j <= je; j++) {
+            geom_t geom;
+            double ucon[NDIM];
+            const double *pz = PV(o, o->p, h, is - 1, j);
+            get_geometry(o, is - 1, j, CENT, &geom);
+            ucon_calc(pz, &geom, ucon, &fatal);
+            r->mdot_in += pz[RHO] * ucon[1] * geom.g * o->dx[TH] * o->dx[PH];
+            pz = PV(o, o->p, h, ie + 1, j);
+            get_geometry(o, ie + 1, j, CENT, &geom);
+            ucon_calc(pz, &geom, ucon, &fatal);
+            r->mdot_out -= pz[RHO] * ucon[1] * geom.g * o->dx[TH] * o->dx[PH];
+        }
+    r->t = o->t;
+    r->mass_start = o->mass_start;
+    r->mass_in = o->mass_in;
+    r->mass_out = o->mass_out;
+    r->flux_valid = o->flux_valid;
+}
+
 int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F3, double *ndt)
 {
     int64_t cnt[ORC_NCOUNTERS];
@@ -2337,6 +2422,7 @@ int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F
     ndt[2] = (o->N3 > 2) ? fluxcalc(o, o->p, o->F3, 3, fullstep, cnt) : 1.e9;
     fix_flux(o);
     flux_ct(o);
+    o->flux_valid = 0;
     const size_t nh = o->N3, nj = o->SN2, ni = o->SN1;
     double *dst[3] = {F1, F2, F3};
     const double *src[3] = {o->F1, o->F2, o->F3};
@@ -2456,7 +2542,15 @@ int orc_set_state(orc_ctx *o, const double *prim, const int8_t *pflag)
                     for (int k = 0; k < NPR; k++) z[k] = prim[(((size_t)k * ni + i) * nj + j) * nh + h];
                 if (pflag) o->pflag[zi(o, (int)h, (int)i, (int)j)] = pflag[(i * nj + j) * nh + h];
             }
+    o->flux_valid = 0;
     return 0;
+}
+
+void orc_set_mass_budget(orc_ctx *o, double mass_start, double mass_in, double mass_out)
+{
+    o->mass_start = mass_start;
+    o->mass_in = mass_in;
+    o->mass_out = mass_out;
 }
 
 void orc_get_time(const orc_ctx *o, double *t, double *dt)

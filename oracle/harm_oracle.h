@@ -80,6 +80,17 @@ int      orc_debug_bound(orc_ctx *o);
 /* out[0] = mass (sum rho gdet dV), out[1] = sum U[UU] dV, out[2] = mdot_in,
  * out[3] = mdot_out (diag.c:477-580, diag.c:59-230). */
 void     orc_diag(const orc_ctx *o, double *out);
+/* The ener.out record of diag() (diag.c:59-326), diag_flux (diag.c:437-465)
+ * and diag_mass (diag.c:477-580); imax = zero-based (i, j, h) of the first
+ * zone with the largest |div B|, -1 when it is 0 everywhere. */
+typedef struct orc_diag_rec {
+    double t, rmed, pp, e, mdot, edot, ldot, lnu, lnu_rho, lnu_rho_2, lnu2;
+    double mdot_in, mdot_out, mass_start, mass_curr, mass_in, mass_out, divbmax;
+    int32_t imax[3], flux_valid;
+} orc_diag_rec;
+void     orc_diag_log(orc_ctx *o, orc_diag_rec *r);
+/* mass_start (main.c:173-176) and the accumulated mass_in / mass_out. */
+void     orc_set_mass_budget(orc_ctx *o, double mass_start, double mass_in, double mass_out);
 
 /* Single-zone primitive solvers; gcov/gcon are row-major 4x4. */
 int  orc_utoprim_2d(const double U[8], const double *gcov, const double *gcon,

@@ -48,6 +48,14 @@ def orc_eos(tab: dict, gam: float) -> OrcEos:
 _lib = None
 
 
+class OrcDiagRec(ctypes.Structure):
+    """orc_diag_rec (oracle/harm_oracle.h)."""
+    _fields_ = [(n, c_double) for n in (
+        "t", "rmed", "pp", "e", "mdot", "edot", "ldot", "lnu", "lnu_rho", "lnu_rho_2", "lnu2",
+        "mdot_in", "mdot_out", "mass_start", "mass_curr", "mass_in", "mass_out", "divbmax")] + [
+        ("imax", ctypes.c_int32 * 3), ("flux_valid", ctypes.c_int32)]
+
+
 class OrcRestartHdr(ctypes.Structure):
     """orc_restart_hdr (oracle/harm_oracle.h), restart.c:105-138 in file order."""
     _fields_ = [
@@ -90,6 +98,8 @@ def lib():
         L.orc_divb_max.restype = c_double
         L.orc_divb_max.argtypes = [c_void_p]
         L.orc_diag.argtypes = [c_void_p, _dp]
+        L.orc_diag_log.argtypes = [c_void_p, POINTER(OrcDiagRec)]
+        L.orc_set_mass_budget.argtypes = [c_void_p, c_double, c_double, c_double]
         L.orc_set_threads.argtypes = [c_int]
         L.orc_utoprim_2d.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, POINTER(c_int), POINTER(c_int)]
         L.orc_utoprim_1dvsq2fix1.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, c_double]
@@ -249,6 +259,17 @@ class Oracle:
         out = np.zeros(4)
         self.L.orc_diag(self.h, _ptr(out))
         return dict(mass=out[0], energy=out[1], mdot_in=out[2], mdot_out=out[3])
+
+    def diag_log(self) -> dict:
+        r = OrcDiagRec()
+        self.L.orc_diag_log(self.h, ctypes.byref(r))
+        d = {n: getattr(r, n) for n, _ in r._fields_}
+        d["imax"] = tuple(r.imax)
+        d["flux_valid"] = bool(r.flux_valid)
+        return d
+
+    def set_mass_budget(self, mass_start: float, mass_in: float = 0.0, mass_out: float = 0.0) -> None:
+        self.L.orc_set_mass_budget(self.h, mass_start, mass_in, mass_out)
 
 
 def utoprim_2d(U, gcov, gcon, gdet, prim, gam):

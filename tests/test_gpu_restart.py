@@ -24,7 +24,7 @@ def _torch():
 
 
 DRIVER = dict(nstep=3, DTd=5.0, DTl=0.5, DTi=1.0, DTr=100, dump_cnt=1, image_cnt=2, rdump_cnt=0, failed=0,
-              mass_start=1.25, tdump=5.0, timage=1.0, tlog=0.5, nfailed=0, cyl_R0=0.0, cyl_Th0=0.0, ener_size=4096)
+              tdump=5.0, timage=1.0, tlog=0.5, nfailed=0, cyl_R0=0.0, cyl_Th0=0.0, ener_size=4096)
 
 
 def test_restart_file_matches_oracle_and_resumes_bitwise(tmp_path):
@@ -46,6 +46,7 @@ def test_restart_file_matches_oracle_and_resumes_bitwise(tmp_path):
     for k, v in DRIVER.items():
         if k != "ener_size":  # follows the body: the header peek does not read it
             assert h[k] == v, k
+    assert h["mass_start"] == g.diag_log().mass_start > 0.0  # the context's, not the driver's
     # the oracle's restart_write() of its own state under the same header
     oh = OrcRestartHdr()
     for k, v in h.items():
