@@ -229,6 +229,32 @@ def tabulated_torus(hc, steps: int, warmup: int) -> dict:
             "frac_zones_in_table_rho_range": inside}
 
 
+def tracer_section(g, steps: int) -> dict:
+    """The reference's default build advances Lagrangian tracers every step
+    (TRACERSDUMP, tracers.c:298): one per zone with rho > RHOMIN, moved with
+    the interpolated u^i/u^t.  Reported apart from the headline step."""
+    n_live = g.tracers_init()
+    g.time()
+    t0 = time.perf_counter()
+    g.step_ch(steps)
+    el = time.perf_counter() - t0
+    g.set_timing(True)
+    g.step_ch(2)
+    tim = g.timing()
+    g.set_timing(False)
+    ms, cnt = tim["tracers"]
+    t0 = time.perf_counter()
+    recs = g.tracers_dump()
+    el_dump = time.perf_counter() - t0
+    ev = g.tracers_events()
+    g.tracers_stop()
+    upd = ms / max(cnt, 1)
+    return {"n_live_at_seed": n_live, "ms_per_step_with_tracers": el / steps * 1e3, "steps": steps,
+            "update_ms": round(upd, 4), "tracer_updates_per_s": n_live / (upd * 1e-3) if upd else None,
+            "dump_ms_with_host_copy": round(el_dump * 1e3, 2), "dump_records": int(recs.size),
+            "removed_during_run": int(ev.size)}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +268,7 @@ def main() -> None:
     ap.add_argument("--train-batch", type=int, default=65536)
     ap.add_argument("--stress-n", type=int, default=1 << 26)
     ap.add_argument("--no-tabulated", action="store_true", help="skip the tabulated-EOS torus section")
+    ap.add_argument("--no-tracers", action="store_true", help="skip the tracer section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -317,6 +344,7 @@ def main() -> None:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(per[dom], 4)}
     kernels = {k: {"avg_ms": round(per[k], 4), "launches": int(tim[k][1])} for k in per if tim[k][1]}
+    tracers = None if args.no_tracers else tracer_section(g, min(args.steps, 10))
     g.close()
     if roof is not None:
         roof["traffic"] = measured_traffic(dom, n1, n2, n3)
@@ -352,7 +380,7 @@ def main() -> None:
         "newton_kconv_per_zone": c["newton_kconv"] / max(c["c2p_calls"], 1),
         "newton_stopping_rule": "reference rule for both starts: |dW/W|+|dv2/v2| <= 1e-10, then 2 extra iterations",
         "learned_seed_torus": learned_rep, "stress_batch": st, "tabulated_eos_torus": tabulated,
-        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "tracers": tracers, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

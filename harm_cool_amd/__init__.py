@@ -24,7 +24,7 @@ COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_f
                  "fail_font2", "fixup_gamma", "fixup_repairs", "fixup_nostencil",
                  "floor_zones", "gamma_cap", "fatal"]
 KERNEL_NAMES = ["flux1", "flux2", "flux3", "emf", "update", "bound", "fixutoprim", "stepend",
-                "c2p_batch", "mlp", "fontfix", "div"]
+                "c2p_batch", "mlp", "fontfix", "div", "tracers"]
 NKERNELS = len(KERNEL_NAMES)
 
 _dp = POINTER(c_double)
@@ -92,6 +92,20 @@ class DiagRec(ctypes.Structure):
         d["imax"] = tuple(self.imax)
         d["flux_valid"] = bool(self.flux_valid)
         return d
+
+
+# harm_tracer_rec: one tracer-file line (tracers.c:470-510) and its tracer
+TRACER_DTYPE = np.dtype([("t", "f8"), ("rho", "f8"), ("T", "f8"), ("ye", "f8"), ("r", "f8"), ("th", "f8"),
+                         ("ph", "f8"), ("h", "i4"), ("i", "i4"), ("j", "i4"), ("kind", "i4")])
+TRACER_NONE, TRACER_LIVE, TRACER_IN, TRACER_OUT, TRACER_JET, TRACER_LOST = range(6)
+_TRACER_DIRS = {TRACER_IN: "in", TRACER_OUT: "out", TRACER_JET: "jet"}
+
+
+def tracer_name(root: str, h: int, i: int, j: int, n3: int) -> str:
+    """tracer_name (tracers.c:100-107)."""
+    if n3 > 2:
+        return os.path.join(root, "tracer_%03d_%03d_%03d.dat" % (i, j, h))
+    return os.path.join(root, "tracer_%03d_%03d.dat" % (i, j))
 
 
 def restart_peek(path: str) -> dict:
@@ -261,6 +275,13 @@ def lib() -> ctypes.CDLL:
     L.harm_diag_log.argtypes = [c_void_p, POINTER(DiagRec)]
     L.harm_ener_line.argtypes = [POINTER(DiagRec), c_char_p, c_size_t]
     L.harm_set_mass_budget.argtypes = [c_void_p, c_double, c_double, c_double]
+    L.harm_tracers_init.argtypes = [c_void_p, c_double, c_double, POINTER(c_int64)]
+    L.harm_tracers_stop.argtypes = [c_void_p]
+    L.harm_tracers_dump.argtypes = [c_void_p, c_void_p, c_int64, POINTER(c_int64)]
+    L.harm_tracers_events.argtypes = [c_void_p, c_void_p, c_int64, POINTER(c_int64)]
+    L.harm_tracers_get.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.harm_tracers_update.argtypes = [c_void_p, c_double]
+    L.harm_tracers_set.argtypes = [c_void_p, c_void_p, c_void_p]
     L.harm_device_state.argtypes = [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p)]
     L.harm_set_timing.argtypes = [c_void_p, c_int]
     L.harm_get_timing.argtypes = [c_void_p, _dp, POINTER(c_int64)]
@@ -432,6 +453,74 @@ class Harm:
 
     def set_mass_budget(self, mass_start: float, mass_in: float = 0.0, mass_out: float = 0.0) -> None:
         _check(self.L.harm_set_mass_budget(self.h, mass_start, mass_in, mass_out), "harm_set_mass_budget")
+
+    # -- tracers (tracers.c:109-654) -------------------------------------------
+    def tracers_init(self, rho_unit: float = 1.0, t_unit: float = 1.0) -> int:
+        """init_tracers(): seed the tracers; every later step advances them."""
+        n = c_int64()
+        _check(self.L.harm_tracers_init(self.h, rho_unit, t_unit, ctypes.byref(n)), "harm_tracers_init")
+        self._tracer_buf = np.zeros(n.value, dtype=TRACER_DTYPE)
+        return n.value
+
+    def tracers_stop(self) -> None:
+        _check(self.L.harm_tracers_stop(self.h), "harm_tracers_stop")
+
+    def _tracer_call(self, fn, what: str) -> np.ndarray:
+        buf = getattr(self, "_tracer_buf", None)
+        if buf is None:
+            raise HarmError(f"{what}: tracers_init has not been called")
+        n = c_int64()
+        rc = fn(self.h, buf.ctypes.data, buf.size, ctypes.byref(n))
+        if rc == -1:
+            self._tracer_buf = buf = np.zeros(n.value, dtype=TRACER_DTYPE)
+            rc = fn(self.h, buf.ctypes.data, buf.size, ctypes.byref(n))
+        _check(rc, what)
+        return buf[:n.value].copy()
+
+    def tracers_dump(self) -> np.ndarray:
+        """dump_tracers(): one TRACER_DTYPE record per live tracer, slot order."""
+        return self._tracer_call(self.L.harm_tracers_dump, "harm_tracers_dump")
+
+    def tracers_events(self) -> np.ndarray:
+        """Final records of the tracers that left since the previous call."""
+        return self._tracer_call(self.L.harm_tracers_events, "harm_tracers_events")
+
+    def tracers_update(self, dt: float) -> None:
+        """update_tracers(dt) on the current state (harm_step already does it)."""
+        _check(self.L.harm_tracers_update(self.h, dt), "harm_tracers_update")
+
+    def tracers_get(self):
+        """(X[3, N3, N1, N2], state[N3, N1, N2]) of every slot."""
+        n1, n2, n3 = self.params.n1, self.params.n2, self.params.n3
+        X = np.empty((3, n3, n1, n2))
+        st = np.empty((n3, n1, n2), dtype=np.int8)
+        _check(self.L.harm_tracers_get(self.h, X.ctypes.data, st.ctypes.data), "harm_tracers_get")
+        return X, st
+
+    def tracers_set(self, X: np.ndarray, state: np.ndarray) -> None:
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        st = np.ascontiguousarray(state, dtype=np.int8)
+        n = self.params.n1 * self.params.n2 * self.params.n3
+        if X.size != 3 * n or st.size != n:
+            raise HarmError("tracers_set: X must hold 3*N1*N2*N3 values and state N1*N2*N3")
+        _check(self.L.harm_tracers_set(self.h, X.ctypes.data, st.ctypes.data), "harm_tracers_set")
+
+    def write_tracer_files(self, recs: np.ndarray, events: "np.ndarray | None" = None,
+                           root: str = "tracers") -> None:
+        """dump_tracers()/update_tracers() file output (tracers.c:300-380,
+        470-510): append each record to its tracer's file; a tracer that left
+        gets its last line, then its file moves to root/in, out or jet."""
+        n3 = self.params.n3
+        for r in recs:
+            with open(tracer_name(root, r["h"], r["i"], r["j"], n3), "a") as f:
+                f.write(("%32.22e" * 7 + "\n") % tuple(float(r[k]) for k in ("t", "rho", "T", "ye", "r", "th", "ph")))
+        for r in events if events is not None else ():
+            src = tracer_name(root, r["h"], r["i"], r["j"], n3)
+            with open(src, "a") as f:
+                f.write(("%32.22e" * 7 + "\n") % tuple(float(r[k]) for k in ("t", "rho", "T", "ye", "r", "th", "ph")))
+            d = os.path.join(root, _TRACER_DIRS[int(r["kind"])])
+            os.makedirs(d, exist_ok=True)
+            os.replace(src, tracer_name(d, r["h"], r["i"], r["j"], n3))
 
     def set_timing(self, on: bool) -> None:
         _check(self.L.harm_set_timing(self.h, 1 if on else 0), "harm_set_timing")

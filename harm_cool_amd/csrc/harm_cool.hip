@@ -1528,6 +1528,296 @@ __global__ void __launch_bounds__(TPB) k_diag_seq(DGrid g, const double *__restr
         for (int k = 0; k < 5; k++) out[k] = acc[k];
 }
 
+// ---------------------------------------------------------------------------
+// Lagrangian tracers (tracers.c:37-510).  Slot k of the tracer arrays is the
+// seed zone k = (h N1 + i) N2 + j of init_tracers' loop (tracers.c:150-170);
+// a tracer that leaves keeps its slot with a non-live state, so slot order is
+// the reference's vector order without compaction.  Positions are code
+// coordinates, SoA X[3][n].
+// ---------------------------------------------------------------------------
+struct TrArgs {
+    double *X;
+    int8_t *state;
+    harm_tracer_rec *ev;  // removal records, appended
+    unsigned long long *nev;
+    unsigned long long cap_ev;
+    long long n;
+    double sx1, sx2, sx3;  // startx
+    double R0, hslope, Rbr;
+    double Rin, Rout, thmin, thmax;  // tracers.c:126-131
+    double rho_unit, t_unit;
+};
+
+// metric.c:62-115 centre coordinate of active-relative index q
+__device__ __forceinline__ double tr_cent(double sx, int q, double dx) { return sx + ((double)q + 0.5) * dx; }
+__device__ __forceinline__ double tr_phcent(const DGrid &g, const TrArgs &a, int h)
+{
+    return g.N3 > 2 ? a.sx3 + ((double)h + 0.5) * g.dx3 : a.sx3 + (double)h * g.dx3;
+}
+
+// The k in [lo, hi] with edge(k) <= x < edge(k+1) for increasing edges, started
+// from a guess: the same k as the reference's linear scans (tracers.c:50-95).
+template <class F>
+__device__ bool tr_interval(double x, int lo, int hi, double guess, F edge, int &k)
+{
+    const double gk = fmin(fmax(floor(guess), (double)lo), (double)hi);
+    int q = (int)gk;
+    while (q > lo && x < edge(q)) q--;
+    while (q < hi && x >= edge(q + 1)) q++;
+    if (x >= edge(q) && x < edge(q + 1)) {
+        k = q;
+        return true;
+    }
+    return false;
+}
+
+// search_tracer (tracers.c:37-98): hh = phi plane (0-based), ii, jj = array
+// indices of the cell corner below the tracer.
+__device__ bool tr_search(const DGrid &g, const TrArgs &a, const double *X, int &hh, int &ii, int &jj)
+{
+    if (X[0] < tr_cent(a.sx1, g.is - 3, g.dx1) || X[1] < tr_cent(a.sx2, g.js - 3, g.dx2)) return false;
+    if (!isfinite(X[0]) || !isfinite(X[1])) return false;
+    hh = 0;
+    if (!tr_interval(X[0], g.is - 1, g.ie, (X[0] - a.sx1) / g.dx1 + 1.5,
+                     [&](int q) { return tr_cent(a.sx1, q - 2, g.dx1); }, ii))
+        return false;
+    if (!tr_interval(X[1], g.js - 1, g.je, (X[1] - a.sx2) / g.dx2 + 1.5,
+                     [&](int q) { return tr_cent(a.sx2, q - 2, g.dx2); }, jj))
+        return false;
+    if (g.N3 > 2) {
+        if (!isfinite(X[2])) return false;
+        if (!tr_interval(X[2], 0, g.N3 - 1, (X[2] - a.sx3) / g.dx3, [&](int q) { return a.sx3 + (double)q * g.dx3; },
+                         hh))
+            return false;
+    }
+    return true;
+}
+
+// bl_coord (coord.c:61-125) for r and theta
+__device__ void tr_bl(const TrArgs &a, const double *X, double &r, double &th)
+{
+    double theexp = X[0];
+    if (a.Rbr >= 0.) {
+        const double x1br = log(a.Rbr - a.R0);
+        if (X[0] > x1br) theexp += 1.0 * pow(X[0] - x1br, 4.0);
+    }
+    r = exp(theexp) + a.R0;
+    th = M_PI_2 * (1. + X[1]) + ((1. - a.hslope) / 2.) * sin(M_PI * (1. + X[1]));
+    if (fabs(th) < 1.E-20) th = (th >= 0) ? 1.E-20 : -1.E-20;
+    if (fabs(M_PI - th) < 1.E-20) th = (th >= M_PI) ? M_PI + 1.E-20 : M_PI - 1.E-20;
+}
+
+// linear_interpolation (tracers.c:174-245) of NV values of the zones around
+// (hh, ii, jj) to X; the plane past the last is the periodic ghost plane.
+template <int NV, class VF>
+__device__ void tr_interp(const DGrid &g, const TrArgs &a, const double *X, int hh, int ii, int jj, VF vf,
+                          double *val)
+{
+    const int wph = g.N3 > 2 ? 2 : 1;
+    double tmp2[2][NV];
+    const double x0 = tr_cent(a.sx2, jj - 2, g.dx2), x1 = tr_cent(a.sx2, jj - 1, g.dx2);
+    const double r0 = tr_cent(a.sx1, ii - 2, g.dx1), r1 = tr_cent(a.sx1, ii - 1, g.dx1);
+    for (int l3 = 0; l3 < wph; l3++) {
+        const int hz = (hh + l3 < g.N3) ? hh + l3 : hh + l3 - g.N3;
+        double tmp1[2][NV];
+        for (int l1 = 0; l1 < 2; l1++) {
+            double v0[NV], v1[NV];
+            vf(hz, ii + l1, jj, v0);
+            vf(hz, ii + l1, jj + 1, v1);
+            const double T1 = 1.0 / (x1 - x0), T2 = X[1] - x0;
+            for (int k = 0; k < NV; k++) tmp1[l1][k] = v0[k] + T2 * ((v1[k] - v0[k]) * T1);
+        }
+        const double T1 = 1.0 / (r1 - r0), T2 = X[0] - r0;
+        for (int k = 0; k < NV; k++) tmp2[l3][k] = tmp1[0][k] + T2 * ((tmp1[1][k] - tmp1[0][k]) * T1);
+    }
+    if (wph == 1) {
+        for (int k = 0; k < NV; k++) val[k] = tmp2[0][k];
+        return;
+    }
+    const double p0 = tr_phcent(g, a, hh), p1 = tr_phcent(g, a, hh + 1);
+    const double T1 = 1.0 / (p1 - p0), T2 = X[2] - p0;
+    for (int k = 0; k < NV; k++) val[k] = tmp2[0][k] + T2 * ((tmp2[1][k] - tmp2[0][k]) * T1);
+}
+
+// ucon_fun (tracers.c:248-260): u^t, u^i / u^t of one zone
+__device__ void tr_vel(const DGrid &g, const double *__restrict__ p, int h, int i, int j, double *v)
+{
+    const size_t z = zidx(g, i, j, h), nz = g.nz;
+    double pr[NPR];
+    for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+    Geom geom;
+    load_geom(g.geom[CENT], g.ncol, (size_t)i * g.SN2 + j, geom);
+    int fatal = 0;
+    double ucon[4];
+    ucon_calc(pr, geom, ucon, fatal);
+    v[0] = ucon[0];
+    v[1] = ucon[1] / ucon[0];
+    v[2] = ucon[2] / ucon[0];
+    v[3] = ucon[3] / ucon[0];
+}
+
+// rhoTYe_fun (tracers.c:432-466): rho RHO_UNIT, T, Ye of one zone
+template <bool TAB>
+__device__ void tr_rhoT(const DGrid &g, const TrArgs &a, const double *__restrict__ p, int h, int i, int j, double *v)
+{
+    const size_t z = zidx(g, i, j, h), nz = g.nz;
+    const double rho = p[RHO * nz + z], u = p[UU * nz + z];
+    v[0] = rho * a.rho_unit;
+    if (TAB) {
+        double pp, d1, d2, T;
+        eos_table_p(g.tab, rho, u, pp, d1, d2, &T);
+        v[1] = T;
+    } else {
+        constexpr double CL = 2.99792458e10, KBOL = 1.38e-16, mn = 1.67492729e-24, me = 9.109565e-28,
+                         mp = 1.673e-24, mhe = 6.6465e-24;
+        constexpr double mav = (mp + mn + 2. * me + mhe) / 5.;
+        v[1] = (g.gam - 1.) * u / rho / KBOL * mav * CL * CL;
+    }
+    v[2] = -1.;
+}
+
+// dump_one_tracer (tracers.c:470-510) at X with the cell found before the move
+template <bool TAB>
+__device__ void tr_record(const DGrid &g, const TrArgs &a, const double *__restrict__ p, const double *X, int hh,
+                          int ii, int jj, double t, long long k, int kind, harm_tracer_rec &rec)
+{
+    double val[3];
+    tr_interp<3>(g, a, X, hh, ii, jj, [&](int h, int i, int j, double *v) { tr_rhoT<TAB>(g, a, p, h, i, j, v); },
+                 val);
+    double r, th, ph;
+    tr_bl(a, X, r, th);
+    if (g.N3 > 2) {
+        ph = X[2];
+        if (ph > 2. * M_PI)
+            ph -= 2. * M_PI;
+        else if (ph < 0.)
+            ph += 2. * M_PI;
+    } else {
+        ph = 0.;
+    }
+    const long long n12 = (long long)g.N1 * g.N2;
+    rec.t = t * a.t_unit;
+    rec.rho = val[0];
+    rec.T = val[1];
+    rec.ye = val[2];
+    rec.r = r;
+    rec.th = th;
+    rec.ph = ph;
+    rec.h = (int32_t)(k / n12);
+    rec.i = (int32_t)((k % n12) / g.N2);
+    rec.j = (int32_t)(k % g.N2);
+    rec.kind = kind;
+}
+
+// init_tracers (tracers.c:109-172)
+__global__ void __launch_bounds__(TPB) k_tracer_init(DGrid g, TrArgs a, const double *__restrict__ p,
+                                                     unsigned long long *count)
+{
+    const long long k = (long long)blockIdx.x * TPB + threadIdx.x;
+    int live = 0;
+    if (k < a.n) {
+        const long long n12 = (long long)g.N1 * g.N2;
+        const int h = (int)(k / n12), i = (int)((k % n12) / g.N2), j = (int)(k % g.N2);
+        double X[3];
+        X[0] = a.sx1 + (i + 0.5) * g.dx1;
+        X[1] = a.sx2 + (j + 0.5) * g.dx2;
+        X[2] = a.sx3 + h * g.dx3;
+        int hh, ii, jj;
+        if (tr_search(g, a, X, hh, ii, jj)) live = p[RHO * g.nz + zidx(g, ii, jj, hh)] > RHOMIN;
+        a.X[k] = X[0];
+        a.X[a.n + k] = X[1];
+        a.X[2 * a.n + k] = X[2];
+        a.state[k] = live ? HARM_TRACER_LIVE : HARM_TRACER_NONE;
+    }
+    const unsigned long long b = __ballot(live);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, (unsigned long long)__popcll(b));
+}
+
+// update_tracers (tracers.c:298-430) after a step: t and dt are the step's
+// (tstate before k_stepend advances t).
+template <bool TAB>
+__global__ void __launch_bounds__(TPB) k_tracer_update(DGrid g, TrArgs a, const double *__restrict__ p,
+                                                       const double *__restrict__ tstate)
+{
+    const long long k = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (k >= a.n || a.state[k] != HARM_TRACER_LIVE) return;
+    const double t = tstate[0], Dt = tstate[1];
+    double X[3] = {a.X[k], a.X[a.n + k], a.X[2 * a.n + k]};
+    int hh, ii, jj;
+    if (!tr_search(g, a, X, hh, ii, jj)) {
+        a.state[k] = HARM_TRACER_LOST;
+        return;
+    }
+    double ucon[4];
+    tr_interp<4>(g, a, X, hh, ii, jj, [&](int h, int i, int j, double *v) { tr_vel(g, p, h, i, j, v); }, ucon);
+    for (int d = 0; d < 3; d++) X[d] += ucon[d + 1] * Dt;
+    double r, th;
+    tr_bl(a, X, r, th);
+    int kind = 0;
+    if (r < a.Rin || th < 0 || th > M_PI)
+        kind = HARM_TRACER_IN;
+    else if (r >= a.Rout)
+        kind = (th >= a.thmin && th <= a.thmax) ? HARM_TRACER_OUT : HARM_TRACER_JET;
+    if (kind) {
+        harm_tracer_rec rec;
+        tr_record<TAB>(g, a, p, X, hh, ii, jj, t, k, kind, rec);
+        const unsigned long long slot = atomicAdd(a.nev, 1ull);
+        if (slot < a.cap_ev) a.ev[slot] = rec;
+        a.state[k] = (int8_t)kind;
+    } else if (g.N3 > 2) {  // tracers.c:386-392 (phi-periodic on one rank)
+        if (X[2] < 0.)
+            X[2] += 2. * M_PI;
+        else if (X[2] > 2. * M_PI)
+            X[2] -= 2. * M_PI;
+    }
+    a.X[k] = X[0];
+    a.X[a.n + k] = X[1];
+    a.X[2 * a.n + k] = X[2];
+}
+
+// dump_tracers (tracers.c:512-560), pass 1: per-block count of the live
+// tracers that still lie in the grid (the others become LOST).
+__global__ void __launch_bounds__(TPB) k_tracer_count(DGrid g, TrArgs a, unsigned *bcount)
+{
+    __shared__ unsigned s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const long long k = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (k < a.n && a.state[k] == HARM_TRACER_LIVE) {
+        const double X[3] = {a.X[k], a.X[a.n + k], a.X[2 * a.n + k]};
+        int hh, ii, jj;
+        if (tr_search(g, a, X, hh, ii, jj))
+            atomicAdd(&s, 1u);
+        else
+            a.state[k] = HARM_TRACER_LOST;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = s;
+}
+
+// pass 2: records at base[block] + rank of the tracer in its block
+template <bool TAB>
+__global__ void __launch_bounds__(TPB) k_tracer_dump(DGrid g, TrArgs a, const double *__restrict__ p,
+                                                     const double *__restrict__ tstate,
+                                                     const unsigned long long *__restrict__ base,
+                                                     harm_tracer_rec *__restrict__ out)
+{
+    __shared__ unsigned wsum[TPB / 64];
+    const long long k = (long long)blockIdx.x * TPB + threadIdx.x;
+    const bool live = k < a.n && a.state[k] == HARM_TRACER_LIVE;
+    const unsigned long long b = __ballot(live);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = (unsigned)__popcll(b);
+    __syncthreads();
+    if (!live) return;
+    unsigned off = (unsigned)__popcll(b & ((1ull << lane) - 1ull));
+    for (int q = 0; q < w; q++) off += wsum[q];
+    const double X[3] = {a.X[k], a.X[a.n + k], a.X[2 * a.n + k]};
+    int hh, ii, jj;
+    tr_search(g, a, X, hh, ii, jj);  // found in pass 1
+    tr_record<TAB>(g, a, p, X, hh, ii, jj, tstate[0], k, HARM_TRACER_LIVE, out[base[blockIdx.x] + off]);
+}
+
 // Batched Noble 2D inversion over SoA arrays (utoprim_2d.c:126, plus the Font fix).
 // Batched inversion; LEARNED seeds Newton from the network (all lanes take
 // part in the wave-level forward pass, so there is no early exit).
@@ -1871,6 +2161,13 @@ struct harm_ctx {
     long long launches[HARM_NKERNELS];
     double mass_start;  // main.c:173-176
     bool flux_valid;    // F1 holds the fluxes of a completed full step (diag_flux)
+    struct Tracers {
+        bool on;
+        TrArgs a;                   // device arrays and parameters
+        harm_tracer_rec *dump;      // dump_tracers output, a.cap_ev records
+        unsigned *bcount;           // per-block live counts
+        unsigned long long *base;   // per-block output offsets
+    } tr;
 };
 
 namespace {
@@ -2119,6 +2416,15 @@ int one_step(harm_ctx *c)
     if ((rc = half_step(c, c->p, c->p, c->ph, 0.5, 0))) return rc;
     if ((rc = half_step(c, c->p, c->ph, c->p, 1.0, 1))) return rc;
     c->flux_valid = true;
+    if (c->tr.on) {
+        const unsigned nb = nblocks((size_t)c->tr.a.n);
+        if (c->eos_kind == 1)
+            LAUNCH(HARM_K_TRACERS, hipLaunchKernelGGL(k_tracer_update<true>, dim3(nb), dim3(TPB), 0, c->stream, g,
+                                                      c->tr.a, c->p, c->tstate));
+        else
+            LAUNCH(HARM_K_TRACERS, hipLaunchKernelGGL(k_tracer_update<false>, dim3(nb), dim3(TPB), 0, c->stream, g,
+                                                      c->tr.a, c->p, c->tstate));
+    }
     LAUNCH(HARM_K_STEPEND,
            hipLaunchKernelGGL(k_mdot_partials, dim3(g.N3), dim3(TPB), 0, c->stream, g, c->p, c->part);
            hipLaunchKernelGGL(k_stepend, dim3(1), dim3(1), 0, c->stream, g, c->tstate, c->ndt_slot, c->part, g.N3));
@@ -2370,10 +2676,22 @@ harm_ctx *harm_create(const harm_params *p, int device)
     return c;
 }
 
+namespace {
+void tracers_free(harm_ctx *c)
+{
+    auto &t = c->tr;
+    for (void *q : {(void *)t.a.X, (void *)t.a.state, (void *)t.a.ev, (void *)t.a.nev, (void *)t.dump,
+                    (void *)t.bcount, (void *)t.base})
+        if (q) (void)hipFree(q);
+    std::memset(&t, 0, sizeof(t));
+}
+}  // namespace
+
 void harm_destroy(harm_ctx *c)
 {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
+    tracers_free(c);
     ctx_free(c);
     delete c;
 }
@@ -3052,3 +3370,180 @@ int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tracers (tracers.c:109-654)
+// ---------------------------------------------------------------------------
+int harm_tracers_init(harm_ctx *c, double rho_unit, double t_unit, int64_t *n_live)
+{
+    if (!c) return set_err("null context");
+    if (c->P.n3 == 2) return set_err("tracers: N3 == 2 is not supported", 3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    tracers_free(c);
+    auto &t = c->tr;
+    TrArgs &a = t.a;
+    const auto &hg = c->hg;
+    const harm_params &P = c->P;
+    int rc;
+    a.n = (long long)hg.N1 * hg.N2 * hg.N3;
+    const unsigned nb = nblocks((size_t)a.n);
+    if ((rc = dalloc(&a.X, 3 * (size_t)a.n))) return rc;
+    if ((rc = dalloc(&a.state, (size_t)a.n))) return rc;
+    if ((rc = dalloc(&a.nev, 1))) return rc;
+    if ((rc = dalloc(&t.bcount, nb))) return rc;
+    if ((rc = dalloc(&t.base, nb))) return rc;
+    a.sx1 = hg.startx[1];
+    a.sx2 = hg.startx[2];
+    a.sx3 = hg.startx[3];
+    a.R0 = P.R0;
+    a.hslope = P.hslope;
+    a.Rbr = P.Rbr;
+    a.Rin = exp(hg.startx[1] - 0.5 * hg.dx[1]) + P.R0;        // tracers.c:126
+    a.Rout = 0.8 * (exp(hg.startx[1] + hg.N1 * hg.dx[1]) + P.R0);  // tracers.c:127
+    a.thmin = 0.02 * M_PI;
+    a.thmax = 0.98 * M_PI;
+    a.rho_unit = rho_unit;
+    a.t_unit = t_unit;
+    hipLaunchKernelGGL(k_tracer_init, dim3(nb), dim3(TPB), 0, c->stream, c->g, a, c->p, a.nev);
+    HIPCHK(hipGetLastError());
+    unsigned long long live = 0;
+    HIPCHK(hipMemcpyAsync(&live, a.nev, sizeof(live), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemsetAsync(a.nev, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // each tracer leaves at most once, and the live count never grows
+    a.cap_ev = live;
+    if ((rc = dalloc(&a.ev, (size_t)live))) return rc;
+    if ((rc = dalloc(&t.dump, (size_t)live))) return rc;
+    t.on = true;
+    if (n_live) *n_live = (int64_t)live;
+    return 0;
+}
+
+int harm_tracers_stop(harm_ctx *c)
+{
+    if (!c) return set_err("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    tracers_free(c);
+    return 0;
+}
+
+int harm_tracers_dump(harm_ctx *c, harm_tracer_rec *recs, int64_t cap, int64_t *n)
+{
+    if (!c || !n) return set_err("null argument");
+    auto &t = c->tr;
+    if (!t.on) return set_err("tracers: harm_tracers_init has not been called");
+    const unsigned nb = nblocks((size_t)t.a.n);
+    hipLaunchKernelGGL(k_tracer_count, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, t.bcount);
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned> cnt(nb);
+    std::vector<unsigned long long> base(nb);
+    HIPCHK(hipMemcpyAsync(cnt.data(), t.bcount, nb * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long total = 0;
+    for (unsigned b = 0; b < nb; b++) {
+        base[b] = total;
+        total += cnt[b];
+    }
+    *n = (int64_t)total;
+    if ((int64_t)total > cap || (total && !recs)) return set_err("harm_tracers_dump: record buffer too small", -1);
+    if (total > t.a.cap_ev) return set_err("harm_tracers_dump: more live tracers than were seeded");
+    HIPCHK(hipMemcpyAsync(t.base, base.data(), nb * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    if (c->eos_kind == 1)
+        hipLaunchKernelGGL(k_tracer_dump<true>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->tstate, t.base,
+                           t.dump);
+    else
+        hipLaunchKernelGGL(k_tracer_dump<false>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->tstate,
+                           t.base, t.dump);
+    HIPCHK(hipGetLastError());
+    if (total)
+        HIPCHK(hipMemcpyAsync(recs, t.dump, total * sizeof(harm_tracer_rec), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_tracers_events(harm_ctx *c, harm_tracer_rec *recs, int64_t cap, int64_t *n)
+{
+    if (!c || !n) return set_err("null argument");
+    auto &t = c->tr;
+    if (!t.on) return set_err("tracers: harm_tracers_init has not been called");
+    unsigned long long ne = 0;
+    HIPCHK(hipMemcpyAsync(&ne, t.a.nev, sizeof(ne), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *n = (int64_t)ne;
+    if ((int64_t)ne > cap || (ne && !recs)) return set_err("harm_tracers_events: record buffer too small", -1);
+    if (ne > t.a.cap_ev) return set_err("harm_tracers_events: more removals than tracers");
+    if (ne) HIPCHK(hipMemcpyAsync(recs, t.a.ev, ne * sizeof(harm_tracer_rec), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemsetAsync(t.a.nev, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // the atomic append order is arbitrary within a step: restore (step, slot)
+    const long long n12 = (long long)c->hg.N1 * c->hg.N2;
+    auto slot = [&](const harm_tracer_rec &r) { return (long long)r.h * n12 + (long long)r.i * c->hg.N2 + r.j; };
+    std::sort(recs, recs + ne, [&](const harm_tracer_rec &x, const harm_tracer_rec &y) {
+        return x.t < y.t || (x.t == y.t && slot(x) < slot(y));
+    });
+    return 0;
+}
+
+int harm_tracers_update(harm_ctx *c, double dt)
+{
+    if (!c) return set_err("null context");
+    auto &t = c->tr;
+    if (!t.on) return set_err("tracers: harm_tracers_init has not been called");
+    // k_tracer_update reads (t, dt) from a two-double state: the divergence
+    // buffer's first two doubles hold them for this launch
+    double ts[2];
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ts[1] = dt;
+    HIPCHK(hipMemcpyAsync(c->S, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    const unsigned nb = nblocks((size_t)t.a.n);
+    if (c->eos_kind == 1)
+        hipLaunchKernelGGL(k_tracer_update<true>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->S);
+    else
+        hipLaunchKernelGGL(k_tracer_update<false>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->S);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_tracers_get(harm_ctx *c, double *X, int8_t *state)
+{
+    if (!c) return set_err("null context");
+    auto &t = c->tr;
+    if (!t.on) return set_err("tracers: harm_tracers_init has not been called");
+    if (X) HIPCHK(hipMemcpyAsync(X, t.a.X, 3 * (size_t)t.a.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (state) HIPCHK(hipMemcpyAsync(state, t.a.state, (size_t)t.a.n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_tracers_set(harm_ctx *c, const double *X, const int8_t *state)
+{
+    if (!c) return set_err("null context");
+    auto &t = c->tr;
+    if (!t.on) return set_err("tracers: harm_tracers_init has not been called");
+    if (state) {
+        long long live = 0;
+        for (long long k = 0; k < t.a.n; k++) {
+            if (state[k] < HARM_TRACER_NONE || state[k] > HARM_TRACER_LOST)
+                return set_err("harm_tracers_set: unknown tracer state", 3);
+            live += state[k] == HARM_TRACER_LIVE;
+        }
+        if ((unsigned long long)live > t.a.cap_ev) {
+            // room for every live tracer's removal record and dump line
+            for (void *q : {(void *)t.a.ev, (void *)t.dump})
+                if (q) HIPCHK(hipFree(q));
+            t.a.ev = nullptr;
+            t.dump = nullptr;
+            int rc;
+            if ((rc = dalloc(&t.a.ev, (size_t)live))) return rc;
+            if ((rc = dalloc(&t.dump, (size_t)live))) return rc;
+            t.a.cap_ev = (unsigned long long)live;
+        }
+    }
+    if (X) HIPCHK(hipMemcpyAsync(t.a.X, X, 3 * (size_t)t.a.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (state) HIPCHK(hipMemcpyAsync(t.a.state, state, (size_t)t.a.n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(t.a.nev, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}

@@ -64,7 +64,7 @@ enum {
 enum {
     HARM_K_FLUX1 = 0, HARM_K_FLUX2, HARM_K_FLUX3, HARM_K_EMF, HARM_K_UPDATE,
     HARM_K_BOUND, HARM_K_FIXUTOPRIM, HARM_K_STEPEND, HARM_K_C2P_BATCH, HARM_K_MLP, HARM_K_FONTFIX,
-    HARM_K_DIV, HARM_NKERNELS
+    HARM_K_DIV, HARM_K_TRACERS, HARM_NKERNELS
 };
 
 typedef struct harm_ctx harm_ctx;
@@ -123,6 +123,50 @@ int harm_ener_line(const harm_diag_rec *r, char *buf, size_t n);
 /* mass_start (main.c:173-176; set by harm_init_torus and harm_restart_read) and
  * the mass_in / mass_out that diag_mass accumulates (diag.c:577-578). */
 int harm_set_mass_budget(harm_ctx *c, double mass_start, double mass_in, double mass_out);
+
+/* Lagrangian tracers (tracers.c:109-654).  harm_tracers_init seeds one tracer
+ * at every active zone whose density exceeds RHOMIN, at X = (zone centre in
+ * X1, X2; phi face in X3), in (h, i, j) order (init_tracers, tracers.c:150-170);
+ * afterwards every harm_step advances them with the interpolated u^i/u^t of
+ * the new state and the step's dt (update_tracers, tracers.c:298-430) and
+ * records the ones that leave (r < Rin_tr or theta outside [0, pi]: IN;
+ * r >= Rout_tr: OUT, or JET outside [0.02 pi, 0.98 pi]).  Slot k of the
+ * per-tracer arrays is the seed zone k = (h N1 + i) N2 + j (zero-based). */
+enum {
+    HARM_TRACER_NONE = 0, /* no tracer seeded at this zone */
+    HARM_TRACER_LIVE = 1,
+    HARM_TRACER_IN = 2,
+    HARM_TRACER_OUT = 3,
+    HARM_TRACER_JET = 4,
+    HARM_TRACER_LOST = 5 /* position outside the searchable grid (tracers.c:322-328) */
+};
+/* One line of a tracer file (dump_one_tracer, tracers.c:470-510): t T_UNIT,
+ * rho RHO_UNIT, T, Ye, r, theta, phi; (h, i, j) name the tracer's file. */
+typedef struct harm_tracer_rec {
+    double t, rho, T, ye, r, th, ph;
+    int32_t h, i, j, kind;
+} harm_tracer_rec;
+/* rho_unit, t_unit: the run's RHO_UNIT and T_UNIT (cooleos.c:120-140).  T is
+ * the Gamma-law temperature of rhoTYe_fun (tracers.c:446-459) with Ye = -1, or
+ * with a tabulated EOS the table's temperature (Ye = -1: the table carries no
+ * electron fraction).  Not supported for N3 == 2.  *n_live may be NULL. */
+int harm_tracers_init(harm_ctx *c, double rho_unit, double t_unit, int64_t *n_live);
+int harm_tracers_stop(harm_ctx *c);
+/* dump_tracers (tracers.c:512-560): one record per live tracer, in slot order;
+ * live tracers whose position no longer lies in the grid become LOST.
+ * Returns -1 with *n = the count if cap is too small. */
+int harm_tracers_dump(harm_ctx *c, harm_tracer_rec *recs, int64_t cap, int64_t *n);
+/* The records of tracers that left since the previous call, in (step, slot)
+ * order, as update_tracers writes them before it renames the files into
+ * tracers/in, out or jet.  Returns -1 with *n = the count if cap is too small. */
+int harm_tracers_events(harm_ctx *c, harm_tracer_rec *recs, int64_t cap, int64_t *n);
+/* update_tracers(dt) on the current state and time without a step (harm_step
+ * already calls it after every step). */
+int harm_tracers_update(harm_ctx *c, double dt);
+/* Positions (X[3][N1 N2 N3], X1..X3) and states (HARM_TRACER_*) of every slot. */
+int harm_tracers_get(harm_ctx *c, double *X, int8_t *state);
+/* Replace positions and states (restart_read_tracers, tracers.c:608-654). */
+int harm_tracers_set(harm_ctx *c, const double *X, const int8_t *state);
 
 /* Device pointers of the state (canonical layout, var stride (N1+4)*(N2+4)*N3). */
 int harm_device_state(harm_ctx *c, double **prim, int8_t **pflag, void **stream);

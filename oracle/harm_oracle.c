@@ -125,6 +125,13 @@ struct orc_ctx {
     int64_t cnt[ORC_NCOUNTERS];
     double mass_in, mass_out, mass_start;
     int flux_valid; /* F1 holds the fluxes of a completed full step (diag_flux) */
+    /* tracers (tracers.c): slot k = (h N1 + i) N2 + j, X[3][n], state per slot */
+    int tr_on;
+    long long tr_n, tr_nev, tr_cap;
+    double *tr_X;
+    int8_t *tr_state;
+    orc_tracer_rec *tr_ev;
+    double tr_Rin, tr_Rout, tr_thmin, tr_thmax, tr_rho_unit, tr_t_unit;
     const orc_eos *eos; /* tabulated EOS of the step (NULL: Gamma-law) */
     int eos_refd;
 };
@@ -1950,6 +1957,283 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
     return ndt;
 }
 
+/* ------------------------------------------------------------------------ */
+/* tracers.c: Lagrangian tracers                                             */
+/* ------------------------------------------------------------------------ */
+/* metric.c:62-115, CENT, for array indices (i, j) and 0-based plane h */
+static double tr_c1(const orc_ctx *o, int i) { return o->startx[RR] + ((i - 2) + 0.5) * o->dx[RR]; }
+static double tr_c2(const orc_ctx *o, int j) { return o->startx[TH] + ((j - 2) + 0.5) * o->dx[TH]; }
+static double tr_c3(const orc_ctx *o, int h)
+{
+    return o->N3 > 2 ? o->startx[PH] + (h + 0.5) * o->dx[PH] : o->startx[PH] + h * o->dx[PH];
+}
+
+/* search_tracer, tracers.c:37-98 (linear scans) */
+static int tr_search(const orc_ctx *o, const double *X, int *hh, int *ii, int *jj)
+{
+    int ok;
+    if (X[RR] < tr_c1(o, o->istart - 1) || X[TH] < tr_c2(o, o->jstart - 1)) return 0;
+    *hh = 0;
+    ok = 0;
+    for (int i = o->istart - 1; i <= o->istop; i++)
+        if (X[RR] >= tr_c1(o, i) && X[RR] < tr_c1(o, i + 1)) {
+            *ii = i;
+            ok = 1;
+            break;
+        }
+    if (!ok) return 0;
+    ok = 0;
+    for (int j = o->jstart - 1; j <= o->jstop; j++)
+        if (X[TH] >= tr_c2(o, j) && X[TH] < tr_c2(o, j + 1)) {
+            *jj = j;
+            ok = 1;
+            break;
+        }
+    if (!ok) return 0;
+    if (o->N3 > 2) {
+        ok = 0;
+        for (int h = 0; h < o->N3; h++)
+            if (X[PH] >= o->startx[PH] + h * o->dx[PH] && X[PH] < o->startx[PH] + (h + 1) * o->dx[PH]) {
+                *hh = h;
+                ok = 1;
+                break;
+            }
+        if (!ok) return 0;
+    }
+    return 1;
+}
+
+/* ucon_fun, tracers.c:248-260 */
+static void tr_ucon(const orc_ctx *o, int h, int i, int j, double *v)
+{
+    geom_t geom;
+    int64_t fatal = 0;
+    get_geometry(o, i, j, CENT, &geom);
+    ucon_calc(PV(o, o->p, h, i, j), &geom, v, &fatal);
+    v[RR] /= v[TT];
+    v[TH] /= v[TT];
+    v[PH] /= v[TT];
+}
+
+/* rhoTYe_fun, tracers.c:432-466 (Ye = -1; with a table, its temperature) */
+static void tr_rhoT(const orc_ctx *o, int h, int i, int j, double *v)
+{
+    const double *pz = PV(o, o->p, h, i, j);
+    v[0] = pz[RHO] * o->tr_rho_unit;
+    if (o->eos) {
+        double pp, T;
+        orc_eos_table_p(o->eos, pz[RHO], pz[UU], &pp, NULL, NULL, &T);
+        v[1] = T;
+    } else {
+        const double CL = 2.99792458e10, KBOL = 1.38e-16, mn = 1.67492729e-24, me = 9.109565e-28, mp = 1.673e-24,
+                     mhe = 6.6465e-24;
+        const double mav = (mp + mn + 2. * me + mhe) / 5.;
+        v[1] = (o->P.gam - 1.) * pz[UU] / pz[RHO] / KBOL * mav * CL * CL;
+    }
+    v[2] = -1.;
+}
+
+/* linear_interpolation, tracers.c:174-245; X is [TT, RR, TH, PH] */
+static void tr_interp(const orc_ctx *o, const double *X, int hh, int ii, int jj, int nv, double *val,
+                      void (*fun)(const orc_ctx *, int, int, int, double *))
+{
+    const int wph = o->N3 > 2 ? 2 : 1;
+    double XX[2][2][2][NDIM], valt[2][2][2][NDIM], tmp1[2][2][NDIM], tmp2[2][NDIM];
+    for (int l3 = 0; l3 < wph; l3++)
+        for (int l1 = 0; l1 < 2; l1++)
+            for (int l2 = 0; l2 < 2; l2++) {
+                XX[l3][l1][l2][RR] = tr_c1(o, ii + l1);
+                XX[l3][l1][l2][TH] = tr_c2(o, jj + l2);
+                XX[l3][l1][l2][PH] = tr_c3(o, hh + l3);
+                fun(o, hwrap(o, hh + l3), ii + l1, jj + l2, valt[l3][l1][l2]);
+            }
+    for (int l3 = 0; l3 < wph; l3++)
+        for (int l1 = 0; l1 < 2; l1++) {
+            const double T1 = 1.0 / (XX[l3][l1][1][TH] - XX[l3][l1][0][TH]);
+            const double T2 = X[TH] - XX[l3][l1][0][TH];
+            for (int k = 0; k < nv; k++) {
+                const double ypval = (valt[l3][l1][1][k] - valt[l3][l1][0][k]) * T1;
+                tmp1[l3][l1][k] = valt[l3][l1][0][k] + T2 * ypval;
+            }
+        }
+    for (int l3 = 0; l3 < wph; l3++) {
+        const double T1 = 1.0 / (XX[l3][1][1][RR] - XX[l3][0][0][RR]);
+        const double T2 = X[RR] - XX[l3][0][0][RR];
+        for (int k = 0; k < nv; k++) {
+            const double ypval = (tmp1[l3][1][k] - tmp1[l3][0][k]) * T1;
+            tmp2[l3][k] = tmp1[l3][0][k] + T2 * ypval;
+        }
+    }
+    if (wph == 1) {
+        for (int k = 0; k < nv; k++) val[k] = tmp2[0][k];
+        return;
+    }
+    const double T1 = 1.0 / (XX[1][1][1][PH] - XX[0][0][0][PH]);
+    const double T2 = X[PH] - XX[0][0][0][PH];
+    for (int k = 0; k < nv; k++) {
+        const double ypval = (tmp2[1][k] - tmp2[0][k]) * T1;
+        val[k] = tmp2[0][k] + T2 * ypval;
+    }
+}
+
+/* dump_one_tracer, tracers.c:470-510 */
+static void tr_record(const orc_ctx *o, long long k, const double *X, int hh, int ii, int jj, int kind,
+                      orc_tracer_rec *r)
+{
+    double val[3], V[NDIM];
+    const long long n12 = (long long)o->N1 * o->N2;
+    tr_interp(o, X, hh, ii, jj, 3, val, tr_rhoT);
+    bl_coord_vec(&o->P, X, V);
+    double ph = V[PH];
+    if (o->N3 > 2) {
+        if (ph > 2. * M_PI)
+            ph -= 2. * M_PI;
+        else if (ph < 0.)
+            ph += 2. * M_PI;
+    } else {
+        ph = 0.0;
+    }
+    r->t = o->t * o->tr_t_unit;
+    r->rho = val[0];
+    r->T = val[1];
+    r->ye = val[2];
+    r->r = V[RR];
+    r->th = V[TH];
+    r->ph = ph;
+    r->h = (int32_t)(k / n12);
+    r->i = (int32_t)((k % n12) / o->N2);
+    r->j = (int32_t)(k % o->N2);
+    r->kind = kind;
+}
+
+/* update_tracers, tracers.c:298-430 (one rank: phi wraps in place) */
+static void tracers_update(orc_ctx *o, double Dt)
+{
+    const long long n = o->tr_n;
+    for (long long k = 0; k < n; k++) {
+        if (o->tr_state[k] != ORC_TRACER_LIVE) continue;
+        double X[NDIM] = {0., o->tr_X[k], o->tr_X[n + k], o->tr_X[2 * n + k]}, ucon[NDIM], V[NDIM];
+        int hh, ii, jj, kind = 0;
+        if (!tr_search(o, X, &hh, &ii, &jj)) {
+            o->tr_state[k] = ORC_TRACER_LOST;
+            continue;
+        }
+        tr_interp(o, X, hh, ii, jj, NDIM, ucon, tr_ucon);
+        for (int d = 1; d < NDIM; d++) X[d] += ucon[d] * Dt;
+        bl_coord_vec(&o->P, X, V);
+        if (V[RR] < o->tr_Rin || V[TH] < 0 || V[TH] > M_PI)
+            kind = ORC_TRACER_IN;
+        else if (V[RR] >= o->tr_Rout)
+            kind = (V[TH] >= o->tr_thmin && V[TH] <= o->tr_thmax) ? ORC_TRACER_OUT : ORC_TRACER_JET;
+        if (kind) {
+            if (o->tr_nev < o->tr_cap) tr_record(o, k, X, hh, ii, jj, kind, &o->tr_ev[o->tr_nev++]);
+            o->tr_state[k] = (int8_t)kind;
+        } else if (o->N3 > 2) {
+            if (X[PH] < 0.)
+                X[PH] += 2. * M_PI;
+            else if (X[PH] > 2. * M_PI)
+                X[PH] -= 2. * M_PI;
+        }
+        o->tr_X[k] = X[RR];
+        o->tr_X[n + k] = X[TH];
+        o->tr_X[2 * n + k] = X[PH];
+    }
+}
+
+/* init_tracers, tracers.c:109-172 */
+long long orc_tracers_init(orc_ctx *o, double rho_unit, double t_unit)
+{
+    const long long n = (long long)o->N1 * o->N2 * o->N3;
+    long long live = 0;
+    free(o->tr_X);
+    free(o->tr_state);
+    free(o->tr_ev);
+    o->tr_n = n;
+    o->tr_X = (double *)calloc(3 * (size_t)n, sizeof(double));
+    o->tr_state = (int8_t *)calloc((size_t)n, 1);
+    o->tr_Rin = exp(o->startx[RR] - 0.5 * o->dx[RR]) + o->P.R0;
+    o->tr_Rout = 0.8 * (exp(o->startx[RR] + o->N1 * o->dx[RR]) + o->P.R0);
+    o->tr_thmin = 0.02 * M_PI;
+    o->tr_thmax = 0.98 * M_PI;
+    o->tr_rho_unit = rho_unit;
+    o->tr_t_unit = t_unit;
+    long long k = 0;
+    for (int h = 0; h < o->N3; h++)
+        for (int i = 0; i < o->N1; i++)
+            for (int j = 0; j < o->N2; j++, k++) {
+                double X[NDIM];
+                int hh, ii, jj;
+                X[TT] = 0.;
+                X[RR] = o->startx[RR] + (i + 0.5) * o->dx[RR];
+                X[TH] = o->startx[TH] + (j + 0.5) * o->dx[TH];
+                X[PH] = o->startx[PH] + h * o->dx[PH];
+                o->tr_X[k] = X[RR];
+                o->tr_X[n + k] = X[TH];
+                o->tr_X[2 * n + k] = X[PH];
+                if (tr_search(o, X, &hh, &ii, &jj) && PV(o, o->p, hh, ii, jj)[RHO] > RHOMIN) {
+                    o->tr_state[k] = ORC_TRACER_LIVE;
+                    live++;
+                }
+            }
+    o->tr_cap = live;
+    o->tr_nev = 0;
+    o->tr_ev = (orc_tracer_rec *)calloc(live > 0 ? (size_t)live : 1, sizeof(orc_tracer_rec));
+    o->tr_on = 1;
+    return live;
+}
+
+/* dump_tracers, tracers.c:512-560 */
+long long orc_tracers_dump(orc_ctx *o, orc_tracer_rec *recs, long long cap)
+{
+    const long long n = o->tr_n;
+    long long m = 0;
+    g_step_eos = o->eos;
+    for (long long k = 0; k < n; k++) {
+        if (o->tr_state[k] != ORC_TRACER_LIVE) continue;
+        const double X[NDIM] = {0., o->tr_X[k], o->tr_X[n + k], o->tr_X[2 * n + k]};
+        int hh, ii, jj;
+        if (!tr_search(o, X, &hh, &ii, &jj)) {
+            o->tr_state[k] = ORC_TRACER_LOST;
+            continue;
+        }
+        if (m < cap) tr_record(o, k, X, hh, ii, jj, ORC_TRACER_LIVE, &recs[m]);
+        m++;
+    }
+    g_step_eos = NULL;
+    return m;
+}
+
+/* records of the tracers that left since the previous call, in order */
+long long orc_tracers_events(orc_ctx *o, orc_tracer_rec *recs, long long cap)
+{
+    const long long m = o->tr_nev;
+    for (long long q = 0; q < m && q < cap; q++) recs[q] = o->tr_ev[q];
+    o->tr_nev = 0;
+    return m;
+}
+
+void orc_tracers_update(orc_ctx *o, double Dt) { tracers_update(o, Dt); }
+
+void orc_tracers_set(orc_ctx *o, const double *X, const int8_t *state)
+{
+    long long live = 0;
+    if (X) memcpy(o->tr_X, X, 3 * (size_t)o->tr_n * sizeof(double));
+    if (state) memcpy(o->tr_state, state, (size_t)o->tr_n);
+    for (long long k = 0; k < o->tr_n; k++) live += o->tr_state[k] == ORC_TRACER_LIVE;
+    if (live > o->tr_cap) {
+        free(o->tr_ev);
+        o->tr_ev = (orc_tracer_rec *)calloc((size_t)live, sizeof(orc_tracer_rec));
+        o->tr_cap = live;
+    }
+    o->tr_nev = 0;
+}
+
+void orc_tracers_get(const orc_ctx *o, double *X, int8_t *state)
+{
+    if (X) memcpy(X, o->tr_X, 3 * (size_t)o->tr_n * sizeof(double));
+    if (state) memcpy(state, o->tr_state, (size_t)o->tr_n);
+}
+
 /* step_ch.c:90-140 */
 void orc_set_eos(orc_ctx *o, const orc_eos *eos, int ref_derivs)
 {
@@ -1979,6 +2263,7 @@ int orc_step(orc_ctx *o)
         o->mass_in += out[2] * o->dt;
         o->mass_out += out[3] * o->dt;
     }
+    if (o->tr_on) tracers_update(o, o->dt); /* step_ch.c:121-123 */
     g_step_eos = NULL;
     if (o->dt < 1.e-9) return 11;
     o->t += o->dt;
@@ -2514,6 +2799,7 @@ void orc_destroy(orc_ctx *o)
     free(o->p); free(o->ph); free(o->dq); free(o->F1); free(o->F2); free(o->F3);
     free(o->e1); free(o->e2); free(o->e3); free(o->pflag);
     free(o->gcov); free(o->gcon); free(o->gdet); free(o->conn); free(o->coords); free(o->Katm);
+    free(o->tr_X); free(o->tr_state); free(o->tr_ev);
     free(o);
 }
 

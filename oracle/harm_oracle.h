@@ -91,6 +91,20 @@ typedef struct orc_diag_rec {
 void     orc_diag_log(orc_ctx *o, orc_diag_rec *r);
 /* mass_start (main.c:173-176) and the accumulated mass_in / mass_out. */
 void     orc_set_mass_budget(orc_ctx *o, double mass_start, double mass_in, double mass_out);
+/* Lagrangian tracers (tracers.c:109-560): slot k = (h N1 + i) N2 + j of the
+ * seed zone; after orc_tracers_init every orc_step advances them. */
+enum { ORC_TRACER_NONE = 0, ORC_TRACER_LIVE, ORC_TRACER_IN, ORC_TRACER_OUT, ORC_TRACER_JET, ORC_TRACER_LOST };
+typedef struct orc_tracer_rec {
+    double t, rho, T, ye, r, th, ph;
+    int32_t h, i, j, kind;
+} orc_tracer_rec;
+long long orc_tracers_init(orc_ctx *o, double rho_unit, double t_unit);
+long long orc_tracers_dump(orc_ctx *o, orc_tracer_rec *recs, long long cap);
+long long orc_tracers_events(orc_ctx *o, orc_tracer_rec *recs, long long cap);
+void      orc_tracers_get(const orc_ctx *o, double *X, int8_t *state);
+void      orc_tracers_set(orc_ctx *o, const double *X, const int8_t *state);
+/* update_tracers(Dt) on the current state, without a step */
+void      orc_tracers_update(orc_ctx *o, double Dt);
 
 /* Single-zone primitive solvers; gcov/gcon are row-major 4x4. */
 int  orc_utoprim_2d(const double U[8], const double *gcov, const double *gcon,

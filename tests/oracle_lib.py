@@ -100,6 +100,14 @@ def lib():
         L.orc_diag.argtypes = [c_void_p, _dp]
         L.orc_diag_log.argtypes = [c_void_p, POINTER(OrcDiagRec)]
         L.orc_set_mass_budget.argtypes = [c_void_p, c_double, c_double, c_double]
+        L.orc_tracers_init.restype = ctypes.c_longlong
+        L.orc_tracers_init.argtypes = [c_void_p, c_double, c_double]
+        for name in ("orc_tracers_dump", "orc_tracers_events"):
+            getattr(L, name).restype = ctypes.c_longlong
+            getattr(L, name).argtypes = [c_void_p, c_void_p, ctypes.c_longlong]
+        L.orc_tracers_get.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.orc_tracers_set.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.orc_tracers_update.argtypes = [c_void_p, c_double]
         L.orc_set_threads.argtypes = [c_int]
         L.orc_utoprim_2d.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, POINTER(c_int), POINTER(c_int)]
         L.orc_utoprim_1dvsq2fix1.argtypes = [_dp, _dp, _dp, c_double, _dp, c_double, c_double]
@@ -270,6 +278,35 @@ class Oracle:
 
     def set_mass_budget(self, mass_start: float, mass_in: float = 0.0, mass_out: float = 0.0) -> None:
         self.L.orc_set_mass_budget(self.h, mass_start, mass_in, mass_out)
+
+    def tracers_init(self, rho_unit: float = 1.0, t_unit: float = 1.0) -> int:
+        return self.L.orc_tracers_init(self.h, rho_unit, t_unit)
+
+    def _tracer_recs(self, fn):
+        from harm_cool_amd import TRACER_DTYPE
+        buf = np.zeros(self.n1 * self.n2 * self.n3, dtype=TRACER_DTYPE)
+        m = fn(self.h, buf.ctypes.data, buf.size)
+        return buf[:m].copy()
+
+    def tracers_dump(self):
+        return self._tracer_recs(self.L.orc_tracers_dump)
+
+    def tracers_events(self):
+        return self._tracer_recs(self.L.orc_tracers_events)
+
+    def tracers_set(self, X, state) -> None:
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        st = np.ascontiguousarray(state, dtype=np.int8)
+        self.L.orc_tracers_set(self.h, X.ctypes.data, st.ctypes.data)
+
+    def tracers_update(self, dt: float) -> None:
+        self.L.orc_tracers_update(self.h, dt)
+
+    def tracers_get(self):
+        X = np.empty((3, self.n3, self.n1, self.n2))
+        st = np.empty((self.n3, self.n1, self.n2), dtype=np.int8)
+        self.L.orc_tracers_get(self.h, X.ctypes.data, st.ctypes.data)
+        return X, st
 
 
 def utoprim_2d(U, gcov, gcon, gdet, prim, gam):
