@@ -98,6 +98,7 @@ class DiagRec(ctypes.Structure):
 TRACER_DTYPE = np.dtype([("t", "f8"), ("rho", "f8"), ("T", "f8"), ("ye", "f8"), ("r", "f8"), ("th", "f8"),
                          ("ph", "f8"), ("h", "i4"), ("i", "i4"), ("j", "i4"), ("kind", "i4")])
 TRACER_NONE, TRACER_LIVE, TRACER_IN, TRACER_OUT, TRACER_JET, TRACER_LOST = range(6)
+DUMP_NCOL = 46  # columns of an ASCII dump line (HARM_DUMP_NCOL)
 _TRACER_DIRS = {TRACER_IN: "in", TRACER_OUT: "out", TRACER_JET: "jet"}
 
 
@@ -281,6 +282,8 @@ def lib() -> ctypes.CDLL:
     L.harm_tracers_events.argtypes = [c_void_p, c_void_p, c_int64, POINTER(c_int64)]
     L.harm_tracers_get.argtypes = [c_void_p, c_void_p, c_void_p]
     L.harm_tracers_update.argtypes = [c_void_p, c_double]
+    L.harm_dump_write.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
+    L.harm_dump_columns.argtypes = [c_void_p, c_size_t, c_size_t, c_void_p]
     L.harm_tracers_set.argtypes = [c_void_p, c_void_p, c_void_p]
     L.harm_device_state.argtypes = [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p)]
     L.harm_set_timing.argtypes = [c_void_p, c_int]
@@ -550,6 +553,20 @@ class Harm:
         the run driver owns (nstep, dump counters, clocks, ener_size, ...)."""
         h = RestartHdr.of(driver)
         _check(self.L.harm_restart_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_write")
+
+    def dump_write(self, path: str, **driver) -> None:
+        """dump() ASCII file (dump.c:1710-1990); ``driver`` gives nstep, the
+        DT* clocks, the dump/image/rdump counters and ``failed``."""
+        h = RestartHdr.of(driver)
+        _check(self.L.harm_dump_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_dump_write")
+
+    def dump_columns(self, z0: int = 0, n: "int | None" = None) -> np.ndarray:
+        """The dump's 46 columns of active zones z0..z0+n-1 in (i, j, h) order."""
+        total = self.params.n1 * self.params.n2 * self.params.n3
+        n = total - z0 if n is None else n
+        out = np.empty((n, DUMP_NCOL))
+        _check(self.L.harm_dump_columns(self.h, z0, n, out.ctypes.data), "harm_dump_columns")
+        return out
 
     def restart_read(self, path: str) -> dict:
         """restart_init()/restart_read() (restart.c:195-376) into this context;

@@ -1818,6 +1818,121 @@ __global__ void __launch_bounds__(TPB) k_tracer_dump(DGrid g, TrArgs a, const do
     tr_record<TAB>(g, a, p, X, hh, ii, jj, tstate[0], k, HARM_TRACER_LIVE, out[base[blockIdx.x] + off]);
 }
 
+// ---------------------------------------------------------------------------
+// dump() (dump.c:1710-1990): the per-zone columns of the ASCII dump, zones in
+// (i, j, h) order.  Columns 0-3 (code and BL coordinates) are filled on the
+// host from its coordinate tables; this kernel writes 4-45: the 8 primitives,
+// the 7 cooling values (Lambda, P, T, H_d, Qnu, tau, Ye: zeros with the Gamma
+// law; the table's P and T with the tabulated EOS), the dump's own corner
+// |div B| and, unless the run failed, u^mu, u_mu, b^mu, b_mu, the dir-1 and
+// dir-2 characteristic speeds, gdet, |v|, and the (p, c_s, u^r, Mach) block.
+// ---------------------------------------------------------------------------
+constexpr int DUMP_NCOL = 46, DUMP_NCOL_FAILED = 20;
+
+template <bool TAB>
+__global__ void __launch_bounds__(TPB) k_dump_cols(DGrid g, const double *__restrict__ p, size_t z0, size_t n,
+                                                   double *__restrict__ out)
+{
+    const size_t t = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= n) return;
+    const size_t zl = z0 + t, nj = (size_t)(g.je - g.js + 1);
+    const int h = (int)(zl % g.N3);
+    const size_t rest = zl / g.N3;
+    const int j = g.js + (int)(rest % nj), i = g.is + (int)(rest / nj);
+    const size_t nz = g.nz, z = zidx(g, i, j, h);
+    double *o = out + t * DUMP_NCOL;
+    double pr[NPR];
+    for (int k = 0; k < NPR; k++) {
+        pr[k] = p[k * nz + z];
+        o[4 + k] = pr[k];
+    }
+    for (int k = 0; k < 7; k++) o[12 + k] = 0.0;
+    if (TAB) {
+        double pp, d1, d2, T;
+        eos_table_p(g.tab, pr[RHO], pr[UU], pp, d1, d2, &T);
+        o[13] = pp;
+        o[14] = T;
+    }
+    const double *GD = g.geom[CENT] + 32 * g.ncol;
+    const size_t col = (size_t)i * g.SN2 + j;
+    const double g00 = GD[col], gjm = GD[col - 1], gim = GD[col - g.SN2], gimjm = GD[col - g.SN2 - 1];
+    auto P = [&](int k, int ii, int jj, int hh) { return p[k * nz + zidx(g, ii, jj, hh)]; };
+    double divb;
+    if (g.N3 == 1) {
+        // dump.c:1864-1879: interior corners only (array indices)
+        if (i > 0 && j > 0 && i < g.N1 && j < g.N2)
+            divb = fabs(0.5 *
+                            (P(B1, i, j, h) * g00 + P(B1, i, j - 1, h) * gjm - P(B1, i - 1, j, h) * gim -
+                             P(B1, i - 1, j - 1, h) * gimjm) /
+                            g.dx1 +
+                        0.5 *
+                            (P(B2, i, j, h) * g00 + P(B2, i - 1, j, h) * gim - P(B2, i, j - 1, h) * gjm -
+                             P(B2, i - 1, j - 1, h) * gimjm) /
+                            g.dx2);
+        else
+            divb = 0.;
+    } else {
+        // dump.c:1881-1915, term for term (the (i, j-1) B3 term takes gdet of column (i, j))
+        const int m = (h + g.N3 - 1) % g.N3;
+        divb = fabs(0.25 *
+                        (+P(B1, i, j, m) * g00 + P(B1, i, j - 1, m) * gjm - P(B1, i - 1, j, m) * gim -
+                         P(B1, i - 1, j - 1, m) * gimjm
+
+                         + P(B1, i, j, h) * g00 + P(B1, i, j - 1, h) * gjm - P(B1, i - 1, j, h) * gim -
+                         P(B1, i - 1, j - 1, h) * gimjm) /
+                        g.dx1 +
+                    0.25 *
+                        (+P(B2, i, j, m) * g00 + P(B2, i - 1, j, m) * gim - P(B2, i, j - 1, m) * gjm -
+                         P(B2, i - 1, j - 1, m) * gimjm
+
+                         + P(B2, i, j, h) * g00 + P(B2, i - 1, j, h) * gim - P(B2, i, j - 1, h) * gjm -
+                         P(B2, i - 1, j - 1, h) * gimjm) /
+                        g.dx2 +
+                    0.25 *
+                        (+P(B3, i, j, h) * g00 + P(B3, i - 1, j, h) * gim - P(B3, i, j, m) * g00 -
+                         P(B3, i - 1, j, m) * gim
+
+                         + P(B3, i, j - 1, h) * g00 + P(B3, i - 1, j - 1, h) * gimjm - P(B3, i, j - 1, m) * gjm -
+                         P(B3, i - 1, j - 1, m) * gimjm) /
+                        g.dx3);
+    }
+    o[19] = divb;
+    Geom geom;
+    load_geom(g.geom[CENT], g.ncol, col, geom);
+    State q;
+    int fatal = 0;
+    get_state(pr, geom, q, fatal);
+    for (int k = 0; k < 4; k++) {
+        o[20 + k] = q.ucon[k];
+        o[24 + k] = q.ucov[k];
+        o[28 + k] = q.bcon[k];
+        o[32 + k] = q.bcov[k];
+    }
+    double vmax, vmin;
+    for (int dir = 1; dir <= 2; dir++) {
+        if (TAB)
+            vchar(pr, q, geom, dir, EosTable{g.tab, g.ref_derivs}, &vmax, &vmin);
+        else
+            vchar(pr, q, geom, dir, g.gam, &vmax, &vmin);
+        o[36 + 2 * (dir - 1)] = vmin;
+        o[37 + 2 * (dir - 1)] = vmax;
+    }
+    o[40] = geom.g;
+    double vsq = 0.;
+    for (int a = 1; a < 4; a++)
+        for (int b = 1; b < 4; b++) vsq += geom.gcov[a][b] * q.ucon[a] * q.ucon[b];
+    const double gsq = 1. + vsq;
+    vsq /= gsq;
+    o[41] = sqrt(vsq);
+    const double ps_p = pr[UU] * (g.gam - 1.0);
+    const double ps_cs = sqrt(4.0 / 3.0 * ps_p / (pr[RHO] + 4.0 * ps_p));
+    const double uur = sqrt(geom.gcov[1][1]) * q.ucon[1] / q.ucon[0];
+    o[42] = ps_p;
+    o[43] = ps_cs;
+    o[44] = uur;
+    o[45] = -uur / ps_cs;
+}
+
 // Batched Noble 2D inversion over SoA arrays (utoprim_2d.c:126, plus the Font fix).
 // Batched inversion; LEARNED seeds Newton from the network (all lanes take
 // part in the wave-level forward pass, so there is no early exit).
@@ -3546,4 +3661,106 @@ int harm_tracers_set(harm_ctx *c, const double *X, const int8_t *state)
     HIPCHK(hipMemsetAsync(t.a.nev, 0, sizeof(unsigned long long), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ASCII dump (dump.c:1710-1990)
+// ---------------------------------------------------------------------------
+namespace {
+
+// The 46 columns of zones z0..z0+n-1 of the active grid in (i, j, h) order.
+int dump_fill(harm_ctx *c, double *dcols, size_t cap, size_t z0, size_t n, double *cols)
+{
+    const DGrid &g = c->g;
+    const auto &hg = c->hg;
+    for (size_t a = 0; a < n; a += cap) {
+        const size_t m = std::min(cap, n - a);
+        if (c->eos_kind == 1)
+            hipLaunchKernelGGL(k_dump_cols<true>, dim3(nblocks(m)), dim3(TPB), 0, c->stream, g, c->p, z0 + a, m, dcols);
+        else
+            hipLaunchKernelGGL(k_dump_cols<false>, dim3(nblocks(m)), dim3(TPB), 0, c->stream, g, c->p, z0 + a, m,
+                               dcols);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(cols + a * DUMP_NCOL, dcols, m * DUMP_NCOL * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    // coord() and bl_coord() (metric.c:62-115, coord.c:115-125) from the host tables
+    const size_t nj = (size_t)(g.je - g.js + 1), ncol = (size_t)hg.SN1 * hg.SN2;
+    const double *bl = c->tab.coords.data();
+    for (size_t t = 0; t < n; t++) {
+        const size_t zl = z0 + t;
+        const int h = (int)(zl % g.N3);
+        const size_t rest = zl / g.N3;
+        const int j = g.js + (int)(rest % nj), i = g.is + (int)(rest / nj);
+        const size_t col = (size_t)i * hg.SN2 + j;
+        double *o = cols + t * DUMP_NCOL;
+        const double X1 = hg.startx[1] + ((i - 2) + 0.5) * hg.dx[1];
+        const double X2 = hg.startx[2] + ((j - 2) + 0.5) * hg.dx[2];
+        const double X3 = hg.N3 > 2 ? hg.startx[3] + (h + 0.5) * hg.dx[3] : hg.startx[3] + h * hg.dx[3];
+        o[0] = X1;
+        o[1] = hg.N3 > 1 ? X3 : X2;
+        o[2] = bl[1 * ncol + col];
+        o[3] = hg.N3 > 1 ? X3 : bl[2 * ncol + col];
+    }
+    return 0;
+}
+
+}  // namespace
+
+int harm_dump_columns(harm_ctx *c, size_t z0, size_t n, double *cols)
+{
+    if (!c || (n && !cols)) return set_err("null argument");
+    const DGrid &g = c->g;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    if (z0 > na || n > na - z0) return set_err("harm_dump_columns: zone range outside the grid", 3);
+    if (!n) return 0;
+    const size_t cap = std::min<size_t>(n, (size_t)1 << 18);
+    double *d = nullptr;
+    int rc;
+    if ((rc = dalloc(&d, cap * DUMP_NCOL))) return rc;
+    rc = dump_fill(c, d, cap, z0, n, cols);
+    HIPCHK(hipFree(d));
+    return rc;
+}
+
+int harm_dump_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr)
+{
+    if (!c || !path) return set_err("null argument");
+    harm_restart_hdr h;
+    if (hdr) {
+        h = *hdr;
+    } else {
+        std::memset(&h, 0, sizeof(h));
+    }
+    const harm_params &P = c->P;
+    h.N1 = P.n1;
+    h.N2 = P.n2;
+    h.N3 = P.n3;
+    h.tf = P.tf;
+    h.a = P.a;
+    h.gam = P.gam;
+    h.cour = P.cour;
+    h.lim = P.lim;
+    h.Rin = P.Rin;
+    h.Rout = P.Rout;
+    h.hslope = P.hslope;
+    h.R0 = P.R0;
+    double ts[2];
+    HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    h.t = ts[0];
+    h.dt = ts[1];
+    const DGrid &g = c->g;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    const size_t cap = std::min<size_t>(na, (size_t)1 << 18);
+    double *d = nullptr;
+    int rc;
+    if ((rc = dalloc(&d, std::max<size_t>(cap, 1) * DUMP_NCOL))) return rc;
+    // dump.c:1917: the derived block only while the run has not failed
+    const int ncol = h.failed ? DUMP_NCOL_FAILED : DUMP_NCOL;
+    rc = harm_host::dump_write_file(path, harm_host::dump_header_text(h, c->hg.startx, c->hg.dx), na, ncol, DUMP_NCOL,
+                                    [&](size_t z0, size_t n, double *cols) { return dump_fill(c, d, cap, z0, n, cols); });
+    HIPCHK(hipFree(d));
+    return rc;
 }

@@ -1,6 +1,7 @@
 // harm_host.hpp -- host-side grid description and setup tables of libharmcool.so.
 #pragma once
 #include <cstddef>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,11 @@ int set_error(const std::string &msg, int code);
 // Restart file text (harm_restart.cpp); prim/temp in the canonical layout.
 int restart_write_file(const char *path, const harm_restart_hdr &h, const double *prim, const double *temp);
 int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp);
+// dump() text (dump.c:1710-1990): the header line, then one line of ncol
+// values per zone; fill(z0, n, cols) puts zones z0..z0+n-1, stride values each.
+std::string dump_header_text(const harm_restart_hdr &h, const double *startx, const double *dx);
+int dump_write_file(const char *path, const std::string &header, size_t nzones, int ncol, int stride,
+                    const std::function<int(size_t, size_t, double *)> &fill);
 
 int build_torus2d(const harm_params &P, const Grid &g, const GeomTables &t, std::vector<double> &tor, double &rhomax,
                   double &umax);

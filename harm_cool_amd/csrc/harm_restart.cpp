@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -240,6 +241,82 @@ int restart_write_file(const char *path, const harm_restart_hdr &h, const double
     }
     if (fclose(fp) != 0) ok = false;
     return ok ? 0 : set_error(std::string("error writing restart file ") + path, 2);
+}
+
+std::string dump_header_text(const harm_restart_hdr &h, const double *startx, const double *dx)
+{
+    // dump.c:1745-1790: 2D files name N2 and the theta origin and spacing, 3D
+    // files N3 and the phi ones
+    std::string s;
+    char b[64];
+    auto I = [&](int32_t v) {
+        snprintf(b, sizeof(b), "%10d", v);
+        s += b;
+    };
+    auto D = [&](double v) {
+        snprintf(b, sizeof(b), "%32.22e", v);
+        s += b;
+    };
+    const bool d3 = h.N3 > 1;
+    D(h.t);
+    I(h.N1);
+    I(d3 ? h.N3 : h.N2);
+    D(startx[1]);
+    D(d3 ? startx[3] : startx[2]);
+    D(dx[1]);
+    D(d3 ? dx[3] : dx[2]);
+    D(h.tf);
+    I(h.nstep);
+    D(h.a);
+    D(h.gam);
+    D(h.cour);
+    D(h.DTd);
+    D(h.DTl);
+    D(h.DTi);
+    I(h.DTr);
+    I(h.dump_cnt);
+    I(h.image_cnt);
+    I(h.rdump_cnt);
+    D(h.dt);
+    I(h.lim);
+    I(h.failed);
+    D(h.Rin);
+    D(h.Rout);
+    D(h.hslope);
+    D(h.R0);
+    s += "\n";
+    return s;
+}
+
+int dump_write_file(const char *path, const std::string &header, size_t nzones, int ncol, int stride,
+                    const std::function<int(size_t, size_t, double *)> &fill)
+{
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return set_error(std::string("cannot open dump file for writing: ") + path, 2);
+    bool ok = fwrite(header.data(), 1, header.size(), fp) == header.size();
+    const size_t line = (size_t)ncol * W + 1, chunk = CHUNK / 4;
+    std::vector<double> cols;
+    std::vector<char> buf;
+    for (size_t z0 = 0; ok && z0 < nzones; z0 += chunk) {
+        const size_t nq = std::min(chunk, nzones - z0);
+        cols.resize(nq * stride);
+        int rc = fill(z0, nq, cols.data());
+        if (rc) {
+            fclose(fp);
+            return rc;
+        }
+        buf.resize(nq * line);
+        parallel_for(nq, [&](size_t a, size_t b) {
+            for (size_t r = a; r < b; r++) {
+                char *ln = buf.data() + r * line;
+                for (int k = 0; k < ncol; k++) fmt_dbl(ln + k * W, cols[r * stride + k]);
+                ln[ncol * W] = '\n';
+            }
+        });
+        ok = fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+    }
+    if (fclose(fp) != 0) ok = false;
+    return ok ? 0 : set_error(std::string("error writing dump file ") + path, 2);
 }
 
 int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp)

@@ -373,6 +373,24 @@ int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hd
  * file as the reference does.  Loads p, pflag = 0, t, dt, mass_start, mass_in,
  * mass_out, then bound_prim (restart.c:278-279).  hdr (may be NULL) receives the header. */
 int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr);
+/* dump() ASCII file (dump.c:1710-1990; the HDFDUMP=0 build): a header line
+ * (t, N1, N2 and the theta origin/spacing in 2D or N3 and the phi ones in 3D,
+ * then tf, nstep, a, gam, cour, DTd, DTl, DTi, DTr, dump/image/rdump counters,
+ * dt, lim, failed, Rin, Rout, hslope, R0), then one line per active zone in
+ * (i, j, h) order of HARM_DUMP_NCOL "%32.22e" values (HARM_DUMP_NCOL_FAILED when
+ * hdr->failed: the derived block is skipped, dump.c:1917).  The context owns
+ * N1..N3, t, dt, tf, a, gam, cour, lim, Rin, Rout, hslope and R0; hdr (may be
+ * NULL) gives the driver's counters and clocks. */
+#define HARM_DUMP_NCOL 46
+#define HARM_DUMP_NCOL_FAILED 20
+int harm_dump_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr);
+/* The same columns for active zones z0 .. z0+n-1 in (i, j, h) order, row-major
+ * [n][HARM_DUMP_NCOL]: X1, X2 (3D: X3), r, theta (3D: phi), the 8 primitives,
+ * Lambda, P, T, H_d, Qnu, tau, Ye (zero with the Gamma law; the table's P and
+ * T with the tabulated EOS, the others zero), |div B| (the dump's corner
+ * stencil), u^mu, u_mu, b^mu, b_mu, vmin/vmax along X1 and X2, gdet, |v|,
+ * p = (gam-1) u, c_s, u^r (sqrt(g_rr) u^r/u^t), Mach. */
+int harm_dump_columns(harm_ctx *c, size_t z0, size_t n, double *cols);
 /* The header alone (ener_size, which follows the body, reads as -1). */
 int harm_restart_peek(const char *path, harm_restart_hdr *hdr);
 /* Host-array forms (no device): prim in the canonical layout of

@@ -3656,3 +3656,168 @@ int orc_restart_read(const char *path, orc_restart_hdr *h, double *prim, double 
     fclose(fp);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* dump(), dump.c:1710-1990 (HDFDUMP = 0), one rank                          */
+/* ------------------------------------------------------------------------ */
+int orc_dump_write(orc_ctx *o, const char *path, const orc_restart_hdr *hin)
+{
+    const int N3 = o->N3;
+    orc_restart_hdr h = *hin;
+    FILE *fp = fopen(path, "wt");
+    int i, j, hh, k;
+    int64_t fatal = 0;
+    if (!fp) return 2;
+    h.N1 = o->N1;
+    h.N2 = o->N2;
+    h.N3 = o->N3;
+    h.t = o->t;
+    h.dt = o->dt;
+    h.tf = o->P.tf;
+    h.a = o->P.a;
+    h.gam = o->P.gam;
+    h.cour = o->P.cour;
+    h.lim = o->P.lim;
+    h.Rin = o->P.Rin;
+    h.Rout = o->P.Rout;
+    h.hslope = o->P.hslope;
+    h.R0 = o->P.R0;
+    /* dump.c:1745-1790 */
+    fprintf(fp, ORC_FMT_DBL_OUT, h.t);
+    fprintf(fp, ORC_FMT_INT_OUT, h.N1);
+    if (N3 == 1) fprintf(fp, ORC_FMT_INT_OUT, h.N2);
+    if (N3 > 1) fprintf(fp, ORC_FMT_INT_OUT, h.N3);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->startx[RR]);
+    if (N3 == 1) fprintf(fp, ORC_FMT_DBL_OUT, o->startx[TH]);
+    if (N3 > 1) fprintf(fp, ORC_FMT_DBL_OUT, o->startx[PH]);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->dx[RR]);
+    if (N3 == 1) fprintf(fp, ORC_FMT_DBL_OUT, o->dx[TH]);
+    if (N3 > 1) fprintf(fp, ORC_FMT_DBL_OUT, o->dx[PH]);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.tf);
+    fprintf(fp, ORC_FMT_INT_OUT, h.nstep);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.a);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.gam);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.cour);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.DTd);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.DTl);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.DTi);
+    fprintf(fp, ORC_FMT_INT_OUT, h.DTr);
+    fprintf(fp, ORC_FMT_INT_OUT, h.dump_cnt);
+    fprintf(fp, ORC_FMT_INT_OUT, h.image_cnt);
+    fprintf(fp, ORC_FMT_INT_OUT, h.rdump_cnt);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.dt);
+    fprintf(fp, ORC_FMT_INT_OUT, h.lim);
+    fprintf(fp, ORC_FMT_INT_OUT, h.failed);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.Rin);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.Rout);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.hslope);
+    fprintf(fp, ORC_FMT_DBL_OUT, h.R0);
+    fprintf(fp, "\n");
+    g_step_eos = o->eos;
+    g_step_refd = o->eos_refd;
+#define P_(hq, iq, jq, kq) PV(o, o->p, hq, iq, jq)[kq]
+#define G_(iq, jq) GDET(o, iq, jq, CENT)
+    for (i = o->istart; i <= o->istop; i++)
+        for (j = o->jstart; j <= o->jstop; j++)
+            for (hh = 0; hh < N3; hh++) {
+                double X[NDIM], V[NDIM], divb, vmin, vmax;
+                const double *pz = PV(o, o->p, hh, i, j);
+                geom_t geom;
+                state_t q;
+                coord(o, hh, i, j, CENT, X);
+                bl_coord_vec(&o->P, X, V);
+                fprintf(fp, ORC_FMT_DBL_OUT, X[1]);
+                if (N3 == 1) fprintf(fp, ORC_FMT_DBL_OUT, X[2]);
+                if (N3 > 1) fprintf(fp, ORC_FMT_DBL_OUT, X[3]);
+                fprintf(fp, ORC_FMT_DBL_OUT, V[RR]);
+                if (N3 == 1) fprintf(fp, ORC_FMT_DBL_OUT, V[TH]);
+                if (N3 > 1) fprintf(fp, ORC_FMT_DBL_OUT, V[PH]);
+                for (k = 0; k < NPR; k++) fprintf(fp, ORC_FMT_DBL_OUT, pz[k]);
+                /* dump.c:1835-1860: the cooling values of the COOL_EOS build
+                 * (only P and T exist in this table), zeros otherwise */
+                {
+                    double cv[7] = {0., 0., 0., 0., 0., 0., 0.};
+                    if (o->eos) {
+                        double pp, T;
+                        orc_eos_table_p(o->eos, pz[RHO], pz[UU], &pp, NULL, NULL, &T);
+                        cv[1] = pp;
+                        cv[2] = T;
+                    }
+                    for (k = 0; k < 7; k++) fprintf(fp, ORC_FMT_DBL_OUT, cv[k]);
+                }
+                if (N3 == 1) {
+                    if (i > 0 && j > 0 && i < o->N1 && j < o->N2)
+                        divb = fabs(0.5 *
+                                        (P_(hh, i, j, B1) * G_(i, j) + P_(hh, i, j - 1, B1) * G_(i, j - 1) -
+                                         P_(hh, i - 1, j, B1) * G_(i - 1, j) -
+                                         P_(hh, i - 1, j - 1, B1) * G_(i - 1, j - 1)) /
+                                        o->dx[RR] +
+                                    0.5 *
+                                        (P_(hh, i, j, B2) * G_(i, j) + P_(hh, i - 1, j, B2) * G_(i - 1, j) -
+                                         P_(hh, i, j - 1, B2) * G_(i, j - 1) -
+                                         P_(hh, i - 1, j - 1, B2) * G_(i - 1, j - 1)) /
+                                        o->dx[TH]);
+                    else
+                        divb = 0.;
+                } else {
+                    const int m = hwrap(o, hh - 1);
+                    divb = fabs(0.25 *
+                                    (+P_(m, i, j, B1) * G_(i, j) + P_(m, i, j - 1, B1) * G_(i, j - 1) -
+                                     P_(m, i - 1, j, B1) * G_(i - 1, j) - P_(m, i - 1, j - 1, B1) * G_(i - 1, j - 1)
+
+                                     + P_(hh, i, j, B1) * G_(i, j) + P_(hh, i, j - 1, B1) * G_(i, j - 1) -
+                                     P_(hh, i - 1, j, B1) * G_(i - 1, j) -
+                                     P_(hh, i - 1, j - 1, B1) * G_(i - 1, j - 1)) /
+                                    o->dx[RR] +
+                                0.25 *
+                                    (+P_(m, i, j, B2) * G_(i, j) + P_(m, i - 1, j, B2) * G_(i - 1, j) -
+                                     P_(m, i, j - 1, B2) * G_(i, j - 1) - P_(m, i - 1, j - 1, B2) * G_(i - 1, j - 1)
+
+                                     + P_(hh, i, j, B2) * G_(i, j) + P_(hh, i - 1, j, B2) * G_(i - 1, j) -
+                                     P_(hh, i, j - 1, B2) * G_(i, j - 1) -
+                                     P_(hh, i - 1, j - 1, B2) * G_(i - 1, j - 1)) /
+                                    o->dx[TH] +
+                                0.25 *
+                                    (+P_(hh, i, j, B3) * G_(i, j) + P_(hh, i - 1, j, B3) * G_(i - 1, j) -
+                                     P_(m, i, j, B3) * G_(i, j) - P_(m, i - 1, j, B3) * G_(i - 1, j)
+
+                                     + P_(hh, i, j - 1, B3) * G_(i, j) + P_(hh, i - 1, j - 1, B3) * G_(i - 1, j - 1) -
+                                     P_(m, i, j - 1, B3) * G_(i, j - 1) - P_(m, i - 1, j - 1, B3) * G_(i - 1, j - 1)) /
+                                    o->dx[PH]);
+                }
+                fprintf(fp, ORC_FMT_DBL_OUT, divb);
+                if (!h.failed) {
+                    double vsq = 0., gsqtmp, ps_p, ps_cs, uur;
+                    get_geometry(o, i, j, CENT, &geom);
+                    get_state(pz, &geom, &q, &fatal);
+                    for (k = 0; k < NDIM; k++) fprintf(fp, ORC_FMT_DBL_OUT, q.ucon[k]);
+                    for (k = 0; k < NDIM; k++) fprintf(fp, ORC_FMT_DBL_OUT, q.ucov[k]);
+                    for (k = 0; k < NDIM; k++) fprintf(fp, ORC_FMT_DBL_OUT, q.bcon[k]);
+                    for (k = 0; k < NDIM; k++) fprintf(fp, ORC_FMT_DBL_OUT, q.bcov[k]);
+                    vchar(pz, &q, &geom, 1, o->P.gam, &vmax, &vmin);
+                    fprintf(fp, ORC_FMT_DBL_OUT, vmin);
+                    fprintf(fp, ORC_FMT_DBL_OUT, vmax);
+                    vchar(pz, &q, &geom, 2, o->P.gam, &vmax, &vmin);
+                    fprintf(fp, ORC_FMT_DBL_OUT, vmin);
+                    fprintf(fp, ORC_FMT_DBL_OUT, vmax);
+                    fprintf(fp, ORC_FMT_DBL_OUT, geom.g);
+                    for (int id = 1; id < 4; id++)
+                        for (int jd = 1; jd < 4; jd++) vsq += geom.gcov[id][jd] * q.ucon[id] * q.ucon[jd];
+                    gsqtmp = 1. + vsq;
+                    vsq /= gsqtmp;
+                    fprintf(fp, ORC_FMT_DBL_OUT, sqrt(vsq));
+                    ps_p = pz[UU] * (o->P.gam - 1.0);
+                    ps_cs = sqrt(4.0 / 3.0 * ps_p / (pz[RHO] + 4.0 * ps_p));
+                    uur = sqrt(geom.gcov[RR][RR]) * q.ucon[RR] / q.ucon[TT];
+                    fprintf(fp, ORC_FMT_DBL_OUT, ps_p);
+                    fprintf(fp, ORC_FMT_DBL_OUT, ps_cs);
+                    fprintf(fp, ORC_FMT_DBL_OUT, uur);
+                    fprintf(fp, ORC_FMT_DBL_OUT, -uur / ps_cs);
+                }
+                fprintf(fp, "\n");
+            }
+#undef P_
+#undef G_
+    g_step_eos = NULL;
+    return fclose(fp) == 0 ? 0 : 2;
+}

@@ -184,6 +184,9 @@ typedef struct orc_restart_hdr {
  * the ener.out size when ener_size >= 0 (restart.c:150-157), the COOL_EOS
  * temperatures when temp != NULL (cooleos.c:996-1014). */
 int orc_restart_write(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp);
+/* dump() (dump.c:1710-1990, HDFDUMP = 0): hdr supplies the driver's counters;
+ * the context's parameters, t and dt replace the rest. */
+int orc_dump_write(orc_ctx *o, const char *path, const orc_restart_hdr *hdr);
 /* restart_read() (restart.c:317-376) with fscanf; read_ener as rank 0 does,
  * temp (may be NULL) as restart_read_CoolEOS (cooleos.c:1018-1030). */
 int orc_restart_read(const char *path, orc_restart_hdr *h, double *prim, double *temp, int read_ener);

@@ -134,6 +134,7 @@ def lib():
         L.orc_set_eos.argtypes = [c_void_p, POINTER(OrcEos), c_int]
         L.orc_restart_write.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp]
         L.orc_restart_read.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_int]
+        L.orc_dump_write.argtypes = [c_void_p, ctypes.c_char_p, POINTER(OrcRestartHdr)]
         _lib = L
     return _lib
 
