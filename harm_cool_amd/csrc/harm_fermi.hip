@@ -1,0 +1,199 @@
+// harm_fermi.hip -- device batches of the tabulated-EOS generator's gas
+// physics (harm_fermi.hpp): generalized Fermi-Dirac integrals and the
+// per-species equations of state, one element per thread.
+//
+// The 20-point Gauss-Legendre and Gauss-Laguerre rules are computed here once
+// (Newton iteration on P_20 and L_20 in long double, then rounded), not read
+// from tables: the reference's rules were evaluated in 100-digit arithmetic
+// and rounded to double, which the long-double roots reproduce.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <string>
+
+#include "../../include/harm_cool.h"
+#include "harm_fermi.hpp"
+#include "harm_host.hpp"
+
+using harm::FermiRules;
+
+namespace {
+
+constexpr int TPB_F = 256;
+
+#define FCHK(x)                                                                              \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) return harm_host::set_error(std::string(#x) + ": " + hipGetErrorString(e_), 2); \
+    } while (0)
+
+void legendre20(long double *x, long double *w)
+{
+    const int n = 20;
+    for (int i = 1; i <= n / 2; i++) {
+        long double z = cosl(3.14159265358979323846264338327950288L * (i - 0.25L) / (n + 0.5L)), z1, pp;
+        for (int it = 0; it < 100; it++) {
+            long double p1 = 1.0L, p2 = 0.0L;
+            for (int j = 1; j <= n; j++) {
+                const long double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0L);
+            z1 = z;
+            z = z1 - p1 / pp;
+            if (fabsl(z - z1) <= 1e-21L) break;
+        }
+        long double p1 = 1.0L, p2 = 0.0L;
+        for (int j = 1; j <= n; j++) {
+            const long double p3 = p2;
+            p2 = p1;
+            p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0L);
+        x[n / 2 - i] = z;  // increasing order
+        w[n / 2 - i] = 2.0L / ((1.0L - z * z) * pp * pp);
+    }
+}
+
+void laguerre20(long double *x, long double *w)
+{
+    const int n = 20;
+    long double z = 0.0L;
+    for (int i = 0; i < n; i++) {
+        // starting values of the usual Laguerre-root recurrence
+        if (i == 0)
+            z = 3.0L / (1.0L + 2.4L * n);
+        else if (i == 1)
+            z += 15.0L / (1.0L + 2.5L * n);
+        else
+            z += ((1.0L + 2.55L * (i - 1)) / (1.9L * (i - 1))) * (z - x[i - 2]);
+        for (int it = 0; it < 200; it++) {
+            long double p1 = 1.0L, p2 = 0.0L;
+            for (int j = 1; j <= n; j++) {
+                const long double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0L * j - 1.0L - z) * p2 - (j - 1.0L) * p3) / j;
+            }
+            const long double pp = n * (p1 - p2) / z;
+            const long double z1 = z;
+            z = z1 - p1 / pp;
+            if (fabsl(z - z1) <= 1e-21L * fabsl(z)) break;
+        }
+        x[i] = z;
+        // w = x / ((n+1)^2 L_{n+1}(x)^2)
+        long double p1 = 1.0L, p2 = 0.0L;
+        for (int j = 1; j <= n + 1; j++) {
+            const long double p3 = p2;
+            p2 = p1;
+            p1 = ((2.0L * j - 1.0L - z) * p2 - (j - 1.0L) * p3) / j;
+        }
+        w[i] = z / ((n + 1.0L) * (n + 1.0L) * p1 * p1);
+    }
+}
+
+const FermiRules &rules()
+{
+    static FermiRules R;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        long double lx[10], lw[10], gx[20], gw[20];
+        legendre20(lx, lw);
+        laguerre20(gx, gw);
+        for (int i = 0; i < 10; i++) {
+            R.lx[i] = (double)lx[i];
+            R.lw[i] = (double)lw[i];
+        }
+        for (int i = 0; i < 20; i++) {
+            R.gx[i] = (double)gx[i];
+            R.gw[i] = (double)(gw[i] * expl(gx[i]));
+        }
+    });
+    return R;
+}
+
+__global__ void __launch_bounds__(TPB_F) k_fermi(FermiRules R, const double *__restrict__ dk,
+                                                 const double *__restrict__ eta, const double *__restrict__ theta,
+                                                 double *__restrict__ fd, double *__restrict__ fdeta,
+                                                 double *__restrict__ fdtheta, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * TPB_F + threadIdx.x;
+    if (i >= n) return;
+    double a, b, c;
+    harm::dfermi(dk[i], eta[i], theta[i], R, a, b, c);
+    fd[i] = a;
+    if (fdeta) fdeta[i] = b;
+    if (fdtheta) fdtheta[i] = c;
+}
+
+// in: [nin][n], out: [3][n] (e, p, s)
+__global__ void __launch_bounds__(TPB_F) k_eos_species(FermiRules R, int kind, const double *__restrict__ in,
+                                                       double *__restrict__ out, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * TPB_F + threadIdx.x;
+    if (i >= n) return;
+    auto I = [&](int k) { return in[(size_t)k * n + i]; };
+    double e = 0., p = 0., s = 0.;
+    switch (kind) {
+    case HARM_EOS_NPE:
+        harm::eos_npe(I(0), I(1), I(2), I(3), I(4), I(5), I(6), I(7), R, e, p, s);
+        break;
+    case HARM_EOS_RAD:
+        harm::eos_rad(I(0), e, p, s);
+        break;
+    case HARM_EOS_NEU:
+        harm::eos_neu(I(0), I(1), I(2), e, p, s);
+        break;
+    case HARM_EOS_HE:
+        harm::eos_he(I(0), I(1), e, p, s);
+        break;
+    }
+    out[i] = e;
+    out[n + i] = p;
+    out[2 * n + i] = s;
+}
+
+unsigned nb(size_t n) { return (unsigned)((n + TPB_F - 1) / TPB_F); }
+
+}  // namespace
+
+extern "C" {
+
+int harm_fermi_rules(double *out)
+{
+    if (!out) return harm_host::set_error("null argument", 1);
+    const FermiRules &R = rules();
+    std::copy(R.lx, R.lx + 10, out);
+    std::copy(R.lw, R.lw + 10, out + 10);
+    std::copy(R.gx, R.gx + 20, out + 20);
+    std::copy(R.gw, R.gw + 20, out + 40);
+    return 0;
+}
+
+int harm_fermi_batch(const double *dk, const double *eta, const double *theta, double *fd, double *fdeta,
+                     double *fdtheta, size_t n, void *stream)
+{
+    if (!n) return 0;
+    if (!dk || !eta || !theta || !fd) return harm_host::set_error("null argument", 1);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fermi, dim3(nb(n)), dim3(TPB_F), 0, s, rules(), dk, eta, theta, fd, fdeta, fdtheta, n);
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int harm_eos_species_batch(int kind, const double *in, double *out, size_t n, void *stream)
+{
+    if (kind < HARM_EOS_NPE || kind > HARM_EOS_HE) return harm_host::set_error("harm_eos_species_batch: unknown kind", 3);
+    if (!n) return 0;
+    if (!in || !out) return harm_host::set_error("null argument", 1);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_eos_species, dim3(nb(n)), dim3(TPB_F), 0, s, rules(), kind, in, out, n);
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"

@@ -373,6 +373,30 @@ int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hd
  * file as the reference does.  Loads p, pflag = 0, t, dt, mass_start, mass_in,
  * mass_out, then bound_prim (restart.c:278-279).  hdr (may be NULL) receives the header. */
 int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr);
+/* Gas physics of the cooling-table generator (the reference's Fortran chain
+ * behind cooleos.c's cache, abundance.f90:6-243).  Device pointers; stream
+ * may be NULL.
+ * harm_fermi_batch: dfermi (gauss_integral.f90:241-349), the generalized
+ * Fermi-Dirac integral F_k(eta, theta) = int_0^inf x^k sqrt(1 + x theta / 2)
+ * / (exp(x - eta) + 1) dx and its eta and theta derivatives (fdeta, fdtheta
+ * may be NULL), by Aparicio's four-interval 20-point quadrature. */
+int harm_fermi_batch(const double *dk, const double *eta, const double *theta, double *fd, double *fdeta,
+                     double *fdtheta, size_t n, void *stream);
+/* The 20-point rules it uses: out[0..9] Gauss-Legendre positive nodes,
+ * [10..19] their weights, [20..39] Gauss-Laguerre nodes, [40..59] their weights
+ * times exp(node) (host call). */
+int harm_fermi_rules(double *out /* 60 */);
+/* Per-species EOS of eos.f90 (energy density e and pressure p in MeV fm^-3,
+ * entropy density s in k_B fm^-3) for n points; in is [nin][n], out [3][n]:
+ *   HARM_EOS_NPE (eos_npe, eos.f90:31-113) nin 8: kT [MeV], eta_n, eta_p,
+ *     eta_e (chemical potentials over kT, rest mass excluded), n_n, n_p, n_e-,
+ *     n_e+ [fm^-3]; free nucleons and e-/e+ pairs;
+ *   HARM_EOS_RAD (eos_rad, eos.f90:122-132) nin 1: kT;
+ *   HARM_EOS_NEU (eos_neu, eos.f90:143-153) nin 3: kT, blocking factors b1, b2;
+ *   HARM_EOS_HE  (eos_he, eos.f90:164-181) nin 2: kT, n_He. */
+enum { HARM_EOS_NPE = 0, HARM_EOS_RAD = 1, HARM_EOS_NEU = 2, HARM_EOS_HE = 3 };
+int harm_eos_species_batch(int kind, const double *in, double *out, size_t n, void *stream);
+
 /* dump() ASCII file (dump.c:1710-1990; the HDFDUMP=0 build): a header line
  * (t, N1, N2 and the theta origin/spacing in 2D or N3 and the phi ones in 3D,
  * then tf, nstep, a, gam, cour, DTd, DTl, DTi, DTr, dump/image/rdump counters,

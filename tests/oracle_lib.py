@@ -15,6 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libharmref.so")
+EOSREF_SO = os.path.join(ORACLE_DIR, "_ref", "libeosref.so")
 
 NCOUNTERS = 12
 COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_font1",
@@ -320,3 +321,41 @@ def utoprim_2d(U, gcov, gcon, gdet, prim, gam):
     ni, kc = c_int(), c_int()
     r = L.orc_utoprim_2d(_ptr(U), _ptr(gcov), _ptr(gcon), gdet, _ptr(p), gam, ctypes.byref(ni), ctypes.byref(kc))
     return r, p, ni.value, kc.value
+
+
+class EosRef:
+    """The reference's own Fortran EOS chain (oracle/_ref/libeosref.so, built
+    from abundance.f90 and its modules by oracle/Makefile); every argument is
+    passed by reference, as Fortran does."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL(EOSREF_SO)
+
+    def _call(self, name, ins, nout):
+        args = [c_double(float(v)) for v in ins] + [c_double() for _ in range(nout)]
+        getattr(self.L, name)(*[ctypes.byref(a) for a in args])
+        return tuple(a.value for a in args[len(ins):])
+
+    def dfermi(self, dk, eta, theta):
+        """(fd, fdeta, fdtheta), gauss_integral.f90:241-349."""
+        return self._call("_QMgauss_integralPdfermi", (dk, eta, theta), 3)
+
+    def eos_npe(self, xkt, etan, etap, etae, nn, np_, nee, nep):
+        return self._call("_QMeosPeos_npe", (xkt, etan, etap, etae, nn, np_, nee, nep), 3)
+
+    def eos_rad(self, xkt):
+        return self._call("_QMeosPeos_rad", (xkt,), 3)
+
+    def eos_neu(self, xkt, b1, b2):
+        return self._call("_QMeosPeos_neu", (xkt, b1, b2), 3)
+
+    def eos_he(self, xkt, nhe):
+        return self._call("_QMeosPeos_he", (xkt, nhe), 3)
+
+    def abundance(self, rho, T, hdfac):
+        """(e, p, Qnu, tau, n_e-, n_e+) of one table point, abundance.f90:6-243."""
+        return self._call("abundance_", (rho, T, hdfac), 6)
+
+
+def eosref_available() -> bool:
+    return os.path.exists(EOSREF_SO)
