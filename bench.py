@@ -255,6 +255,51 @@ def tracer_section(g, steps: int) -> dict:
             "removed_during_run": int(ev.size)}
 
 
+def eos_table(hc, torch, n_rho: int = 200, n_t: int = 200) -> dict:
+    """The cooling-table generator (abundance.f90, one point per GPU thread) on
+    a log-spaced rho x T grid (1e8-1e13 g/cm^3, 5e9-2e11 K, H_d factor 0.5)."""
+    import ctypes
+
+    import numpy as np
+    rho, T = np.meshgrid(np.geomspace(1e8, 1e13, n_rho), np.geomspace(5e9, 2e11, n_t), indexing="ij")
+    rho, T = rho.ravel(), T.ravel()
+    n = rho.size
+    d = [torch.as_tensor(a, device="cuda:0") for a in (rho, T, np.full(n, 0.5))]
+    out = torch.empty(6 * n, dtype=torch.float64, device="cuda:0")
+    ptr = [ctypes.c_void_p(t.data_ptr()) for t in d + [out]]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = hc.lib().harm_eos_point_batch(*ptr, n, None)
+    el = time.perf_counter() - t0
+    if rc:
+        raise RuntimeError("harm_eos_point_batch failed")
+    vals = out.cpu().numpy().reshape(6, n)
+    return {"workload": f"cooling-table points {n_rho}x{n_t} (rho 1e8-1e13 g/cm^3, T 5e9-2e11 K, HdFac 0.5), "
+                        "chemical equilibrium + neutrino trapping + rates per point, FP64",
+            "points": n, "seconds": round(el, 3), "points_per_s": n / el,
+            "finite": bool(np.isfinite(vals).all()), "_rho": rho, "_T": T, "_vals": vals}
+
+
+def eos_table_cpu(tab: dict, sample: int = 24) -> dict:
+    """The reference's own abundance_ (oracle/_ref/libeosref.so) on an evenly
+    strided sample of the same grid, on one core; parity of that sample."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    from oracle_lib import EosRef, eosref_available
+    if not eosref_available():
+        return None
+    R = EosRef()
+    idx = np.linspace(0, tab["points"] - 1, sample).astype(int)
+    t0 = time.perf_counter()
+    ref = np.array([R.abundance(tab["_rho"][i], tab["_T"][i], 0.5) for i in idx]).T
+    el = time.perf_counter() - t0
+    got = tab["_vals"][:, idx]
+    rel = float((np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)).max())
+    return {"kind": "reference", "cores": 1, "sample": f"{sample} strided points of the grid",
+            "ms_per_point": el / sample * 1e3, "points_per_s": sample / el, "max_rel_diff_gpu_vs_reference": rel}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +314,7 @@ def main() -> None:
     ap.add_argument("--stress-n", type=int, default=1 << 26)
     ap.add_argument("--no-tabulated", action="store_true", help="skip the tabulated-EOS torus section")
     ap.add_argument("--no-tracers", action="store_true", help="skip the tracer section")
+    ap.add_argument("--no-eos-table", action="store_true", help="skip the cooling-table generator section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -353,6 +399,10 @@ def main() -> None:
     if not args.no_tabulated:
         tabulated = tabulated_torus(hc, args.steps, args.warmup)
 
+    eos_tab = None
+    if not args.no_eos_table:
+        eos_tab = eos_table(hc, torch)
+
     st = None
     if not args.no_learned and args.stress_n > 0:
         st = stress(hc, torch, args.stress_n, m_torus, args.train_steps, args.train_batch)
@@ -367,6 +417,11 @@ def main() -> None:
     if st is not None:
         st.pop("_host", None)
         st.pop("_sample", None)
+    if eos_tab is not None:
+        if not args.no_cpu:
+            eos_tab["cpu_reference"] = eos_table_cpu(eos_tab)
+        for k in ("_rho", "_T", "_vals"):
+            eos_tab.pop(k)
 
     out = {
         "metric": "zone-cycles/s at 288x256x256; con2prim Newton iters/zone, MLP-guess vs plain",
@@ -380,7 +435,7 @@ def main() -> None:
         "newton_kconv_per_zone": c["newton_kconv"] / max(c["c2p_calls"], 1),
         "newton_stopping_rule": "reference rule for both starts: |dW/W|+|dv2/v2| <= 1e-10, then 2 extra iterations",
         "learned_seed_torus": learned_rep, "stress_batch": st, "tabulated_eos_torus": tabulated,
-        "roofline": roof, "kernels": kernels, "tracers": tracers, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "tracers": tracers, "eos_table": eos_tab, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -285,6 +285,7 @@ def lib() -> ctypes.CDLL:
     L.harm_fermi_rules.argtypes = [POINTER(c_double)]
     L.harm_fermi_batch.argtypes = [c_void_p] * 6 + [c_size_t, c_void_p]
     L.harm_eos_species_batch.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p]
+    L.harm_eos_point_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
     L.harm_dump_write.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
     L.harm_dump_columns.argtypes = [c_void_p, c_size_t, c_size_t, c_void_p]
     L.harm_tracers_set.argtypes = [c_void_p, c_void_p, c_void_p]

@@ -16,8 +16,10 @@
 #include "../../include/harm_cool.h"
 #include "harm_fermi.hpp"
 #include "harm_host.hpp"
+#include "harm_nse.hpp"
 
 using harm::FermiRules;
+using harm::NseRules;
 
 namespace {
 
@@ -29,9 +31,9 @@ constexpr int TPB_F = 256;
         if (e_ != hipSuccess) return harm_host::set_error(std::string(#x) + ": " + hipGetErrorString(e_), 2); \
     } while (0)
 
-void legendre20(long double *x, long double *w)
+// positive nodes (increasing) and weights of the n-point Gauss-Legendre rule
+void legendre(int n, long double *x, long double *w)
 {
-    const int n = 20;
     for (int i = 1; i <= n / 2; i++) {
         long double z = cosl(3.14159265358979323846264338327950288L * (i - 0.25L) / (n + 0.5L)), z1, pp;
         for (int it = 0; it < 100; it++) {
@@ -94,13 +96,19 @@ void laguerre20(long double *x, long double *w)
     }
 }
 
-const FermiRules &rules()
+const NseRules &nse_rules();
+
+const FermiRules &rules() { return nse_rules().f; }
+
+const NseRules &nse_rules()
 {
-    static FermiRules R;
+    static NseRules N;
     static std::once_flag once;
     std::call_once(once, [] {
-        long double lx[10], lw[10], gx[20], gw[20];
-        legendre20(lx, lw);
+        FermiRules &R = N.f;
+        long double lx[10], lw[10], gx[20], gw[20], l40x[20], l40w[20];
+        legendre(20, lx, lw);
+        legendre(40, l40x, l40w);
         laguerre20(gx, gw);
         for (int i = 0; i < 10; i++) {
             R.lx[i] = (double)lx[i];
@@ -109,12 +117,35 @@ const FermiRules &rules()
         for (int i = 0; i < 20; i++) {
             R.gx[i] = (double)gx[i];
             R.gw[i] = (double)(gw[i] * expl(gx[i]));
+            N.lx40[i] = (double)l40x[i];
+            N.lw40[i] = (double)l40w[i];
         }
     });
-    return R;
+    return N;
 }
 
-__global__ void __launch_bounds__(TPB_F) k_fermi(FermiRules R, const double *__restrict__ dk,
+// The rules on the device, copied once per device before the first launch
+// (kernels read them in place rather than copying a by-value argument to
+// their stack for the noinline routines that take them by reference).
+__constant__ NseRules c_rules;
+
+int upload_rules()
+{
+    static bool done[64] = {};
+    static std::mutex mu;
+    int dev = 0;
+    FCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev < 0 || dev >= 64) return harm_host::set_error("device index out of range", 3);
+    if (!done[dev]) {
+        const NseRules &N = nse_rules();
+        FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules), &N, sizeof(NseRules)));
+        done[dev] = true;
+    }
+    return 0;
+}
+
+__global__ void __launch_bounds__(TPB_F) k_fermi(const double *__restrict__ dk,
                                                  const double *__restrict__ eta, const double *__restrict__ theta,
                                                  double *__restrict__ fd, double *__restrict__ fdeta,
                                                  double *__restrict__ fdtheta, size_t n)
@@ -122,19 +153,20 @@ __global__ void __launch_bounds__(TPB_F) k_fermi(FermiRules R, const double *__r
     const size_t i = (size_t)blockIdx.x * TPB_F + threadIdx.x;
     if (i >= n) return;
     double a, b, c;
-    harm::dfermi(dk[i], eta[i], theta[i], R, a, b, c);
+    harm::dfermi(dk[i], eta[i], theta[i], c_rules.f, a, b, c);
     fd[i] = a;
     if (fdeta) fdeta[i] = b;
     if (fdtheta) fdtheta[i] = c;
 }
 
 // in: [nin][n], out: [3][n] (e, p, s)
-__global__ void __launch_bounds__(TPB_F) k_eos_species(FermiRules R, int kind, const double *__restrict__ in,
+__global__ void __launch_bounds__(TPB_F) k_eos_species(int kind, const double *__restrict__ in,
                                                        double *__restrict__ out, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * TPB_F + threadIdx.x;
     if (i >= n) return;
     auto I = [&](int k) { return in[(size_t)k * n + i]; };
+    const FermiRules &R = c_rules.f;
     double e = 0., p = 0., s = 0.;
     switch (kind) {
     case HARM_EOS_NPE:
@@ -155,6 +187,19 @@ __global__ void __launch_bounds__(TPB_F) k_eos_species(FermiRules R, int kind, c
     out[2 * n + i] = s;
 }
 
+// One cooling-table point per thread (abundance.f90:6-243); out [6][n]: e, p,
+// Qnu, tau, n_e-, n_e+.
+__global__ void __launch_bounds__(64) k_abundance(const double *__restrict__ rho,
+                                                  const double *__restrict__ T, const double *__restrict__ hdfac,
+                                                  double *__restrict__ out, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    double v[6];
+    harm::abundance(rho[i], T[i], hdfac[i], c_rules, v[0], v[1], v[2], v[3], v[4], v[5]);
+    for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = v[k];
+}
+
 unsigned nb(size_t n) { return (unsigned)((n + TPB_F - 1) / TPB_F); }
 
 }  // namespace
@@ -164,11 +209,14 @@ extern "C" {
 int harm_fermi_rules(double *out)
 {
     if (!out) return harm_host::set_error("null argument", 1);
-    const FermiRules &R = rules();
+    const NseRules &N = nse_rules();
+    const FermiRules &R = N.f;
     std::copy(R.lx, R.lx + 10, out);
     std::copy(R.lw, R.lw + 10, out + 10);
     std::copy(R.gx, R.gx + 20, out + 20);
     std::copy(R.gw, R.gw + 20, out + 40);
+    std::copy(N.lx40, N.lx40 + 20, out + 60);
+    std::copy(N.lw40, N.lw40 + 20, out + 80);
     return 0;
 }
 
@@ -178,7 +226,23 @@ int harm_fermi_batch(const double *dk, const double *eta, const double *theta, d
     if (!n) return 0;
     if (!dk || !eta || !theta || !fd) return harm_host::set_error("null argument", 1);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fermi, dim3(nb(n)), dim3(TPB_F), 0, s, rules(), dk, eta, theta, fd, fdeta, fdtheta, n);
+    int rc = upload_rules();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fermi, dim3(nb(n)), dim3(TPB_F), 0, s, dk, eta, theta, fd, fdeta, fdtheta, n);
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
+                         void *stream)
+{
+    if (!n) return 0;
+    if (!rho || !T || !hdfac || !out) return harm_host::set_error("null argument", 1);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = upload_rules();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_abundance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, rho, T, hdfac, out, n);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(s));
     return 0;
@@ -190,7 +254,9 @@ int harm_eos_species_batch(int kind, const double *in, double *out, size_t n, vo
     if (!n) return 0;
     if (!in || !out) return harm_host::set_error("null argument", 1);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_eos_species, dim3(nb(n)), dim3(TPB_F), 0, s, rules(), kind, in, out, n);
+    int rc = upload_rules();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eos_species, dim3(nb(n)), dim3(TPB_F), 0, s, kind, in, out, n);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(s));
     return 0;

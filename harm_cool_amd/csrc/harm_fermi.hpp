@@ -20,6 +20,9 @@
 #ifndef HD
 #define HD __host__ __device__ inline
 #endif
+// Heavy routines keep their own register allocation on the device: inlined
+// into one table-point kernel they exceed the register file and spill.
+#define HD_CALL __host__ __device__ __attribute__((noinline)) inline
 
 namespace harm {
 
@@ -123,7 +126,7 @@ HD void fd_lag20(const FermiRules &R, double a, double b, double dk, double eta,
 // sqrt(1 + x theta / 2) / (exp(x - eta) + 1) dx and its eta and theta
 // derivatives, over Aparicio's four subintervals.  The breakpoint fit
 // parameters are the published ones.
-HD void dfermi(double dk, double eta, double theta, const FermiRules &R, double &fd, double &fdeta, double &fdtheta)
+HD_CALL void dfermi(double dk, double eta, double theta, const FermiRules &R, double &fd, double &fdeta, double &fdtheta)
 {
     constexpr double d = 3.3609, sg = 9.1186e-2, a1 = 6.7774, b1 = 1.1418, c1 = 2.9826, a2 = 3.7601,
                      b2 = 9.3719e-2, c2 = 2.1063e-2, d2 = 3.1084e1, e2 = 1.0056, a3 = 7.5669, b3 = 1.1695,
@@ -162,7 +165,7 @@ HD void fd_species(double eta, double beta, double cp, double ce, double xkt, do
 // eos_npe (eos.f90:31-113): free neutrons, protons and e-/e+ pairs.
 // xkt [MeV]; eta's are chemical potentials over kT without rest mass; n's in
 // fm^-3.  e, p in MeV fm^-3, s in k_B fm^-3.
-HD void eos_npe(double xkt, double xetan, double xetap, double xetae, double xnn, double xnp, double xnee, double xnep,
+HD_CALL void eos_npe(double xkt, double xetan, double xetap, double xetae, double xnn, double xnp, double xnee, double xnep,
                 const FermiRules &R, double &e, double &p, double &s)
 {
     const double betae = xkt / FD_XME, betap = xkt / FD_XMP, betan = xkt / FD_XMN;

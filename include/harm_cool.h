@@ -382,10 +382,18 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr);
  * may be NULL), by Aparicio's four-interval 20-point quadrature. */
 int harm_fermi_batch(const double *dk, const double *eta, const double *theta, double *fd, double *fdeta,
                      double *fdtheta, size_t n, void *stream);
-/* The 20-point rules it uses: out[0..9] Gauss-Legendre positive nodes,
- * [10..19] their weights, [20..39] Gauss-Laguerre nodes, [40..59] their weights
- * times exp(node) (host call). */
-int harm_fermi_rules(double *out /* 60 */);
+/* The rules it uses: out[0..9] 20-point Gauss-Legendre positive nodes,
+ * [10..19] their weights, [20..39] 20-point Gauss-Laguerre nodes, [40..59] their
+ * weights times exp(node), [60..79] / [80..99] the 40-point Gauss-Legendre
+ * positive nodes / weights of the neutrino-rate integrals (host call). */
+int harm_fermi_rules(double *out /* 100 */);
+/* One cooling-table point per element (abundance.f90:6-243): rho [g cm^-3], T
+ * [K] and the disk-height factor HdFac -> out[6][n]: e, p [erg cm^-3], Qnu
+ * [erg cm^-2 s^-1], optical depth tau, n_e-, n_e+ [fm^-3].  The chemical
+ * equilibrium solve, the neutrino trapping iteration and all rates run in the
+ * thread of their point. */
+int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
+                         void *stream);
 /* Per-species EOS of eos.f90 (energy density e and pressure p in MeV fm^-3,
  * entropy density s in k_B fm^-3) for n points; in is [nin][n], out [3][n]:
  *   HARM_EOS_NPE (eos_npe, eos.f90:31-113) nin 8: kT [MeV], eta_n, eta_p,

@@ -23,7 +23,7 @@ from oracle_lib import EosRef, eosref_available
 
 
 def _rules():
-    out = np.zeros(60)
+    out = np.zeros(100)
     assert hc.lib().harm_fermi_rules(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     return out[:10], out[10:20], out[20:40], out[40:60]
 

@@ -77,3 +77,32 @@ def test_species_eos_matches_reference(kind, nin):
     ok = (np.abs(got - ref) <= TOL * scale) | (got == ref)
     print(f"kind {kind}: {int(np.all(got == ref, axis=0).sum())}/{n} bitwise")
     assert ok.all(), np.argwhere(~ok)[:5]
+
+
+def test_table_points_match_reference():
+    """Whole table points (abundance.f90): the chemical-equilibrium Newton solve,
+    ten trapping passes and the rates, one thread per point.  The reference's
+    Newton solves stop at residuals of 1e-8 of the density, so an ulp of the
+    device's exp or pow can move a solve's stopping iterate: the bar is 1e-6
+    relative, and the test reports how many points agree bit for bit."""
+    import time
+    torch = _torch()
+    rho, T = np.meshgrid(np.geomspace(1e8, 1e13, 9), np.geomspace(5e9, 2e11, 8), indexing="ij")
+    rho, T = rho.ravel(), T.ravel()
+    hd = np.full(rho.size, 0.5)
+    n = rho.size
+    ins = [_dev(torch, a) for a in (rho, T, hd)]
+    out = torch.empty(6 * n, dtype=torch.float64, device="cuda:0")
+    t0 = time.perf_counter()
+    assert hc.lib().harm_eos_point_batch(*(ctypes_ptr(t) for t in ins), ctypes_ptr(out), n, None) == 0
+    el = time.perf_counter() - t0
+    got = out.cpu().numpy().reshape(6, n)
+    R = EosRef()
+    t0 = time.perf_counter()
+    ref = np.array([R.abundance(a, b, 0.5) for a, b in zip(rho, T)]).T
+    el_ref = time.perf_counter() - t0
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+    print(f"table points: {n}, {int(np.all(got == ref, axis=0).sum())} bitwise, max rel {rel.max():.2e}; "
+          f"GPU {el * 1e3:.1f} ms, reference on one core {el_ref * 1e3:.0f} ms")
+    assert np.all(np.isfinite(got))
+    assert rel.max() <= 1e-6
