@@ -13,6 +13,9 @@
 // Layout: SoA, phi fastest.  Zone z = (i*SN2 + j)*N3 + h, variable stride nz.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -3185,6 +3188,54 @@ int harm_set_eos(harm_ctx *c, const harm_eos_table *tab, int ref_derivs)
     return 0;
 }
 
+namespace {
+
+// tracer_name (tracers.c:100-107) under the run directory's tracers/
+std::string tracer_file(const harm_ctx *c, int h, int i, int j)
+{
+    char b[96];
+    if (c->hg.N3 > 2)
+        snprintf(b, sizeof(b), "tracers/tracer_%03d_%03d_%03d.dat", i, j, h);
+    else
+        snprintf(b, sizeof(b), "tracers/tracer_%03d_%03d.dat", i, j);
+    return b;
+}
+
+// restart_write_tracers (tracers.c:565-606): the live tracers in slot order,
+// with the current size of their files (0 when absent)
+int tracer_block(harm_ctx *c, harm_host::TracerBlock &tb)
+{
+    const auto &hg = c->hg;
+    const TrArgs &a = c->tr.a;
+    std::vector<double> X(3 * (size_t)a.n);
+    std::vector<int8_t> st((size_t)a.n);
+    HIPCHK(hipMemcpyAsync(X.data(), a.X, X.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(st.data(), a.state, st.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const bool d3 = hg.N3 > 2;
+    // tracers.c:126-145 (the phi limits are set only in 3D)
+    const double par[8] = {a.Rin, a.Rout, a.thmin, a.thmax, hg.startx[1] + (0 + 0.5) * hg.dx[1],
+                           hg.startx[1] + (hg.N1 + 0.5) * hg.dx[1], d3 ? hg.startx[3] + (0 + 0.5) * hg.dx[3] : 0.0,
+                           d3 ? hg.startx[3] + (hg.N3 + 0.5) * hg.dx[3] : 0.0};
+    std::copy(par, par + 8, tb.par);
+    const long long n12 = (long long)hg.N1 * hg.N2;
+    for (long long k = 0; k < a.n; k++) {
+        if (st[k] != HARM_TRACER_LIVE) continue;
+        const int h = (int)(k / n12), i = (int)((k % n12) / hg.N2), j = (int)(k % hg.N2);
+        tb.i.push_back(i);
+        tb.j.push_back(j);
+        tb.h.push_back(h);
+        tb.X.push_back(0.0);  // X[TT] (tracers.c:153)
+        for (int d = 0; d < 3; d++) tb.X.push_back(X[(size_t)d * a.n + k]);
+        struct stat sb;
+        const std::string f = tracer_file(c, h, i, j);
+        tb.size.push_back(stat(f.c_str(), &sb) == 0 ? (int32_t)sb.st_size : 0);
+    }
+    return 0;
+}
+
+}  // namespace
+
 int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr)
 {
     if (!c || !path) return set_err("null argument");
@@ -3227,7 +3278,13 @@ int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hd
     h.mass_in = ts[2];
     h.mass_out = ts[3];
     h.mass_start = c->mass_start;
-    return harm_host::restart_write_file(path, h, prim.data(), temp.empty() ? nullptr : temp.data());
+    harm_host::TracerBlock tb;
+    if (c->tr.on) {
+        int rc = tracer_block(c, tb);
+        if (rc) return rc;
+    }
+    return harm_host::restart_write_file(path, h, prim.data(), temp.empty() ? nullptr : temp.data(),
+                                         c->tr.on ? &tb : nullptr);
 }
 
 int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
@@ -3251,7 +3308,9 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
                            3);
     if (h.lim < 0 || h.lim > 2) return set_err("restart file: unknown slope limiter", 3);
     std::vector<double> prim(8 * c->nz);
-    if ((rc = harm_host::restart_read_file(path, h, prim.data(), nullptr, nullptr))) return rc;
+    harm_host::TracerBlock tb;
+    int has_tb = 0;
+    if ((rc = harm_host::restart_read_file(path, h, prim.data(), nullptr, nullptr, &tb, &has_tb))) return rc;
     // cour, lim, tf, defcon come from the file (restart.c:335-367)
     c->P.cour = c->g.cour = h.cour;
     c->P.lim = c->g.lim = h.lim;
@@ -3264,6 +3323,25 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
     HIPCHK(hipStreamSynchronize(c->stream));
     c->mass_start = h.mass_start;
     c->flux_valid = false;
+    if (has_tb && c->tr.on) {
+        // restart_read_tracers (tracers.c:608-654): a listed tracer whose file is
+        // gone is dropped; an existing file is cut back to its recorded size
+        const auto &hg = c->hg;
+        const long long n = c->tr.a.n;
+        std::vector<double> X(3 * (size_t)n, 0.0);
+        std::vector<int8_t> st((size_t)n, (int8_t)HARM_TRACER_NONE);
+        for (size_t q = 0; q < tb.i.size(); q++) {
+            if (tb.i[q] < 0 || tb.i[q] >= hg.N1 || tb.j[q] < 0 || tb.j[q] >= hg.N2 || tb.h[q] < 0 || tb.h[q] >= hg.N3)
+                return set_err("restart file: tracer outside the grid", 3);
+            const std::string f = tracer_file(c, tb.h[q], tb.i[q], tb.j[q]);
+            if (access(f.c_str(), F_OK) != 0) continue;
+            if (truncate(f.c_str(), (off_t)tb.size[q]) != 0) return set_err("restart: cannot truncate " + f, 2);
+            const long long k = ((long long)tb.h[q] * hg.N1 + tb.i[q]) * hg.N2 + tb.j[q];
+            st[k] = HARM_TRACER_LIVE;
+            for (int d = 0; d < 3; d++) X[(size_t)d * n + k] = tb.X[4 * q + 1 + d];
+        }
+        if ((rc = harm_tracers_set(c, X.data(), st.data()))) return rc;
+    }
     if ((rc = bound(c, c->p))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     if (hdr) *hdr = h;

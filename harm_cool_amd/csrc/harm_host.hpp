@@ -29,8 +29,20 @@ int build_geometry(const harm_params &P, Grid &g, GeomTables &t);
 int set_error(const std::string &msg, int code);
 
 // Restart file text (harm_restart.cpp); prim/temp in the canonical layout.
-int restart_write_file(const char *path, const harm_restart_hdr &h, const double *prim, const double *temp);
-int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp);
+// The tracer block of a restart file (restart_write_tracers, tracers.c:565-606):
+// Rin, Rout, ThetaMin, ThetaMax, Rank_Rin, Rank_Rout, Rank_Phin, Rank_Phout,
+// the count, then per tracer i, j, h (zero-based), X[0..3] and its file size.
+struct TracerBlock {
+    double par[8];
+    std::vector<int32_t> i, j, h, size;
+    std::vector<double> X;  // 4 per tracer
+};
+int restart_write_file(const char *path, const harm_restart_hdr &h, const double *prim, const double *temp,
+                       const TracerBlock *tb = nullptr);
+// tb (may be NULL) receives the tracer block when the file holds one; *has_tb
+// says whether it did.
+int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp,
+                      TracerBlock *tb = nullptr, int *has_tb = nullptr);
 // dump() text (dump.c:1710-1990): the header line, then one line of ncol
 // values per zone; fill(z0, n, cols) puts zones z0..z0+n-1, stride values each.
 std::string dump_header_text(const harm_restart_hdr &h, const double *startx, const double *dx);

@@ -196,7 +196,8 @@ int slurp(const char *path, std::vector<char> &buf, size_t max_bytes = SIZE_MAX)
 
 }  // namespace
 
-int restart_write_file(const char *path, const harm_restart_hdr &h, const double *prim, const double *temp)
+int restart_write_file(const char *path, const harm_restart_hdr &h, const double *prim, const double *temp,
+                       const TracerBlock *tb)
 {
     const int sn1 = h.N1 + 4, sn2 = h.N2 + 4, n3 = h.N3;
     const size_t nz = (size_t)sn1 * sn2 * n3;
@@ -224,6 +225,28 @@ int restart_write_file(const char *path, const harm_restart_hdr &h, const double
         ok = fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
     }
     if (ok && h.ener_size >= 0) ok = fprintf(fp, "%10d", h.ener_size) > 0;
+    if (ok && tb) {
+        // restart_write_tracers (tracers.c:565-606), no separators but the fields' own
+        std::string t;
+        char b[64];
+        for (double v : tb->par) {
+            snprintf(b, sizeof(b), "%32.22e", v);
+            t += b;
+        }
+        snprintf(b, sizeof(b), "%10d", (int)tb->i.size());
+        t += b;
+        for (size_t q = 0; q < tb->i.size(); q++) {
+            snprintf(b, sizeof(b), "%10d%10d%10d", tb->i[q], tb->j[q], tb->h[q]);
+            t += b;
+            for (int k = 0; k < 4; k++) {
+                snprintf(b, sizeof(b), "%32.22e", tb->X[4 * q + k]);
+                t += b;
+            }
+            snprintf(b, sizeof(b), "%10d", tb->size[q]);
+            t += b;
+        }
+        ok = fwrite(t.data(), 1, t.size(), fp) == t.size();
+    }
     if (ok && temp) {
         for (size_t q0 = 0; ok && q0 < nz; q0 += CHUNK) {
             const size_t nq = std::min(CHUNK, nz - q0);
@@ -319,8 +342,10 @@ int dump_write_file(const char *path, const std::string &header, size_t nzones, 
     return ok ? 0 : set_error(std::string("error writing dump file ") + path, 2);
 }
 
-int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp)
+int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, double *temp, int *has_temp,
+                      TracerBlock *tb, int *has_tb)
 {
+    if (has_tb) *has_tb = 0;
     std::vector<char> buf;
     int rc = slurp(path, buf, prim ? SIZE_MAX : (size_t)65536);
     if (rc) return rc;
@@ -388,6 +413,35 @@ int restart_read_file(const char *path, harm_restart_hdr &h, double *prim, doubl
         h.ener_size = es;
     else
         tk = save;
+    // a tracer block: eight doubles then an integer count (a temperature block
+    // holds "%32.22e" values only, whose tokens always carry an exponent)
+    {
+        Tokens probe = tk;
+        double par[8];
+        bool blk = true;
+        for (int k = 0; k < 8 && blk; k++) blk = probe.dbl(par[k]);
+        int32_t cnt = 0;
+        bool is_int = false;
+        blk = blk && probe.i32(cnt, &is_int) && is_int && cnt >= 0;
+        if (blk) {
+            TracerBlock local;
+            TracerBlock &T = tb ? *tb : local;
+            std::copy(par, par + 8, T.par);
+            T.i.resize(cnt);
+            T.j.resize(cnt);
+            T.h.resize(cnt);
+            T.size.resize(cnt);
+            T.X.resize(4 * (size_t)cnt);
+            for (int32_t q = 0; q < cnt; q++) {
+                bool good = probe.i32(T.i[q]) && probe.i32(T.j[q]) && probe.i32(T.h[q]);
+                for (int k = 0; k < 4 && good; k++) good = probe.dbl(T.X[4 * (size_t)q + k]);
+                good = good && probe.i32(T.size[q]);
+                if (!good) return set_error("restart file: tracer block truncated or malformed", 3);
+            }
+            tk = probe;
+            if (has_tb) *has_tb = 1;
+        }
+    }
     save = tk;
     const char *s;
     size_t n;

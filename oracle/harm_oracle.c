@@ -24,6 +24,8 @@
  * reference's measured config-1 Newton-iteration and Font-fix counts.
  */
 #define _GNU_SOURCE
+#include <sys/stat.h>
+
 #include "harm_oracle.h"
 
 #include <math.h>
@@ -3525,7 +3527,15 @@ void orc_dudp_5d(const double prim[8], const double *gcov, const double *gcon, d
 #define ORC_FMT_DBL_OUT "%32.22e" /* decs.h:175 */
 #define ORC_FMT_INT_OUT "%10d"    /* decs.h:176 */
 
+static void restart_write_tracers(FILE *fp, const orc_ctx *o);
+
 int orc_restart_write(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp)
+{
+    return orc_restart_write_tr(path, h, prim, temp, NULL);
+}
+
+int orc_restart_write_tr(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp,
+                         const orc_ctx *o)
 {
     const int sn1 = h->N1 + 4, sn2 = h->N2 + 4, sn3 = h->N3;
     FILE *fp = fopen(path, "wt");
@@ -3575,6 +3585,7 @@ int orc_restart_write(const char *path, const orc_restart_hdr *h, const double *
                 fprintf(fp, "\n");
             }
     if (h->ener_size >= 0) fprintf(fp, ORC_FMT_INT_OUT, h->ener_size); /* restart.c:150-157 */
+    if (o && o->tr_on) restart_write_tracers(fp, o);                   /* restart.c:159-161 */
     if (temp)                                                          /* cooleos.c:1002-1005 */
         for (hh = 0; hh < sn3; ++hh)
             for (i = 0; i < sn1; ++i)
@@ -3820,4 +3831,39 @@ int orc_dump_write(orc_ctx *o, const char *path, const orc_restart_hdr *hin)
 #undef G_
     g_step_eos = NULL;
     return fclose(fp) == 0 ? 0 : 2;
+}
+
+/* tracers.c:565-606, one rank: the live tracers in their vector order */
+static void restart_write_tracers(FILE *fp, const orc_ctx *o)
+{
+    const long long n = o->tr_n, n12 = (long long)o->N1 * o->N2;
+    long long cnt = 0, k;
+    int d;
+    fprintf(fp, ORC_FMT_DBL_OUT, o->tr_Rin);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->tr_Rout);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->tr_thmin);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->tr_thmax);
+    /* tracers.c:133-145: coord() of the first and one-past-last centres */
+    fprintf(fp, ORC_FMT_DBL_OUT, o->startx[RR] + (0 + 0.5) * o->dx[RR]);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->startx[RR] + (o->N1 + 0.5) * o->dx[RR]);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->N3 > 2 ? o->startx[PH] + (0 + 0.5) * o->dx[PH] : 0.0);
+    fprintf(fp, ORC_FMT_DBL_OUT, o->N3 > 2 ? o->startx[PH] + (o->N3 + 0.5) * o->dx[PH] : 0.0);
+    for (k = 0; k < n; k++) cnt += o->tr_state[k] == ORC_TRACER_LIVE;
+    fprintf(fp, ORC_FMT_INT_OUT, (int)cnt);
+    for (k = 0; k < n; k++) {
+        if (o->tr_state[k] != ORC_TRACER_LIVE) continue;
+        const int h = (int)(k / n12), i = (int)((k % n12) / o->N2), j = (int)(k % o->N2);
+        char name[96];
+        struct stat st;
+        fprintf(fp, ORC_FMT_INT_OUT, i);
+        fprintf(fp, ORC_FMT_INT_OUT, j);
+        fprintf(fp, ORC_FMT_INT_OUT, h);
+        fprintf(fp, ORC_FMT_DBL_OUT, 0.0);
+        for (d = 0; d < 3; d++) fprintf(fp, ORC_FMT_DBL_OUT, o->tr_X[(size_t)d * n + k]);
+        if (o->N3 > 2)
+            snprintf(name, sizeof(name), "tracers/tracer_%03d_%03d_%03d.dat", i, j, h);
+        else
+            snprintf(name, sizeof(name), "tracers/tracer_%03d_%03d.dat", i, j);
+        fprintf(fp, ORC_FMT_INT_OUT, stat(name, &st) == 0 ? (int)st.st_size : 0);
+    }
 }

@@ -184,6 +184,10 @@ typedef struct orc_restart_hdr {
  * the ener.out size when ener_size >= 0 (restart.c:150-157), the COOL_EOS
  * temperatures when temp != NULL (cooleos.c:996-1014). */
 int orc_restart_write(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp);
+/* The same with, after the ener.out size, the tracer block of o's tracers
+ * (restart_write_tracers, tracers.c:565-606) when o has them. */
+int orc_restart_write_tr(const char *path, const orc_restart_hdr *h, const double *prim, const double *temp,
+                         const orc_ctx *o);
 /* dump() (dump.c:1710-1990, HDFDUMP = 0): hdr supplies the driver's counters;
  * the context's parameters, t and dt replace the rest. */
 int orc_dump_write(orc_ctx *o, const char *path, const orc_restart_hdr *hdr);

@@ -136,6 +136,7 @@ def lib():
         L.orc_restart_write.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp]
         L.orc_restart_read.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_int]
         L.orc_dump_write.argtypes = [c_void_p, ctypes.c_char_p, POINTER(OrcRestartHdr)]
+        L.orc_restart_write_tr.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_void_p]
         _lib = L
     return _lib
 

@@ -110,3 +110,55 @@ def test_restart_with_tabulated_eos_writes_zone_temperatures(tmp_path):
     assert np.array_equal(g2.prims(), g.prims())
     g2.close()
     g.close()
+
+
+def test_restart_tracer_block(tmp_path, monkeypatch):
+    """restart_write_tracers / restart_read_tracers (tracers.c:565-654): the
+    block after the ener.out size equals the oracle's, tracer file sizes are
+    those of tracers/ under the run directory, and reading it back keeps the
+    tracers whose files exist (cut back to the recorded size) at their
+    positions."""
+    _torch()
+    monkeypatch.chdir(tmp_path)
+    P = torus_params(16, 16, 64)
+    g = hc.Harm(P)
+    o = Oracle(P)
+    olib().orc_set_threads(8)
+    g.init()
+    o.init_torus()
+    assert g.tracers_init() == o.tracers_init() > 0
+    for _ in range(3):
+        g.step_ch(1)
+        assert o.step() == 0
+    recs = g.tracers_dump()
+    os.makedirs("tracers")
+    g.write_tracer_files(recs[:40], root="tracers")
+    g.write_tracer_files(recs[:20], root="tracers")  # the first 20 files hold two lines
+    g.restart_write("rdump", ener_size=123, **{k: v for k, v in DRIVER.items() if k != "ener_size"})
+    h = hc.restart_peek("rdump")
+    oh = OrcRestartHdr()
+    for k, v in h.items():
+        setattr(oh, k, v)
+    oh.ener_size = 123
+    po = np.ascontiguousarray(o.prims())
+    assert olib().orc_restart_write_tr(b"orc", ctypes.byref(oh), po.ctypes.data_as(_dp), None, o.h) == 0
+    a, b = open("rdump", "rb").read(), open("orc", "rb").read()
+    assert a == b
+    # later lines appended after the checkpoint are cut away on restart
+    g.write_tracer_files(recs[:5], root="tracers")
+    g2 = hc.Harm(hc.params_from_restart(h))
+    g2.tracers_init()
+    g2.restart_read("rdump")
+    X, st = g.tracers_get()
+    X2, st2 = g2.tracers_get()
+    keep = np.zeros(st.size, dtype=bool)
+    n12 = P.n1 * P.n2
+    for r in recs[:40]:
+        keep[r["h"] * n12 + r["i"] * P.n2 + r["j"]] = True
+    assert np.array_equal(st2.reshape(-1) == 1, keep)
+    assert np.array_equal(X2.reshape(3, -1)[:, keep], X.reshape(3, -1)[:, keep])
+    for r in recs[:6]:
+        lines = open(hc.tracer_name("tracers", r["h"], r["i"], r["j"], P.n3)).read().splitlines()
+        assert len(lines) == 2
+    g.close()
+    g2.close()
