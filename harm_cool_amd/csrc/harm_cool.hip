@@ -880,9 +880,13 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
 // Font fix (step_ch.c:277-300) + fixup1zone for the zones k_update deferred.
 // Grid-stride over the list; the list count is reset by the next kernel on the
 // stream (k_bound), after every thread here has read it.
-// A small grid: the list is usually empty or short, and a full-size launch of
-// this register-heavy kernel costs tens of microseconds even then.
-constexpr int FONTFIX_BLOCKS = 16;
+// A fixed grid-stride grid: a launch sized for the list's capacity costs tens of
+// microseconds even when the list is empty, and every listed zone is a serial
+// 1D Newton solve, so too few threads serialize the long lists of later times.
+// Over 20 steps of the 288x256x256 torus: 16 blocks 121.5 us per call on
+// average (5.1 ms worst), 256 blocks 15.0 us (0.42 ms), 1024 blocks 18.8 us;
+// an empty list costs 3.4 us with 16 or 256 blocks, 7.2 us with 1024.
+constexpr int FONTFIX_BLOCKS = 256;
 __global__ void __launch_bounds__(TPB) k_fontfix(DGrid g, double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                  FontList fl, unsigned long long *__restrict__ cnt)
 {
