@@ -33,15 +33,16 @@ C5_BYTES = 64 + 128 + 128 + 8 + 64 + 64 + 3  # batched inversion, per zone (see 
 #        non-B fluxes of the zone's six faces (5 x 3 x 2 faces, each face shared
 #        by two zones -> 5 x 3 x 1) and the three EMFs it owns, write the 8 flux
 #        divergences.
-#   update (the plain wave-uniform kernel, which forms the divergence itself):
-#        read pb (8), the 15 non-B fluxes and 3 EMFs of the divergence, in the
-#        corrector also pi (8); write pf (8) and the flag (1 B): 273 B predictor,
-#        337 B corrector, 305 B per launch on average.
+#   update (the plain wave-uniform kernel, which forms the corner EMFs and the
+#        divergence itself): read pb (8), the 15 non-B fluxes and the 6 B-flux
+#        components of the corners the zone owns, in the corrector also pi (8);
+#        write pf (8) and the flag (1 B): 297 B predictor, 361 B corrector,
+#        329 B per launch on average.
 ALG_BYTES = {
     "flux1": 128.0, "flux2": 128.0, "flux3": 128.0,
     "emf": 6 * 8 + 3 * 8,
     "div": 15 * 8 + 3 * 8 + 8 * 8,
-    "update": ((8 + 18) * 8 + 8 * 8 + 1 + (8 + 8 + 18) * 8 + 8 * 8 + 1) / 2,
+    "update": ((8 + 21) * 8 + 8 * 8 + 1 + (8 + 8 + 21) * 8 + 8 * 8 + 1) / 2,
 }
 
 

@@ -627,24 +627,41 @@ __device__ __forceinline__ void flux_div(const DGrid &g, const DivArgs &a, int i
 }
 
 // flux_div with 32-bit zone offsets (nz < OFF32_MAX_ZONES, N3 > 2); the same
-// operations in the same order.
+// operations in the same order.  The twelve corner EMFs it needs are formed
+// here from the face B-fluxes with k_emf's expressions (step_ch.c:773-781), so
+// the fused update needs no EMF pass.
 __device__ __forceinline__ void flux_div32(const DGrid &g, const DivArgs &a, int i, int j, int h, double *d)
 {
     const unsigned sI = (unsigned)g.SN2 * (unsigned)g.N3, sJ = (unsigned)g.N3;
     const unsigned z = (unsigned)zidx(g, i, j, h);
     const unsigned zh1 = z - (unsigned)h + (unsigned)((h + 1) % g.N3);
+    const unsigned zhm = z - (unsigned)h + (unsigned)((h + g.N3 - 1) % g.N3);
     const unsigned zi1 = z + sI, zj1 = z + sJ, zi1h1 = zh1 + sI, zj1h1 = zh1 + sJ, zi1j1 = zi1 + sJ;
     const size_t nz = g.nz;
+    const double *F1B2 = a.F1 + B2 * nz, *F1B3 = a.F1 + B3 * nz, *F2B1 = a.F2 + B1 * nz, *F2B3 = a.F2 + B3 * nz,
+                 *F3B1 = a.F3 + B1 * nz, *F3B2 = a.F3 + B2 * nz;
+    // e1 = ((F2[B3] + F2[B3](h-1)) - (F3[B2] + F3[B2](j-1))) / 4 at (j, h), (j, h+1), (j+1, h), (j+1, h+1)
+    const double e1z = 0.25 * ((at32(F2B3, z) + at32(F2B3, zhm)) - (at32(F3B2, z) + at32(F3B2, z - sJ)));
+    const double e1h = 0.25 * ((at32(F2B3, zh1) + at32(F2B3, z)) - (at32(F3B2, zh1) + at32(F3B2, zh1 - sJ)));
+    const double e1j = 0.25 * ((at32(F2B3, zj1) + at32(F2B3, zhm + sJ)) - (at32(F3B2, zj1) + at32(F3B2, z)));
+    const double e1jh = 0.25 * ((at32(F2B3, zj1h1) + at32(F2B3, zj1)) - (at32(F3B2, zj1h1) + at32(F3B2, zh1)));
+    // e2 = ((F3[B1] + F3[B1](i-1)) - (F1[B3] + F1[B3](h-1))) / 4 at (i, h), (i, h+1), (i+1, h), (i+1, h+1)
+    const double e2z = 0.25 * ((at32(F3B1, z) + at32(F3B1, z - sI)) - (at32(F1B3, z) + at32(F1B3, zhm)));
+    const double e2h = 0.25 * ((at32(F3B1, zh1) + at32(F3B1, zh1 - sI)) - (at32(F1B3, zh1) + at32(F1B3, z)));
+    const double e2i = 0.25 * ((at32(F3B1, zi1) + at32(F3B1, z)) - (at32(F1B3, zi1) + at32(F1B3, zhm + sI)));
+    const double e2ih = 0.25 * ((at32(F3B1, zi1h1) + at32(F3B1, zh1)) - (at32(F1B3, zi1h1) + at32(F1B3, zi1)));
+    // e3 = ((F1[B2] + F1[B2](j-1)) - (F2[B1] + F2[B1](i-1))) / 4 at (i, j), (i, j+1), (i+1, j), (i+1, j+1)
+    const double e3z = 0.25 * ((at32(F1B2, z) + at32(F1B2, z - sJ)) - (at32(F2B1, z) + at32(F2B1, z - sI)));
+    const double e3j = 0.25 * ((at32(F1B2, zj1) + at32(F1B2, z)) - (at32(F2B1, zj1) + at32(F2B1, zj1 - sI)));
+    const double e3i = 0.25 * ((at32(F1B2, zi1) + at32(F1B2, zi1 - sJ)) - (at32(F2B1, zi1) + at32(F2B1, z)));
+    const double e3ij = 0.25 * ((at32(F1B2, zi1j1) + at32(F1B2, zi1)) - (at32(F2B1, zi1j1) + at32(F2B1, zj1)));
     // ct_f1 at (i,j,h) and (i+1,j,h), ct_f2 at (i,j,h) and (i,j+1,h), ct_f3 at (i,j,h) and (i,j,h+1)
-    const double f1a1 = 0.5 * (at32(a.e3, z) + at32(a.e3, zj1)), f1a2 = -0.5 * (at32(a.e2, z) + at32(a.e2, zh1));
-    const double f1b1 = 0.5 * (at32(a.e3, zi1) + at32(a.e3, zi1j1)),
-                 f1b2 = -0.5 * (at32(a.e2, zi1) + at32(a.e2, zi1h1));
-    const double f2a0 = -0.5 * (at32(a.e3, z) + at32(a.e3, zi1)), f2a2 = 0.5 * (at32(a.e1, z) + at32(a.e1, zh1));
-    const double f2b0 = -0.5 * (at32(a.e3, zj1) + at32(a.e3, zi1j1)),
-                 f2b2 = 0.5 * (at32(a.e1, zj1) + at32(a.e1, zj1h1));
-    const double f3a0 = 0.5 * (at32(a.e2, z) + at32(a.e2, zi1)), f3a1 = -0.5 * (at32(a.e1, z) + at32(a.e1, zj1));
-    const double f3b0 = 0.5 * (at32(a.e2, zh1) + at32(a.e2, zi1h1)),
-                 f3b1 = -0.5 * (at32(a.e1, zh1) + at32(a.e1, zj1h1));
+    const double f1a1 = 0.5 * (e3z + e3j), f1a2 = -0.5 * (e2z + e2h);
+    const double f1b1 = 0.5 * (e3i + e3ij), f1b2 = -0.5 * (e2i + e2ih);
+    const double f2a0 = -0.5 * (e3z + e3i), f2a2 = 0.5 * (e1z + e1h);
+    const double f2b0 = -0.5 * (e3j + e3ij), f2b2 = 0.5 * (e1j + e1jh);
+    const double f3a0 = 0.5 * (e2z + e2i), f3a1 = -0.5 * (e1z + e1j);
+    const double f3b0 = 0.5 * (e2h + e2ih), f3b1 = -0.5 * (e1h + e1jh);
     const double f1a[3] = {0., f1a1, f1a2}, f1b[3] = {0., f1b1, f1b2};
     const double f2a[3] = {f2a0, 0., f2a2}, f2b[3] = {f2b0, 0., f2b2};
     const double f3a[3] = {f3a0, f3a1, 0.}, f3b[3] = {f3b0, f3b1, 0.};
@@ -2414,13 +2431,15 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
     return 0;
 }
 
-int fluxes_and_emf(harm_ctx *c, const double *pb, int fullstep)
+// emf = false: the update that follows forms the corner EMFs itself (flux_div32)
+int fluxes_and_emf(harm_ctx *c, const double *pb, int fullstep, bool emf = true)
 {
     const DGrid &g = c->g;
     int rc;
     if ((rc = flux_launch(c, 1, pb, c->F1, fullstep))) return rc;
     if ((rc = flux_launch(c, 2, pb, c->F2, fullstep))) return rc;
     if (g.N3 > 2 && (rc = flux_launch(c, 3, pb, c->F3, fullstep))) return rc;
+    if (!emf) return 0;
     const size_t ne = (size_t)(g.ie + 2 - g.is) * (g.je + 2 - g.js) * g.N3;
     LAUNCH(HARM_K_EMF, hipLaunchKernelGGL(k_emf, dim3(nblocks8(ne)), dim3(TPB), 0, c->stream, g, c->F1, c->F2, c->F3,
                                           c->e1, c->e2, c->e3));
@@ -2454,7 +2473,6 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
 {
     const DGrid &g = c->g;
     int rc;
-    if ((rc = fluxes_and_emf(c, pb, fullstep))) return rc;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
     const int same = (pi == pb) ? 1 : 0;
     UpdateExtra ex;
@@ -2480,8 +2498,10 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
         c->cap_phase = (int)((ph + 7919) % (size_t)c->cap_stride);
     }
     const DivArgs dv = {c->F1, c->F2, c->F3, c->e1, c->e2, c->e3};
-    // the plain wave-uniform update forms the divergence itself; the others read S
+    // the plain wave-uniform update forms the EMFs and the divergence itself; the
+    // others read S, which k_div forms from the EMF arrays
     const bool fused = c->uniform && (mode != UPD_CAPTURE || c->eos_kind == 1) && c->nz < OFF32_MAX_ZONES;
+    if ((rc = fluxes_and_emf(c, pb, fullstep, !fused))) return rc;
     if (!fused)
         LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, dv, c->S));
 #define UPD_LAUNCH(U_, M_, T_, F_)                                                                                  \
@@ -2571,6 +2591,10 @@ int dalloc(T **p, size_t n)
 {
     HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
     HIPCHK(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    // hipMemset runs on the null stream, which the library's non-blocking stream
+    // does not wait for: finish it before any kernel on that stream can touch
+    // the buffer (allocation happens at set-up and in occasional calls only)
+    HIPCHK(hipDeviceSynchronize());
     return 0;
 }
 
