@@ -262,7 +262,10 @@ def eos_table(hc, torch, n_rho: int = 200, n_t: int = 200) -> dict:
     import ctypes
 
     import numpy as np
-    rho, T = np.meshgrid(np.geomspace(1e8, 1e13, n_rho), np.geomspace(5e9, 2e11, n_t), indexing="ij")
+    # the grid of a constant-grid cache with logarithmic rho and T axes
+    ax_rho = hc.cache_axis(hc.AXIS_LOG, n_rho, 1e8, 1e13)
+    ax_T = hc.cache_axis(hc.AXIS_LOG, n_t, 5e9, 2e11)
+    rho, T = np.meshgrid(ax_rho, ax_T, indexing="ij")
     rho, T = rho.ravel(), T.ravel()
     n = rho.size
     d = [torch.as_tensor(a, device="cuda:0") for a in (rho, T, np.full(n, 0.5))]
@@ -278,7 +281,37 @@ def eos_table(hc, torch, n_rho: int = 200, n_t: int = 200) -> dict:
     return {"workload": f"cooling-table points {n_rho}x{n_t} (rho 1e8-1e13 g/cm^3, T 5e9-2e11 K, HdFac 0.5), "
                         "chemical equilibrium + neutrino trapping + rates per point, FP64",
             "points": n, "seconds": round(el, 3), "points_per_s": n / el,
-            "finite": bool(np.isfinite(vals).all()), "_rho": rho, "_T": T, "_vals": vals}
+            "finite": bool(np.isfinite(vals).all()), "_rho": rho, "_T": T, "_vals": vals,
+            "_axes": (ax_rho, ax_T)}
+
+
+def cache_lookups(hc, torch, tab: dict, n: int = 1 << 22) -> dict:
+    """getvalues() on the device over the generator's table held as a cache
+    (one z plane, 6 values per point): n random (rho, T) lookups, the spline
+    for e with de/dT (the call of the reference's T solve) and for all six."""
+    import numpy as np
+
+    ax_rho, ax_T = tab["_axes"]
+    desc = hc.cache_desc(ax_rho.size, ax_rho[0], ax_rho[-1], ax_T.size, ax_T[0], ax_T[-1], 1, 0.5, 0.5, d=6,
+                         ztype=hc.AXIS_CUSTOM)
+    values = tab["_vals"].T.reshape(1, ax_rho.size, ax_T.size, 6)
+    cache = hc.Cache.create(desc, ax_rho, ax_T, np.array([0.5]), values, np.ones(values.shape[:3], np.uint8))
+    rng = np.random.default_rng(5)
+    x = torch.as_tensor(10 ** rng.uniform(8, 13, n), device="cuda:0")
+    y = torch.as_tensor(10 ** rng.uniform(np.log10(5e9), np.log10(2e11), n), device="cuda:0")
+    zi = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    res = {"workload": f"{n} random (rho, T) lookups in the {ax_rho.size}x{ax_T.size} cooling table (Akima "
+                       "spline over 4x4 grid points, FP64)"}
+    for name, mask, der in (("e_and_dedT", [1, 0, 0, 0, 0, 0], True), ("all6", None, False)):
+        cache.lookup(x, y, None, zi, hc.INTERP_SPLINE, mask, der)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, st = cache.lookup(x, y, None, zi, hc.INTERP_SPLINE, mask, der)
+        el = time.perf_counter() - t0
+        res[name] = {"ms": round(el * 1e3, 3), "lookups_per_s": n / el,
+                     "ok": int((st == hc.CACHE_OK).sum().item())}
+    cache.close()
+    return res
 
 
 def eos_table_cpu(tab: dict, sample: int = 24) -> dict:
@@ -403,6 +436,7 @@ def main() -> None:
     eos_tab = None
     if not args.no_eos_table:
         eos_tab = eos_table(hc, torch)
+        eos_tab["cache_lookup"] = cache_lookups(hc, torch, eos_tab)
 
     st = None
     if not args.no_learned and args.stress_n > 0:
@@ -421,7 +455,7 @@ def main() -> None:
     if eos_tab is not None:
         if not args.no_cpu:
             eos_tab["cpu_reference"] = eos_table_cpu(eos_tab)
-        for k in ("_rho", "_T", "_vals"):
+        for k in ("_rho", "_T", "_vals", "_axes"):
             eos_tab.pop(k)
 
     out = {

@@ -185,6 +185,138 @@ class DeviceEos:
                              T=self.t["T"].data_ptr(), e=self.t["e"].data_ptr(), p=self.t["p"].data_ptr(), gam=gam)
 
 
+class CacheDesc(ctypes.Structure):
+    """Binary image of ``harm_cache_desc``: the cooling-table cache header, the
+    reference's packed 229-byte record (cache4d.h:17-53)."""
+
+    _pack_ = 1
+    _fields_ = [
+        ("mode", ctypes.c_int32), ("ver", ctypes.c_char * 13),
+        ("x", ctypes.c_int32), ("xmin", c_double), ("xmax", c_double), ("xrat", c_double), ("xtype", ctypes.c_int32),
+        ("y", ctypes.c_int32), ("ymin", c_double), ("ymax", c_double), ("yrat", c_double), ("ytype", ctypes.c_int32),
+        ("z", ctypes.c_int32), ("zmin", c_double), ("zmax", c_double), ("zrat", c_double), ("ztype", ctypes.c_int32),
+        ("d", ctypes.c_int32), ("unused", ctypes.c_int32), ("autosave", ctypes.c_int32),
+        ("wholecachecalculated", ctypes.c_int32),
+        ("xcust", ctypes.c_uint64), ("ycust", ctypes.c_uint64), ("zcust", ctypes.c_uint64),
+        ("reserved", ctypes.c_char * 76),
+    ]
+
+
+AXIS_LOG, AXIS_LINEAR, AXIS_CUSTOM, AXIS_NONUNIFORM, AXIS_ROLLING = range(5)
+INTERP_LIN, INTERP_SPLINE = 0, 1
+CACHE_OUT_OF_RANGE, CACHE_OK, CACHE_NEEDS_RECENTRE, CACHE_MISSING, CACHE_BAD_ZINDEX = range(5)
+
+
+def cache_desc(x: int, xmin: float, xmax: float, y: int, ymin: float, ymax: float, z: int = 1, zmin: float = 1.0,
+               zmax: float = 1.0, d: int = 5, xtype: int = AXIS_LOG, ytype: int = AXIS_LOG, ztype: int = AXIS_LINEAR,
+               xrat: float = 0.0, yrat: float = 0.0, zrat: float = 0.0, whole: int = 0) -> CacheDesc:
+    """A constant-grid cache header of the reference's version 2.0."""
+    return CacheDesc(mode=0, ver=b"AGACACHE_2.0", x=x, xmin=xmin, xmax=xmax, xrat=xrat, xtype=xtype, y=y, ymin=ymin,
+                     ymax=ymax, yrat=yrat, ytype=ytype, z=z, zmin=zmin, zmax=zmax, zrat=zrat, ztype=ztype, d=d,
+                     wholecachecalculated=whole)
+
+
+def cache_axis(kind: int, n: int, vmin: float, vmax: float, rat: float = 0.0, cust=None) -> np.ndarray:
+    """Axis values of a cache axis type (setindices, cache4d.c:329-388)."""
+    out = np.empty(n)
+    c = None if cust is None else _ptr(np.ascontiguousarray(cust, dtype=np.float64))
+    _check(lib().harm_cache_axis(kind, n, vmin, vmax, rat, c, _ptr(out)), "harm_cache_axis")
+    return out
+
+
+def cache_axes(desc: CacheDesc):
+    return (cache_axis(desc.xtype, desc.x, desc.xmin, desc.xmax, desc.xrat),
+            cache_axis(desc.ytype, desc.y, desc.ymin, desc.ymax, desc.yrat),
+            cache_axis(desc.ztype, desc.z, desc.zmin, desc.zmax, desc.zrat))
+
+
+def cache_peek(path: str) -> CacheDesc:
+    d = CacheDesc()
+    _check(lib().harm_cache_peek(os.fsencode(path), ctypes.byref(d)), "harm_cache_peek")
+    return d
+
+
+def cache_read(path: str) -> dict:
+    """Header, axes, values [z, x, y, d] and computed flags [z, x, y] of a cache file."""
+    d = cache_peek(path)
+    vx, vy, vz = np.empty(d.x), np.empty(d.y), np.empty(d.z)
+    values = np.empty((d.z, d.x, d.y, d.d))
+    counted = np.empty((d.z, d.x, d.y), dtype=np.uint8)
+    _check(lib().harm_cache_read(os.fsencode(path), ctypes.byref(d), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(values),
+                                 counted.ctypes.data), "harm_cache_read")
+    return {"desc": d, "vx": vx, "vy": vy, "vz": vz, "values": values, "counted": counted}
+
+
+def _cache_arrays(vx, vy, vz, values, counted):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz, values)]
+    return a + [np.ascontiguousarray(counted, dtype=np.uint8)]
+
+
+def cache_write(path: str, desc: CacheDesc, vx, vy, vz, values, counted) -> None:
+    vx, vy, vz, values, counted = _cache_arrays(vx, vy, vz, values, counted)
+    _check(lib().harm_cache_write(os.fsencode(path), ctypes.byref(desc), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(values),
+                                  counted.ctypes.data), "harm_cache_write")
+
+
+class Cache:
+    """A cooling-table cache on the current device (harm_cache_*)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise HarmError(f"harm_cache: {lib().harm_last_error().decode(errors='replace')}")
+        self.h = handle
+        self.desc = CacheDesc()
+        w = c_int()
+        _check(lib().harm_cache_info(self.h, ctypes.byref(self.desc), ctypes.byref(w)), "harm_cache_info")
+        self.whole = bool(w.value)
+
+    @classmethod
+    def create(cls, desc: CacheDesc, vx, vy, vz, values, counted) -> "Cache":
+        vx, vy, vz, values, counted = _cache_arrays(vx, vy, vz, values, counted)
+        return cls(lib().harm_cache_create(ctypes.byref(desc), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(values),
+                                           counted.ctypes.data))
+
+    @classmethod
+    def open(cls, path: str) -> "Cache":
+        return cls(lib().harm_cache_open(os.fsencode(path)))
+
+    def lookup(self, x, y, z=None, zindex=None, mode: int = INTERP_SPLINE, dimmask=None, dery: bool = False):
+        """getvalues() for n points (device tensors) -> out [d, n], dery [d, n]
+        or None, status [n]; entries not written hold NaN."""
+        import torch
+
+        n = x.shape[0]
+        _need(x, (n,), "f64", "x")
+        _need(y, (n,), "f64", "y")
+        if z is not None:
+            _need(z, (n,), "f64", "z")
+        if zindex is not None:
+            _need(zindex, (n,), "i32", "zindex")
+        dv = self.desc.d
+        out = torch.full((dv, n), float("nan"), dtype=torch.float64, device=x.device)
+        der = torch.full((dv, n), float("nan"), dtype=torch.float64, device=x.device) if dery else None
+        st = torch.full((n,), -1, dtype=torch.int32, device=x.device)
+        mask = None if dimmask is None else (c_int * dv)(*[int(m) for m in dimmask])
+        _sync_torch()
+        _check(lib().harm_cache_lookup(self.h, mode, mask, x.data_ptr(), y.data_ptr(),
+                                       None if z is None else z.data_ptr(),
+                                       None if zindex is None else zindex.data_ptr(), out.data_ptr(),
+                                       None if der is None else der.data_ptr(), st.data_ptr(), n, None),
+               "harm_cache_lookup")
+        return out, der, st
+
+    def close(self) -> None:
+        if self.h:
+            lib().harm_cache_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def eos_eval(tab: DeviceEos, rho, u):
     """p, dp/drho, dp/du on device tensors (harm_eos_eval)."""
     import torch
@@ -337,6 +469,19 @@ def lib() -> ctypes.CDLL:
     L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
     L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
     L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]
+    L.harm_cache_axis.argtypes = [c_int, c_int, c_double, c_double, c_double, _dp, _dp]
+    L.harm_cache_peek.argtypes = [c_char_p, POINTER(CacheDesc)]
+    L.harm_cache_read.argtypes = [c_char_p, POINTER(CacheDesc), _dp, _dp, _dp, _dp, c_void_p]
+    L.harm_cache_write.argtypes = [c_char_p, POINTER(CacheDesc), _dp, _dp, _dp, _dp, c_void_p]
+    L.harm_cache_create.argtypes = [POINTER(CacheDesc), _dp, _dp, _dp, _dp, c_void_p]
+    L.harm_cache_create.restype = c_void_p
+    L.harm_cache_open.argtypes = [c_char_p]
+    L.harm_cache_open.restype = c_void_p
+    L.harm_cache_info.argtypes = [c_void_p, POINTER(CacheDesc), POINTER(c_int)]
+    L.harm_cache_free.argtypes = [c_void_p]
+    L.harm_cache_free.restype = None
+    L.harm_cache_lookup.argtypes = [c_void_p, c_int, POINTER(c_int), c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
     _lib = L
     return L
 
@@ -665,7 +810,7 @@ def _need(t, shape, dtype_name: str, what: str) -> None:
     """The C ABI reads device tensors as dense row-major arrays: reject views."""
     import torch
 
-    dt = {"f64": torch.float64, "f32": torch.float32, "u8": torch.uint8}[dtype_name]
+    dt = {"f64": torch.float64, "f32": torch.float32, "u8": torch.uint8, "i32": torch.int32}[dtype_name]
     if t.dtype != dt or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != tuple(shape):
         raise ValueError(f"{what}: expected a contiguous {dtype_name} device tensor of shape {tuple(shape)}, "
                          f"got {tuple(t.shape)} {t.dtype} contiguous={t.is_contiguous()}")

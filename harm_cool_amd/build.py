@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libharmcool.so")
-SOURCES = ["harm_cool.hip", "harm_geometry.cpp", "harm_mlp.hip", "harm_eos.cpp", "harm_restart.cpp", "harm_fermi.hip"]
+SOURCES = ["harm_cool.hip", "harm_geometry.cpp", "harm_mlp.hip", "harm_eos.cpp", "harm_restart.cpp", "harm_fermi.hip", "harm_cache.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HARM_OFFLOAD_ARCH", "gfx950")
 

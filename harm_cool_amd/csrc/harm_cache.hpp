@@ -1,0 +1,239 @@
+// harm_cache.hpp -- lookups in the cooling-table cache: a grid of D values at
+// every point of a 3D (x, y, z) = (rho, T, H_d factor) grid, stored
+// [z][x][y][D] (cache4d.c:152-260). One lookup is cache4d.c's getvalues() in
+// CONSTGRID mode (cache4d.c:863-1134): a window of grid points around the
+// query along each axis, interpolated along y, then x, then z, either
+// piecewise-linearly (akima.c:300-370) or with Akima's local spline (H. Akima,
+// J. ACM 17(4), 589-602, 1970) fitted to the window alone, with the
+// two-point extrapolation at each end and the slope rule of akima.c:9-160.
+//
+// Every operation is +, -, * or / in the reference's order, so with
+// contraction off the host and the device give the reference's bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace harm {
+
+#define HC_HD __host__ __device__ inline
+
+enum CacheStatus : int {
+    CACHE_OUT_OF_RANGE = 0,  // getvalues() returns 0 (cache4d.c:896-929)
+    CACHE_OK = 1,
+    CACHE_NEEDS_RECENTRE = 2,  // outside a rolling axis' inner range: the reference moves the grid
+    CACHE_MISSING = 3,         // the window holds points not yet computed
+    CACHE_BAD_ZINDEX = 4,      // z index outside [0, Z)
+};
+
+enum { CACHE_LIN = 0, CACHE_SPLINE = 1 };
+enum { AXIS_LOG = 0, AXIS_LINEAR, AXIS_CUSTOM, AXIS_NONUNIFORM, AXIS_ROLLING };
+
+// 1-based index `left` of the bracket [x[left-1], x[left]] of xval in the
+// increasing x[0..n): the first i in 2..n-1 with xval < x[i-1] gives i-1,
+// none gives n-1 (akima.c:234-298).
+HC_HD int cache_bracket(int n, const double *x, double xval)
+{
+    for (int i = 2; i <= n - 1; i++)
+        if (xval < x[i - 1]) return i - 1;
+    return n - 1;
+}
+
+// Window [lo, hi] of grid indices a lookup reads along one axis
+// (cache4d.c:408-440). The linear rule's window is [left-1, left+1]; its third
+// point is never used except at xval == x[n-1], where the reference reads one
+// point past the axis: the window stops at n-1 here and that point takes the
+// last interval.
+HC_HD void cache_window(int n, const double *x, double xval, int mode, int &lo, int &hi)
+{
+    const int left = cache_bracket(n, x, xval);
+    if (mode == CACHE_LIN) {
+        lo = left - 1;
+        hi = left + 1 < n - 1 ? left + 1 : n - 1;
+    } else {
+        lo = left - 2 < 0 ? 0 : left - 2;
+        hi = left + 1 >= n ? n - 1 : left + 1;
+    }
+}
+
+// Piecewise-linear value at t from m points (akima.c:300-370).
+HC_HD double cache_lin(int m, const double *xs, const double *ys, double t)
+{
+    const int left = cache_bracket(m, xs, t);  // 1-based
+    const double T1 = 1.0 / (xs[left] - xs[left - 1]);
+    const double T2 = t - xs[left - 1];
+    const double yp = (ys[left] - ys[left - 1]) * T1;
+    return ys[left - 1] + T2 * yp;
+}
+
+// Akima's spline through m (3 or 4) points, evaluated at nt increasing-search
+// points ts (akima.c:9-160). Returns false when two abscissae are closer than
+// 1e-12 (the reference then leaves the output unwritten).
+HC_HD bool cache_akima(int m, const double *xs, const double *ys, int nt, const double *ts, double *out)
+{
+    // secants, with two extrapolated on each side: S = g0, g1, m_0 .. m_{m-2}, m_{m-1}, m_m
+    double S[7];
+    for (int k = 0; k < m - 1; k++) {
+        const double dx = xs[k + 1] - xs[k];
+        if (dx < 1e-12) return false;
+        S[k + 2] = (ys[k + 1] - ys[k]) / dx;
+    }
+    {
+        const double x0 = xs[0], x1 = xs[1], y0 = ys[0];
+        const double xl1 = x0 + x1 - xs[2];
+        const double xl0 = xl1 + x0 - x1;
+        const double yl1 = (x0 - xl1) * (S[3] - 2.0 * S[2]) + y0;
+        const double g1 = (y0 - yl1) / (x0 - xl1);
+        const double yl0 = (xl1 - xl0) * (S[2] - 2.0 * g1) + yl1;
+        S[0] = (yl1 - yl0) / (xl1 - xl0);
+        S[1] = g1;
+    }
+    {
+        const double xa = xs[m - 2], xb = xs[m - 1], yb = ys[m - 1];
+        const double xr2 = xb + xa - xs[m - 3];
+        const double xr3 = xr2 + xb - xa;
+        const double yr2 = (2.0 * S[m] - S[m - 1]) * (xr2 - xb) + yb;
+        S[m + 1] = (yr2 - yb) / (xr2 - xb);
+        const double yr3 = (2.0 * S[m + 1] - S[m]) * (xr3 - xr2) + yr2;
+        S[m + 2] = (yr3 - yr2) / (xr3 - xr2);
+    }
+    // node slopes from the four secants around each node
+    double sl[4];
+    for (int k = 0; k < m; k++) {
+        const double a = S[k], b = S[k + 1], c = S[k + 2], d = S[k + 3];
+        double d0 = d - c, d1 = b - a;
+        if (d0 < 0.0) d0 *= -1.0;
+        if (d1 < 0.0) d1 *= -1.0;
+        sl[k] = (d0 + d1) < 1e-9 ? 0.5 * (b + c) : (d0 * b + d1 * c) / (d0 + d1);
+    }
+    // the interval search continues from one output point to the next
+    int i = -1;
+    for (int q = 0; q < nt; q++) {
+        const double t = ts[q];
+        while (t > xs[i + 1] && i < m - 2) i++;
+        if (i < 0) i = 0;
+        const double d0 = xs[i + 1] - xs[i];
+        const double t2 = (ys[i + 1] - ys[i]) / d0;
+        const double g0 = sl[i], g1 = sl[i + 1];
+        const double p1 = (3.0 * t2 - 2.0 * g0 - g1) / d0;
+        const double p2 = (g0 + g1 - 2.0 * t2) / (d0 * d0);
+        const double t1 = t - xs[i];
+        out[q] = ys[i] + g0 * t1 + p1 * t1 * t1 + p2 * t1 * t1 * t1;
+    }
+    return true;
+}
+
+// Value and centred difference (step t * 1e-6) of the spline at t
+// (akima.c:196-231).
+HC_HD bool cache_akima_der(int m, const double *xs, const double *ys, double t, double &y, double &dy)
+{
+    const double h = t * 1.0E-6;
+    const double ts[3] = {t - h, t, t + h};
+    double v[3];
+    if (!cache_akima(m, xs, ys, 3, ts, v)) return false;
+    y = v[1];
+    dy = (v[2] - v[0]) / (2.0 * h);
+    return true;
+}
+
+struct CacheView {
+    int X, Y, Z, D;
+    int xtype, ytype, ztype;
+    int whole;                    // every point computed: the window check is skipped
+    const double *vx, *vy, *vz;   // axes
+    const double *values;         // [Z][X][Y][D]
+    const uint8_t *counted;       // [Z][X][Y], 1 = computed
+};
+
+HC_HD int cache_axis_check(int type, int n, const double *v, double t, bool checked)
+{
+    if (type == AXIS_ROLLING) return (t < v[1] || t > v[n - 2]) ? CACHE_NEEDS_RECENTRE : CACHE_OK;
+    if (checked && (t < v[0] || t > v[n - 1])) return CACHE_OUT_OF_RANGE;
+    return CACHE_OK;
+}
+
+// One getvalues() call (cache4d.c:863-1134, CONSTGRID). zindex >= 0 selects a
+// z plane (no z interpolation), -1 interpolates in z. out[k * ostride] and,
+// with dery, dery[k * ostride] receive the masked components (the reference's
+// dimmask; the linear rule fills every component and no derivative). The
+// derivative along y is interpolated in x only and comes from the last z plane
+// of the window, as the reference's does.
+HC_HD int cache_lookup(const CacheView &c, double x, double y, double z, int zindex, int mode, uint64_t mask,
+                       double *out, double *dery, size_t ostride)
+{
+    int rolling = 0;
+    int s = cache_axis_check(c.xtype, c.X, c.vx, x, true);
+    if (s == CACHE_OUT_OF_RANGE) return s;
+    rolling |= s == CACHE_NEEDS_RECENTRE;
+    s = cache_axis_check(c.ytype, c.Y, c.vy, y, true);
+    if (s == CACHE_OUT_OF_RANGE) return s;
+    rolling |= s == CACHE_NEEDS_RECENTRE;
+    s = cache_axis_check(c.ztype, c.Z, c.vz, z, zindex == -1);
+    if (s == CACHE_OUT_OF_RANGE) return s;
+    rolling |= s == CACHE_NEEDS_RECENTRE;
+    if (rolling) return CACHE_NEEDS_RECENTRE;
+    if (zindex < -1 || zindex >= c.Z) return CACHE_BAD_ZINDEX;
+    // a window needs 3 points for the spline and 2 for the linear rule (x and y
+    // are checked when the cache is built)
+    if (zindex == -1 && c.Z < (mode == CACHE_LIN ? 2 : 3)) return CACHE_BAD_ZINDEX;
+
+    int x0, x1, y0, y1, z0, z1;
+    cache_window(c.X, c.vx, x, mode, x0, x1);
+    cache_window(c.Y, c.vy, y, mode, y0, y1);
+    if (zindex != -1)
+        z0 = z1 = zindex;
+    else
+        cache_window(c.Z, c.vz, z, mode, z0, z1);
+    const long XY = (long)c.X * c.Y;
+    if (!c.whole) {
+        for (int iz = z0; iz <= z1; iz++)
+            for (int ix = x0; ix <= x1; ix++)
+                for (int iy = y0; iy <= y1; iy++) {
+                    const uint8_t f = c.counted[iz * XY + (long)ix * c.Y + iy];
+                    if (f == 0) return CACHE_MISSING;
+                    if (f != 1) return CACHE_OUT_OF_RANGE;  // a broken entry (cache4d.c:1018-1021)
+                }
+    }
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+    const double *xs = c.vx + x0, *ys = c.vy + y0, *zs = c.vz + z0;
+    for (int k = 0; k < c.D; k++) {
+        const bool lin = mode == CACHE_LIN;
+        if (!lin && !((mask >> k) & 1)) continue;
+        double zt[4], dlast = 0.0;
+        bool ok = true;
+        for (int a = 0; a < nz; a++) {
+            double xt[4], xd[4];
+            for (int b = 0; b < nx; b++) {
+                const double *row = c.values + ((size_t)(z0 + a) * XY + (size_t)(x0 + b) * c.Y + y0) * c.D + k;
+                double yv[4];
+                for (int q = 0; q < ny; q++) yv[q] = row[(size_t)q * c.D];
+                if (lin)
+                    xt[b] = cache_lin(ny, ys, yv, y);
+                else if (dery)
+                    ok &= cache_akima_der(ny, ys, yv, y, xt[b], xd[b]);
+                else
+                    ok &= cache_akima(ny, ys, yv, 1, &y, &xt[b]);
+            }
+            if (lin) {
+                zt[a] = cache_lin(nx, xs, xt, x);
+            } else {
+                ok &= cache_akima(nx, xs, xt, 1, &x, &zt[a]);
+                if (dery) ok &= cache_akima(nx, xs, xd, 1, &x, &dlast);
+            }
+        }
+        if (!ok) return CACHE_OUT_OF_RANGE;
+        double v = zt[0];
+        if (zindex == -1) {
+            if (lin)
+                v = cache_lin(nz, zs, zt, z);
+            else if (!cache_akima(nz, zs, zt, 1, &z, &v))
+                return CACHE_OUT_OF_RANGE;
+        }
+        out[k * ostride] = v;
+        if (dery && !lin) dery[k * ostride] = dlast;
+    }
+    return CACHE_OK;
+}
+
+}  // namespace harm

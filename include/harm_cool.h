@@ -405,6 +405,67 @@ int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac
 enum { HARM_EOS_NPE = 0, HARM_EOS_RAD = 1, HARM_EOS_NEU = 2, HARM_EOS_HE = 3 };
 int harm_eos_species_batch(int kind, const double *in, double *out, size_t n, void *stream);
 
+/* Cooling-table cache (cache4d.h:17-53, cache4d.c): D values at every point of
+ * an (x, y, z) = (rho, T, HdFac) grid.  The header is the reference's packed
+ * record, byte for byte; the file holds it, the three axes, the values
+ * [z][x][y][d] and one computed flag per grid point (cache4d.c:263-316). */
+#pragma pack(push, 1)
+typedef struct harm_cache_desc {
+    int32_t mode; /* 0: constant grid (the only kind looked up by grid), 1: prepared points */
+    char ver[13]; /* "AGACACHE_2.0" ("AGACACHE_1.0" files store int flags) */
+    int32_t x;
+    double xmin, xmax, xrat;
+    int32_t xtype; /* HARM_AXIS_* */
+    int32_t y;
+    double ymin, ymax, yrat;
+    int32_t ytype;
+    int32_t z;
+    double zmin, zmax, zrat;
+    int32_t ztype;
+    int32_t d; /* values per grid point */
+    int32_t unused, autosave, wholecachecalculated;
+    uint64_t xcust, ycust, zcust; /* the writer's pointers to custom axes, kept as stored, never read */
+    char reserved[76];
+} harm_cache_desc;
+#pragma pack(pop)
+enum { HARM_AXIS_LOG = 0, HARM_AXIS_LINEAR = 1, HARM_AXIS_CUSTOM = 2, HARM_AXIS_NONUNIFORM = 3, HARM_AXIS_ROLLING = 4 };
+enum { HARM_INTERP_LIN = 0, HARM_INTERP_SPLINE = 1 };
+/* Lookup status per point: 1 done; 0 outside the grid (getvalues() returns 0);
+ * 2 outside a rolling axis' inner range (the reference re-centres the grid and
+ * recomputes it); 3 the window holds points not yet computed (the reference
+ * computes them first); 4 z index outside [0, z). */
+enum { HARM_CACHE_OUT_OF_RANGE = 0, HARM_CACHE_OK = 1, HARM_CACHE_NEEDS_RECENTRE = 2, HARM_CACHE_MISSING = 3,
+       HARM_CACHE_BAD_ZINDEX = 4 };
+typedef struct harm_cache harm_cache;
+/* Axis values for an axis type (setindices, cache4d.c:329-388); host call. */
+int harm_cache_axis(int type, int n, double vmin, double vmax, double rat, const double *cust, double *out);
+/* Header of a cache file; then harm_cache_read fills host arrays sized from it
+ * (vx[x], vy[y], vz[z], values[z*x*y*d], counted[z*x*y]) (load, cache4d.c:152-260). */
+int harm_cache_peek(const char *path, harm_cache_desc *desc);
+int harm_cache_read(const char *path, harm_cache_desc *desc, double *vx, double *vy, double *vz, double *values,
+                    uint8_t *counted);
+/* save(), cache4d.c:263-316: the same layout from host arrays. */
+int harm_cache_write(const char *path, const harm_cache_desc *desc, const double *vx, const double *vy,
+                     const double *vz, const double *values, const uint8_t *counted);
+/* A device copy of a constant-grid cache on the current device (x, y >= 3
+ * points, d <= 64, axes increasing by >= 1e-12), from host arrays or a file;
+ * NULL on error.  It counts as whole when its header says so or every flag is 1. */
+harm_cache *harm_cache_create(const harm_cache_desc *desc, const double *vx, const double *vy, const double *vz,
+                              const double *values, const uint8_t *counted);
+harm_cache *harm_cache_open(const char *path);
+int harm_cache_info(const harm_cache *c, harm_cache_desc *desc, int *whole);
+void harm_cache_free(harm_cache *c);
+/* n lookups, one getvalues() each (cache4d.c:863-1134): x, y (device, n); z
+ * (device, n; may be NULL when zindex is given) and zindex (device, n, may be
+ * NULL: every point interpolates in z) where zindex[i] >= 0 takes that z plane
+ * and -1 interpolates in z; out[d][n]; dery[d][n] (may be NULL) the derivative
+ * along y (spline only); status[n] (may be NULL) HARM_CACHE_*.  dimmask (host,
+ * d ints, may be NULL = all) selects the components the spline computes; the
+ * others, like every output of a point whose status is not 1, are not written. */
+int harm_cache_lookup(const harm_cache *c, int mode, const int *dimmask, const double *x, const double *y,
+                      const double *z, const int *zindex, double *out, double *dery, int *status, size_t n,
+                      void *stream);
+
 /* dump() ASCII file (dump.c:1710-1990; the HDFDUMP=0 build): a header line
  * (t, N1, N2 and the theta origin/spacing in 2D or N3 and the phi ones in 3D,
  * then tf, nstep, a, gam, cour, DTd, DTl, DTi, DTr, dump/image/rdump counters,

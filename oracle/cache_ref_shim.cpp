@@ -1,0 +1,87 @@
+/*
+ * cache_ref_shim.cpp -- TEST INFRASTRUCTURE.  extern "C" entry points into the
+ * reference's own cooling-table cache (src/cache4d.c, src/akima.c, compiled
+ * where they lie in the reference tree into oracle/_ref/libcacheref.so).
+ *
+ * Used only to pin oracle/cache_oracle.c and the cache file layout: a Cache4d
+ * opened on a complete file (every point computed) interpolates without
+ * starting its worker processes; one built on a file name that does not exist
+ * only lays out its axes.  Its destructor saves nothing unless a point was
+ * computed, which never happens here.
+ */
+#include <cstring>
+#include <ctime>  // cache4d.h uses time_t without including it
+
+#include "akima.h"
+#include "cache4d.h"
+
+static int no_calc(double, double, double, double *) { return 1; }
+
+extern "C" {
+
+int cref_desc_size(void) { return (int)sizeof(CacheDesc); }
+
+/* A Cache4d on an existing complete cache file (custom axes, if any, given). */
+void *cref_open(const char *path, double *xcust, double *ycust, double *zcust)
+{
+    CacheDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.xcust = xcust;
+    d.ycust = ycust;
+    d.zcust = zcust;
+    return new Cache4d(&d, no_calc, path);
+}
+
+/* The axes a new cache lays out for a header (path must not exist). */
+int cref_axes(const void *desc, const char *path, double *vx, double *vy, double *vz)
+{
+    CacheDesc d;
+    std::memcpy(&d, desc, sizeof d);
+    Cache4d *c = new Cache4d(&d, no_calc, path);
+    std::memcpy(vx, c->VectorX, sizeof(double) * c->X);
+    std::memcpy(vy, c->VectorY, sizeof(double) * c->Y);
+    std::memcpy(vz, c->VectorZ, sizeof(double) * c->Z);
+    delete c;
+    return 0;
+}
+
+int cref_get(void *c, double x, double y, double z, int spline, double *ret, double *der, int zindic, int *dimmask)
+{
+    return ((Cache4d *)c)->getvalues(x, y, z, spline ? SPLINE : LIN, ret, der, zindic, dimmask);
+}
+
+/* What load() read: the header, axes, values and flags (arrays sized from it). */
+int cref_arrays(void *cv, void *desc, double *vx, double *vy, double *vz, double *values, unsigned char *counted)
+{
+    Cache4d *c = (Cache4d *)cv;
+    const size_t np = (size_t)c->X * c->Y * c->Z;
+    std::memcpy(desc, &c->cdesc, sizeof(CacheDesc));
+    if (vx) {
+        std::memcpy(vx, c->VectorX, sizeof(double) * c->X);
+        std::memcpy(vy, c->VectorY, sizeof(double) * c->Y);
+        std::memcpy(vz, c->VectorZ, sizeof(double) * c->Z);
+        std::memcpy(values, c->MValues, sizeof(double) * np * c->D);
+        std::memcpy(counted, c->MCounted, np);
+    }
+    return 0;
+}
+
+void cref_close(void *c) { delete (Cache4d *)c; }
+
+void cref_spline_akima(int ndim, int *dimmask, int ndata, double *t, double *y, double tval, double *out)
+{
+    spline_akima(ndim, dimmask, ndata, t, y, tval, out);
+}
+
+void cref_spline_akima_der(int ndim, int *dimmask, int ndata, double *t, double *y, double tval, double *out,
+                           double *der)
+{
+    spline_akima_der(ndim, dimmask, ndata, t, y, tval, out, der);
+}
+
+void cref_linear(int ndim, int ndata, double *t, double *y, double tval, double *out)
+{
+    spline_linear_val2(ndim, ndata, t, y, tval, out);
+}
+
+}  // extern "C"

@@ -5,7 +5,7 @@ from __future__ import annotations
 import ctypes
 import os
 import subprocess
-from ctypes import POINTER, c_double, c_int, c_int8, c_int64, c_uint32, c_void_p
+from ctypes import POINTER, c_double, c_int, c_int8, c_int64, c_size_t, c_uint32, c_void_p
 
 import numpy as np
 
@@ -360,3 +360,127 @@ class EosRef:
 
 def eosref_available() -> bool:
     return os.path.exists(EOSREF_SO)
+
+
+# ---------------------------------------------------------------------------
+# Cooling-table cache: the oracle's restatement (oracle/cache_oracle.c) and the
+# reference's own cache4d.c/akima.c (oracle/_ref/libcacheref.so).
+
+CACHEREF_SO = os.path.join(ORACLE_DIR, "_ref", "libcacheref.so")
+
+
+def cacheref_available() -> bool:
+    return os.path.exists(CACHEREF_SO)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def cache_lib():
+    L = lib()
+    L.orc_akima.argtypes = [c_int, _dp, _dp, c_int, _dp, _dp]
+    L.orc_akima_der.argtypes = [c_int, _dp, _dp, c_double, _dp, _dp]
+    L.orc_lin.argtypes = [c_int, _dp, _dp, c_double]
+    L.orc_lin.restype = c_double
+    L.orc_cache_get.argtypes = [c_int, c_int, c_int, c_int, POINTER(c_int), c_int, _dp, _dp, _dp, _dp, c_void_p,
+                                c_double, c_double, c_double, c_int, c_int, POINTER(c_int), _dp, _dp, c_size_t]
+    return L
+
+
+class OracleCache:
+    """The oracle's getvalues() over host arrays (values [z, x, y, d])."""
+
+    def __init__(self, desc, vx, vy, vz, values, counted):
+        self.d = desc
+        self.vx, self.vy, self.vz, self.values = _f64(vx), _f64(vy), _f64(vz), _f64(values)
+        self.counted = np.ascontiguousarray(counted, dtype=np.uint8)
+        self.whole = int(desc.wholecachecalculated or bool(np.all(self.counted == 1)))
+        self.types = (c_int * 3)(desc.xtype, desc.ytype, desc.ztype)
+        self.L = cache_lib()
+
+    def get(self, x, y, z=0.0, zindex=-1, spline=True, dimmask=None, der=False):
+        D = self.d.d
+        out = np.full(D, np.nan)
+        dv = np.full(D, np.nan) if der else None
+        mask = None if dimmask is None else (c_int * D)(*[int(m) for m in dimmask])
+        p = lambda a: a.ctypes.data_as(_dp)
+        s = self.L.orc_cache_get(self.d.x, self.d.y, self.d.z, D, self.types, self.whole, p(self.vx), p(self.vy),
+                                 p(self.vz), p(self.values), self.counted.ctypes.data, float(x), float(y), float(z),
+                                 int(zindex), int(spline), mask, p(out), None if dv is None else p(dv), 1)
+        return s, out, dv
+
+
+class CacheRef:
+    """The reference's Cache4d and akima.c routines through cache_ref_shim.cpp."""
+
+    def __init__(self):
+        L = ctypes.CDLL(CACHEREF_SO)
+        L.cref_open.restype = c_void_p
+        L.cref_open.argtypes = [ctypes.c_char_p, _dp, _dp, _dp]
+        L.cref_axes.argtypes = [c_void_p, ctypes.c_char_p, _dp, _dp, _dp]
+        L.cref_get.argtypes = [c_void_p, c_double, c_double, c_double, c_int, _dp, _dp, c_int, POINTER(c_int)]
+        L.cref_arrays.argtypes = [c_void_p, c_void_p, _dp, _dp, _dp, _dp, c_void_p]
+        L.cref_close.argtypes = [c_void_p]
+        L.cref_spline_akima.argtypes = [c_int, POINTER(c_int), c_int, _dp, _dp, c_double, _dp]
+        L.cref_spline_akima_der.argtypes = [c_int, POINTER(c_int), c_int, _dp, _dp, c_double, _dp, _dp]
+        L.cref_linear.argtypes = [c_int, c_int, _dp, _dp, c_double, _dp]
+        self.L = L
+
+    def desc_size(self) -> int:
+        return self.L.cref_desc_size()
+
+    def open(self, path: str):
+        h = self.L.cref_open(os.fsencode(path), None, None, None)
+        assert h
+        return h
+
+    def close(self, h) -> None:
+        self.L.cref_close(h)
+
+    def axes(self, desc, missing_path: str):
+        vx, vy, vz = np.empty(desc.x), np.empty(desc.y), np.empty(desc.z)
+        p = lambda a: a.ctypes.data_as(_dp)
+        self.L.cref_axes(ctypes.addressof(desc), os.fsencode(missing_path), p(vx), p(vy), p(vz))
+        return vx, vy, vz
+
+    def arrays(self, h, desc_type):
+        d = desc_type()
+        self.L.cref_arrays(h, ctypes.addressof(d), None, None, None, None, None)
+        vx, vy, vz = np.empty(d.x), np.empty(d.y), np.empty(d.z)
+        values = np.empty((d.z, d.x, d.y, d.d))
+        counted = np.empty((d.z, d.x, d.y), dtype=np.uint8)
+        p = lambda a: a.ctypes.data_as(_dp)
+        self.L.cref_arrays(h, ctypes.addressof(d), p(vx), p(vy), p(vz), p(values), counted.ctypes.data)
+        return {"desc": d, "vx": vx, "vy": vy, "vz": vz, "values": values, "counted": counted}
+
+    def get(self, h, D, x, y, z=0.0, zindex=-1, spline=True, dimmask=None, der=False):
+        out = np.full(D, np.nan)
+        dv = np.full(D, np.nan) if der else None
+        mask = None if dimmask is None else (c_int * D)(*[int(m) for m in dimmask])
+        p = lambda a: a.ctypes.data_as(_dp)
+        rc = self.L.cref_get(h, float(x), float(y), float(z), int(spline), p(out), None if dv is None else p(dv),
+                             int(zindex), mask)
+        return rc, out, dv
+
+    def spline_akima(self, t, y, tval, ndim=1, dimmask=None):
+        out = np.full(ndim, np.nan)
+        t, y = _f64(t), _f64(y)
+        mask = None if dimmask is None else (c_int * ndim)(*dimmask)
+        self.L.cref_spline_akima(ndim, mask, t.size, t.ctypes.data_as(_dp), y.ctypes.data_as(_dp), float(tval),
+                                 out.ctypes.data_as(_dp))
+        return out
+
+    def spline_akima_der(self, t, y, tval, ndim=1):
+        out, der = np.full(ndim, np.nan), np.full(ndim, np.nan)
+        t, y = _f64(t), _f64(y)
+        self.L.cref_spline_akima_der(ndim, None, t.size, t.ctypes.data_as(_dp), y.ctypes.data_as(_dp), float(tval),
+                                     out.ctypes.data_as(_dp), der.ctypes.data_as(_dp))
+        return out, der
+
+    def linear(self, t, y, tval, ndim=1):
+        out = np.full(ndim, np.nan)
+        t, y = _f64(t), _f64(y)
+        self.L.cref_linear(ndim, t.size, t.ctypes.data_as(_dp), y.ctypes.data_as(_dp), float(tval),
+                           out.ctypes.data_as(_dp))
+        return out
