@@ -50,8 +50,9 @@ def query_points(rng, vx, vy, vz, n):
                     10 ** rng.uniform(np.log10(vy[0]), np.log10(vy[-1])), rng.uniform(vz[0], vz[-1])))
     for a in (0, 1, 2, len(vx) - 2, len(vx) - 1):
         pts.append((vx[a], vy[min(a, len(vy) - 1)], vz[min(a, len(vz) - 1)]))
-    pts.append((vx[0] * 0.999, vy[2], vz[1]))   # outside in x
-    pts.append((vx[3], vy[-1] * 1.001, vz[1]))  # outside in y
+    z1 = vz[min(1, len(vz) - 1)]
+    pts.append((vx[0] * 0.999, vy[2], z1))      # outside in x
+    pts.append((vx[3], vy[-1] * 1.001, z1))     # outside in y
     pts.append((vx[3], vy[2], vz[-1] + 0.1))    # outside in z (matters only when interpolating in z)
     return pts
 
