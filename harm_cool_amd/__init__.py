@@ -305,6 +305,29 @@ class Cache:
                "harm_cache_lookup")
         return out, der, st
 
+    def solve_T(self, rho, e, T, zindex=None, Tlo: float = 0.0, Thi: float = 0.0, ntrial: int = 300,
+                tolf: float = 1.0e-6):
+        """The COOL_EOS temperature solve (harm_cache_solve_T) for device
+        tensors; T (the start) is overwritten. Bounds default to the y axis
+        ends moved in by 1e-6, as the reference sets them. -> (berr, ret)."""
+        import torch
+
+        n = rho.shape[0]
+        _need(rho, (n,), "f64", "rho")
+        _need(e, (n,), "f64", "e")
+        _need(T, (n,), "f64", "T")
+        if zindex is not None:
+            _need(zindex, (n,), "i32", "zindex")
+        if Tlo == 0.0 and Thi == 0.0:
+            Tlo, Thi = self.desc.ymin * (1.0 + 1.0e-6), self.desc.ymax * (1.0 - 1.0e-6)
+        berr = torch.empty(n, dtype=torch.float64, device=rho.device)
+        ret = torch.empty(n, dtype=torch.int32, device=rho.device)
+        _sync_torch()
+        _check(lib().harm_cache_solve_T(self.h, rho.data_ptr(), e.data_ptr(),
+                                        None if zindex is None else zindex.data_ptr(), T.data_ptr(), Tlo, Thi,
+                                        ntrial, tolf, berr.data_ptr(), ret.data_ptr(), n, None), "harm_cache_solve_T")
+        return berr, ret
+
     def close(self) -> None:
         if self.h:
             lib().harm_cache_free(self.h)
@@ -482,6 +505,8 @@ def lib() -> ctypes.CDLL:
     L.harm_cache_free.restype = None
     L.harm_cache_lookup.argtypes = [c_void_p, c_int, POINTER(c_int), c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
+    L.harm_cache_solve_T.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int,
+                                     c_double, c_void_p, c_void_p, c_size_t, c_void_p]
     _lib = L
     return L
 

@@ -65,6 +65,21 @@ __global__ void __launch_bounds__(TPB_C) k_cache_lookup(harm::CacheView c, int m
     if (status) status[i] = s;
 }
 
+__global__ void __launch_bounds__(TPB_C) k_cache_solve_T(harm::CacheView c, const double *__restrict__ rho,
+                                                         const double *__restrict__ e,
+                                                         const int *__restrict__ zindex, double *__restrict__ T,
+                                                         double lo, double hi, int ntrial, double tolf,
+                                                         double *__restrict__ berr, int *__restrict__ ret, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * TPB_C + threadIdx.x;
+    if (i >= n) return;
+    double t, b;
+    const int r = harm::cache_solve_T(c, rho[i], e[i], zindex ? zindex[i] : 0, T[i], lo, hi, ntrial, tolf, t, b);
+    T[i] = t;
+    if (berr) berr[i] = b;
+    if (ret) ret[i] = r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -277,6 +292,23 @@ int harm_cache_lookup(const harm_cache *c, int mode, const int *dimmask, const d
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_cache_lookup, dim3((unsigned)((n + TPB_C - 1) / TPB_C)), dim3(TPB_C), 0, s, v, mode, mask, x,
                        y, z, zindex, out, dery, status, n);
+    CCHK(hipGetLastError());
+    CCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int harm_cache_solve_T(const harm_cache *c, const double *rho, const double *e, const int *zindex, double *T,
+                       double Tlo, double Thi, int ntrial, double tolf, double *berr, int *ret, size_t n, void *stream)
+{
+    if (!c) return harm_host::set_error("null cache", 1);
+    if (!n) return 0;
+    if (!rho || !e || !T) return harm_host::set_error("harm_cache_solve_T: rho, e and T are required", 1);
+    if (ntrial < 1) return harm_host::set_error("harm_cache_solve_T: ntrial >= 1", 3);
+    harm::CacheView v{c->desc.x, c->desc.y, c->desc.z, c->desc.d, c->desc.xtype, c->desc.ytype, c->desc.ztype,
+                      c->whole, c->vx, c->vy, c->vz, c->values, c->counted};
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_cache_solve_T, dim3((unsigned)((n + TPB_C - 1) / TPB_C)), dim3(TPB_C), 0, s, v, rho, e,
+                       zindex, T, Tlo, Thi, ntrial, tolf, berr, ret, n);
     CCHK(hipGetLastError());
     CCHK(hipStreamSynchronize(s));
     return 0;

@@ -236,4 +236,54 @@ HC_HD int cache_lookup(const CacheView &c, double x, double y, double z, int zin
     return CACHE_OK;
 }
 
+// Temperature at which the cache's energy density (component 0) equals e, by
+// the bounded Newton iteration the reference's COOL_EOS pressure and cooling
+// routines run (mnewt.h:104-205 mnewtc<1> with cooleos.c:228-262/296-306 as
+// its function: F = (e - e(rho, T)) / e, dF/dT = -de/dT / e from the spline's
+// centred difference; cooleos.c:365-404, 545-577). A step is taken when it
+// stays inside [lo, hi] and moves T; otherwise every later trial would repeat
+// this one, so the loop stops with the result those trials give. Returns the
+// reference's code (1 converged, errf <= tolf, T the last iterate; 2 or 0 not
+// converged, T the iterate of least error) or -1 when a lookup fails (the
+// reference reads values it did not get).
+HC_HD int cache_solve_T(const CacheView &c, double rho, double e, int zindex, double T0, double lo, double hi,
+                        int ntrial, double tolf, double &T, double &berr)
+{
+    double x = T0, best = T0, besterr = 1.0E100;
+    int ret = 0;
+    for (int k = 1; k <= ntrial; k++) {
+        double val = 0.0, der = 0.0;
+        if (cache_lookup(c, rho, x, 0.0, zindex, CACHE_SPLINE, 1ull, &val, &der, 0) != CACHE_OK) {
+            T = x;
+            berr = besterr;
+            return -1;
+        }
+        const double F = (e - val) / e;
+        const double D = -der / e;
+        double bet = -F;
+        const double errf = 0.0 + fabs(bet);
+        if (errf <= tolf) {
+            T = x;
+            berr = errf;
+            return 1;
+        }
+        if (errf < besterr) {
+            if (k > 1) ret = 2;
+            besterr = errf;
+            best = x;
+        }
+        // 1x1 LU solve: a zero (or NaN) derivative is a singular matrix, and the
+        // step stays -F (mnewt.h:46-62, 149-150)
+        if (fabs(D) > 0.0) bet = bet / D;
+        const double nv = x + bet;
+        if (nv >= lo && nv <= hi && nv != x)
+            x = nv;
+        else
+            break;
+    }
+    T = best;
+    berr = besterr;
+    return ret;
+}
+
 }  // namespace harm

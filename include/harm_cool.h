@@ -465,6 +465,18 @@ void harm_cache_free(harm_cache *c);
 int harm_cache_lookup(const harm_cache *c, int mode, const int *dimmask, const double *x, const double *y,
                       const double *z, const int *zindex, double *out, double *dery, int *status, size_t n,
                       void *stream);
+/* Temperature solve of the COOL_EOS pressure and cooling routines
+ * (cooleos.c:365-404, 545-577; mnewt.h:104-205 mnewtc<1>): for n zones (device
+ * arrays, cgs) find T with cache component 0 (e) at (rho, T, z plane zindex
+ * (NULL: plane 0)) equal to e, by up to ntrial bounded Newton steps from the
+ * T given (the zone's last T), each kept inside [Tlo, Thi] (the reference uses
+ * 300, TMin (1 + 1e-6), TMax (1 - 1e-6), tolf 1e-6).  T is overwritten with the
+ * result, berr (may be NULL) gets |F| at it and ret (may be NULL) the
+ * reference's code: 1 converged, 2 or 0 not (T = the iterate of least error),
+ * -1 a lookup outside the cache.  The density-range and negative-e branches of
+ * those routines, and the unit conversions, stay with the caller. */
+int harm_cache_solve_T(const harm_cache *c, const double *rho, const double *e, const int *zindex, double *T,
+                       double Tlo, double Thi, int ntrial, double tolf, double *berr, int *ret, size_t n, void *stream);
 
 /* dump() ASCII file (dump.c:1710-1990; the HDFDUMP=0 build): a header line
  * (t, N1, N2 and the theta origin/spacing in 2D or N3 and the phi ones in 3D,

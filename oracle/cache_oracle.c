@@ -202,3 +202,49 @@ int orc_cache_get(int X, int Y, int Z, int D, const int *types, int whole, const
     }
     return 1;
 }
+
+/*
+ * orc_cache_solve_T -- the temperature solve of cooleos.c:365-404 and 545-577:
+ * mnewtc<1> (mnewt.h:104-205) over F = (e - e(rho, T)) / e with the Jacobian
+ * -de/dT / e (cooleos.c:228-262, 296-306), every trial run as the reference
+ * runs it (a rejected step leaves T where it is for the next trial).
+ */
+int orc_cache_solve_T(int X, int Y, int Z, int D, const int *types, int whole, const double *vx, const double *vy,
+                      const double *vz, const double *values, const uint8_t *counted, double rho, double e,
+                      int zindex, double T0, double lo, double hi, int ntrial, double tolf, double *T, double *berr)
+{
+    int mask[64] = {0};
+    mask[0] = 1;
+    double x = T0, best = T0, besterr = 1.0E100;
+    int ret = 0;
+    for (int k = 1; k <= ntrial; k++) {
+        double val[64], der[64];
+        if (orc_cache_get(X, Y, Z, D, types, whole, vx, vy, vz, values, counted, rho, x, 0.0, zindex, 1, mask, val,
+                          der, 1) != 1) {
+            *T = x;
+            *berr = besterr;
+            return -1;
+        }
+        double alpha = -der[0] / e;
+        double bet = -((e - val[0]) / e);
+        double errf = 0.0;
+        errf += bet < 0.0 ? -bet : bet;
+        if (errf <= tolf) {
+            *T = x;
+            *berr = errf;
+            return 1;
+        }
+        if (errf < besterr) {
+            if (k > 1) ret = 2;
+            besterr = errf;
+            best = x;
+        }
+        double big = alpha < 0.0 ? -alpha : alpha;
+        if (big > 0.0) bet = bet / alpha; /* the 1x1 LU solve; big == 0 (or NaN) is singular */
+        double nv = x + bet;
+        if (nv >= lo && nv <= hi && nv != x) x = nv;
+    }
+    *T = best;
+    *berr = besterr;
+    return ret;
+}

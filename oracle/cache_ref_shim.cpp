@@ -14,6 +14,7 @@
 
 #include "akima.h"
 #include "cache4d.h"
+#include "mnewt.h"
 
 static int no_calc(double, double, double, double *) { return 1; }
 
@@ -82,6 +83,35 @@ void cref_spline_akima_der(int ndim, int *dimmask, int ndata, double *t, double 
 void cref_linear(int ndim, int ndata, double *t, double *y, double tval, double *out)
 {
     spline_linear_val2(ndim, ndata, t, y, tval, out);
+}
+
+struct SolveArgs {
+    Cache4d *c;
+    double rho, e;
+    int zidx;
+};
+
+/* The function cooleos.c hands mnewtc<1> (usrfun_mnewt_rho_e with
+ * mnewt_mini_rho_e_der, cooleos.c:228-262, 296-306), over a Cache4d. */
+static void rho_e_fun(double x[2], double a[2][2], double b[2], void *par)
+{
+    SolveArgs *s = (SolveArgs *)par;
+    double val[64], der[64];
+    int mask[64] = {1};
+    s->c->getvalues(s->rho, x[1], 0.0, SPLINE, val, der, s->zidx, mask);
+    a[1][1] = -der[0] / s->e;
+    b[1] = -((s->e - val[0]) / s->e);
+}
+
+/* One temperature solve by the reference's own mnewtc<1> (mnewt.h:104-205). */
+int cref_solve_T(void *c, double rho, double e, int zidx, double T0, double lo, double hi, int ntrial, double tolf,
+                 double *T, double *berr)
+{
+    SolveArgs s{(Cache4d *)c, rho, e, zidx};
+    double x[2] = {0.0, T0}, mn[2] = {0.0, lo}, mx[2] = {0.0, hi};
+    int r = mnewtc<1>(ntrial, x, tolf, mn, mx, rho_e_fun, &s, berr, 1, nullptr);
+    *T = x[1];
+    return r;
 }
 
 }  // extern "C"

@@ -385,6 +385,9 @@ def cache_lib():
     L.orc_lin.restype = c_double
     L.orc_cache_get.argtypes = [c_int, c_int, c_int, c_int, POINTER(c_int), c_int, _dp, _dp, _dp, _dp, c_void_p,
                                 c_double, c_double, c_double, c_int, c_int, POINTER(c_int), _dp, _dp, c_size_t]
+    L.orc_cache_solve_T.argtypes = [c_int, c_int, c_int, c_int, POINTER(c_int), c_int, _dp, _dp, _dp, _dp,
+                                    c_void_p, c_double, c_double, c_int, c_double, c_double, c_double, c_int,
+                                    c_double, _dp, _dp]
     return L
 
 
@@ -411,6 +414,20 @@ class OracleCache:
         return s, out, dv
 
 
+def _solve_args(o):
+    p = lambda a: a.ctypes.data_as(_dp)
+    return (o.d.x, o.d.y, o.d.z, o.d.d, o.types, o.whole, p(o.vx), p(o.vy), p(o.vz), p(o.values),
+            o.counted.ctypes.data)
+
+
+def oracle_solve_T(o, rho, e, zindex, T0, lo, hi, ntrial=300, tolf=1.0e-6):
+    """The oracle's temperature solve on an OracleCache -> (ret, T, berr)."""
+    T, b = c_double(), c_double()
+    r = o.L.orc_cache_solve_T(*_solve_args(o), float(rho), float(e), int(zindex), float(T0), float(lo), float(hi),
+                              int(ntrial), float(tolf), ctypes.byref(T), ctypes.byref(b))
+    return r, T.value, b.value
+
+
 class CacheRef:
     """The reference's Cache4d and akima.c routines through cache_ref_shim.cpp."""
 
@@ -425,6 +442,8 @@ class CacheRef:
         L.cref_spline_akima.argtypes = [c_int, POINTER(c_int), c_int, _dp, _dp, c_double, _dp]
         L.cref_spline_akima_der.argtypes = [c_int, POINTER(c_int), c_int, _dp, _dp, c_double, _dp, _dp]
         L.cref_linear.argtypes = [c_int, c_int, _dp, _dp, c_double, _dp]
+        L.cref_solve_T.argtypes = [c_void_p, c_double, c_double, c_int, c_double, c_double, c_double, c_int,
+                                   c_double, _dp, _dp]
         self.L = L
 
     def desc_size(self) -> int:
@@ -484,3 +503,10 @@ class CacheRef:
         self.L.cref_linear(ndim, t.size, t.ctypes.data_as(_dp), y.ctypes.data_as(_dp), float(tval),
                            out.ctypes.data_as(_dp))
         return out
+
+    def solve_T(self, h, rho, e, zindex, T0, lo, hi, ntrial=300, tolf=1.0e-6):
+        """The reference's own mnewtc<1> over its getvalues() -> (ret, T, berr)."""
+        T, b = c_double(), c_double()
+        r = self.L.cref_solve_T(h, float(rho), float(e), int(zindex), float(T0), float(lo), float(hi), int(ntrial),
+                                float(tolf), ctypes.byref(T), ctypes.byref(b))
+        return r, T.value, b.value

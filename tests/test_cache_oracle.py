@@ -223,3 +223,48 @@ def test_missing_rolling_and_broken_points():
     assert o2.get(0.5 * (vx2[0] + vx2[1]), vy2[2], zindex=0)[0] == hc.CACHE_NEEDS_RECENTRE
     assert o2.get(vx2[2], vy2[2], zindex=0)[0] == hc.CACHE_OK
     assert o.get(vx[2], vy[2], zindex=desc.z)[0] == hc.CACHE_BAD_ZINDEX
+
+
+def _solve_cases(rng, desc, vx, vy, values, n):
+    """(rho, e, z plane, T start) with e taken from the table near a random T,
+    starts near and far, some e outside what the table reaches."""
+    lo, hi = vy[0] * (1 + 1e-6), vy[-1] * (1 - 1e-6)
+    out = []
+    for k in range(n):
+        ix = int(rng.integers(0, desc.x))
+        zi = int(rng.integers(0, desc.z))
+        rho = vx[ix] * (1 + 0.3 * rng.random()) if ix < desc.x - 1 else vx[ix]
+        e = values[zi, ix, int(rng.integers(0, desc.y)), 0] * (1 + 0.02 * rng.standard_normal())
+        if k % 10 == 0:
+            e *= 1e3  # beyond the table: no root inside the bounds
+        T0 = 10 ** rng.uniform(np.log10(lo), np.log10(hi))
+        out.append((rho, e, zi, T0))
+    return out, lo, hi
+
+
+@needs_ref
+def test_temperature_solve_matches_reference(tmp_path):
+    from oracle_lib import oracle_solve_T
+    rng = np.random.default_rng(17)
+    desc, vx, vy, vz, values, counted = random_cache(rng, X=12, Y=20, Z=3, D=5)
+    # e increasing with T, as an energy density is, plus wiggles
+    values[..., 0] = np.sort(values[..., 0], axis=2)
+    path = str(tmp_path / "c.dat")
+    hc.cache_write(path, desc, vx, vy, vz, values, counted)
+    o = OracleCache(desc, vx, vy, vz, values, counted)
+    ref = CacheRef()
+    h = ref.open(path)
+    cases, lo, hi = _solve_cases(rng, desc, vx, vy, values, 400)
+    codes = []
+    try:
+        for rho, e, zi, T0 in cases:
+            r = ref.solve_T(h, rho, e, zi, T0, lo, hi)
+            mine = oracle_solve_T(o, rho, e, zi, T0, lo, hi)
+            assert mine == r, (rho, e, zi, T0)
+            codes.append(r[0])
+            if r[0] == 1:  # converged: the table's e at T matches to the tolerance
+                s, v, _ = o.get(rho, r[1], zindex=zi)
+                assert s == 1 and abs(e - v[0]) / e <= 1e-6
+    finally:
+        ref.close(h)
+    assert codes.count(1) > 200 and codes.count(1) < len(codes), np.bincount(codes)

@@ -87,3 +87,29 @@ def test_file_cache_and_generator_values(tmp_path):
     _check(c, o, pts, [0] * len(pts), True, [1, 0, 0, 0, 0, 0], True)  # e and de/dT, as the T solve asks
     _check(c, o, pts, [0] * len(pts), True, None, False)
     c.close()
+
+
+def test_temperature_solve_matches_oracle():
+    """harm_cache_solve_T against the oracle's run of every Newton trial (which
+    is bitwise the reference's own mnewtc<1>): code, T and |F| bitwise."""
+    import torch
+    from oracle_lib import oracle_solve_T
+    from test_cache_oracle import _solve_cases
+
+    rng = np.random.default_rng(23)
+    desc, vx, vy, vz, values, counted = random_cache(rng, X=30, Y=40, Z=3, D=5)
+    values[..., 0] = np.sort(values[..., 0], axis=2)
+    c = hc.Cache.create(desc, vx, vy, vz, values, counted)
+    o = OracleCache(desc, vx, vy, vz, values, counted)
+    cases, lo, hi = _solve_cases(rng, desc, vx, vy, values, 3000)
+    cases.append((vx[0] * 0.5, 1e20, 0, vy[3]))  # rho outside the cache: the lookup fails
+    dev = lambda k, dt=torch.float64: torch.tensor([q[k] for q in cases], dtype=dt, device="cuda:0")
+    T = dev(3)
+    berr, ret = c.solve_T(dev(0), dev(1), T, dev(2, torch.int32), lo, hi)
+    T, berr, ret = T.cpu().numpy(), berr.cpu().numpy(), ret.cpu().numpy()
+    for i, (rho, e, zi, T0) in enumerate(cases):
+        r, t, b = oracle_solve_T(o, rho, e, zi, T0, lo, hi)
+        assert (ret[i], T[i], berr[i]) == (r, t, b), (i, rho, e, zi, T0)
+    counts = np.bincount(ret + 1, minlength=4)
+    assert counts[2] > 1500 and counts[0] == 1, counts  # converged, and the one failed lookup
+    c.close()
