@@ -310,6 +310,22 @@ def cache_lookups(hc, torch, tab: dict, n: int = 1 << 22) -> dict:
         el = time.perf_counter() - t0
         res[name] = {"ms": round(el * 1e3, 3), "lookups_per_s": n / el,
                      "ok": int((st == hc.CACHE_OK).sum().item())}
+    # the COOL_EOS temperature solve: e of the table at a random T, solved from a
+    # start 5% away (a zone's T of the previous step)
+    e_true, _, _ = cache.lookup(x, y, None, zi, hc.INTERP_SPLINE, [1, 0, 0, 0, 0, 0], False)
+    e = e_true[0].contiguous()
+    lo, hi = ax_T[0] * (1 + 1e-6), ax_T[-1] * (1 - 1e-6)
+    T0 = torch.clamp(y * (1 + 0.05 * torch.as_tensor(rng.uniform(-1, 1, n), device="cuda:0")), lo, hi)
+    T = T0.clone()
+    cache.solve_T(x, e, T, zi, lo, hi)  # warm-up
+    T = T0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    berr, ret = cache.solve_T(x, e, T, zi, lo, hi)
+    el = time.perf_counter() - t0
+    conv = ret == 1
+    res["solve_T"] = {"ms": round(el * 1e3, 3), "solves_per_s": n / el, "converged": int(conv.sum().item()),
+                      "max_rel_T_err_converged": float(((T - y).abs() / y)[conv].max().item())}
     cache.close()
     return res
 
