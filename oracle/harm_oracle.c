@@ -161,6 +161,14 @@ static inline size_t zi(const orc_ctx *o, int h, int i, int j)
     return ((size_t)h * o->SN1 + (size_t)i) * o->SN2 + (size_t)j;
 }
 #define PV(o, arr, h, i, j) (&(arr)[zi(o, h, i, j) * NPR])
+/* Planted mistakes for the tests that show a pin catches them: built with
+ * -DORC_MUTANT=n, site MUT(n) takes its wrong branch (tests/oracle_lib.py
+ * build_mutant).  The default build has none. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(n) (ORC_MUTANT == (n))
+
 static inline int hwrap(const orc_ctx *o, int h)
 {
     return (h + o->N3) % o->N3;
@@ -1532,8 +1540,8 @@ static void fixup(orc_ctx *o, double *pv, int64_t *cnt)
 #define AVG4_1(pr, h, i, j, k) \
     (0.25 * (PV(o, pr, h, i, j + 1)[k] + PV(o, pr, h, i, j - 1)[k] + PV(o, pr, h, i - 1, j)[k] + PV(o, pr, h, i + 1, j)[k]))
 #define AVG4_2(pr, h, i, j, k)                                                                          \
-    (0.25 * (PV(o, pr, h, i + 1, j + 1)[k] + PV(o, pr, h, i + 1, j - 1)[k] + PV(o, pr, h, i - 1, j + 1)[k] + \
-             PV(o, pr, h, i - 1, j - 1)[k]))
+    (0.25 * (PV(o, pr, h, i + 1, MUT(9) ? j : j + 1)[k] + PV(o, pr, h, i + 1, j - 1)[k] +                   \
+             PV(o, pr, h, i - 1, j + 1)[k] + PV(o, pr, h, i - 1, j - 1)[k]))
 static void fixup_utoprim(orc_ctx *o, double *pv, int64_t *cnt)
 {
     int h, i, j, k, pf[9];
@@ -1541,33 +1549,36 @@ static void fixup_utoprim(orc_ctx *o, double *pv, int64_t *cnt)
     for (h = 0; h < o->N3; h++)
         for (i = 0; i < o->SN1; i++)
             for (j = 0; j < o->SN2; j++) pflag[zi(o, h, i, j)] = !pflag[zi(o, h, i, j)];
+    const int ni = o->istop - o->istart + 1, nj = o->jstop - o->jstart + 1;
     for (h = 0; h < o->N3; h++)
-        for (i = o->istart; i <= o->istop; i++)
-            for (j = o->jstart; j <= o->jstop; j++) {
-                if (pflag[zi(o, h, i, j)] == 0) {
-                    double *z = PV(o, pv, h, i, j);
-                    pf[1] = pflag[zi(o, h, i - 1, j + 1)];
-                    pf[2] = pflag[zi(o, h, i, j + 1)];
-                    pf[3] = pflag[zi(o, h, i + 1, j + 1)];
-                    pf[8] = pflag[zi(o, h, i - 1, j)];
-                    pf[4] = pflag[zi(o, h, i + 1, j)];
-                    pf[7] = pflag[zi(o, h, i - 1, j - 1)];
-                    pf[6] = pflag[zi(o, h, i, j - 1)];
-                    pf[5] = pflag[zi(o, h, i + 1, j - 1)];
-                    if (pf[2] && pf[4] && pf[6] && pf[8]) {
-                        for (k = 0; k < B1; k++) z[k] = AVG4_1(pv, h, i, j, k);
-                    } else if (pf[1] && pf[3] && pf[5] && pf[7]) {
-                        for (k = 0; k < B1; k++) z[k] = AVG4_2(pv, h, i, j, k);
-                    } else {
-                        for (k = RHO; k <= UU; k++) z[k] = 0.5 * (AVG4_1(pv, h, i, j, k) + AVG4_2(pv, h, i, j, k));
-                        z[U1] = z[U2] = z[U3] = 0.;
-                        cnt[ORC_FIXUP_NOSTENCIL]++;
-                    }
-                    pflag[zi(o, h, i, j)] = 0;
-                    cnt[ORC_FIXUP_REPAIRS]++;
-                    fixup1zone(o, h, i, j, z, cnt);
+        for (int q = 0; q < ni * nj; q++) {
+            /* ZLOOP order (h, i, j); mutant 8 walks (h, j, i) */
+            i = o->istart + (MUT(8) ? q % ni : q / nj);
+            j = o->jstart + (MUT(8) ? q / ni : q % nj);
+            if (pflag[zi(o, h, i, j)] == 0) {
+                double *z = PV(o, pv, h, i, j);
+                pf[1] = pflag[zi(o, h, i - 1, j + 1)];
+                pf[2] = pflag[zi(o, h, i, j + 1)];
+                pf[3] = pflag[zi(o, h, i + 1, j + 1)];
+                pf[8] = pflag[zi(o, h, i - 1, j)];
+                pf[4] = pflag[zi(o, h, i + 1, j)];
+                pf[7] = pflag[zi(o, h, i - 1, j - 1)];
+                pf[6] = pflag[zi(o, h, i, j - 1)];
+                pf[5] = pflag[zi(o, h, i + 1, j - 1)];
+                if (pf[2] && pf[4] && pf[6] && pf[8]) {
+                    for (k = 0; k < B1; k++) z[k] = AVG4_1(pv, h, i, j, k);
+                } else if (pf[1] && pf[3] && pf[5] && pf[7]) {
+                    for (k = 0; k < B1; k++) z[k] = AVG4_2(pv, h, i, j, k);
+                } else {
+                    for (k = RHO; k <= UU; k++) z[k] = 0.5 * (AVG4_1(pv, h, i, j, k) + AVG4_2(pv, h, i, j, k));
+                    z[U1] = z[U2] = z[U3] = 0.;
+                    cnt[ORC_FIXUP_NOSTENCIL]++;
                 }
+                pflag[zi(o, h, i, j)] = MUT(7) ? 1 : 0; /* fixup.c:530: stays "bad" */
+                cnt[ORC_FIXUP_REPAIRS]++;
+                fixup1zone(o, h, i, j, z, cnt);
             }
+        }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1634,12 +1645,14 @@ static void bound_prim(orc_ctx *o, double *prim, int64_t *fatal)
                 for (j = 0; j < jref; j++)
                     for (k = 0; k < NPR; k++) {
                         if (k == B1 || k == B2 || (N3 > 1 && k == B3)) continue;
+                        const int jr = MUT(4) ? jref - 1 : jref;
                         if (k == U2) {
-                            PV(o, prim, h, i, js + j)[k] = (j + 0.5) / (jref + 0.5) * PV(o, prim, h, i, js + jref)[k];
-                            PV(o, prim, h, i, je - j)[k] = (j + 0.5) / (jref + 0.5) * PV(o, prim, h, i, je - jref)[k];
+                            const double w = (j + 0.5) / (MUT(3) ? jref + 1. : jref + 0.5);
+                            PV(o, prim, h, i, js + j)[k] = w * PV(o, prim, h, i, js + jr)[k];
+                            PV(o, prim, h, i, je - j)[k] = w * PV(o, prim, h, i, je - jr)[k];
                         } else {
-                            PV(o, prim, h, i, js + j)[k] = PV(o, prim, h, i, js + jref)[k];
-                            PV(o, prim, h, i, je - j)[k] = PV(o, prim, h, i, je - jref)[k];
+                            PV(o, prim, h, i, js + j)[k] = PV(o, prim, h, i, js + jr)[k];
+                            PV(o, prim, h, i, je - j)[k] = PV(o, prim, h, i, je - jr)[k];
                         }
                     }
     }
@@ -1647,7 +1660,7 @@ static void bound_prim(orc_ctx *o, double *prim, int64_t *fatal)
     if (N3 > 1) {
         /* bounds.c:161-275: theta ghosts from the phi+pi plane */
         for (h = 0; h < N3; h++) {
-            const int hop = (h + N3 / 2) % N3;
+            const int hop = MUT(1) ? (h + N3 / 2 + 1) % N3 : MUT(2) ? h : (h + N3 / 2) % N3;
             for (i = is - 2; i <= ie + 2; i++) {
                 for (k = 0; k < NPR; k++) {
                     PV(o, prim, h, i, js - 1)[k] = PV(o, prim, hop, i, js)[k];
@@ -1745,7 +1758,7 @@ static double fluxcalc(orc_ctx *o, const double *pr, double *F, int dir, int ful
             for (int k = 0; k < NPR; k++)
                 PV(o, o->dq, h, i, j)[k] =
                     slope_lim(o->P.lim, PV(o, pr, hwrap(o, h - hdel), i - idel, j - jdel)[k], PV(o, pr, h, i, j)[k],
-                              PV(o, pr, hwrap(o, h + hdel), i + idel, j + jdel)[k]);
+                              PV(o, pr, (MUT(11) && h + hdel >= N3) ? h : hwrap(o, h + hdel), i + idel, j + jdel)[k]);
     }
 
     double ndt = 1.e9;
@@ -1836,10 +1849,11 @@ static void flux_ct(orc_ctx *o)
     }
     double *e1 = o->e1, *e2 = o->e2, *e3 = o->e3;
     for (h = 0; h < N3; h++) {
-        const int hm = hwrap(o, h - 1);
+        const int hm = MUT(10) ? (h > 0 ? h - 1 : 0) : hwrap(o, h - 1);
+        const int hm1 = MUT(5) ? hwrap(o, h + 1) : hm;
         for (i = is; i <= ie + 1; i++)
             for (j = js; j <= je + 1; j++) {
-                EM(e1, h, i, j) = 0.25 * ((PV(o, F2, h, i, j)[B3] + PV(o, F2, hm, i, j)[B3]) -
+                EM(e1, h, i, j) = 0.25 * ((PV(o, F2, h, i, j)[B3] + PV(o, F2, hm1, i, j)[B3]) -
                                           (PV(o, F3, h, i, j)[B2] + PV(o, F3, h, i, j - 1)[B2]));
                 EM(e2, h, i, j) = 0.25 * ((PV(o, F3, h, i, j)[B1] + PV(o, F3, h, i - 1, j)[B1]) -
                                           (PV(o, F1, h, i, j)[B3] + PV(o, F1, hm, i, j)[B3]));
@@ -1863,7 +1877,7 @@ static void flux_ct(orc_ctx *o)
             }
         for (i = is; i <= ie; i++)
             for (j = js; j <= je; j++) {
-                PV(o, F3, h, i, j)[B1] = 0.5 * (EM(e2, h, i, j) + EM(e2, h, i + 1, j));
+                PV(o, F3, h, i, j)[B1] = (MUT(6) ? -0.5 : 0.5) * (EM(e2, h, i, j) + EM(e2, h, i + 1, j));
                 PV(o, F3, h, i, j)[B2] = -0.5 * (EM(e1, h, i, j) + EM(e1, h, i, j + 1));
                 PV(o, F3, h, i, j)[B3] = 0.;
             }
@@ -1910,7 +1924,8 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
                 if (N3 > 2)
                     U[k] += Dt * (-(PV(o, o->F1, h, i + 1, j)[k] - PV(o, o->F1, h, i, j)[k]) / o->dx[RR] -
                                   (PV(o, o->F2, h, i, j + 1)[k] - PV(o, o->F2, h, i, j)[k]) / o->dx[TH] -
-                                  (PV(o, o->F3, hwrap(o, h + 1), i, j)[k] - PV(o, o->F3, h, i, j)[k]) / o->dx[PH] +
+                                  (PV(o, o->F3, (MUT(12) && h + 1 == N3) ? h : hwrap(o, h + 1), i, j)[k] -
+                                   PV(o, o->F3, h, i, j)[k]) / o->dx[PH] +
                                   dU[k]);
                 else
                     U[k] += Dt * (-(PV(o, o->F1, h, i + 1, j)[k] - PV(o, o->F1, h, i, j)[k]) / o->dx[RR] -
@@ -2259,6 +2274,12 @@ int orc_step(orc_ctx *o)
     bound_prim(o, o->p, &o->cnt[ORC_FATAL]);
     fixup_utoprim(o, o->p, o->cnt);
     bound_prim(o, o->p, &o->cnt[ORC_FATAL]);
+    /* The reference exits inside fail() (diag.c:381-397), before diag_mass and
+     * the clock update; here the step's state is kept and 14 returned. */
+    if (o->cnt[ORC_FATAL] != fatal0) {
+        g_step_eos = NULL;
+        return 14;
+    }
     {
         double out[4];
         orc_diag(o, out);
@@ -2267,12 +2288,12 @@ int orc_step(orc_ctx *o)
     }
     if (o->tr_on) tracers_update(o, o->dt); /* step_ch.c:121-123 */
     g_step_eos = NULL;
-    if (o->dt < 1.e-9) return 11;
+    if (o->dt < 1.e-9) return 11;           /* step_ch.c:126-129: exit before t += dt */
     o->t += o->dt;
     if (ndt > SAFE * o->dt) ndt = SAFE * o->dt;
     o->dt = ndt;
     if (o->t + o->dt > o->P.tf) o->dt = o->P.tf - o->t;
-    return (o->cnt[ORC_FATAL] != fatal0) ? 1 : 0;
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -56,6 +56,9 @@ orc_ctx *orc_create(const orc_params *p);
 void     orc_destroy(orc_ctx *o);
 void     orc_set_threads(int n);
 int      orc_init_torus(orc_ctx *o);
+/* One step_ch(): 0; 11 when dt < 1e-9 (state and mass budget of the step kept,
+ * t not advanced, step_ch.c:126-129); 14 when a fail() event happened in the
+ * step (state kept; no mass budget, tracer or clock update). */
 int      orc_step(orc_ctx *o);
 int      orc_get_state(const orc_ctx *o, double *prim, int8_t *pflag);
 int      orc_set_state(orc_ctx *o, const double *prim, const int8_t *pflag);

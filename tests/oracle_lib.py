@@ -78,7 +78,28 @@ def lib():
     if _lib is None:
         if not os.path.exists(ORACLE_SO):
             build_oracle()
-        L = ctypes.CDLL(ORACLE_SO)
+        _lib = _bind(ctypes.CDLL(ORACLE_SO))
+    return _lib
+
+
+def lib_at(path: str):
+    """Another build of the oracle (e.g. one with a planted mistake), bound
+    like lib()."""
+    return _bind(ctypes.CDLL(path))
+
+
+def build_mutant(n: int, outdir: str) -> str:
+    """The oracle built with ORC_MUTANT=n: the planted mistake n of
+    harm_oracle.c (its MUT(n) sites), for tests that show a pin catches it."""
+    so = os.path.join(outdir, f"liboracle_mut{n}.so")
+    subprocess.run(["gcc", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-fopenmp", "-w", "-shared",
+                    f"-DORC_MUTANT={n}", "-o", so, os.path.join(ORACLE_DIR, "harm_oracle.c"),
+                    os.path.join(ORACLE_DIR, "cache_oracle.c"), "-lm"], check=True)
+    return so
+
+
+def _bind(L):
+    if True:
         L.orc_create.restype = c_void_p
         L.orc_create.argtypes = [POINTER(HarmParams)]
         L.orc_destroy.argtypes = [c_void_p]
@@ -137,8 +158,7 @@ def lib():
         L.orc_restart_read.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_int]
         L.orc_dump_write.argtypes = [c_void_p, ctypes.c_char_p, POINTER(OrcRestartHdr)]
         L.orc_restart_write_tr.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_void_p]
-        _lib = L
-    return _lib
+    return L
 
 
 _ref = None
@@ -175,9 +195,9 @@ def ref():
 class Oracle:
     """One oracle simulation context (the restated step_ch state)."""
 
-    def __init__(self, params: HarmParams):
+    def __init__(self, params: HarmParams, L=None):
         self.params = params
-        self.L = lib()
+        self.L = L if L is not None else lib()
         self.h = self.L.orc_create(ctypes.byref(params))
         if not self.h:
             raise RuntimeError("orc_create failed")
