@@ -199,15 +199,18 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
     return res
 
 
-def tabulated_torus(hc, steps: int, warmup: int) -> dict:
+def tabulated_torus(hc, host: dict, steps: int, warmup: int) -> dict:
     """The torus on the tabulated-EOS path (COOL_EOS build): 192x192x128,
-    density scale 1e11 g/cm^3, table over log rho [6, 13] x log T [9, 12] on
-    200 x 200 points, Ye = 0.5."""
+    density scale 1e11 g/cm^3, the cooling-table generator's own table over
+    log rho [6, 13] x log T [9, 12] on 200 x 200 points (eos_table())."""
+    import ctypes
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     from harm_cool_amd.params import torus_params
 
     P = torus_params(192, 192, 128)
-    spec = hc.eos_spec()
-    host = hc.eos_table_build(spec)
     tab = hc.DeviceEos(host, P.gam)
     g = hc.Harm(P, device=0)
     g.set_eos(tab)
@@ -220,14 +223,24 @@ def tabulated_torus(hc, steps: int, warmup: int) -> dict:
     el = time.perf_counter() - t0
     c = g.counters()
     pr = g.prims()
-    rho = pr[0][2:194, 2:194]
-    inside = float(((rho >= host["rho"][0]) & (rho <= host["rho"][-1])).mean())
     g.close()
+    rho = pr[0][2:194, 2:194]
+    u = pr[1][2:194, 2:194]
+    inside = (rho >= host["rho"][0]) & (rho <= host["rho"][-1])
+    # zones whose (rho, u) the table covers (the others take the Gamma-law branch)
+    k = np.clip(np.searchsorted(host["rho"], rho) - 1, 0, host["rho"].size - 2)
+    a = (rho - host["rho"][k]) / (host["rho"][k + 1] - host["rho"][k])
+    elo = host["e"][k, 0] + a * (host["e"][k + 1, 0] - host["e"][k, 0])
+    ehi = host["e"][k, -1] + a * (host["e"][k + 1, -1] - host["e"][k, -1])
+    on = inside & (u >= elo) & (u <= ehi)
+    del pr, rho, u
     nz = 192 * 192 * 128
-    return {"workload": "FM torus 192x192x128, tabulated EOS (rho_unit 1e11 g/cm^3, 200x200 table), Utoprim_5d",
+    return {"workload": "FM torus 192x192x128, tabulated EOS from the cooling-table generator (abundance() per "
+                        "point; rho_unit 1e11 g/cm^3, log rho [6,13] x log T [9,12], 200x200, HdFac at r=12), "
+                        "Utoprim_5d",
             "steps": steps, "zone_cycles_per_s": nz * steps / el, "ms_per_step": el / steps * 1e3,
             "n_iter_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1), "fail_5d": c["fail_2d"],
-            "frac_zones_in_table_rho_range": inside}
+            "frac_zones_in_table_rho_range": float(inside.mean()), "frac_zones_on_table": float(on.mean())}
 
 
 def tracer_section(g, steps: int) -> dict:
@@ -256,33 +269,26 @@ def tracer_section(g, steps: int) -> dict:
             "removed_during_run": int(ev.size)}
 
 
-def eos_table(hc, torch, n_rho: int = 200, n_t: int = 200) -> dict:
+def eos_table(hc, n_rho: int = 200, n_t: int = 200) -> dict:
     """The cooling-table generator (abundance.f90, one point per GPU thread) on
-    a log-spaced rho x T grid (1e8-1e13 g/cm^3, 5e9-2e11 K, H_d factor 0.5)."""
-    import ctypes
-
+    the tabulated-EOS workload's grid: log rho [6, 13] g/cm^3 x log T [9, 12] K,
+    HdFac of the torus's pressure maximum; the table the 192x192x128 torus runs
+    on."""
     import numpy as np
-    # the grid of a constant-grid cache with logarithmic rho and T axes
-    ax_rho = hc.cache_axis(hc.AXIS_LOG, n_rho, 1e8, 1e13)
-    ax_T = hc.cache_axis(hc.AXIS_LOG, n_t, 5e9, 2e11)
-    rho, T = np.meshgrid(ax_rho, ax_T, indexing="ij")
-    rho, T = rho.ravel(), T.ravel()
-    n = rho.size
-    d = [torch.as_tensor(a, device="cuda:0") for a in (rho, T, np.full(n, 0.5))]
-    out = torch.empty(6 * n, dtype=torch.float64, device="cuda:0")
-    ptr = [ctypes.c_void_p(t.data_ptr()) for t in d + [out]]
-    torch.cuda.synchronize()
+
+    spec = hc.eos_spec(nr=n_rho, nt=n_t)
     t0 = time.perf_counter()
-    rc = hc.lib().harm_eos_point_batch(*ptr, n, None)
+    host, raw = hc.fermi_eos_table(spec, raw=True)
     el = time.perf_counter() - t0
-    if rc:
-        raise RuntimeError("harm_eos_point_batch failed")
-    vals = out.cpu().numpy().reshape(6, n)
-    return {"workload": f"cooling-table points {n_rho}x{n_t} (rho 1e8-1e13 g/cm^3, T 5e9-2e11 K, HdFac 0.5), "
-                        "chemical equilibrium + neutrino trapping + rates per point, FP64",
+    n = n_rho * n_t
+    rho = np.repeat(host["rho"] * spec.rho_unit, n_t)
+    T = np.tile(host["T"], n_rho)
+    return {"workload": f"cooling-table points {n_rho}x{n_t} (rho 1e6-1e13 g/cm^3, T 1e9-1e12 K, "
+                        f"HdFac {hc.c3_hdfac():.4e} s), chemical equilibrium + neutrino trapping + rates per point, "
+                        "FP64",
             "points": n, "seconds": round(el, 3), "points_per_s": n / el,
-            "finite": bool(np.isfinite(vals).all()), "_rho": rho, "_T": T, "_vals": vals,
-            "_axes": (ax_rho, ax_T)}
+            "finite": bool(np.isfinite(raw).all()), "_rho": rho, "_T": T, "_vals": raw,
+            "_axes": (host["rho"] * spec.rho_unit, host["T"]), "_table": host}
 
 
 def cache_lookups(hc, torch, tab: dict, n: int = 1 << 22) -> dict:
@@ -292,13 +298,14 @@ def cache_lookups(hc, torch, tab: dict, n: int = 1 << 22) -> dict:
     import numpy as np
 
     ax_rho, ax_T = tab["_axes"]
-    desc = hc.cache_desc(ax_rho.size, ax_rho[0], ax_rho[-1], ax_T.size, ax_T[0], ax_T[-1], 1, 0.5, 0.5, d=6,
+    hd = hc.c3_hdfac()
+    desc = hc.cache_desc(ax_rho.size, ax_rho[0], ax_rho[-1], ax_T.size, ax_T[0], ax_T[-1], 1, hd, hd, d=6,
                          ztype=hc.AXIS_CUSTOM)
     values = tab["_vals"].T.reshape(1, ax_rho.size, ax_T.size, 6)
-    cache = hc.Cache.create(desc, ax_rho, ax_T, np.array([0.5]), values, np.ones(values.shape[:3], np.uint8))
+    cache = hc.Cache.create(desc, ax_rho, ax_T, np.array([hd]), values, np.ones(values.shape[:3], np.uint8))
     rng = np.random.default_rng(5)
-    x = torch.as_tensor(10 ** rng.uniform(8, 13, n), device="cuda:0")
-    y = torch.as_tensor(10 ** rng.uniform(np.log10(5e9), np.log10(2e11), n), device="cuda:0")
+    x = torch.as_tensor(10 ** rng.uniform(np.log10(ax_rho[0]), np.log10(ax_rho[-1]), n), device="cuda:0")
+    y = torch.as_tensor(10 ** rng.uniform(np.log10(ax_T[0]), np.log10(ax_T[-1]), n), device="cuda:0")
     zi = torch.zeros(n, dtype=torch.int32, device="cuda:0")
     res = {"workload": f"{n} random (rho, T) lookups in the {ax_rho.size}x{ax_T.size} cooling table (Akima "
                        "spline over 4x4 grid points, FP64)"}
@@ -325,12 +332,13 @@ def cache_lookups(hc, torch, tab: dict, n: int = 1 << 22) -> dict:
     el = time.perf_counter() - t0
     conv = ret == 1
     res["solve_T"] = {"ms": round(el * 1e3, 3), "solves_per_s": n / el, "converged": int(conv.sum().item()),
-                      "max_rel_T_err_converged": float(((T - y).abs() / y)[conv].max().item())}
+                      "max_rel_T_err_converged": (float(((T - y).abs() / y)[conv].max().item())
+                                                  if bool(conv.any()) else None)}
     cache.close()
     return res
 
 
-def eos_table_cpu(tab: dict, sample: int = 24) -> dict:
+def eos_table_cpu(hc, tab: dict, sample: int = 24) -> dict:
     """The reference's own abundance_ (oracle/_ref/libeosref.so) on an evenly
     strided sample of the same grid, on one core; parity of that sample."""
     import numpy as np
@@ -342,7 +350,7 @@ def eos_table_cpu(tab: dict, sample: int = 24) -> dict:
     R = EosRef()
     idx = np.linspace(0, tab["points"] - 1, sample).astype(int)
     t0 = time.perf_counter()
-    ref = np.array([R.abundance(tab["_rho"][i], tab["_T"][i], 0.5) for i in idx]).T
+    ref = np.array([R.abundance(tab["_rho"][i], tab["_T"][i], hc.c3_hdfac()) for i in idx]).T
     el = time.perf_counter() - t0
     got = tab["_vals"][:, idx]
     rel = float((np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)).max())
@@ -445,14 +453,16 @@ def main() -> None:
     if roof is not None:
         roof["traffic"] = measured_traffic(dom, n1, n2, n3)
 
+    eos_tab = None
+    if not (args.no_eos_table and args.no_tabulated):
+        eos_tab = eos_table(hc)
     tabulated = None
     if not args.no_tabulated:
-        tabulated = tabulated_torus(hc, args.steps, args.warmup)
-
-    eos_tab = None
+        tabulated = tabulated_torus(hc, eos_tab["_table"], args.steps, args.warmup)
     if not args.no_eos_table:
-        eos_tab = eos_table(hc, torch)
         eos_tab["cache_lookup"] = cache_lookups(hc, torch, eos_tab)
+    else:
+        eos_tab = None
 
     st = None
     if not args.no_learned and args.stress_n > 0:
@@ -470,8 +480,8 @@ def main() -> None:
         st.pop("_sample", None)
     if eos_tab is not None:
         if not args.no_cpu:
-            eos_tab["cpu_reference"] = eos_table_cpu(eos_tab)
-        for k in ("_rho", "_T", "_vals", "_axes"):
+            eos_tab["cpu_reference"] = eos_table_cpu(hc, eos_tab)
+        for k in ("_rho", "_T", "_vals", "_axes", "_table"):
             eos_tab.pop(k)
 
     out = {

@@ -31,7 +31,12 @@ _dp = POINTER(c_double)
 
 
 class HarmError(RuntimeError):
-    pass
+    """A library call returned nonzero; .code holds the returned code (11: the
+    step's dt fell below 1e-9, 14: a fail() event; see harm_step)."""
+
+    def __init__(self, msg: str, code: int = 1):
+        super().__init__(msg)
+        self.code = code
 
 
 class U2POpts(ctypes.Structure):
@@ -173,6 +178,32 @@ def eos_table_build(spec: EosSpec) -> dict:
     return {"rho": rho, "T": T, "e": e.reshape(spec.nr, spec.nt), "p": p.reshape(spec.nr, spec.nt)}
 
 
+def c3_hdfac(a: float = 0.9375, r: float = 12.0, m_bh_sun: float = 3.0) -> float:
+    """The reference's cooling-table disk-height factor at radius r (in GM/c^2):
+    HdFac = 1/Omega = (a + r^1.5) G M / c^3 seconds (cooleos.c:948-962, with its
+    constants GNEWT, MSUN, CL of cooleos.c:83-87)."""
+    G, MSUN, CL = 6.6742867e-8, 1.9892e33, 2.99792458e10
+    return (a + r ** 1.5) * G * m_bh_sun * MSUN / CL ** 3
+
+
+def fermi_eos_table(spec: EosSpec, hdfac: "float | None" = None, raw: bool = False):
+    """Host arrays of the step's tabulated EOS filled by the cooling-table
+    generator (harm_eos_table_fermi; abundance() per grid point on the GPU).
+    hdfac defaults to c3_hdfac(): the torus's pressure maximum, r = 12.  With
+    raw, also the generator's [6][nr*nt] output (e, p, Qnu, tau, n_e-, n_e+)."""
+    L = lib()
+    hd = c3_hdfac() if hdfac is None else hdfac
+    rho = np.empty(spec.nr)
+    T = np.empty(spec.nt)
+    e = np.empty(spec.nr * spec.nt)
+    p = np.empty(spec.nr * spec.nt)
+    r = np.empty(6 * spec.nr * spec.nt) if raw else None
+    _check(L.harm_eos_table_fermi(ctypes.byref(spec), hd, _ptr(rho), _ptr(T), _ptr(e), _ptr(p),
+                                  _ptr(r) if raw else None), "harm_eos_table_fermi")
+    tab = {"rho": rho, "T": T, "e": e.reshape(spec.nr, spec.nt), "p": p.reshape(spec.nr, spec.nt)}
+    return (tab, r.reshape(6, spec.nr * spec.nt)) if raw else tab
+
+
 class DeviceEos:
     """A table resident on the device (torch tensors) and its C descriptor."""
 
@@ -308,8 +339,9 @@ class Cache:
     def solve_T(self, rho, e, T, zindex=None, Tlo: float = 0.0, Thi: float = 0.0, ntrial: int = 300,
                 tolf: float = 1.0e-6):
         """The COOL_EOS temperature solve (harm_cache_solve_T) for device
-        tensors; T (the start) is overwritten. Bounds default to the y axis
-        ends moved in by 1e-6, as the reference sets them. -> (berr, ret)."""
+        tensors; T (the start) is overwritten. Bounds (both 0) default to the
+        ends of the cache's y axis moved in by 1e-6, as the reference sets them.
+        -> (berr, ret)."""
         import torch
 
         n = rho.shape[0]
@@ -318,8 +350,6 @@ class Cache:
         _need(T, (n,), "f64", "T")
         if zindex is not None:
             _need(zindex, (n,), "i32", "zindex")
-        if Tlo == 0.0 and Thi == 0.0:
-            Tlo, Thi = self.desc.ymin * (1.0 + 1.0e-6), self.desc.ymax * (1.0 - 1.0e-6)
         berr = torch.empty(n, dtype=torch.float64, device=rho.device)
         ret = torch.empty(n, dtype=torch.int32, device=rho.device)
         _sync_torch()
@@ -479,6 +509,7 @@ def lib() -> ctypes.CDLL:
     L.harm_stress_batch.argtypes = [POINTER(StressSpec), c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p]
     L.harm_eos_table_build.argtypes = [POINTER(EosSpec), _dp, _dp, _dp, _dp]
+    L.harm_eos_table_fermi.argtypes = [POINTER(EosSpec), c_double, _dp, _dp, _dp, _dp, _dp]
     L.harm_eos_eval.argtypes = [POINTER(EosTable), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                 c_void_p]
     L.harm_utoprim5d_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -514,7 +545,7 @@ def lib() -> ctypes.CDLL:
 def _check(rc: int, what: str) -> None:
     if rc != 0:
         msg = lib().harm_last_error().decode(errors="replace")
-        raise HarmError(f"{what} failed ({rc}): {msg}")
+        raise HarmError(f"{what} failed ({rc}): {msg}", rc)
 
 
 def _ptr(a: np.ndarray):

@@ -304,6 +304,16 @@ int harm_cache_solve_T(const harm_cache *c, const double *rho, const double *e, 
     if (!n) return 0;
     if (!rho || !e || !T) return harm_host::set_error("harm_cache_solve_T: rho, e and T are required", 1);
     if (ntrial < 1) return harm_host::set_error("harm_cache_solve_T: ntrial >= 1", 3);
+    if (Tlo == 0. && Thi == 0.) {
+        // the reference's bounds TMin (1 + 1e-6), TMax (1 - 1e-6) (cooleos.c:368-369)
+        // from the y axis the cache holds (a rolling axis' header range is not it)
+        double ends[2];
+        CCHK(hipMemcpy(&ends[0], c->vy, sizeof(double), hipMemcpyDeviceToHost));
+        CCHK(hipMemcpy(&ends[1], c->vy + (c->desc.y - 1), sizeof(double), hipMemcpyDeviceToHost));
+        Tlo = ends[0] * (1. + 1.e-6);
+        Thi = ends[1] * (1. - 1.e-6);
+    }
+    if (!(Tlo < Thi)) return harm_host::set_error("harm_cache_solve_T: needs Tlo < Thi", 3);
     harm::CacheView v{c->desc.x, c->desc.y, c->desc.z, c->desc.d, c->desc.xtype, c->desc.ytype, c->desc.ztype,
                       c->whole, c->vx, c->vy, c->vz, c->values, c->counted};
     hipStream_t s = (hipStream_t)stream;

@@ -65,7 +65,16 @@ struct DGrid {
     const double *fblk[3];  // column-major FACE1/FACE2/FACE3 blocks, FACE_STRIDE per column
     EosTab tab;             // tabulated EOS (used by the TAB kernel variants)
     int ref_derivs;         // tabulated EOS: zero dp/drho, dp/du as the reference's
+    // Inside harm_step: the step-state error word (tstate[4]).  Once a step has
+    // ended in an error every later kernel of the batch returns at once, so the
+    // state stays as the reference leaves it when it exits (step_ch.c:126-129).
+    // nullptr outside harm_step.
+    const double *halt;
 };
+
+// Wave-uniform: true when the batch has stopped (see DGrid::halt).
+__device__ __forceinline__ bool halted(const double *halt) { return halt && *halt != 0.; }
+__device__ __forceinline__ bool halted(const DGrid &g) { return halted(g.halt); }
 
 __device__ __forceinline__ size_t zidx(const DGrid &g, int i, int j, int h)
 {
@@ -207,8 +216,10 @@ struct CntMap {
 // counter k and adds its total with one atomic.
 constexpr int RED_BLOCKS = 64;
 __global__ void __launch_bounds__(256) k_sum_partials(const unsigned long long *__restrict__ part, size_t nb,
-                                                      CntMap map, unsigned long long *__restrict__ cnt)
+                                                      CntMap map, unsigned long long *__restrict__ cnt,
+                                                      const double *__restrict__ halt)
 {
+    if (halted(halt)) return;
     __shared__ unsigned long long red[256];
     const int k = blockIdx.y;
     unsigned long long s = 0;
@@ -226,8 +237,10 @@ __global__ void __launch_bounds__(256) k_sum_partials(const unsigned long long *
 // slot = min(slot, min over blocks of part[b]) where that minimum is < 1e9
 // (the CFL limit of a direction, step_ch.c:477-484); grid RED_BLOCKS.
 __global__ void __launch_bounds__(256) k_min_partials(const double *__restrict__ part, size_t nb,
-                                                      unsigned long long *__restrict__ slot)
+                                                      unsigned long long *__restrict__ slot,
+                                                      const double *__restrict__ halt)
 {
+    if (halted(halt)) return;
     __shared__ double red[256];
     double m = 1.e300;
     for (size_t b = (size_t)blockIdx.x * 256 + threadIdx.x; b < nb; b += (size_t)RED_BLOCKS * 256)
@@ -260,6 +273,7 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
        unsigned long long *__restrict__ cnt)
 {
     static_assert(!ROWS || DIR == 3, "row-local reconstruction is along phi only");
+    if (halted(g)) return;
     const int i0 = (DIR == 1) ? g.is : g.is - 1;
     const int j0 = (DIR == 2) ? g.js : g.js - 1;
     const int ni = g.ie + 1 - i0 + 1;
@@ -394,6 +408,7 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
              int fullstep, unsigned long long *__restrict__ cnt)
 {
     static_assert(DIR == 1 || DIR == 2, "strips run along X1 or X2");
+    if (halted(g)) return;
     // faces along DIR: [fs, fe]; the other index o over [os, oe]
     const int fs = (DIR == 1) ? g.is : g.js, fe = (DIR == 1) ? g.ie + 1 : g.je + 1;
     const int os = (DIR == 1) ? g.js - 1 : g.is - 1, oe = (DIR == 1) ? g.je + 1 : g.ie + 1;
@@ -517,6 +532,7 @@ __global__ void __launch_bounds__(TPB) k_emf(DGrid g, const double *__restrict__
                                              const double *__restrict__ F3, double *__restrict__ e1,
                                              double *__restrict__ e2, double *__restrict__ e3)
 {
+    if (halted(g)) return;
     const int ni = g.ie + 1 - g.is + 1, nj = g.je + 1 - g.js + 1;
     const unsigned idx = xcd_block() * TPB + threadIdx.x;
     if (idx >= (unsigned)ni * nj * g.N3) return;
@@ -693,6 +709,7 @@ __device__ __forceinline__ void flux_div32(const DGrid &g, const DivArgs &a, int
 // the update variants that read it instead of forming it.
 __global__ void __launch_bounds__(TPB) k_div(DGrid g, DivArgs a, double *__restrict__ S)
 {
+    if (halted(g)) return;
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
     const unsigned idx = xcd_block() * TPB + threadIdx.x;
@@ -734,6 +751,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
                                                 const double *__restrict__ tstate, double dtfac, int pi_is_pb,
                                                 FontList fl, unsigned long long *__restrict__ cpart, UpdateExtra ex)
 {
+    if (halted(g)) return;
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
@@ -907,6 +925,7 @@ constexpr int FONTFIX_BLOCKS = 256;
 __global__ void __launch_bounds__(TPB) k_fontfix(DGrid g, double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                  FontList fl, unsigned long long *__restrict__ cnt)
 {
+    if (halted(g)) return;
     const int n = min(*fl.count, fl.cap);
     int c_ff1 = 0, c_ff2 = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
     const int stride = gridDim.x * TPB;
@@ -1009,6 +1028,7 @@ __device__ __forceinline__ size_t bound_nthreads(const DGrid &g)
 __global__ void __launch_bounds__(TPB) k_bound(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag,
                                                int *__restrict__ font_count, unsigned long long *__restrict__ cnt)
 {
+    if (halted(g)) return;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx == 0 && font_count) *font_count = 0;
     int fatal = 0;
@@ -1123,11 +1143,18 @@ __device__ __forceinline__ void repair_zone(const DGrid &g, double *p, int8_t *p
     pf_live[z] = fl;
 }
 
+// Clustered bad zones are marked in a per-plane bitmap, bit (i*SN2 + j) of
+// plane h's row of nwords 64-bit words, and counted per plane: the repair walk
+// of a plane reads and writes that plane only (fixup_utoprim's stencil is
+// i+-1, j+-1 at fixed h), so the planes are repaired independently and each in
+// the reference's (i, j) order with no list, sort or capacity.
 __global__ void __launch_bounds__(TPB) k_fixutoprim_scan(DGrid g, double *__restrict__ p,
                                                          const int8_t *__restrict__ fin, int8_t *__restrict__ fout,
-                                                         int *__restrict__ list, int *__restrict__ count, int cap,
+                                                         unsigned long long *__restrict__ bits,
+                                                         int *__restrict__ pcount, unsigned nwords,
                                                          unsigned long long *__restrict__ cnt)
 {
+    if (halted(g)) return;
     // 8 zones per thread: the common all-good octet is one 8-byte load and store.
     const size_t z0 = ((size_t)blockIdx.x * TPB + threadIdx.x) * 8;
     int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
@@ -1154,8 +1181,8 @@ __global__ void __launch_bounds__(TPB) k_fixutoprim_scan(DGrid g, double *__rest
                     if (isolated) {
                         repair_zone(g, p, fout, i, j, h, false, fin, c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
                     } else {
-                        const int slot = atomicAdd(count, 1);
-                        if (slot < cap) list[slot] = (int)idx;
+                        atomicOr(&bits[(size_t)h * nwords + (col >> 6)], 1ull << (col & 63u));
+                        atomicAdd(&pcount[h], 1);
                     }
                 }
             }
@@ -1169,47 +1196,53 @@ __global__ void __launch_bounds__(TPB) k_fixutoprim_scan(DGrid g, double *__rest
     add_counter(cnt, HARM_FATAL, fatal);
 }
 
-// Serial repair of clustered bad zones in the reference's (h,i,j) order.
-__global__ void __launch_bounds__(1024) k_fixutoprim_serial(DGrid g, double *__restrict__ p, int8_t *__restrict__ f,
-                                                            int *__restrict__ list, int *__restrict__ count, int cap,
-                                                            unsigned long long *__restrict__ cnt,
-                                                            int *__restrict__ err)
+// Serial repair of the clustered bad zones of plane h = blockIdx.x (one wave
+// per plane) in the reference's (i, j) order.  The wave reads 64 bitmap words
+// at a time and lane 0 repairs the marked zones of the nonzero words in word
+// and bit order; each word is cleared after its zones are done, and the plane
+// count is reset, ready for the next call.
+__global__ void __launch_bounds__(64) k_fixutoprim_serial(DGrid g, double *__restrict__ p, int8_t *__restrict__ f,
+                                                          unsigned long long *__restrict__ bits,
+                                                          int *__restrict__ pcount, unsigned nwords,
+                                                          unsigned long long *__restrict__ cnt)
 {
-    const int n_all = *count;
-    if (n_all == 0) return;
-    if (n_all > cap) {
-        if (threadIdx.x == 0) atomicOr(err, 2);
-        return;
-    }
-    const int n = n_all;
-    int *sorted = list + cap;
-    auto key = [&](int z) -> long long {
-        const int h = z % g.N3;
-        const int col = z / g.N3;
-        return (long long)h * g.ncol + col;  // (h, i, j) lexicographic
-    };
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const long long ke = key(list[e]);
-        int rank = 0;
-        for (int q = 0; q < n; q++) rank += (key(list[q]) < ke);
-        sorted[rank] = list[e];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
-        for (int e = 0; e < n; e++) {
-            const int z = sorted[e];
-            const int h = z % g.N3;
-            const int col = z / g.N3;
-            repair_zone(g, p, f, col / g.SN2, col % g.SN2, h, true, nullptr, c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
+    if (halted(g)) return;
+    const int h = blockIdx.x;
+    if (pcount[h] == 0) return;
+    __shared__ unsigned long long sw[64];
+    unsigned long long *row = bits + (size_t)h * nwords;
+    int c_rep = 0, c_nost = 0, c_flr = 0, c_cap = 0, c_gam = 0, fatal = 0;
+    for (unsigned w0 = 0; w0 < nwords; w0 += 64) {
+        const unsigned w = w0 + threadIdx.x;
+        const unsigned long long word = (w < nwords) ? row[w] : 0ull;
+        sw[threadIdx.x] = word;
+        unsigned long long lanes = __ballot(word != 0ull);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            while (lanes) {
+                const int l = __ffsll((long long)lanes) - 1;
+                lanes &= lanes - 1;
+                unsigned long long b = sw[l];
+                while (b) {
+                    const int bit = __ffsll((long long)b) - 1;
+                    b &= b - 1;
+                    const unsigned col = (w0 + (unsigned)l) * 64u + (unsigned)bit;
+                    repair_zone(g, p, f, (int)(col / (unsigned)g.SN2), (int)(col % (unsigned)g.SN2), h, true, nullptr,
+                                c_rep, c_nost, c_flr, c_cap, c_gam, fatal);
+                }
+            }
         }
+        __syncthreads();
+        if (word) row[w] = 0ull;
+    }
+    if (threadIdx.x == 0) {
         atomicAdd(&cnt[HARM_FIXUP_REPAIRS], (unsigned long long)c_rep);
         atomicAdd(&cnt[HARM_FIXUP_NOSTENCIL], (unsigned long long)c_nost);
         atomicAdd(&cnt[HARM_FLOOR_ZONES], (unsigned long long)c_flr);
         atomicAdd(&cnt[HARM_GAMMA_CAP], (unsigned long long)c_cap);
         atomicAdd(&cnt[HARM_FIXUP_GAMMA], (unsigned long long)c_gam);
         atomicAdd(&cnt[HARM_FATAL], (unsigned long long)fatal);
-        *count = 0;
+        pcount[h] = 0;
     }
 }
 
@@ -1219,6 +1252,7 @@ __global__ void __launch_bounds__(1024) k_fixutoprim_serial(DGrid g, double *__r
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(TPB) k_mdot_partials(DGrid g, const double *__restrict__ p, double *__restrict__ part)
 {
+    if (halted(g)) return;
     __shared__ double s_in[TPB], s_out[TPB];
     const int h = blockIdx.x;
     double a_in = 0., a_out = 0.;
@@ -1253,11 +1287,32 @@ __global__ void __launch_bounds__(TPB) k_mdot_partials(DGrid g, const double *__
     }
 }
 
-// End of step_ch (step_ch.c:119-137): mass budget, dt control.  One thread.
-// tstate: [0] t, [1] dt, [2] mass_in, [3] mass_out, [4] error flags.
-__global__ void k_stepend(DGrid g, double *tstate, unsigned long long *ndt_slot, const double *part, int nparts)
+// Start of a harm_step batch: tstate[5] = the fail() event count so far, so
+// that k_stepend sees the events of its own step.  One thread.
+__global__ void k_step_begin(double *tstate, const unsigned long long *cnt)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (tstate[4] == 0.) tstate[5] = (double)cnt[HARM_FATAL];
+}
+
+// End of step_ch (step_ch.c:119-137): mass budget, dt control.  One thread.
+// tstate: [0] t, [1] dt, [2] mass_in, [3] mass_out, [4] error code (11: dt
+// below 1e-9, 14: a fail() event in this step), [5] fail() count at the start
+// of the step.  The reference exits inside fail() (diag.c:381-397); here the
+// step in which such an event happened completes, t is not advanced, and the
+// batch stops.  A dt below 1e-9 exits after diag_mass and before t += dt
+// (step_ch.c:119-129), which is where this kernel stops too.
+__global__ void k_stepend(DGrid g, double *tstate, unsigned long long *ndt_slot, const double *part, int nparts,
+                          const unsigned long long *cnt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (halted(g)) return;
+    const double nfatal = (double)cnt[HARM_FATAL];
+    if (nfatal != tstate[5]) {
+        tstate[5] = nfatal;
+        tstate[4] = 14.;
+        return;
+    }
     double mdot_in = 0., mdot_out = 0.;
     for (int q = 0; q < nparts; q++) {
         mdot_in += part[2 * q];
@@ -1761,8 +1816,13 @@ __global__ void __launch_bounds__(TPB) k_tracer_init(DGrid g, TrArgs a, const do
 // (tstate before k_stepend advances t).
 template <bool TAB>
 __global__ void __launch_bounds__(TPB) k_tracer_update(DGrid g, TrArgs a, const double *__restrict__ p,
-                                                       const double *__restrict__ tstate)
+                                                       const double *__restrict__ tstate,
+                                                       const unsigned long long *__restrict__ cnt)
 {
+    if (halted(g)) return;
+    // a step with a fail() event stops before update_tracers (tstate[5]: the
+    // event count at the start of the step, k_step_begin / k_stepend)
+    if (cnt && (double)cnt[HARM_FATAL] != tstate[5]) return;
     const long long k = (long long)blockIdx.x * TPB + threadIdx.x;
     if (k >= a.n || a.state[k] != HARM_TRACER_LIVE) return;
     const double t = tstate[0], Dt = tstate[1];
@@ -2278,10 +2338,13 @@ struct harm_ctx {
     unsigned long long *cnt;          // HARM_NCOUNTERS
     double *part;                     // reduction partials
     size_t npart;
-    int *fix_list, *fix_count, *fix_err;
+    unsigned long long *fix_bits;  // fixup_utoprim: clustered bad zones, [N3][fix_nwords] bitmap
+    int *fix_pcount;               // [N3] marked zones per plane
+    unsigned fix_nwords;
+    int *fix_err;
     FontList font;
     bool uniform;  // N3 % 64 == 0: a wave's lanes share one (i,j) column
-    int fix_cap;
+    int halted;    // error code of the step that stopped the last batch (11, 14), else 0
     int eos_kind;  // 0 Gamma-law, 1 tabulated (the table's device arrays belong to the caller)
     // learned Newton seed and training-pair capture
     int guess_mode;
@@ -2425,7 +2488,7 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
         LAUNCH(HARM_K_FLUX3, flux_kernel_any<3>(c, grid, pb, F, fullstep));
     if (fullstep) {
         hipLaunchKernelGGL(k_min_partials, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->dtpart, (size_t)grid.x,
-                           c->ndt_slot + (dir - 1));
+                           c->ndt_slot + (dir - 1), c->g.halt);
         HIPCHK(hipGetLastError());
     }
     return 0;
@@ -2460,9 +2523,9 @@ int fixup_utoprim(harm_ctx *c, double *p)
     const DGrid &g = c->g;
     LAUNCH(HARM_K_FIXUTOPRIM,
            hipLaunchKernelGGL(k_fixutoprim_scan, dim3(nblocks((g.nz + 7) / 8)), dim3(TPB), 0, c->stream, g, p, c->pflag,
-                              c->pflag2, c->fix_list, c->fix_count, c->fix_cap, c->cnt);
-           hipLaunchKernelGGL(k_fixutoprim_serial, dim3(1), dim3(1024), 0, c->stream, g, p, c->pflag2, c->fix_list,
-                              c->fix_count, c->fix_cap, c->cnt, c->fix_err));
+                              c->pflag2, c->fix_bits, c->fix_pcount, c->fix_nwords, c->cnt);
+           hipLaunchKernelGGL(k_fixutoprim_serial, dim3(g.N3), dim3(64), 0, c->stream, g, p, c->pflag2, c->fix_bits,
+                              c->fix_pcount, c->fix_nwords, c->cnt));
     std::swap(c->pflag, c->pflag2);
     return 0;
 }
@@ -2539,7 +2602,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
         CntMap map = {8, {HARM_C2P_CALLS, HARM_NEWTON_ITERS, HARM_NEWTON_KCONV, HARM_FAIL_2D, HARM_FIXUP_GAMMA,
                           HARM_FLOOR_ZONES, HARM_GAMMA_CAP, HARM_FATAL}};
         hipLaunchKernelGGL(k_sum_partials, dim3(RED_BLOCKS, 8), dim3(256), 0, c->stream, c->cpart, (size_t)nblocks(na), map,
-                           c->cnt);
+                           c->cnt, g.halt);
         HIPCHK(hipGetLastError());
     }
     if (g.font_fix)
@@ -2562,15 +2625,23 @@ int one_step(harm_ctx *c)
         const unsigned nb = nblocks((size_t)c->tr.a.n);
         if (c->eos_kind == 1)
             LAUNCH(HARM_K_TRACERS, hipLaunchKernelGGL(k_tracer_update<true>, dim3(nb), dim3(TPB), 0, c->stream, g,
-                                                      c->tr.a, c->p, c->tstate));
+                                                      c->tr.a, c->p, c->tstate, c->cnt));
         else
             LAUNCH(HARM_K_TRACERS, hipLaunchKernelGGL(k_tracer_update<false>, dim3(nb), dim3(TPB), 0, c->stream, g,
-                                                      c->tr.a, c->p, c->tstate));
+                                                      c->tr.a, c->p, c->tstate, c->cnt));
     }
     LAUNCH(HARM_K_STEPEND,
            hipLaunchKernelGGL(k_mdot_partials, dim3(g.N3), dim3(TPB), 0, c->stream, g, c->p, c->part);
-           hipLaunchKernelGGL(k_stepend, dim3(1), dim3(1), 0, c->stream, g, c->tstate, c->ndt_slot, c->part, g.N3));
+           hipLaunchKernelGGL(k_stepend, dim3(1), dim3(1), 0, c->stream, g, c->tstate, c->ndt_slot, c->part, g.N3,
+                              c->cnt));
     return 0;
+}
+
+const char *halt_message(int code)
+{
+    return code == 11 ? "timestep too small (dt < 1e-9, step_ch.c:126-129); the batch stopped after that step"
+                      : "a fail() event (diag.c:381-397: gamma, vchar or metric failure) in the last step; the batch "
+                        "stopped after that step";
 }
 
 int check_flags(harm_ctx *c)
@@ -2580,9 +2651,21 @@ int check_flags(harm_ctx *c)
     HIPCHK(hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&fe, c->fix_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (ts[4] == 11.) return set_err("timestep too small (dt < 1e-9, step_ch.c:126)", 11);
-    if (fe & 2) return set_err("fixup_utoprim: more clustered bad zones than the serial list holds", 12);
+    if (ts[4] == 11. || ts[4] == 14.) {
+        c->halted = (int)ts[4];
+        return set_err(halt_message(c->halted), c->halted);
+    }
     if (fe & 1) return set_err("more failed inversions in one half-step than the Font-fix list holds", 13);
+    return 0;
+}
+
+// Clears a stopped batch's error (the caller resets the clock or the state).
+int clear_halt(harm_ctx *c)
+{
+    const double z = 0.;
+    HIPCHK(hipMemcpyAsync(c->tstate + 4, &z, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->halted = 0;
     return 0;
 }
 
@@ -2673,16 +2756,17 @@ int ctx_alloc(harm_ctx *c)
     if ((rc = dalloc(&c->cnt, HARM_NCOUNTERS))) return rc;
     c->npart = std::max<size_t>(DIAG_NPART * nblocks(c->nz) + 16, 2 * (size_t)hg.N3 + 16);
     if ((rc = dalloc(&c->part, c->npart))) return rc;
-    c->font.cap = (int)std::min<size_t>(c->nz / 16 + 65536, (size_t)1 << 24);
+    // one slot per active zone: every inversion of a half-step may fail
+    c->font.cap = (int)((size_t)hg.N1 * hg.N2 * hg.N3);
     if ((rc = dalloc(&c->font.count, 1))) return rc;
     if ((rc = dalloc(&c->font.zone, (size_t)c->font.cap))) return rc;
     if ((rc = dalloc(&c->font.U, 8 * (size_t)c->font.cap))) return rc;
     c->uniform = (hg.N3 % 64) == 0;
-    c->fix_cap = 1 << 16;
-    if ((rc = dalloc(&c->fix_list, 2 * (size_t)c->fix_cap))) return rc;
-    if ((rc = dalloc(&c->fix_count, 1))) return rc;
+    c->fix_nwords = (unsigned)((ncol + 63) / 64);
+    if ((rc = dalloc(&c->fix_bits, (size_t)hg.N3 * c->fix_nwords))) return rc;
+    if ((rc = dalloc(&c->fix_pcount, (size_t)hg.N3))) return rc;
     if ((rc = dalloc(&c->fix_err, 1))) return rc;
-    c->font.overflow = c->fix_err;  // bit 1: fixup_utoprim list, bit 0: Font-fix list (atomicOr 1 / 2)
+    c->font.overflow = c->fix_err;  // bit 0: Font-fix list (cannot fill: one slot per active zone)
     unsigned long long big[3];
     const double b = 1.e9;
     for (int k = 0; k < 3; k++) std::memcpy(&big[k], &b, 8);
@@ -2731,7 +2815,7 @@ void ctx_free(harm_ctx *c)
     for (void *q : {(void *)c->p, (void *)c->ph, (void *)c->F1, (void *)c->F2, (void *)c->F3, (void *)c->e1,
                     (void *)c->e2, (void *)c->e3, (void *)c->S, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
                     (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->dtpart, (void *)c->cpart, (void *)c->cnt,
-                    (void *)c->part, (void *)c->fix_list, (void *)c->fix_count, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
+                    (void *)c->part, (void *)c->fix_bits, (void *)c->fix_pcount, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk})
         if (q) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
@@ -2888,6 +2972,7 @@ int harm_init_torus(harm_ctx *c)
     double ts[8] = {0., c->P.dt0, 0., 0., 0., 0., 0., 0.};
     HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->halted = 0;
     HIPCHK(hipFree(d_tor));
     // main.c:173-176
     double sums[5], dv, key;
@@ -2900,6 +2985,16 @@ int harm_init_torus(harm_ctx *c)
 int harm_step(harm_ctx *c, int nsteps)
 {
     if (!c) return set_err("null context");
+    if (c->halted) return set_err(halt_message(c->halted), c->halted);
+    if (nsteps <= 0) return 0;
+    // the step kernels read the error word and return once a step has set it
+    struct Gate {
+        harm_ctx *c;
+        ~Gate() { c->g.halt = nullptr; }
+    } gate{c};
+    c->g.halt = c->tstate + 4;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->tstate, c->cnt);
+    HIPCHK(hipGetLastError());
     for (int s = 0; s < nsteps; s++) {
         int rc = one_step(c);
         if (rc) return rc;
@@ -2932,7 +3027,7 @@ int harm_set_time(harm_ctx *c, double t, double dt)
     double ts[2] = {t, dt};
     HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return 0;
+    return clear_halt(c);
 }
 
 int harm_get_counters(harm_ctx *c, int64_t *out)
@@ -3346,9 +3441,10 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
     c->P.defcon = c->g.defcon = h.defcon;
     HIPCHK(hipMemcpyAsync(c->p, prim.data(), prim.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemsetAsync(c->pflag, 0, c->nz, c->stream));
-    const double ts[4] = {h.t, h.dt, h.mass_in, h.mass_out};
+    const double ts[5] = {h.t, h.dt, h.mass_in, h.mass_out, 0.};
     HIPCHK(hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->halted = 0;
     c->mass_start = h.mass_start;
     c->flux_valid = false;
     if (has_tb && c->tr.on) {
@@ -3719,9 +3815,11 @@ int harm_tracers_update(harm_ctx *c, double dt)
     HIPCHK(hipMemcpyAsync(c->S, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     const unsigned nb = nblocks((size_t)t.a.n);
     if (c->eos_kind == 1)
-        hipLaunchKernelGGL(k_tracer_update<true>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->S);
+        hipLaunchKernelGGL(k_tracer_update<true>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->S,
+                           (const unsigned long long *)nullptr);
     else
-        hipLaunchKernelGGL(k_tracer_update<false>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->S);
+        hipLaunchKernelGGL(k_tracer_update<false>, dim3(nb), dim3(TPB), 0, c->stream, c->g, t.a, c->p, c->S,
+                           (const unsigned long long *)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/harm_cool.h"
 #include "harm_fermi.hpp"
@@ -245,6 +246,61 @@ int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac
     hipLaunchKernelGGL(k_abundance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, rho, T, hdfac, out, n);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+// The step's tabulated EOS from this generator (the reference's COOL_EOS build
+// fills its per-radius cooling tables with the same abundance() points,
+// cooleos.c:908-973, calc_cache_val): e and p of every (rho, T) grid point at
+// one disk-height factor, in code units (cooleos.c:114-135: U_UNIT = RHO_UNIT
+// c^2, the u of pressure_rho0_u_CoolEOS is e / U_UNIT).
+int harm_eos_table_fermi(const harm_eos_spec *s, double hdfac, double *rho, double *T, double *e, double *p,
+                         double *raw)
+{
+    if (!s || !rho || !T || !e || !p) return harm_host::set_error("null argument", 1);
+    if (s->nr < 2 || s->nt < 2 || !(s->rho_unit > 0.) || !(s->lrho_hi > s->lrho_lo) || !(s->lT_hi > s->lT_lo) ||
+        !(hdfac > 0.))
+        return harm_host::set_error("bad EOS table spec (nr, nt >= 2, increasing ranges, hdfac > 0)", 4);
+    const double c = 2.99792458e10;  // cm/s, cooleos.c:87
+    const double u_unit = s->rho_unit * c * c;
+    const size_t nr = (size_t)s->nr, nt = (size_t)s->nt, n = nr * nt;
+    std::vector<double> in(3 * n), out(6 * n);
+    for (size_t i = 0; i < nr; i++) rho[i] = std::pow(10., s->lrho_lo + (s->lrho_hi - s->lrho_lo) * i / (nr - 1));
+    for (size_t j = 0; j < nt; j++) T[j] = std::pow(10., s->lT_lo + (s->lT_hi - s->lT_lo) * j / (nt - 1));
+    for (size_t i = 0; i < nr; i++)
+        for (size_t j = 0; j < nt; j++) {
+            in[i * nt + j] = rho[i];
+            in[n + i * nt + j] = T[j];
+            in[2 * n + i * nt + j] = hdfac;
+        }
+    double *d = nullptr;
+    FCHK(hipMalloc((void **)&d, 9 * n * sizeof(double)));
+    int rc = 0;
+    if (hipMemcpy(d, in.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = harm_host::set_error("harm_eos_table_fermi: copy to the device failed", 2);
+    if (!rc) rc = harm_eos_point_batch(d, d + n, d + 2 * n, d + 3 * n, n, nullptr);
+    if (!rc && hipMemcpy(out.data(), d + 3 * n, 6 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = harm_host::set_error("harm_eos_table_fermi: copy from the device failed", 2);
+    (void)hipFree(d);
+    if (rc) return rc;
+    if (raw) std::copy(out.begin(), out.end(), raw);
+    for (size_t i = 0; i < nr; i++) {
+        for (size_t j = 0; j < nt; j++) {
+            const double ev = out[i * nt + j], pv = out[n + i * nt + j];
+            if (!std::isfinite(ev) || !std::isfinite(pv) || !(ev > 0.) || !(pv > 0.))
+                return harm_host::set_error("harm_eos_table_fermi: no positive finite e, p at rho " +
+                                                std::to_string(rho[i]) + " T " + std::to_string(T[j]),
+                                            5);
+            // the table's inversion p(rho, u) needs e strictly increasing in T
+            if (j && !(ev > out[i * nt + j - 1]))
+                return harm_host::set_error("harm_eos_table_fermi: e does not grow with T at rho " +
+                                                std::to_string(rho[i]) + " T " + std::to_string(T[j]),
+                                            5);
+            e[i * nt + j] = ev / u_unit;
+            p[i * nt + j] = pv / u_unit;
+        }
+        rho[i] /= s->rho_unit;
+    }
     return 0;
 }
 

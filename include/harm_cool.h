@@ -81,8 +81,17 @@ void      harm_destroy(harm_ctx *c);
 /* Fishbone-Moncrief torus initial data; replaces init() (init.c:70-472). */
 int harm_init_torus(harm_ctx *c);
 
-/* One predictor/corrector step with floors, bounds, fixups and dt control;
- * replaces step_ch() (step_ch.c:90-140, decs.h:600).  nsteps >= 1. */
+/* nsteps predictor/corrector steps with floors, bounds, fixups and dt control;
+ * replaces step_ch() (step_ch.c:90-140, decs.h:600), called nsteps times.
+ * Where the reference exits, the batch stops after that step and returns:
+ *   11  dt < 1e-9 (step_ch.c:126-129): the step's state and mass budget are
+ *       kept, t is not advanced;
+ *   14  a fail() event in the step (diag.c:381-397: gamma, vchar or metric
+ *       failure; the reference exits inside the step, here the step completes),
+ *       t is not advanced.
+ * The steps after it launch nothing that changes the context.  A stopped
+ * context refuses further steps with the same code until harm_set_time,
+ * harm_init_torus or harm_restart_read resets it. */
 int harm_step(harm_ctx *c, int nsteps);
 
 /* Host <-> device state in the canonical layout (8*(N1+4)*(N2+4)*N3 doubles). */
@@ -394,6 +403,18 @@ int harm_fermi_rules(double *out /* 100 */);
  * thread of their point. */
 int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
                          void *stream);
+/* The step's tabulated EOS (harm_eos_table) from this generator: e and p of
+ * abundance() at every point of the spec's log-spaced (rho, T) grid at one
+ * disk-height factor hdfac [s] (the reference's per-radius cooling tables,
+ * cooleos.c:908-973, with HdFac = 1/Omega(r)), in code units: rho / rho_unit,
+ * e and p / (rho_unit c^2) (cooleos.c:114-135).  Host arrays rho[nr], T[nt],
+ * e[nr*nt], p[nr*nt]; raw (may be NULL) receives the generator's own output
+ * [6][nr*nt] in harm_eos_point_batch's units and order.  spec->ye is not used
+ * (the generator's electron fraction follows from chemical equilibrium).
+ * Returns 5 when a point has no positive finite e, p or e does not grow with T
+ * (the inversion along T needs it). */
+int harm_eos_table_fermi(const harm_eos_spec *spec, double hdfac, double *rho, double *T, double *e, double *p,
+                         double *raw);
 /* Per-species EOS of eos.f90 (energy density e and pressure p in MeV fm^-3,
  * entropy density s in k_B fm^-3) for n points; in is [nin][n], out [3][n]:
  *   HARM_EOS_NPE (eos_npe, eos.f90:31-113) nin 8: kT [MeV], eta_n, eta_p,
@@ -473,8 +494,10 @@ int harm_cache_lookup(const harm_cache *c, int mode, const int *dimmask, const d
  * 300, TMin (1 + 1e-6), TMax (1 - 1e-6), tolf 1e-6).  T is overwritten with the
  * result, berr (may be NULL) gets |F| at it and ret (may be NULL) the
  * reference's code: 1 converged, 2 or 0 not (T = the iterate of least error),
- * -1 a lookup outside the cache.  The density-range and negative-e branches of
- * those routines, and the unit conversions, stay with the caller. */
+ * -1 a lookup outside the cache.  Tlo = Thi = 0 selects the reference's bounds
+ * from the ends of the cache's y axis; otherwise Tlo < Thi is required.  The
+ * density-range and negative-e branches of those routines, and the unit
+ * conversions, stay with the caller. */
 int harm_cache_solve_T(const harm_cache *c, const double *rho, const double *e, const int *zindex, double *T,
                        double Tlo, double Thi, int ntrial, double tolf, double *berr, int *ret, size_t n, void *stream);
 
