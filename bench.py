@@ -23,7 +23,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 C5_BYTES = 64 + 128 + 128 + 8 + 64 + 64 + 3  # batched inversion, per zone (see stress())
+# FP64 operations of one Noble 2D inversion counted from the restated formulas
+# (add, sub, mul, div and sqrt one each; the 16-term contractions as 4 mul + 3
+# add per component): setup -- U/gdet and the alpha transform 25, the scalars
+# of utoprim_2d.c:240-266 (lower B, raise Q and n, B^2, Q.B, Q.n, Qt^2) 112,
+# u^2 and the starting W 35, the W_last test 13, x1_of_x0 10, the primitives
+# from (W, v^2) 36 -- 231; per Newton iteration func_vsq 75 (incl. 8 divisions
+# and 2 square roots) and the update with its error norm 5 -- 80, counted for
+# every iteration incl. the two extra ones.
+C5_FLOPS_SETUP, C5_FLOPS_PER_ITER = 231, 80
 
 # Algorithmic HBM bytes per zone of each kernel class (one launch covers every
 # active zone of one half-step).
@@ -60,25 +70,49 @@ def measured_traffic(kernel: str, n1: int, n2: int, n3: int):
     return None
 
 
-def cpu_baseline(threads: int) -> dict:
+# The oracle (C restatement, OpenMP over phi planes) against the reference's
+# own MPI build, both on the same 8-core Xeon (2.70 GHz) with 8 threads / 8
+# ranks at 288x256x256: the reference ran 1.41e6 zone-cycles/s (steady steps,
+# Makefile flags), the oracle 1.15e6-1.18e6 -- the oracle is 0.83x the
+# reference there.  At 64x64x32 the reference's 8 ranks reached 3.06e5, the
+# oracle 1.14e6 (the reference's halo exchanges dominate a grid that small).
+ORACLE_OVER_REFERENCE_FULLSIZE = 1.165e6 / 1.41e6
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(threads: int, nsteps: int = 3) -> dict:
+    """The CPU oracle on the headline grid itself (288x256x256), one warm-up
+    step after init, then nsteps timed steps on `threads` host threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from harm_cool_amd.params import torus_params
     from oracle_lib import Oracle, lib
 
     lib().orc_set_threads(threads)
-    P = torus_params(288, 256, 64)
+    P = torus_params(288, 256, 256)
     o = Oracle(P)
     o.init_torus()
     o.step()
-    nsteps = 10
     t0 = time.perf_counter()
     for _ in range(nsteps):
         o.step()
     el = time.perf_counter() - t0
-    return {"value": P.n1 * P.n2 * P.n3 * nsteps / el, "unit": "zone-cycles/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle (C restatement, OpenMP) on a 288x256x64 slab of the same torus, {nsteps} steps "
-                      f"after 1 warm-up, {el:.1f} s"}
+    del o
+    value = P.n1 * P.n2 * P.n3 * nsteps / el
+    return {"value": value, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the step, OpenMP over phi planes) on the same 288x256x256 torus, "
+                      f"{nsteps} steps after init and 1 warm-up step, {el:.1f} s",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "per_core": value / threads,
+            "oracle_over_reference_8_threads_288x256x256": round(ORACLE_OVER_REFERENCE_FULLSIZE, 3),
+            "reference_equivalent_value": value / ORACLE_OVER_REFERENCE_FULLSIZE}
 
 
 def stress_oracle_check(h: dict, prims: dict, gam: float, nsample: int = 4096) -> dict:
@@ -184,9 +218,12 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
             # algorithmic bytes per zone: U 64, gcov 128, gcon 128, gdet 8, guess in 64 and
             # result out 64, ret/iters/k_conv 3
             gbps = n * C5_BYTES / el / 1e9
-            rows[label] = {"n_iter_per_zone": it.double().mean().item(), "k_conv_per_zone": kc.double().mean().item(),
+            n_iter = it.double().mean().item()
+            tflops = n * (C5_FLOPS_SETUP + C5_FLOPS_PER_ITER * n_iter) / el / 1e12
+            rows[label] = {"n_iter_per_zone": n_iter, "k_conv_per_zone": kc.double().mean().item(),
                            "fail_rate": (ret != 0).double().mean().item(), "ms": round(el * 1e3, 2),
-                           "alg_GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
+                           "alg_GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4),
+                           "fp64_TFLOPs": round(tflops, 2), "fp64_frac": round(tflops / FP64_PEAK_TFLOPS, 4)}
             if kind == 0:
                 keep = slice(0, 1 << 20)
                 sample[label] = (ret[keep].cpu().numpy(), prim[:, keep].cpu().numpy())
@@ -241,6 +278,52 @@ def tabulated_torus(hc, host: dict, steps: int, warmup: int) -> dict:
             "steps": steps, "zone_cycles_per_s": nz * steps / el, "ms_per_step": el / steps * 1e3,
             "n_iter_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1), "fail_5d": c["fail_2d"],
             "frac_zones_in_table_rho_range": float(inside.mean()), "frac_zones_on_table": float(on.mean())}
+
+
+def fast_mode(hc, n: tuple, steps: int, warmup: int, dev_steps: int = 10) -> dict:
+    """The same step built with FMA contraction (libharmcool_fast.so), reported
+    beside the parity build: its step time, per-kernel times, and its deviation
+    from the parity build after dev_steps steps from the same initial state
+    (field-normalized max |diff| per primitive over the active zones)."""
+    import numpy as np
+    from harm_cool_amd.params import torus_params
+
+    P = torus_params(*n)
+    n1, n2, n3 = n
+    sl = (slice(None), slice(2, n1 + 2), slice(2, n2 + 2), slice(None))
+    gp = hc.Harm(P, device=0)
+    gp.init()
+    gp.step_ch(dev_steps)
+    pp, cp = gp.prims()[sl], gp.counters()
+    gp.close()
+    gf = hc.Harm(P, device=0, fast=True)
+    gf.init()
+    gf.step_ch(dev_steps)
+    pf, cf = gf.prims()[sl], gf.counters()
+    dev = {name: float(np.abs(pf[v] - pp[v]).max() / np.abs(pp[v]).max())
+           for v, name in enumerate(("rho", "u", "U1", "U2", "U3", "B1", "B2", "B3"))}
+    torus = pp[0] > 1e-3
+    rel_rho_torus = float((np.abs(pf[0] - pp[0]) / pp[0])[torus].max())
+    del pp, pf
+    gf.step_ch(warmup)
+    gf.reset_counters()
+    gf.time()
+    t0 = time.perf_counter()
+    gf.step_ch(steps)
+    el = time.perf_counter() - t0
+    c = gf.counters()
+    gf.set_timing(True)
+    gf.step_ch(3)
+    tim = gf.timing()
+    gf.set_timing(False)
+    gf.close()
+    kernels = {k: {"avg_ms": round(ms / cnt, 4), "launches": int(cnt)} for k, (ms, cnt) in tim.items() if cnt}
+    return {"workload": f"the same torus {n1}x{n2}x{n3} with FMA contraction in every kernel (libharmcool_fast.so)",
+            "steps": steps, "zone_cycles_per_s": n1 * n2 * n3 * steps / el, "ms_per_step": el / steps * 1e3,
+            "newton_iters_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1),
+            "deviation_after_steps": dev_steps, "field_normalized_max_diff_vs_parity": dev,
+            "max_rel_diff_rho_in_torus_vs_parity": rel_rho_torus,
+            "counters_equal_to_parity_after_steps": cp == cf, "kernels": kernels}
 
 
 def tracer_section(g, steps: int) -> dict:
@@ -373,6 +456,7 @@ def main() -> None:
     ap.add_argument("--no-tabulated", action="store_true", help="skip the tabulated-EOS torus section")
     ap.add_argument("--no-tracers", action="store_true", help="skip the tracer section")
     ap.add_argument("--no-eos-table", action="store_true", help="skip the cooling-table generator section")
+    ap.add_argument("--no-fast", action="store_true", help="skip the FMA-contracted (fast mode) section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -453,6 +537,10 @@ def main() -> None:
     if roof is not None:
         roof["traffic"] = measured_traffic(dom, n1, n2, n3)
 
+    fast = None
+    if not args.no_fast and os.path.exists(hc.LIB_FAST_PATH):
+        fast = fast_mode(hc, (n1, n2, n3), args.steps, args.warmup)
+
     eos_tab = None
     if not (args.no_eos_table and args.no_tabulated):
         eos_tab = eos_table(hc)
@@ -496,7 +584,8 @@ def main() -> None:
         "newton_kconv_per_zone": c["newton_kconv"] / max(c["c2p_calls"], 1),
         "newton_stopping_rule": "reference rule for both starts: |dW/W|+|dv2/v2| <= 1e-10, then 2 extra iterations",
         "learned_seed_torus": learned_rep, "stress_batch": st, "tabulated_eos_torus": tabulated,
-        "roofline": roof, "kernels": kernels, "tracers": tracers, "eos_table": eos_tab, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "fast_mode_fma": fast, "tracers": tracers, "eos_table": eos_tab,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

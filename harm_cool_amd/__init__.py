@@ -18,6 +18,9 @@ from .params import HarmParams, config1, config2, reference_default, torus_param
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("HARMCOOL_LIB", os.path.join(HERE, "libharmcool.so"))
+# The same kernels built with FMA contraction (the fast mode): results within
+# rounding of the parity build, not bit-identical to it or to the oracle.
+LIB_FAST_PATH = os.path.join(HERE, "libharmcool_fast.so")
 
 NCOUNTERS = 12
 COUNTER_NAMES = ["c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_font1",
@@ -444,7 +447,27 @@ def lib() -> ctypes.CDLL:
         import torch  # noqa: F401
     except ImportError:
         pass
-    L = ctypes.CDLL(LIB_PATH)
+    _lib = _bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+_lib_fast = None
+
+
+def lib_fast() -> ctypes.CDLL:
+    """Load libharmcool_fast.so, the FMA-contracted build of the same library
+    (same ABI); raise if absent."""
+    global _lib_fast
+    if _lib_fast is not None:
+        return _lib_fast
+    if not os.path.exists(LIB_FAST_PATH):
+        raise HarmError(f"{LIB_FAST_PATH} is missing: run harm_cool_amd/build.py (hipcc, gfx950)")
+    lib()  # torch's runtime first, as for the parity build
+    _lib_fast = _bind(ctypes.CDLL(LIB_FAST_PATH))
+    return _lib_fast
+
+
+def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_last_error.restype = c_char_p
     L.harm_create.restype = c_void_p
     L.harm_create.argtypes = [POINTER(HarmParams), c_int]
@@ -538,13 +561,12 @@ def lib() -> ctypes.CDLL:
                                     c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
     L.harm_cache_solve_T.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int,
                                      c_double, c_void_p, c_void_p, c_size_t, c_void_p]
-    _lib = L
     return L
 
 
-def _check(rc: int, what: str) -> None:
+def _check(rc: int, what: str, L: "ctypes.CDLL | None" = None) -> None:
     if rc != 0:
-        msg = lib().harm_last_error().decode(errors="replace")
+        msg = (L or lib()).harm_last_error().decode(errors="replace")
         raise HarmError(f"{what} failed ({rc}): {msg}", rc)
 
 
@@ -565,9 +587,10 @@ class Harm:
     resident on the GPU.  Methods mirror the reference driver: init() is the
     torus setup of init.c, step_ch() one predictor/corrector step of step_ch.c."""
 
-    def __init__(self, params: HarmParams, device: int = 0):
+    def __init__(self, params: HarmParams, device: int = 0, fast: bool = False):
         self.params = params
-        self.L = lib()
+        self.fast = fast
+        self.L = lib_fast() if fast else lib()
         self.h = self.L.harm_create(ctypes.byref(params), device)
         if not self.h:
             raise HarmError("harm_create failed: " + self.L.harm_last_error().decode(errors="replace"))
@@ -586,19 +609,19 @@ class Harm:
         return (8, self.n1 + 4, self.n2 + 4, self.n3)
 
     def init(self) -> None:
-        _check(self.L.harm_init_torus(self.h), "harm_init_torus")
+        _check(self.L.harm_init_torus(self.h), "harm_init_torus", self.L)
 
     def step_ch(self, nsteps: int = 1) -> None:
-        _check(self.L.harm_step(self.h, nsteps), "harm_step")
+        _check(self.L.harm_step(self.h, nsteps), "harm_step", self.L)
 
     def prims(self) -> np.ndarray:
         a = np.empty(self.shape, dtype=np.float64)
-        _check(self.L.harm_get_prims(self.h, _ptr(a), None), "harm_get_prims")
+        _check(self.L.harm_get_prims(self.h, _ptr(a), None), "harm_get_prims", self.L)
         return a
 
     def pflag(self) -> np.ndarray:
         f = np.empty(self.shape[1:], dtype=np.int8)
-        _check(self.L.harm_get_prims(self.h, None, f.ctypes.data_as(POINTER(c_int8))), "harm_get_prims")
+        _check(self.L.harm_get_prims(self.h, None, f.ctypes.data_as(POINTER(c_int8))), "harm_get_prims", self.L)
         return f
 
     def set_prims(self, a: np.ndarray, pflag: np.ndarray | None = None) -> None:
@@ -609,27 +632,27 @@ class Harm:
         if pflag is not None:
             pflag = np.ascontiguousarray(pflag, dtype=np.int8)
             fp = pflag.ctypes.data_as(POINTER(c_int8))
-        _check(self.L.harm_set_prims(self.h, _ptr(a), fp), "harm_set_prims")
+        _check(self.L.harm_set_prims(self.h, _ptr(a), fp), "harm_set_prims", self.L)
 
     def time(self):
         t, dt = c_double(), c_double()
-        _check(self.L.harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "harm_get_time")
+        _check(self.L.harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "harm_get_time", self.L)
         return t.value, dt.value
 
     def set_time(self, t: float, dt: float) -> None:
-        _check(self.L.harm_set_time(self.h, t, dt), "harm_set_time")
+        _check(self.L.harm_set_time(self.h, t, dt), "harm_set_time", self.L)
 
     def counters(self) -> dict:
         c = (c_int64 * NCOUNTERS)()
-        _check(self.L.harm_get_counters(self.h, c), "harm_get_counters")
+        _check(self.L.harm_get_counters(self.h, c), "harm_get_counters", self.L)
         return dict(zip(COUNTER_NAMES, list(c)))
 
     def reset_counters(self) -> None:
-        _check(self.L.harm_reset_counters(self.h), "harm_reset_counters")
+        _check(self.L.harm_reset_counters(self.h), "harm_reset_counters", self.L)
 
     def diag(self) -> dict:
         out = np.zeros(7)
-        _check(self.L.harm_diag(self.h, _ptr(out)), "harm_diag")
+        _check(self.L.harm_diag(self.h, _ptr(out)), "harm_diag", self.L)
         return dict(mass=out[0], energy=out[1], mdot_in=out[2], mdot_out=out[3],
                     mass_in=out[4], mass_out=out[5], divb_max=out[6])
 
@@ -638,7 +661,7 @@ class Harm:
         and its zone, the inner-face fluxes of the last full step, the mass
         budget."""
         r = DiagRec()
-        _check(self.L.harm_diag_log(self.h, ctypes.byref(r)), "harm_diag_log")
+        _check(self.L.harm_diag_log(self.h, ctypes.byref(r)), "harm_diag_log", self.L)
         return r
 
     def ener_line(self, rec: "DiagRec | None" = None) -> str:
@@ -660,18 +683,18 @@ class Harm:
         return r
 
     def set_mass_budget(self, mass_start: float, mass_in: float = 0.0, mass_out: float = 0.0) -> None:
-        _check(self.L.harm_set_mass_budget(self.h, mass_start, mass_in, mass_out), "harm_set_mass_budget")
+        _check(self.L.harm_set_mass_budget(self.h, mass_start, mass_in, mass_out), "harm_set_mass_budget", self.L)
 
     # -- tracers (tracers.c:109-654) -------------------------------------------
     def tracers_init(self, rho_unit: float = 1.0, t_unit: float = 1.0) -> int:
         """init_tracers(): seed the tracers; every later step advances them."""
         n = c_int64()
-        _check(self.L.harm_tracers_init(self.h, rho_unit, t_unit, ctypes.byref(n)), "harm_tracers_init")
+        _check(self.L.harm_tracers_init(self.h, rho_unit, t_unit, ctypes.byref(n)), "harm_tracers_init", self.L)
         self._tracer_buf = np.zeros(n.value, dtype=TRACER_DTYPE)
         return n.value
 
     def tracers_stop(self) -> None:
-        _check(self.L.harm_tracers_stop(self.h), "harm_tracers_stop")
+        _check(self.L.harm_tracers_stop(self.h), "harm_tracers_stop", self.L)
 
     def _tracer_call(self, fn, what: str) -> np.ndarray:
         buf = getattr(self, "_tracer_buf", None)
@@ -682,7 +705,7 @@ class Harm:
         if rc == -1:
             self._tracer_buf = buf = np.zeros(n.value, dtype=TRACER_DTYPE)
             rc = fn(self.h, buf.ctypes.data, buf.size, ctypes.byref(n))
-        _check(rc, what)
+        _check(rc, what, self.L)
         return buf[:n.value].copy()
 
     def tracers_dump(self) -> np.ndarray:
@@ -695,14 +718,14 @@ class Harm:
 
     def tracers_update(self, dt: float) -> None:
         """update_tracers(dt) on the current state (harm_step already does it)."""
-        _check(self.L.harm_tracers_update(self.h, dt), "harm_tracers_update")
+        _check(self.L.harm_tracers_update(self.h, dt), "harm_tracers_update", self.L)
 
     def tracers_get(self):
         """(X[3, N3, N1, N2], state[N3, N1, N2]) of every slot."""
         n1, n2, n3 = self.params.n1, self.params.n2, self.params.n3
         X = np.empty((3, n3, n1, n2))
         st = np.empty((n3, n1, n2), dtype=np.int8)
-        _check(self.L.harm_tracers_get(self.h, X.ctypes.data, st.ctypes.data), "harm_tracers_get")
+        _check(self.L.harm_tracers_get(self.h, X.ctypes.data, st.ctypes.data), "harm_tracers_get", self.L)
         return X, st
 
     def tracers_set(self, X: np.ndarray, state: np.ndarray) -> None:
@@ -711,7 +734,7 @@ class Harm:
         n = self.params.n1 * self.params.n2 * self.params.n3
         if X.size != 3 * n or st.size != n:
             raise HarmError("tracers_set: X must hold 3*N1*N2*N3 values and state N1*N2*N3")
-        _check(self.L.harm_tracers_set(self.h, X.ctypes.data, st.ctypes.data), "harm_tracers_set")
+        _check(self.L.harm_tracers_set(self.h, X.ctypes.data, st.ctypes.data), "harm_tracers_set", self.L)
 
     def write_tracer_files(self, recs: np.ndarray, events: "np.ndarray | None" = None,
                            root: str = "tracers") -> None:
@@ -731,12 +754,12 @@ class Harm:
             os.replace(src, tracer_name(d, r["h"], r["i"], r["j"], n3))
 
     def set_timing(self, on: bool) -> None:
-        _check(self.L.harm_set_timing(self.h, 1 if on else 0), "harm_set_timing")
+        _check(self.L.harm_set_timing(self.h, 1 if on else 0), "harm_set_timing", self.L)
 
     def timing(self) -> dict:
         ms = np.zeros(NKERNELS)
         n = (c_int64 * NKERNELS)()
-        _check(self.L.harm_get_timing(self.h, _ptr(ms), n), "harm_get_timing")
+        _check(self.L.harm_get_timing(self.h, _ptr(ms), n), "harm_get_timing", self.L)
         return {k: (ms[i], n[i]) for i, k in enumerate(KERNEL_NAMES)}
 
     def debug_fluxes(self, fullstep: int = 0):
@@ -744,40 +767,40 @@ class Harm:
         F2 = np.zeros(self.shape)
         F3 = np.zeros(self.shape)
         nd = np.zeros(3)
-        _check(self.L.harm_debug_fluxes(self.h, fullstep, _ptr(F1), _ptr(F2), _ptr(F3), _ptr(nd)), "harm_debug_fluxes")
+        _check(self.L.harm_debug_fluxes(self.h, fullstep, _ptr(F1), _ptr(F2), _ptr(F3), _ptr(nd)), "harm_debug_fluxes", self.L)
         return F1, F2, F3, nd
 
     def debug_bound(self) -> None:
-        _check(self.L.harm_debug_bound(self.h), "harm_debug_bound")
+        _check(self.L.harm_debug_bound(self.h), "harm_debug_bound", self.L)
 
     def debug_fixup_utoprim(self) -> None:
-        _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim")
+        _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim", self.L)
 
     def restart_write(self, path: str, **driver) -> None:
         """restart_write() (restart.c:73-184); ``driver`` sets the header fields
         the run driver owns (nstep, dump counters, clocks, ener_size, ...)."""
         h = RestartHdr.of(driver)
-        _check(self.L.harm_restart_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_write")
+        _check(self.L.harm_restart_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_write", self.L)
 
     def dump_write(self, path: str, **driver) -> None:
         """dump() ASCII file (dump.c:1710-1990); ``driver`` gives nstep, the
         DT* clocks, the dump/image/rdump counters and ``failed``."""
         h = RestartHdr.of(driver)
-        _check(self.L.harm_dump_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_dump_write")
+        _check(self.L.harm_dump_write(self.h, os.fsencode(path), ctypes.byref(h)), "harm_dump_write", self.L)
 
     def dump_columns(self, z0: int = 0, n: "int | None" = None) -> np.ndarray:
         """The dump's 46 columns of active zones z0..z0+n-1 in (i, j, h) order."""
         total = self.params.n1 * self.params.n2 * self.params.n3
         n = total - z0 if n is None else n
         out = np.empty((n, DUMP_NCOL))
-        _check(self.L.harm_dump_columns(self.h, z0, n, out.ctypes.data), "harm_dump_columns")
+        _check(self.L.harm_dump_columns(self.h, z0, n, out.ctypes.data), "harm_dump_columns", self.L)
         return out
 
     def restart_read(self, path: str) -> dict:
         """restart_init()/restart_read() (restart.c:195-376) into this context;
         returns the header."""
         h = RestartHdr()
-        _check(self.L.harm_restart_read(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_read")
+        _check(self.L.harm_restart_read(self.h, os.fsencode(path), ctypes.byref(h)), "harm_restart_read", self.L)
         for k in ("cour", "lim", "tf", "defcon"):
             setattr(self.params, k, getattr(h, k))
         return h.as_dict()
@@ -785,13 +808,13 @@ class Harm:
     def set_eos(self, tab: "DeviceEos | None", ref_derivs: bool = False) -> None:
         """Tabulated EOS for the step (None: back to the Gamma-law EOS)."""
         _check(self.L.harm_set_eos(self.h, ctypes.byref(tab.desc) if tab is not None else None, int(ref_derivs)),
-               "harm_set_eos")
+               "harm_set_eos", self.L)
         self._eos_ref = tab
 
     def set_guess(self, mode: int, mlp: "Mlp | None" = None) -> None:
         """Newton start of the inversion: 0 previous-step primitives (the
         reference's guess), 1 learned seed with previous-step fallback."""
-        _check(self.L.harm_set_guess(self.h, mode, mlp.h if mlp is not None else None), "harm_set_guess")
+        _check(self.L.harm_set_guess(self.h, mode, mlp.h if mlp is not None else None), "harm_set_guess", self.L)
         self._mlp_ref = mlp
 
     def set_capture(self, feat=None, targ=None, valid=None, stride: int = 1) -> None:
@@ -799,18 +822,18 @@ class Harm:
         targ[2,cap] (float64), valid[cap] (uint8); feat=None stops capturing."""
         _sync_torch()
         if feat is None:
-            _check(self.L.harm_set_capture(self.h, None, None, None, 0, 1), "harm_set_capture")
+            _check(self.L.harm_set_capture(self.h, None, None, None, 0, 1), "harm_set_capture", self.L)
             self._cap_ref = None
             return
         cap = feat.shape[1]
         _train_set(feat, targ, valid)
         _check(self.L.harm_set_capture(self.h, feat.data_ptr(), targ.data_ptr(), valid.data_ptr(), cap, stride),
-               "harm_set_capture")
+               "harm_set_capture", self.L)
         self._cap_ref = (feat, targ, valid)
 
     def capture_count(self) -> int:
         n = c_size_t()
-        _check(self.L.harm_get_capture_count(self.h, ctypes.byref(n)), "harm_get_capture_count")
+        _check(self.L.harm_get_capture_count(self.h, ctypes.byref(n)), "harm_get_capture_count", self.L)
         return n.value
 
     def geometry(self):
@@ -819,7 +842,7 @@ class Harm:
         gcon = np.zeros((5, 16, ncol))
         gdet = np.zeros((5, ncol))
         conn = np.zeros((64, ncol))
-        _check(self.L.harm_get_geometry(self.h, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn)), "harm_get_geometry")
+        _check(self.L.harm_get_geometry(self.h, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn)), "harm_get_geometry", self.L)
         sh = (self.n1 + 4, self.n2 + 4)
         return (gcov.reshape(5, 4, 4, *sh), gcon.reshape(5, 4, 4, *sh), gdet.reshape(5, *sh),
                 conn.reshape(4, 4, 4, *sh))

@@ -1,10 +1,12 @@
-// harm_eos.cpp -- the EOS table generator of libharmcool.so (host).
+// harm_eos.cpp -- a closed-form EOS table for the tabulated step (host).
 //
 // The reference fills its per-radius tables from a Fortran Fermi-gas /
 // nuclear-statistical-equilibrium code (abundance.f90, eos.f90,
-// gauss_integral.f90); that generator is not part of this build.  This one
-// fills the same kind of table -- e(rho, T) and p(rho, T) on a log-spaced grid,
-// Ye fixed -- from a closed-form EOS: ultra-relativistic degenerate electrons
+// gauss_integral.f90), which this library restates and runs on the GPU
+// (harm_eos_table_fermi, harm_fermi.hip).  This function fills the same kind of
+// table -- e(rho, T) and p(rho, T) on a log-spaced grid, Ye fixed -- from a
+// closed-form EOS instead, for tests of the interpolant and the tabulated step
+// that need no generator run: ultra-relativistic degenerate electrons
 // (p = K (Ye rho)^(4/3), e = 3p), an ideal gas of nucleons and electrons
 // ((1 + Ye) rho/m_u k T, e = 3/2 p), and radiation (a T^4 / 3, e = a T^4).
 // e grows strictly with T at fixed rho, which the inversion p(rho, u) needs.
