@@ -532,6 +532,8 @@ def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_stress_batch.argtypes = [POINTER(StressSpec), c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p]
     L.harm_eos_table_build.argtypes = [POINTER(EosSpec), _dp, _dp, _dp, _dp]
+    L.Utoprim_5d.argtypes = [_dp, _dp, _dp, c_double, _dp]
+    L.harm_set_u2p_eos.argtypes = [c_void_p]
     L.harm_eos_table_fermi.argtypes = [POINTER(EosSpec), c_double, _dp, _dp, _dp, _dp, _dp]
     L.harm_eos_eval.argtypes = [POINTER(EosTable), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                 c_void_p]
@@ -870,15 +872,29 @@ def set_gamma(gam: float) -> None:
     lib().harm_set_gamma(gam)
 
 
-def Utoprim_2d(U, gcov, gcon, gdet: float, prim) -> int:
-    """The reference's Utoprim_2d (utoprim_2d.c:126) for one zone, run on the GPU.
-    prim is updated in place (a float64 numpy array of 8)."""
-    L = lib()
+def set_u2p_eos(tab: "DeviceEos | None") -> None:
+    """The calling thread's tabulated EOS for Utoprim_5d (None: the Gamma law)."""
+    _check(lib().harm_set_u2p_eos(ctypes.byref(tab.desc) if tab is not None else None), "harm_set_u2p_eos")
+
+
+def _u2p_one(fn, U, gcov, gcon, gdet: float, prim) -> int:
     U = np.ascontiguousarray(U, dtype=np.float64)
     gcov = np.ascontiguousarray(gcov, dtype=np.float64)
     gcon = np.ascontiguousarray(gcon, dtype=np.float64)
     assert prim.dtype == np.float64 and prim.flags.c_contiguous
-    return L.Utoprim_2d(_ptr(U), _ptr(gcov), _ptr(gcon), gdet, _ptr(prim))
+    return fn(_ptr(U), _ptr(gcov), _ptr(gcon), gdet, _ptr(prim))
+
+
+def Utoprim_2d(U, gcov, gcon, gdet: float, prim) -> int:
+    """The reference's Utoprim_2d (utoprim_2d.c:126) for one zone, run on the GPU.
+    prim is updated in place (a float64 numpy array of 8)."""
+    return _u2p_one(lib().Utoprim_2d, U, gcov, gcon, gdet, prim)
+
+
+def Utoprim_5d(U, gcov, gcon, gdet: float, prim) -> int:
+    """The reference's Utoprim_5d (utoprim_5d.c:179) for one zone, run on the GPU
+    with the thread's EOS (set_gamma, set_u2p_eos).  prim is updated in place."""
+    return _u2p_one(lib().Utoprim_5d, U, gcov, gcon, gdet, prim)
 
 
 MLP_NPARAMS = 9026

@@ -2244,7 +2244,8 @@ __global__ void __launch_bounds__(TPB) k_stress_gen(StressSpec sp, size_t n, dou
 __global__ void __launch_bounds__(TPB) k_c2p5_batch(const double *__restrict__ U, const double *__restrict__ gcov,
                                                     const double *__restrict__ gcon, const double *__restrict__ gdet,
                                                     double *__restrict__ prim, int8_t *__restrict__ ret,
-                                                    uint8_t *__restrict__ iters, size_t n, double gam, EosTab tab)
+                                                    uint8_t *__restrict__ iters, size_t n, double gam, EosTab tab,
+                                                    int *__restrict__ ret_full)
 {
     const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (z >= n) return;
@@ -2270,6 +2271,7 @@ __global__ void __launch_bounds__(TPB) k_c2p5_batch(const double *__restrict__ U
 #pragma unroll
     for (int k = 0; k < NPR; k++) prim[(size_t)k * n + z] = pr[k];
     ret[z] = (int8_t)r;
+    if (ret_full) ret_full[z] = r;
     if (iters) iters[z] = (uint8_t)niter;
 }
 
@@ -3609,7 +3611,7 @@ int harm_utoprim5d_batch(const double *U, const double *gcov, const double *gcon
     int rc;
     if ((rc = eos_tab_of(tab, e))) return rc;
     hipLaunchKernelGGL(k_c2p5_batch, dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon, gdet, prim,
-                       ret, iters, n, gam, e);
+                       ret, iters, n, gam, e, (int *)nullptr);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -3652,21 +3654,40 @@ int harm_debug_copy(size_t nbytes, int repeats, double *ms_per_copy)
     return 0;
 }
 
-static double g_gamma = 4. / 3.;
+// The reference-signature inversions (Utoprim_2d, Utoprim_5d) take their EOS
+// from the reference's globals (gam, the COOL_EOS tables); here each calling
+// thread sets its own with harm_set_gamma / harm_set_u2p_eos.
+static thread_local double g_gamma = 4. / 3.;
+static thread_local EosTab g_u2p_tab;  // nr == 0: the Gamma law
 static std::mutex g_u2p_mu;
 
 void harm_set_gamma(double gam) { g_gamma = gam; }
 
-int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8])
+int harm_set_u2p_eos(const harm_eos_table *tab) { return eos_tab_of(tab, g_u2p_tab); }
+
+namespace {
+
+// Staging for one zone on the current device: 49 doubles (U, gcov, gcon, gdet,
+// prim), an int8 and an int code; allocated on a device's first call and kept
+// (the caller holds g_u2p_mu).
+struct U2pStage {
+    double *buf;
+    int8_t *ret8;
+    int *ret;
+};
+
+int u2p_stage(U2pStage &st, const double *U, const double (*gcov)[4], const double (*gcon)[4], double gdet,
+              const double *prim)
 {
-    std::lock_guard<std::mutex> lk(g_u2p_mu);
-    static double *dbuf = nullptr;
-    static int8_t *dret = nullptr;
-    static int *dfull = nullptr;
-    if (!dbuf) {
-        HIPCHK(hipMalloc((void **)&dbuf, 49 * sizeof(double)));
-        HIPCHK(hipMalloc((void **)&dret, 8));
-        HIPCHK(hipMalloc((void **)&dfull, sizeof(int)));
+    static U2pStage per_dev[64];
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_err("device index out of range", 4);
+    U2pStage &s = per_dev[dev];
+    if (!s.buf) {
+        HIPCHK(hipMalloc((void **)&s.buf, 49 * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&s.ret8, 8));
+        HIPCHK(hipMalloc((void **)&s.ret, sizeof(int)));
     }
     double h[49];
     std::memcpy(h, U, 8 * sizeof(double));
@@ -3674,15 +3695,48 @@ int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, d
     std::memcpy(h + 24, gcon, 16 * sizeof(double));
     h[40] = gdet;
     std::memcpy(h + 41, prim, 8 * sizeof(double));
-    HIPCHK(hipMemcpy(dbuf, h, sizeof(h), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.buf, h, sizeof(h), hipMemcpyHostToDevice));
+    st = s;
+    return 0;
+}
+
+int u2p_unstage(const U2pStage &s, double *prim, int &r)
+{
+    HIPCHK(hipMemcpy(prim, s.buf + 41, 8 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&r, s.ret, sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // namespace
+
+int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8])
+{
+    std::lock_guard<std::mutex> lk(g_u2p_mu);
+    U2pStage s;
+    int rc = u2p_stage(s, U, gcov, gcon, gdet, prim);
+    if (rc) return rc;
     MlpDev mv;
     std::memset(&mv, 0, sizeof(mv));
-    hipLaunchKernelGGL((k_c2p_batch<false>), dim3(1), dim3(TPB), 0, 0, dbuf, dbuf + 8, dbuf + 24, dbuf + 40,
-                       dbuf + 41, dret, (uint8_t *)nullptr, (uint8_t *)nullptr, (size_t)1, g_gamma, 0, 0., dfull, mv);
+    hipLaunchKernelGGL((k_c2p_batch<false>), dim3(1), dim3(TPB), 0, 0, s.buf, s.buf + 8, s.buf + 24, s.buf + 40,
+                       s.buf + 41, s.ret8, (uint8_t *)nullptr, (uint8_t *)nullptr, (size_t)1, g_gamma, 0, 0., s.ret,
+                       mv);
     HIPCHK(hipGetLastError());
-    int r;
-    HIPCHK(hipMemcpy(prim, dbuf + 41, 8 * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&r, dfull, sizeof(int), hipMemcpyDeviceToHost));
+    int r = 0;
+    if ((rc = u2p_unstage(s, prim, r))) return rc;
+    return r;
+}
+
+int Utoprim_5d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8])
+{
+    std::lock_guard<std::mutex> lk(g_u2p_mu);
+    U2pStage s;
+    int rc = u2p_stage(s, U, gcov, gcon, gdet, prim);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_c2p5_batch, dim3(1), dim3(TPB), 0, 0, s.buf, s.buf + 8, s.buf + 24, s.buf + 40, s.buf + 41,
+                       s.ret8, (uint8_t *)nullptr, (size_t)1, g_gamma, g_u2p_tab, s.ret);
+    HIPCHK(hipGetLastError());
+    int r = 0;
+    if ((rc = u2p_unstage(s, prim, r))) return rc;
     return r;
 }
 

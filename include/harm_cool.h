@@ -528,10 +528,18 @@ int harm_restart_peek(const char *path, harm_restart_hdr *hdr);
 int harm_restart_write_host(const char *path, const harm_restart_hdr *hdr, const double *prim, const double *temp);
 int harm_restart_read_host(const char *path, harm_restart_hdr *hdr, double *prim, double *temp, int *has_temp);
 
-/* Exact reference signature (utoprim_2d.c:126, decs.h:608-609), evaluated on the
- * GPU for one zone; gam is taken from harm_set_gamma() (default 4/3, init.c:91). */
+/* Exact reference signatures, evaluated on the GPU (current device) for one zone.
+ * The reference takes the EOS from its globals (gam, init.c:91; the COOL_EOS
+ * tables); here each calling thread sets its own: harm_set_gamma (default 4/3)
+ * and harm_set_u2p_eos (a tabulated EOS for Utoprim_5d, device arrays that must
+ * stay alive; NULL: the Gamma law).  Calls from several threads are serialized.
+ * Utoprim_2d: utoprim_2d.c:126, decs.h:608-609.
+ * Utoprim_5d: utoprim_5d.c:179-239, decs.h:610-611 (return code i*100 + j as
+ * utoprim_5d.c:162 defines it, not truncated). */
 int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8]);
+int Utoprim_5d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, double prim[8]);
 void harm_set_gamma(double gam);
+int harm_set_u2p_eos(const harm_eos_table *tab);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,14 @@ def test_library_exports_symbol(name):
     assert hasattr(L, name), name
 
 
+def test_fast_build_exports_the_same_abi():
+    """libharmcool_fast.so (FMA-contracted device code) is a drop-in for the
+    parity build: every declared entry point is there too."""
+    F = ctypes.CDLL(hc.LIB_FAST_PATH)
+    missing = [n for n in declared_functions() if not hasattr(F, n)]
+    assert not missing, missing
+
+
 def test_params_struct_layout():
     # 6 int32 + 18 doubles + 4 uint32, no padding holes beyond pad0
     assert ctypes.sizeof(HarmParams) == 6 * 4 + 18 * 8 + 4 * 4
@@ -46,3 +54,5 @@ def test_product_fails_loudly_without_gpu():
         pass
     with pytest.raises(hc.HarmError):
         hc.Harm(hc.config1())
+    with pytest.raises(hc.HarmError):
+        hc.Harm(hc.config1(), fast=True)

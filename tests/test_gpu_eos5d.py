@@ -99,3 +99,48 @@ def test_5d_tabulated_batch_matches_oracle(table):
     assert ok.mean() > 0.95
     rel = np.abs(prim[ok, 0] / P[ok, 0] - 1)
     assert np.median(rel) < 1e-12
+
+
+def _exact_vs_oracle(U, gcov, gcon, gdet, guess, eos_host):
+    """The exact-signature Utoprim_5d zone by zone against the oracle: the full
+    return code (not the char the step stores), primitives bitwise."""
+    L = olib()
+    for z in range(len(gdet)):
+        p = np.ascontiguousarray(guess[z].copy())
+        r = hc.Utoprim_5d(U[z], gcov[z].reshape(4, 4), gcon[z].reshape(4, 4), float(gdet[z]), p)
+        q = np.ascontiguousarray(guess[z].copy())
+        ni = ctypes.c_int()
+        ro = L.orc_utoprim_5d(_ptr(np.ascontiguousarray(U[z])), _ptr(np.ascontiguousarray(gcov[z])),
+                              _ptr(np.ascontiguousarray(gcon[z])), float(gdet[z]), _ptr(q), GAM,
+                              ctypes.byref(eos_host) if eos_host is not None else None, ctypes.byref(ni))
+        assert r == ro, (z, r, ro)
+        assert np.array_equal(p, q), z
+
+
+def test_exact_signature_utoprim_5d_gamma_law_and_table(table):
+    """Utoprim_5d (utoprim_5d.c:179, decs.h:610-611) with the thread's Gamma and
+    tabulated EOS (harm_set_gamma, harm_set_u2p_eos)."""
+    _torch()
+    d = np.load(os.path.join(GOLD, "utoprim.npz"))
+    hc.set_gamma(GAM)
+    hc.set_u2p_eos(None)
+    _exact_vs_oracle(d["U"][:200], d["gcov"][:200], d["gcon"][:200], d["gdet"][:200], d["guess"][:200], None)
+    tab = hc.DeviceEos(table, GAM)
+    eos = orc_eos(table, GAM)
+    L = olib()
+    spec = hc.stress_spec(0, rho_lo=table["rho"][0], rho_hi=table["rho"][-1], urho_lo=1e-2, urho_hi=3.,
+                          lorentz_hi=5., sigma_hi=1.)
+    n = 200
+    b = {k: v.cpu().numpy() for k, v in hc.stress_batch(n, spec).items()}
+    P = b["prim_true"].T.copy()
+    gcov, gcon, gdet = b["gcov"].T.copy(), b["gcon"].T.copy(), b["gdet"].copy()
+    U = np.zeros((n, 8))
+    for z in range(n):
+        L.orc_primtoU_eos(_ptr(np.ascontiguousarray(P[z])), _ptr(np.ascontiguousarray(gcov[z])),
+                          _ptr(np.ascontiguousarray(gcon[z])), float(gdet[z]), GAM, ctypes.byref(eos), _ptr(U[z]))
+    guess = P * (1 + 0.01 * np.random.default_rng(2).uniform(-1, 1, P.shape))
+    hc.set_u2p_eos(tab)
+    try:
+        _exact_vs_oracle(U, gcov, gcon, gdet, guess, eos)
+    finally:
+        hc.set_u2p_eos(None)
