@@ -325,6 +325,51 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
                 p_r[k] = pc[k] - 0.5 * dq[k];
             }
         }
+    } else if (LIM == LIM_PPM && act && !zero_face) {
+        // six zones m-3 .. m+2 along DIR; a zone whose five-zone stencil leaves
+        // the two ghost layers (the first and last radial zones' neighbours
+        // outside) keeps the linear MC state, so m-3 / m+2 are read only where
+        // the stencil is inside the grid
+        size_t zz[6];
+        bool ppm_l = true, ppm_r = true;
+        if (DIR == 3) {
+            const size_t base = z - h;
+#pragma unroll
+            for (int q = 0; q < 6; q++) zz[q] = base + (h + 3 * g.N3 + q - 3) % g.N3;
+        } else {
+            const size_t s = (DIR == 1) ? (size_t)g.SN2 * g.N3 : (size_t)g.N3;
+            const int m = (DIR == 1) ? i : j, lo = (DIR == 1) ? g.is : g.js, hi = (DIR == 1) ? g.ie : g.je;
+            ppm_l = m - 1 >= lo && m - 1 <= hi;
+            ppm_r = m >= lo && m <= hi;
+            zz[0] = ppm_l ? z - 3 * s : z - 2 * s;
+            zz[1] = z - 2 * s;
+            zz[2] = z - s;
+            zz[3] = z;
+            zz[4] = z + s;
+            zz[5] = ppm_r ? z + 2 * s : z + s;
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            double y[6], dq[4];
+#pragma unroll
+            for (int q = 0; q < 6; q++) y[q] = pb[k * nz + zz[q]];
+#pragma unroll
+            for (int q = 0; q < 4; q++) dq[q] = slope_lim(LIM_MC, y[q], y[q + 1], y[q + 2]);
+            // dq[q] is the slope of zone q+1 (zones m-2 .. m+1)
+            double l, r;
+            if (ppm_l) {
+                ppm_para(y[1], y[2], y[3], dq[0], dq[1], dq[2], l, r);
+                p_l[k] = r;
+            } else {
+                p_l[k] = y[2] + 0.5 * dq[1];
+            }
+            if (ppm_r) {
+                ppm_para(y[2], y[3], y[4], dq[1], dq[2], dq[3], l, r);
+                p_r[k] = l;
+            } else {
+                p_r[k] = y[3] - 0.5 * dq[2];
+            }
+        }
     } else if (act && !zero_face) {
         size_t zm2, zm1, zp1;
         if (DIR == 1) {
@@ -2434,8 +2479,21 @@ void flux_kernel(harm_ctx *c, dim3 grid, const double *pb, double *F, int fullst
 template <int DIR>
 void flux_kernel_any(harm_ctx *c, dim3 &grid, const double *pb, double *F, int fullstep)
 {
-    const bool tab = c->eos_kind == 1, mc = c->g.lim == 0;
-    if (c->uniform && DIR != 3) {
+    const bool tab = c->eos_kind == 1, mc = c->g.lim == LIM_MC;
+    if (c->g.lim == LIM_PPM) {
+        // six zones per face, one face per thread
+        if (c->uniform) {
+            if (tab)
+                flux_kernel<DIR, true, true, LIM_PPM>(c, grid, pb, F, fullstep);
+            else
+                flux_kernel<DIR, true, false, LIM_PPM>(c, grid, pb, F, fullstep);
+        } else {
+            if (tab)
+                flux_kernel<DIR, false, true, LIM_PPM>(c, grid, pb, F, fullstep);
+            else
+                flux_kernel<DIR, false, false, LIM_PPM>(c, grid, pb, F, fullstep);
+        }
+    } else if (c->uniform && DIR != 3) {
         // strips of STRIP faces along DIR x one other index x 64 phi zones
         const DGrid &g = c->g;
         const size_t nf = (DIR == 1) ? (size_t)(g.ie + 2 - g.is) : (size_t)(g.je + 2 - g.js);
@@ -2874,7 +2932,7 @@ harm_ctx *harm_create(const harm_params *p, int device)
         set_err("grid too large: (n1+4)*(n2+4)*n3 must stay below 2^31 zones per device (32-bit zone indices)");
         return nullptr;
     }
-    if (p->lim < 0 || p->lim > 2) {
+    if (p->lim < 0 || p->lim > 3) {
         set_err("unknown slope limiter");
         return nullptr;
     }
@@ -3431,7 +3489,7 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
             return set_err(std::string("restart file: ") + f.name +
                                " differs from the context's (create the context from harm_restart_peek's header)",
                            3);
-    if (h.lim < 0 || h.lim > 2) return set_err("restart file: unknown slope limiter", 3);
+    if (h.lim < 0 || h.lim > 3) return set_err("restart file: unknown slope limiter", 3);
     std::vector<double> prim(8 * c->nz);
     harm_host::TracerBlock tb;
     int has_tb = 0;

@@ -471,6 +471,34 @@ HD double slope_lim(int lim, double y1, double y2, double y3)
     return Dqp;
 }
 
+// Slope-limiter codes: the reference's MC, VANL, MINM (decs.h:214-216) and the
+// piecewise-parabolic option, which the reference does not have.
+constexpr int LIM_MC = 0, LIM_PPM = 3;
+
+// Piecewise-parabolic face values of one zone (Colella & Woodward 1984,
+// J. Comput. Phys. 54, 174, eqs. 1.6 and 1.10) on the MC slopes of slope_lim:
+// y0 the zone's value, ym/yp its neighbours, dqm/dq0/dqp the three zones' MC
+// slopes.  Interface a(m+1/2) = (y(m) + y(m+1))/2 - (dq(m+1) - dq(m))/6 (the
+// same expression from both zones of an interface, so both start from the same
+// bits); a local extremum flattens the zone; a parabola that overshoots beyond
+// one face has the other face's value moved to 3 y0 - 2 (overshooting face).
+HD void ppm_para(double ym, double y0, double yp, double dqm, double dq0, double dqp, double &l, double &r)
+{
+    double ql = 0.5 * (ym + y0) - (1. / 6.) * (dq0 - dqm);
+    double qr = 0.5 * (y0 + yp) - (1. / 6.) * (dqp - dq0);
+    const double qd = qr - ql, qe = 6. * (y0 - 0.5 * (ql + qr));
+    if ((qr - y0) * (y0 - ql) <= 0.) {
+        ql = y0;
+        qr = y0;
+    } else if (qd * (qd - qe) < 0.) {
+        ql = 3. * y0 - 2. * qr;
+    } else if (qd * (qd + qe) < 0.) {
+        qr = 3. * y0 - 2. * ql;
+    }
+    l = ql;
+    r = qr;
+}
+
 // One face of step_ch.c:429-475: reconstructed states -> numerical flux and ctop.
 template <class E>
 HD void face_flux(const double *p_l, const double *p_r, const Geom &geom, int dir, const E &eos, double hllf,

@@ -13,6 +13,7 @@ import math
 from ctypes import c_double, c_int32, c_uint32
 
 MC, VANL, MINM = 0, 1, 2  # slope limiters (decs.h:214-216)
+PPM = 3  # piecewise-parabolic reconstruction on MC slopes (no counterpart in the reference)
 
 
 class HarmParams(ctypes.Structure):

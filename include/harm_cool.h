@@ -28,7 +28,8 @@ extern "C" {
  * init.c:90-161, step_ch.c:66-67, u2p_defs.h:79-96). */
 typedef struct harm_params {
     int32_t n1, n2, n3;      /* active zones in X1 (r), X2 (theta), X3 (phi); n3 = 1 or even >= 4 */
-    int32_t lim;             /* slope limiter: 0 MC, 1 van Leer, 2 minmod (interp.c:49-93) */
+    int32_t lim;             /* slope limiter: 0 MC, 1 van Leer, 2 minmod (interp.c:49-93);
+                                3 PPM (Colella & Woodward 1984 on MC slopes; not in the reference) */
     int32_t font_fix;        /* 1: Font-fix fallbacks on inversion failure (step_ch.c:277-300) */
     int32_t pad0;
     double a, gam;           /* BH spin, adiabatic index */

@@ -59,6 +59,7 @@
 #define MC 0
 #define VANL 1
 #define MINM 2
+#define PPM 3 /* not in the reference (interp.c:49-93 has MC, VANL, MINM only) */
 
 /* decs.h:147-171 */
 #define RHOMIN (1.e-6)
@@ -747,6 +748,39 @@ static double slope_lim(int lim, double y1, double y2, double y3)
         else if (fabs(Dqm) < fabs(Dqp)) return Dqm;
         else return Dqp;
     }
+}
+
+/* Piecewise-parabolic reconstruction of one zone (Colella & Woodward 1984,
+ * J. Comput. Phys. 54, 174, eqs. 1.6 and 1.10), built on the MC-limited slopes
+ * slope_lim() gives (interp.c:49-93; the reference has no PPM).  y0 is the
+ * zone's value, ym/yp its neighbours along the direction, dqm/dq0/dqp the MC
+ * slopes of the three zones.  Returns the zone's left and right face values:
+ *   interface  a(m+1/2) = (y(m) + y(m+1))/2 - (dq(m+1) - dq(m))/6
+ *   monotone   a local extremum flattens the zone (l = r = y0); a parabola
+ *              that overshoots moves the far face value to 3 y0 - 2 (near one).
+ * The expression of a shared interface is the same from both zones, so the two
+ * zones start from identical bits. */
+static void para(double ym, double y0, double yp, double dqm, double dq0, double dqp, double *l, double *r)
+{
+    double ql = 0.5 * (ym + y0) - (MUT(13) ? 0.125 : 1. / 6.) * (dq0 - dqm);
+    double qr = 0.5 * (y0 + yp) - (MUT(13) ? 0.125 : 1. / 6.) * (dqp - dq0);
+    const double qd = qr - ql, qe = 6. * (y0 - 0.5 * (ql + qr));
+    if ((qr - y0) * (y0 - ql) <= 0.) {
+        ql = y0;
+        qr = y0;
+    } else if (qd * (qd - qe) < 0.) {
+        ql = MUT(14) ? 3. * y0 - 2. * ql : 3. * y0 - 2. * qr;
+    } else if (qd * (qd + qe) < 0.) {
+        qr = 3. * y0 - 2. * ql;
+    }
+    *l = ql;
+    *r = qr;
+}
+
+double orc_para(double ym, double y0, double yp, double dqm, double dq0, double dqp, double *l, double *r)
+{
+    para(ym, y0, yp, dqm, dq0, dqp, l, r);
+    return *r - *l;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1741,6 +1775,32 @@ static void face_flux(const double *p_l, const double *p_r, const geom_t *geom, 
     *ctop_out = ctop;
 }
 
+/* PPM needs zones m-2 .. m+2 along dir; the grid holds 2 ghost layers in r
+ * and theta (phi is periodic). */
+static int ppm_zone(const orc_ctx *o, int dir, int i, int j)
+{
+    if (dir == 1) return i - 2 >= o->istart - 2 && i + 2 <= o->istop + 2 && !(MUT(15) && i == o->istart);
+    if (dir == 2) return j - 2 >= o->jstart - 2 && j + 2 <= o->jstop + 2;
+    return 1;
+}
+
+/* PPM face values of zone (h, i, j) along the direction (hdel, idel, jdel)
+ * from pr and the MC slopes in o->dq: left into l, right into r (each may be
+ * NULL). */
+static void ppm_face(const orc_ctx *o, const double *pr, int h, int i, int j, int hdel, int idel, int jdel, double *l,
+                     double *r)
+{
+    const int hm = hwrap(o, h - hdel), hp = hwrap(o, h + hdel);
+    for (int k = 0; k < NPR; k++) {
+        double ql, qr;
+        para(PV(o, pr, hm, i - idel, j - jdel)[k], PV(o, pr, h, i, j)[k], PV(o, pr, hp, i + idel, j + jdel)[k],
+             PV(o, o->dq, hm, i - idel, j - jdel)[k], PV(o, o->dq, h, i, j)[k], PV(o, o->dq, hp, i + idel, j + jdel)[k],
+             &ql, &qr);
+        if (l) l[k] = ql;
+        if (r) r[k] = qr;
+    }
+}
+
 /* step_ch.c:331-504 */
 static double fluxcalc(orc_ctx *o, const double *pr, double *F, int dir, int fullstep, int64_t *cnt)
 {
@@ -1757,7 +1817,7 @@ static double fluxcalc(orc_ctx *o, const double *pr, double *F, int dir, int ful
         for (int j = o->jstart - 1; j <= o->jstop + 1; j++)
             for (int k = 0; k < NPR; k++)
                 PV(o, o->dq, h, i, j)[k] =
-                    slope_lim(o->P.lim, PV(o, pr, hwrap(o, h - hdel), i - idel, j - jdel)[k], PV(o, pr, h, i, j)[k],
+                    slope_lim(o->P.lim == PPM ? MC : o->P.lim, PV(o, pr, hwrap(o, h - hdel), i - idel, j - jdel)[k], PV(o, pr, h, i, j)[k],
                               PV(o, pr, (MUT(11) && h + hdel >= N3) ? h : hwrap(o, h + hdel), i + idel, j + jdel)[k]);
     }
 
@@ -1781,6 +1841,13 @@ static double fluxcalc(orc_ctx *o, const double *pr, double *F, int dir, int ful
             for (int k = 0; k < NPR; k++) {
                 p_l[k] = PV(o, pr, hm, i - idel, j - jdel)[k] + 0.5 * PV(o, o->dq, hm, i - idel, j - jdel)[k];
                 p_r[k] = PV(o, pr, h, i, j)[k] - 0.5 * PV(o, o->dq, h, i, j)[k];
+            }
+            if (o->P.lim == PPM) {
+                /* the zone left of the face gives its right value, the zone right
+                 * of it its left value; a zone whose 5-zone stencil leaves the
+                 * grid (the 2 ghost layers) keeps the linear MC state */
+                if (ppm_zone(o, dir, i - idel, j - jdel)) ppm_face(o, pr, hm, i - idel, j - jdel, hdel, idel, jdel, NULL, p_l);
+                if (ppm_zone(o, dir, i, j)) ppm_face(o, pr, h, i, j, hdel, idel, jdel, p_r, NULL);
             }
             get_geometry(o, i, j, face, &geom);
             face_flux(p_l, p_r, &geom, dir, o->P.gam, o->P.hllf, o->P.laxf, Fz, &ctop, &fatal);

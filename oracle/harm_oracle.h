@@ -29,7 +29,7 @@ extern "C" {
 /* Same field layout as harm_params in include/harm_cool.h. */
 typedef struct orc_params {
     int32_t n1, n2, n3;      /* active zones in r (X1), theta (X2), phi (X3)      */
-    int32_t lim;             /* slope limiter: 0 MC, 1 van Leer, 2 minmod          */
+    int32_t lim;             /* slope limiter: 0 MC, 1 van Leer, 2 minmod, 3 PPM   */
     int32_t font_fix;        /* 1: Font fix fallbacks on inversion failure         */
     int32_t pad0;
     double a, gam;           /* BH spin, adiabatic index                          */
@@ -162,6 +162,9 @@ void orc_conn_func(const orc_params *p, const double X[4], const double *gcon, d
 double orc_gdet_func(const double *gcov);
 int  orc_invert_matrix(const double *A, double *Ainv);
 double orc_slope_lim(int lim, double y1, double y2, double y3);
+/* PPM face values of one zone from its value, its neighbours and their MC
+ * slopes (Colella & Woodward 1984; lim 3, not in the reference); returns r - l. */
+double orc_para(double ym, double y0, double yp, double dqm, double dq0, double dqp, double *l, double *r);
 double orc_kiss_uniform(uint32_t z, uint32_t w, uint32_t jsr, uint32_t jcong);
 /* One face of fluxcalc (step_ch.c:407-487): left/right states -> flux, ctop. */
 void orc_face_flux(const double *p_l, const double *p_r, const double *gcov,

@@ -90,9 +90,10 @@ def test_mc_limiter_is_the_textbook_monotonized_central_slope():
         assert L.orc_slope_lim(0, y1, y2, y3) == want
 
 
+@pytest.mark.parametrize("lim", [0, 3])
 @pytest.mark.parametrize("shape", [(64, 64, 1), (24, 16, 8)])
-def test_flux_ct_keeps_divb_at_roundoff(shape):
-    o = Oracle(torus_params(*shape))
+def test_flux_ct_keeps_divb_at_roundoff(shape, lim):
+    o = Oracle(torus_params(*shape, lim=lim))
     o.init_torus()
     d0 = o.divb_max()
     for _ in range(5):
@@ -100,10 +101,13 @@ def test_flux_ct_keeps_divb_at_roundoff(shape):
     assert d0 < 1e-12 and o.divb_max() < 1e-12
 
 
-def test_equatorial_reflection_symmetry():
+@pytest.mark.parametrize("lim", [0, 3])
+def test_equatorial_reflection_symmetry(lim):
     """Without the random perturbation the torus is symmetric about theta=pi/2:
-    rho, u, u^r, u^phi even; u^theta, B^theta even/odd as a poloidal loop field."""
+    rho, u, u^r, u^phi even; u^theta, B^theta even/odd as a poloidal loop field
+    (MC, and PPM's left and right states as mirror images)."""
     P = config1()
+    P.lim = lim
     P.n1, P.n2 = 48, 48
     P.pert_amp = 0.0
     o = Oracle(P)
