@@ -106,6 +106,7 @@ class DiagRec(ctypes.Structure):
 TRACER_DTYPE = np.dtype([("t", "f8"), ("rho", "f8"), ("T", "f8"), ("ye", "f8"), ("r", "f8"), ("th", "f8"),
                          ("ph", "f8"), ("h", "i4"), ("i", "i4"), ("j", "i4"), ("kind", "i4")])
 TRACER_NONE, TRACER_LIVE, TRACER_IN, TRACER_OUT, TRACER_JET, TRACER_LOST = range(6)
+DUMP_NOALLOC_MAX_ZONES = 8192  # HARM_DUMP_NOALLOC_MAX_ZONES
 DUMP_NCOL = 46  # columns of an ASCII dump line (HARM_DUMP_NCOL)
 _TRACER_DIRS = {TRACER_IN: "in", TRACER_OUT: "out", TRACER_JET: "jet"}
 
@@ -496,6 +497,7 @@ def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_eos_point_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
     L.harm_dump_write.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
     L.harm_dump_columns.argtypes = [c_void_p, c_size_t, c_size_t, c_void_p]
+    L.harm_dump_columns_noalloc.argtypes = [c_void_p, c_size_t, c_size_t, c_void_p]
     L.harm_tracers_set.argtypes = [c_void_p, c_void_p, c_void_p]
     L.harm_device_state.argtypes = [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p)]
     L.harm_set_timing.argtypes = [c_void_p, c_int]
@@ -796,6 +798,13 @@ class Harm:
         n = total - z0 if n is None else n
         out = np.empty((n, DUMP_NCOL))
         _check(self.L.harm_dump_columns(self.h, z0, n, out.ctypes.data), "harm_dump_columns", self.L)
+        return out
+
+    def dump_columns_noalloc(self, z0: int, n: int) -> np.ndarray:
+        """dump_columns staged in the library's fixed device array (no device
+        allocation; n <= DUMP_NOALLOC_MAX_ZONES)."""
+        out = np.empty((n, DUMP_NCOL))
+        _check(self.L.harm_dump_columns_noalloc(self.h, z0, n, out.ctypes.data), "harm_dump_columns_noalloc", self.L)
         return out
 
     def restart_read(self, path: str) -> dict:

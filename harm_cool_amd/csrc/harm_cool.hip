@@ -2366,6 +2366,42 @@ inline unsigned nblocks8(size_t n) { return (nblocks(n) + 7u) & ~7u; }
 // ===========================================================================
 // Context
 // ===========================================================================
+// Staging of harm_dump_columns_noalloc: the columns of at most
+// HARM_DUMP_NOALLOC_MAX_ZONES zones (8192 x 46 doubles, 2.9 MiB), one copy per
+// device for the life of the process.  In host code `scratch` names a host-side
+// stand-in: kernels and copies take the device address harm_create looks up for
+// its device (scratch_addr).
+static __device__ __align__(256) double scratch[HARM_DUMP_NOALLOC_MAX_ZONES * DUMP_NCOL];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[SCRATCH_MAX_DEVICES];
+
+// Once per device, with that device current (called from harm_create).
+int scratch_resolve(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return set_err("device index out of range", 4);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev]) return 0;
+    double *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_err(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e), 2);
+    g_scratch_dev[dev] = ptr;
+    return 0;
+}
+
+double *scratch_addr(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[dev];
+}
+
+}  // namespace
+
 struct harm_ctx {
     harm_params P;
     harm_host::Grid hg;
@@ -2951,6 +2987,10 @@ harm_ctx *harm_create(const harm_params *p, int device)
     c->P = *p;
     if (c->P.Rin <= 0.) c->P.Rin = 0.98 * (1. + sqrt(1. - c->P.a * c->P.a));  // init.c:125
     (void)hipGetDevice(&c->device);
+    if (scratch_resolve(c->device)) {
+        delete c;
+        return nullptr;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_err("hipStreamCreate failed");
         delete c;
@@ -3982,6 +4022,7 @@ int harm_tracers_set(harm_ctx *c, const double *X, const int8_t *state)
 // ---------------------------------------------------------------------------
 // ASCII dump (dump.c:1710-1990)
 // ---------------------------------------------------------------------------
+
 namespace {
 
 // The 46 columns of zones z0..z0+n-1 of the active grid in (i, j, h) order.
@@ -4038,6 +4079,25 @@ int harm_dump_columns(harm_ctx *c, size_t z0, size_t n, double *cols)
     rc = dump_fill(c, d, cap, z0, n, cols);
     HIPCHK(hipFree(d));
     return rc;
+}
+
+int harm_dump_columns_noalloc(harm_ctx *c, size_t z0, size_t n, double *cols)
+{
+    if (!c || (n && !cols)) return set_err("null argument");
+    const DGrid &g = c->g;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    if (z0 > na || n > na - z0) return set_err("harm_dump_columns_noalloc: zone range outside the grid", 3);
+    if (n > (size_t)HARM_DUMP_NOALLOC_MAX_ZONES)
+        return set_err("harm_dump_columns_noalloc: n = " + std::to_string(n) +
+                           " zones exceeds HARM_DUMP_NOALLOC_MAX_ZONES (" +
+                           std::to_string(HARM_DUMP_NOALLOC_MAX_ZONES) + "); use harm_dump_columns",
+                       3);
+    if (!n) return 0;
+    double *d = scratch_addr(c->device);
+    if (!d) return set_err("harm_dump_columns_noalloc: no staging address for this device", 2);
+    // every column of the n rows copied back is written by k_dump_cols (4..45)
+    // or on the host (0..3), so nothing left by an earlier call shows through
+    return dump_fill(c, d, n, z0, n, cols);
 }
 
 int harm_dump_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr)

@@ -520,6 +520,15 @@ int harm_dump_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr);
  * stencil), u^mu, u_mu, b^mu, b_mu, vmin/vmax along X1 and X2, gdet, |v|,
  * p = (gam-1) u, c_s, u^r (sqrt(g_rr) u^r/u^t), Mach. */
 int harm_dump_columns(harm_ctx *c, size_t z0, size_t n, double *cols);
+/* harm_dump_columns without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The columns
+ * are staged in a fixed device array of HARM_DUMP_NOALLOC_MAX_ZONES zones
+ * (2.9 MiB per GPU, held until the process ends); n above that is rejected
+ * (error 3).  All calls on one GPU share that array: calls that overlap on one
+ * GPU (two threads, two contexts on the same device) race. */
+#define HARM_DUMP_NOALLOC_MAX_ZONES 8192
+int harm_dump_columns_noalloc(harm_ctx *c, size_t z0, size_t n, double *cols);
 /* The header alone (ener_size, which follows the body, reads as -1). */
 int harm_restart_peek(const char *path, harm_restart_hdr *hdr);
 /* Host-array forms (no device): prim in the canonical layout of
