@@ -206,13 +206,14 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
     for kind, kname in ((0, "perturbed_1pct"), (1, "cold")):
         b = hc.stress_batch(n, hc.stress_spec(kind))
         rows = {}
-        for label, net in (("plain", None), ("learned_stress_net", m), ("learned_torus_net", m_torus)):
+        for label, net, gate in (("plain", None, 0.0), ("learned_stress_net", m, 0.0),
+                                 ("learned_stress_net_gated_0.1", m, 0.1), ("learned_torus_net", m_torus, 0.0)):
             if label == "learned_torus_net" and net is None:
                 continue
             prim = b["guess"].clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            ret, it, kc = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], prim, gam, mlp=net)
+            ret, it, kc = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], prim, gam, mlp=net, gate=gate)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             # algorithmic bytes per zone: U 64, gcov 128, gcon 128, gdet 8, guess in 64 and
@@ -232,6 +233,17 @@ def stress(hc, torch, n, m_torus, train_steps, batch):
             res["_host"] = {k: v[..., :1 << 20].cpu().numpy() for k, v in b.items()}
         del b
     res["_sample"] = sample
+    # how good a start must be: plain Newton iterations/zone from the exact
+    # primitives perturbed by a relative eps (2^22 zones of the same distribution)
+    curve = []
+    for eps in (0.1, 0.03, 0.01, 3e-3, 1e-3, 3e-4, 1e-4, 1e-5, 1e-6, 0.0):
+        b = hc.stress_batch(1 << 22, hc.stress_spec(0, guess_pert=eps))
+        prim = b["guess"].clone()
+        ret, it, kc = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], prim, gam)
+        curve.append({"eps": eps, "n_iter_per_zone": round(it.double().mean().item(), 4),
+                      "fail_rate": (ret != 0).double().mean().item()})
+        del b, prim
+    res["iterations_vs_start_error"] = curve
     m.close()
     return res
 
@@ -457,6 +469,8 @@ def main() -> None:
     ap.add_argument("--no-tracers", action="store_true", help="skip the tracer section")
     ap.add_argument("--no-eos-table", action="store_true", help="skip the cooling-table generator section")
     ap.add_argument("--no-fast", action="store_true", help="skip the FMA-contracted (fast mode) section")
+    ap.add_argument("--lim", choices=("mc", "ppm"), default="mc",
+                    help="reconstruction of the headline torus (the headline workload is MC)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -471,7 +485,8 @@ def main() -> None:
 
     n1, n2, n3 = (int(x) for x in args.n.split("x"))
     nz = n1 * n2 * n3
-    P = torus_params(n1, n2, n3)
+    lim = {"mc": 0, "ppm": 3}[args.lim]
+    P = torus_params(n1, n2, n3, lim=lim)
     g = hc.Harm(P, device=0)
     g.init()
     learned_rep, m_torus = None, None
@@ -491,28 +506,34 @@ def main() -> None:
     value = nz * args.steps / el
     if m_torus is not None:
         p_plain = g.prims()
-        g.set_prims(p_start, f_start)
-        g.set_time(*t_start)
-        g.set_guess(1, m_torus)
-        g.reset_counters()
-        g.time()
-        t0 = time.perf_counter()
-        g.step_ch(args.steps)
-        el_l = time.perf_counter() - t0
-        cl = g.counters()
-        p_l = g.prims()
-        g.set_guess(0)
         sl = (slice(None), slice(2, n1 + 2), slice(2, n2 + 2), slice(None))
-        dev = [float(np.abs(p_l[sl][v] - p_plain[sl][v]).max() / np.abs(p_plain[sl][v]).max()) for v in range(8)]
         rho = p_plain[sl][0]
-        rel_rho = float((np.abs(p_l[sl][0] - rho) / rho).max())
-        del p_l, p_plain, rho
-        learned_rep.update({
-            "zone_cycles_per_s": nz * args.steps / el_l, "ms_per_step": el_l / args.steps * 1e3,
-            "n_iter_per_zone": cl["newton_iters"] / max(cl["c2p_calls"], 1),
-            "k_conv_per_zone": cl["newton_kconv"] / max(cl["c2p_calls"], 1),
-            "fail_2d": cl["fail_2d"], "fail_2d_plain": c["fail_2d"],
-            "max_field_normalized_diff_vs_plain": max(dev), "max_rel_diff_rho_vs_plain": rel_rho})
+        # mode 1: every zone from the network's seed; mode 2: only zones whose first
+        # Newton step from the previous-step guess exceeds the gate
+        for key, mode, gate in (("always", 1, None), ("gated_0.1", 2, 0.1), ("gated_1e-3", 2, 1e-3)):
+            g.set_prims(p_start, f_start)
+            g.set_time(*t_start)
+            g.set_guess(mode, m_torus, gate=gate)
+            g.reset_counters()
+            g.time()
+            t0 = time.perf_counter()
+            g.step_ch(args.steps)
+            el_l = time.perf_counter() - t0
+            cl = g.counters()
+            p_l = g.prims()
+            g.set_guess(0)
+            dev = [float(np.abs(p_l[sl][v] - p_plain[sl][v]).max() / np.abs(p_plain[sl][v]).max()) for v in range(8)]
+            rel_rho = float((np.abs(p_l[sl][0] - rho) / rho).max())
+            del p_l
+            learned_rep[key] = {
+                "zone_cycles_per_s": nz * args.steps / el_l, "ms_per_step": el_l / args.steps * 1e3,
+                "vs_plain_throughput": el / el_l,
+                "n_iter_per_zone": cl["newton_iters"] / max(cl["c2p_calls"], 1),
+                "k_conv_per_zone": cl["newton_kconv"] / max(cl["c2p_calls"], 1),
+                "fail_2d": cl["fail_2d"], "fail_2d_plain": c["fail_2d"],
+                "max_field_normalized_diff_vs_plain": max(dev), "max_rel_diff_rho_vs_plain": rel_rho,
+                "bitwise_equal_to_plain": bool(max(dev) == 0.0)}
+        del p_plain, rho
         g.set_prims(p_start, f_start)  # leave the state as the plain run had it for the timing pass below
         g.set_time(*t_start)
     del p_start, f_start
@@ -577,7 +598,8 @@ def main() -> None:
         "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded FM torus initial data)",
-        "config": {"workload": f"Fishbone-Moncrief torus {n1}x{n2}x{n3}, gamma-law 13/9, a=0.9375, HLL+MC, "
+        "config": {"workload": f"Fishbone-Moncrief torus {n1}x{n2}x{n3}, gamma-law 13/9, a=0.9375, "
+                               f"HLL+{args.lim.upper()}, "
                                "Courant 0.8, FP64, plain previous-step Newton guess",
                    "n1": n1, "n2": n2, "n3": n3},
         "newton_iters_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1),

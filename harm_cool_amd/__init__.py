@@ -44,7 +44,7 @@ class HarmError(RuntimeError):
 
 class U2POpts(ctypes.Structure):
     _fields_ = [("gam", c_double), ("font_fix", ctypes.c_int32), ("guess_mode", ctypes.c_int32),
-                ("mlp", c_void_p), ("K", c_double)]
+                ("mlp", c_void_p), ("K", c_double), ("gate_tau", c_double)]
 
 
 class EosSpec(ctypes.Structure):
@@ -548,6 +548,7 @@ def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_restart_write_host.argtypes = [c_char_p, POINTER(RestartHdr), _dp, _dp]
     L.harm_restart_read_host.argtypes = [c_char_p, POINTER(RestartHdr), _dp, _dp, POINTER(c_int)]
     L.harm_set_guess.argtypes = [c_void_p, c_int, c_void_p]
+    L.harm_set_guess_gate.argtypes = [c_void_p, c_double]
     L.harm_set_capture.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_int]
     L.harm_get_capture_count.argtypes = [c_void_p, POINTER(c_size_t)]
     L.harm_cache_axis.argtypes = [c_int, c_int, c_double, c_double, c_double, _dp, _dp]
@@ -822,9 +823,13 @@ class Harm:
                "harm_set_eos", self.L)
         self._eos_ref = tab
 
-    def set_guess(self, mode: int, mlp: "Mlp | None" = None) -> None:
+    def set_guess(self, mode: int, mlp: "Mlp | None" = None, gate: "float | None" = None) -> None:
         """Newton start of the inversion: 0 previous-step primitives (the
-        reference's guess), 1 learned seed with previous-step fallback."""
+        reference's guess), 1 learned seed with previous-step fallback, 2 the
+        learned seed only where the first Newton step from the previous-step
+        guess exceeds `gate` (|dW/W| + |dv2/v2|)."""
+        if gate is not None:
+            _check(self.L.harm_set_guess_gate(self.h, gate), "harm_set_guess_gate", self.L)
         _check(self.L.harm_set_guess(self.h, mode, mlp.h if mlp is not None else None), "harm_set_guess", self.L)
         self._mlp_ref = mlp
 
@@ -1049,7 +1054,7 @@ class Mlp:
 
 
 def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K: float = 0.0, stream=None,
-                    mlp: Mlp | None = None):
+                    mlp: Mlp | None = None, gate: float = 0.0):
     """Batched inversion on device tensors (torch, float64, SoA: U[8,n], gcov[16,n],
     gcon[16,n], gdet[n], prim[8,n] in/out).  Returns (ret int8[n], iters uint8[n],
     kconv uint8[n]) as device tensors."""
@@ -1062,8 +1067,9 @@ def utoprim2d_batch(U, gcov, gcon, gdet, prim, gam: float, font_fix: int = 0, K:
     ret = torch.empty(n, dtype=torch.int8, device=U.device)
     iters = torch.empty(n, dtype=torch.uint8, device=U.device)
     kconv = torch.empty(n, dtype=torch.uint8, device=U.device)
-    opts = U2POpts(gam=gam, font_fix=font_fix, guess_mode=1 if mlp is not None else 0,
-                   mlp=mlp.h if mlp is not None else None, K=K)
+    mode = 0 if mlp is None else (2 if gate > 0 else 1)
+    opts = U2POpts(gam=gam, font_fix=font_fix, guess_mode=mode, mlp=mlp.h if mlp is not None else None, K=K,
+                   gate_tau=gate)
     s = stream if stream is not None else torch.cuda.current_stream(U.device).cuda_stream
     _check(lib().harm_utoprim2d_batch(U.data_ptr(), gcov.data_ptr(), gcon.data_ptr(), gdet.data_ptr(),
                                       prim.data_ptr(), ret.data_ptr(), iters.data_ptr(), kconv.data_ptr(), n,

@@ -777,6 +777,7 @@ struct UpdateExtra {
     uint8_t *cap_valid; // [cap]
     size_t cap, cap_base;
     int stride, phase;
+    double gate_tau;  // > 0: the network only for zones whose first Newton step exceeds it
 };
 
 enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2 };
@@ -873,6 +874,33 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
             kconv = niter;
         }
     } else if constexpr (MODE == UPD_LEARNED) {
+      if (ex.gate_tau > 0.) {
+        // gated: the plain start and its first Newton iteration; the wave runs
+        // the network only if one of its zones asks for a seed
+        Gnr2 st;
+        bool need = false;
+        int ph1 = 0;
+        if (active) ph1 = u2p_gate_first(U, geom, g.gam, pr, ex.gate_tau, st, need);
+        double W0 = 0., vsq0 = 0.;
+        if (__any(need)) {
+            double f[MLP_NIN];
+            const bool ok = need && mlp_features_U(U, geom, f);
+            float x[MLP_NIN];
+            if (ok)
+                mlp_standardize(ex.mlp, f, x);
+            else
+#pragma unroll
+                for (int k = 0; k < MLP_NIN; k++) x[k] = 0.f;
+            float y0, y1;
+            mlp_forward_wave(ex.mlp, x, y0, y1);
+            if (ok) mlp_unstandardize(ex.mlp, y0, y1, mlp_wref(f), W0, vsq0);
+        }
+        if (active) {
+            int seeded;
+            ret = u2p_gate_finish(U, geom, g.gam, pr, ph1, st, need, W0, vsq0, ex.mlp.vsq_from_W, niter, kconv,
+                                  seeded);
+        }
+      } else {
         double f[MLP_NIN];
         const bool ok = active && mlp_features_U(U, geom, f);
         float x[MLP_NIN];
@@ -889,6 +917,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
             int fb;
             ret = utoprim_2d_learned(U, geom, g.gam, pr, W0, vsq0, ex.mlp.vsq_from_W, niter, kconv, fb);
         }
+      }
     } else {
         if (active) ret = utoprim_2d(U, geom, g.gam, pr, niter, kconv);
         if constexpr (MODE == UPD_CAPTURE) {
@@ -2071,7 +2100,7 @@ __global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U,
                                                    double *__restrict__ prim, int8_t *__restrict__ ret,
                                                    uint8_t *__restrict__ iters, uint8_t *__restrict__ kconv, size_t n,
                                                    double gam, int font_fix, double K, int *__restrict__ ret_full,
-                                                   MlpDev mlp)
+                                                   MlpDev mlp, double gate_tau)
 {
     const size_t z = (size_t)blockIdx.x * TPB + threadIdx.x;
     const bool act = z < n;
@@ -2093,7 +2122,30 @@ __global__ void __launch_bounds__(TPB) k_c2p_batch(const double *__restrict__ U,
         }
     }
     int niter = 0, kc = 0, r = 0;
-    if constexpr (LEARNED) {
+    if (LEARNED && gate_tau > 0.) {
+        // gated: the network only for the waves with a zone whose plain start is poor
+        Gnr2 st;
+        bool need = false;
+        int ph1 = 0;
+        if (act) ph1 = u2p_gate_first(Uz, geom, gam, pr, gate_tau, st, need);
+        double W0 = 0., vsq0 = 0.;
+        if (__any(need)) {
+            double f[MLP_NIN];
+            const bool ok = need && mlp_features_U(Uz, geom, f);
+            float x[MLP_NIN];
+            if (ok)
+                mlp_standardize(mlp, f, x);
+            else
+#pragma unroll
+                for (int k = 0; k < MLP_NIN; k++) x[k] = 0.f;
+            float y0, y1;
+            mlp_forward_wave(mlp, x, y0, y1);
+            if (ok) mlp_unstandardize(mlp, y0, y1, mlp_wref(f), W0, vsq0);
+        }
+        if (!act) return;
+        int seeded;
+        r = u2p_gate_finish(Uz, geom, gam, pr, ph1, st, need, W0, vsq0, mlp.vsq_from_W, niter, kc, seeded);
+    } else if constexpr (LEARNED) {
         double f[MLP_NIN];
         const bool ok = act && mlp_features_U(Uz, geom, f);
         float x[MLP_NIN];
@@ -2431,6 +2483,7 @@ struct harm_ctx {
     int eos_kind;  // 0 Gamma-law, 1 tabulated (the table's device arrays belong to the caller)
     // learned Newton seed and training-pair capture
     int guess_mode;
+    double gate_tau = 0.1;  // guess mode 2
     const harm_mlp *mlp;
     double *cap_feat, *cap_targ;
     uint8_t *cap_valid;
@@ -2639,9 +2692,10 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     int mode = UPD_PLAIN;
     if (c->eos_kind == 1) {
         // the tabulated-EOS path has its own (5D) solver: no learned seed, no capture
-    } else if (c->guess_mode == 1 && c->mlp) {
+    } else if ((c->guess_mode == 1 || c->guess_mode == 2) && c->mlp) {
         mode = UPD_LEARNED;
         ex.mlp = harm_mlp_view(c->mlp);
+        ex.gate_tau = (c->guess_mode == 2) ? c->gate_tau : 0.;
     } else if (c->cap_feat && c->cap_count < c->cap) {
         mode = UPD_CAPTURE;
         ex.cap_feat = c->cap_feat;
@@ -3575,13 +3629,21 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
 int harm_set_guess(harm_ctx *c, int mode, const harm_mlp *m)
 {
     if (!c) return set_err("null context");
-    if (mode != 0 && mode != 1) return set_err("unknown guess mode", 4);
-    if (mode == 1) {
-        if (!m) return set_err("guess mode 1 needs a harm_mlp handle", 4);
+    if (mode < 0 || mode > 2) return set_err("unknown guess mode", 4);
+    if (mode != 0) {
+        if (!m) return set_err("guess modes 1 and 2 need a harm_mlp handle", 4);
         if (harm_mlp_device(m) != c->device) return set_err("network and context are on different devices", 4);
     }
     c->guess_mode = mode;
-    c->mlp = (mode == 1) ? m : nullptr;
+    c->mlp = (mode != 0) ? m : nullptr;
+    return 0;
+}
+
+int harm_set_guess_gate(harm_ctx *c, double tau)
+{
+    if (!c) return set_err("null context");
+    if (!(tau > 0.)) return set_err("the gate threshold must be > 0", 4);
+    c->gate_tau = tau;
     return 0;
 }
 
@@ -3621,14 +3683,17 @@ int harm_utoprim2d_batch(const double *U, const double *gcov, const double *gcon
     if (n == 0) return 0;
     MlpDev mv;
     std::memset(&mv, 0, sizeof(mv));
-    if (opts->guess_mode == 1) {
-        if (!opts->mlp) return set_err("guess_mode 1 needs a harm_mlp handle", 4);
+    double gate = 0.;
+    if (opts->guess_mode == 1 || opts->guess_mode == 2) {
+        if (!opts->mlp) return set_err("guess_mode 1 and 2 need a harm_mlp handle", 4);
+        if (opts->guess_mode == 2 && !(opts->gate_tau > 0.)) return set_err("guess_mode 2 needs gate_tau > 0", 4);
+        if (opts->guess_mode == 2) gate = opts->gate_tau;
         mv = harm_mlp_view((const harm_mlp *)opts->mlp);
         hipLaunchKernelGGL((k_c2p_batch<true>), dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon,
-                           gdet, prim, ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr, mv);
+                           gdet, prim, ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr, mv, gate);
     } else if (opts->guess_mode == 0) {
         hipLaunchKernelGGL((k_c2p_batch<false>), dim3(nblocks(n)), dim3(TPB), 0, (hipStream_t)stream, U, gcov, gcon,
-                           gdet, prim, ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr, mv);
+                           gdet, prim, ret, iters, kconv, n, opts->gam, opts->font_fix, opts->K, (int *)nullptr, mv, gate);
     } else {
         return set_err("unknown guess_mode", 4);
     }
@@ -3817,7 +3882,7 @@ int Utoprim_2d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, d
     std::memset(&mv, 0, sizeof(mv));
     hipLaunchKernelGGL((k_c2p_batch<false>), dim3(1), dim3(TPB), 0, 0, s.buf, s.buf + 8, s.buf + 24, s.buf + 40,
                        s.buf + 41, s.ret8, (uint8_t *)nullptr, (uint8_t *)nullptr, (size_t)1, g_gamma, 0, 0., s.ret,
-                       mv);
+                       mv, 0.);
     HIPCHK(hipGetLastError());
     int r = 0;
     if ((rc = u2p_unstage(s, prim, r))) return rc;

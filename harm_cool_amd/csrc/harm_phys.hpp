@@ -677,39 +677,70 @@ HD void func_vsq(const U2P &s, double W, double vsq, double &dx0, double &dx1, d
     f = -0.5 * df;
 }
 
-// utoprim_2d.c:442-541 (validate_x: utoprim_2d.c:417-431)
+// utoprim_2d.c:442-541 (validate_x: utoprim_2d.c:417-431), as a state that
+// advances one Newton iteration per gnr_step, so a solve can pause after any
+// iteration and resume with the same bits.
+struct Gnr2 {
+    double x0, x1, f, df, errx;
+    int n_iter, i_extra, doing_extra, k_conv;
+    bool keep;
+};
+
+HD void gnr_start(Gnr2 &g, double x0, double x1)
+{
+    g.x0 = x0;
+    g.x1 = x1;
+    g.f = 1.;
+    g.df = 1.;
+    g.errx = 1.;
+    g.n_iter = 0;
+    g.i_extra = 0;
+    g.doing_extra = 0;
+    g.k_conv = -1;
+    g.keep = true;
+}
+
+HD void gnr_step(const U2P &s, Gnr2 &g)
+{
+    double dx0, dx1;
+    func_vsq(s, g.x0, g.x1, dx0, dx1, g.f, g.df);
+    double xo0 = g.x0;
+    g.x0 += dx0;
+    g.x1 += dx1;
+    g.errx = (g.x0 == 0.) ? fabs(dx0) : fabs(dx0 / g.x0);
+    g.errx += (g.x1 == 0.) ? fabs(dx1) : fabs(dx1 / g.x1);
+    g.x0 = fabs(g.x0);
+    g.x0 = (g.x0 > W_TOO_BIG) ? xo0 : g.x0;
+    g.x1 = (g.x1 < 0.) ? 0. : g.x1;
+    g.x1 = (g.x1 > 1.) ? (1. - 1.e-15) : g.x1;
+    if ((fabs(g.errx) <= NEWT_TOL) && (g.doing_extra == 0)) {
+        g.doing_extra = 1;
+        g.k_conv = g.n_iter + 1;
+    }
+    if (g.doing_extra == 1) g.i_extra++;
+    if (((fabs(g.errx) <= NEWT_TOL) && (g.doing_extra == 0)) || (g.i_extra > EXTRA_NEWT_ITER) ||
+        (g.n_iter >= (MAX_NEWT_ITER - 1)))
+        g.keep = false;
+    g.n_iter++;
+}
+
+HD int gnr_end(const Gnr2 &g, int &n_iter_out, int &k_conv_out)
+{
+    n_iter_out = g.n_iter;
+    k_conv_out = (g.k_conv < 0) ? g.n_iter : g.k_conv;
+    if (!isfinite(g.f) || !isfinite(g.df)) return 2;
+    if (fabs(g.errx) > MIN_NEWT_TOL) return 1;
+    return 0;
+}
+
 HD int gnr_2d(const U2P &s, double &x0, double &x1, int &n_iter_out, int &k_conv_out)
 {
-    double f = 1., df = 1., errx = 1.;
-    int n_iter = 0, i_extra = 0, doing_extra = 0, k_conv = -1;
-    bool keep = true;
-    while (keep) {
-        double dx0, dx1;
-        func_vsq(s, x0, x1, dx0, dx1, f, df);
-        double xo0 = x0;
-        x0 += dx0;
-        x1 += dx1;
-        errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
-        errx += (x1 == 0.) ? fabs(dx1) : fabs(dx1 / x1);
-        x0 = fabs(x0);
-        x0 = (x0 > W_TOO_BIG) ? xo0 : x0;
-        x1 = (x1 < 0.) ? 0. : x1;
-        x1 = (x1 > 1.) ? (1. - 1.e-15) : x1;
-        if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) {
-            doing_extra = 1;
-            k_conv = n_iter + 1;
-        }
-        if (doing_extra == 1) i_extra++;
-        if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
-            (n_iter >= (MAX_NEWT_ITER - 1)))
-            keep = false;
-        n_iter++;
-    }
-    n_iter_out = n_iter;
-    k_conv_out = (k_conv < 0) ? n_iter : k_conv;
-    if (!isfinite(f) || !isfinite(df)) return 2;
-    if (fabs(errx) > MIN_NEWT_TOL) return 1;
-    return 0;
+    Gnr2 g;
+    gnr_start(g, x0, x1);
+    while (g.keep) gnr_step(s, g);
+    x0 = g.x0;
+    x1 = g.x1;
+    return gnr_end(g, n_iter_out, k_conv_out);
 }
 
 // Conserved-variable transform of utoprim_2d.c:145-163 (shared by all solvers).
@@ -776,11 +807,10 @@ HD double raise_W(const U2P &s, double W_last)
 
 // utoprim_2d.c:306-356 from a starting W: Newton on (W, v^2), then recovery.
 // Returns the reference's code; prim[0..4] written only on success.
-HD int solve_2d_from_x(U2P &s, const double *Bcon, const double *Qcon, const double *ncon, double QdotB, double x0,
-                       double x1, double *prim, int &n_iter, int &k_conv)
+// utoprim_2d.c:322-356: the primitives from the Newton result (retval, W, vsq).
+HD int finish_2d(U2P &s, const double *Bcon, const double *Qcon, const double *ncon, double QdotB, int retval,
+                 double W, double vsq, double *prim)
 {
-    int retval = gnr_2d(s, x0, x1, n_iter, k_conv);
-    double W = x0, vsq = x1;
     if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
     if (W <= 0. || W > W_TOO_BIG) return 3;
     if (vsq >= 1.) return 4;
@@ -795,6 +825,13 @@ HD int solve_2d_from_x(U2P &s, const double *Bcon, const double *Qcon, const dou
     prim[UU] = u;
     u2p_recover_vel(s, Bcon, Qcon, ncon, QdotB, W, gamma, prim);
     return 0;
+}
+
+HD int solve_2d_from_x(U2P &s, const double *Bcon, const double *Qcon, const double *ncon, double QdotB, double x0,
+                       double x1, double *prim, int &n_iter, int &k_conv)
+{
+    int retval = gnr_2d(s, x0, x1, n_iter, k_conv);
+    return finish_2d(s, Bcon, Qcon, ncon, QdotB, retval, x0, x1, prim);
 }
 
 // utoprim_2d.c:306-307 (x1_of_x0, utoprim_2d.c:395-406): the Newton start from a W guess.
@@ -888,6 +925,94 @@ HD int utoprim_2d_learned(const double *U, const Geom &geom, double gam, double 
     fell_back = 1;
     int n2, k2;
     ret = utoprim_2d(U, geom, gam, prim, n2, k2);
+    n_iter = n1 + n2;
+    k_conv = n1 + k2;
+    return ret;
+}
+
+// The gated learned solve, in two phases around the wave-level network pass.
+// Phase 1 (u2p_gate_first): utoprim_2d's start from the guess primitives and
+// its first Newton iteration.  need: the zone asks for the network's seed --
+// no valid start from the guess (code 2), or a first Newton step
+// |dW/W| + |dv2/v2| above tau.  Returns -100 (nothing to solve), 2 (no start)
+// or 0 (the Newton state st holds the first iteration).
+HD int u2p_gate_first(const double *U, const Geom &geom, double gam, double *prim, double tau, Gnr2 &st,
+                      bool &need)
+{
+    need = false;
+#pragma unroll
+    for (int i = B1; i <= B3; i++) prim[i] = div_r(U[i], geom.g, geom.rg);
+    if (U[0] <= 0.) return -100;
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.rgam = 1. / gam;
+    s.K_atm = 0.;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    double W_last;
+    if (!guess_W(s, geom, prim, W_last)) {
+        need = true;
+        return 2;
+    }
+    W_last = raise_W(s, W_last);
+    gnr_start(st, fabs(W_last), x1_of_x0(s, W_last));
+    gnr_step(s, st);
+    need = st.keep && fabs(st.errx) > tau;
+    return 0;
+}
+
+// Phase 2: a zone that asked for the seed solves from (W0, vsq0) (as
+// utoprim_2d_seeded); where that fails, or for every other zone, the plain
+// solve resumes from its first iteration, so a zone that did not ask gets
+// utoprim_2d's result bit for bit.  Iteration counts add up (the first
+// iteration counts once); seeded = 1 where the seeded solve gave the result.
+HD int u2p_gate_finish(const double *U, const Geom &geom, double gam, double *prim, int ph1, Gnr2 &st, bool need,
+                       double W0, double vsq0, int vsq_from_W, int &n_iter, int &k_conv, int &seeded)
+{
+    n_iter = 0;
+    k_conv = 0;
+    seeded = 0;
+    if (ph1 == -100) return -100;
+    const int n_first = (ph1 == 0) ? 1 : 0;
+    int n1 = 0, k1 = 0;
+    if (need) {
+        double pt[NPR];
+#pragma unroll
+        for (int i = 0; i < NPR; i++) pt[i] = prim[i];
+        const int ret = utoprim_2d_seeded(U, geom, gam, pt, W0, vsq0, vsq_from_W, n1, k1);
+        if (ret == 0) {
+#pragma unroll
+            for (int i = 0; i < NPR; i++) prim[i] = pt[i];
+            n_iter = n_first + n1;
+            k_conv = n_first + k1;
+            seeded = 1;
+            return 0;
+        }
+        if (ph1 == 2) {
+            n_iter = n1;
+            k_conv = n1;
+            return 2;
+        }
+    }
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.rgam = 1. / gam;
+    s.K_atm = 0.;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    while (st.keep) gnr_step(s, st);
+    int n2, k2;
+    const int retval = gnr_end(st, n2, k2);
+    double pt[5];
+    const int ret = finish_2d(s, Bcon, Qcon, ncon, QdotB, retval, st.x0, st.x1, pt);
+    if (ret == 0) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) prim[i] = pt[i];
+    }
     n_iter = n1 + n2;
     k_conv = n1 + k2;
     return ret;

@@ -208,9 +208,12 @@ int harm_get_geometry(harm_ctx *c, double *gcov, double *gcon, double *gdet, dou
 typedef struct harm_u2p_opts {
     double gam;              /* adiabatic index */
     int32_t font_fix;        /* run the Font-fix fallbacks on failure */
-    int32_t guess_mode;      /* 0: guess from prim_in, 1: MLP guess with prim_in fallback */
-    const void *mlp;         /* harm_mlp handle for guess_mode 1 (may be NULL) */
+    int32_t guess_mode;      /* 0: guess from prim_in, 1: MLP guess with prim_in fallback,
+                                2: gated -- the MLP guess only for zones whose first Newton step
+                                from prim_in exceeds gate_tau, the plain solve for the others */
+    const void *mlp;         /* harm_mlp handle for guess_mode 1 and 2 (may be NULL otherwise) */
     double K;                /* isentrope constant for the Font fix */
+    double gate_tau;         /* guess_mode 2: threshold on |dW/W| + |dv2/v2| of the first step (> 0) */
 } harm_u2p_opts;
 
 /* Batched Noble 2D inversion on the device; replaces n calls of Utoprim_2d
@@ -283,9 +286,15 @@ int harm_mlp_pairs(const double *U, const double *gcov, const double *gcon, cons
                    const int8_t *ret, size_t n, double gam, double *feat, double *targ, uint8_t *valid, void *stream);
 
 /* Step integration of the learned seed.  mode 0: previous-step guess (the
- * reference's Utoprim_2d); mode 1: learned seed with previous-step fallback.
+ * reference's Utoprim_2d); mode 1: learned seed with previous-step fallback;
+ * mode 2: gated -- each zone starts from the previous-step guess, and only a
+ * zone whose first Newton step |dW/W| + |dv2/v2| exceeds the gate threshold
+ * (harm_set_guess_gate, default 0.1), or that has no valid start, solves from
+ * the network's seed (a wave runs the network only when one of its zones asks);
+ * the other zones give Utoprim_2d's result bit for bit.
  * The handle must outlive its use by the context. */
 int harm_set_guess(harm_ctx *c, int mode, const harm_mlp *m);
+int harm_set_guess_gate(harm_ctx *c, double tau);
 /* Capture (U, P) training pairs from the plain inversion of later half-steps:
  * every stride-th active zone (phase rotating per half-step) appends its raw
  * features, targets and a valid flag to the device buffers feat[8][cap],

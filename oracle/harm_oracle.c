@@ -941,8 +941,9 @@ static void func_vsq(const u2p_t *s, const double x[], double dx[], double resid
     *df = -resid[0] * resid[0] - resid[1] * resid[1];
     *f = -0.5 * (*df);
 }
-/* utoprim_2d.c:442-541 */
-static int gnr_2d(const u2p_t *s, double x[2], int *n_iter_out, int *k_conv_out)
+/* utoprim_2d.c:442-541; errx_first (may be NULL) receives |dW/W| + |dv2/v2| of
+ * the first iteration (the gate of the learned seed, no reference counterpart). */
+static int gnr_2d_first(const u2p_t *s, double x[2], int *n_iter_out, int *k_conv_out, double *errx_first)
 {
     double f, df, dx[2], x_old[2], resid[2], jac[2][2], errx;
     int n_iter, id, i_extra, doing_extra, keep_iterating, k_conv = -1;
@@ -960,6 +961,7 @@ static int gnr_2d(const u2p_t *s, double x[2], int *n_iter_out, int *k_conv_out)
         errx = (x[0] == 0.) ? fabs(dx[0]) : fabs(dx[0] / x[0]);
         errx += (x[1] == 0.) ? fabs(dx[1]) : fabs(dx[1] / x[1]);
         validate_x_2d(x, x_old);
+        if (errx_first && n_iter == 0) *errx_first = fabs(errx);
         if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) {
             doing_extra = 1;
             k_conv = n_iter + 1;
@@ -975,6 +977,10 @@ static int gnr_2d(const u2p_t *s, double x[2], int *n_iter_out, int *k_conv_out)
     if ((isfinite(f) == 0) || (isfinite(df) == 0)) return 2;
     if (fabs(errx) > MIN_NEWT_TOL) return 1;
     return 0;
+}
+static int gnr_2d(const u2p_t *s, double x[2], int *n_iter_out, int *k_conv_out)
+{
+    return gnr_2d_first(s, x, n_iter_out, k_conv_out, NULL);
 }
 
 /* The scalars shared by all three solvers (utoprim_2d.c:240-266). */
@@ -1006,7 +1012,7 @@ static void u2p_scalars(u2p_t *s, const double U[NPR], double gcov[NDIM][NDIM],
 
 /* utoprim_2d.c:219-362 */
 static int new_body_2d(u2p_t *s, const double U[NPR], double gcov[NDIM][NDIM],
-                       double gcon[NDIM][NDIM], double prim[NPR], int *n_iter, int *k_conv)
+                       double gcon[NDIM][NDIM], double prim[NPR], int *n_iter, int *k_conv, double *errx_first)
 {
     double x_2d[2], QdotB, Bcon[NDIM], Qcon[NDIM], ncon[NDIM], Qtcon[NDIM];
     double rho0, u, p, w, gammasq, gamma, gtmp, W_last, W, utsq, vsq;
@@ -1039,7 +1045,7 @@ static int new_body_2d(u2p_t *s, const double U[NPR], double gcov[NDIM][NDIM],
     }
     x_2d[0] = fabs(W_last);
     x_2d[1] = x1_of_x0(s, W_last);
-    retval = gnr_2d(s, x_2d, n_iter, k_conv);
+    retval = gnr_2d_first(s, x_2d, n_iter, k_conv, errx_first);
     W = x_2d[0];
     vsq = x_2d[1];
     if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
@@ -1077,8 +1083,8 @@ static void u2p_transform(const double U[NPR], double gcon[NDIM][NDIM], double g
     for (i = B1; i <= B3; i++) prim_tmp[i] = alpha * prim[i];
 }
 
-static int Utoprim_2d(const double U[NPR], double gcov[NDIM][NDIM], double gcon[NDIM][NDIM],
-                      double gdet, double prim[NPR], double gam, int *n_iter, int *k_conv)
+static int Utoprim_2d_first(const double U[NPR], double gcov[NDIM][NDIM], double gcon[NDIM][NDIM], double gdet,
+                            double prim[NPR], double gam, int *n_iter, int *k_conv, double *errx_first)
 {
     double U_tmp[NPR], prim_tmp[NPR];
     int i, ret;
@@ -1090,10 +1096,15 @@ static int Utoprim_2d(const double U[NPR], double gcov[NDIM][NDIM], double gcon[
     for (i = B1; i <= B3; i++) prim[i] = U[i] / gdet;
     if (U[0] <= 0.) return -100;
     u2p_transform(U, gcon, gdet, prim, U_tmp, prim_tmp);
-    ret = new_body_2d(&s, U_tmp, gcov, gcon, prim_tmp, n_iter, k_conv);
+    ret = new_body_2d(&s, U_tmp, gcov, gcon, prim_tmp, n_iter, k_conv, errx_first);
     if (ret == 0)
         for (i = 0; i < B1; i++) prim[i] = prim_tmp[i];
     return ret;
+}
+static int Utoprim_2d(const double U[NPR], double gcov[NDIM][NDIM], double gcon[NDIM][NDIM],
+                      double gdet, double prim[NPR], double gam, int *n_iter, int *k_conv)
+{
+    return Utoprim_2d_first(U, gcov, gcon, gdet, prim, gam, n_iter, k_conv, NULL);
 }
 
 /* ---- Font fix #1: utoprim_1dvsq2fix1.c ---- */
@@ -3185,6 +3196,47 @@ int orc_utoprim_2d_learned(const double U[8], const double *gcov, const double *
     *n_iter = n1 + n2;
     *k_conv = n1 + k2;
     return ret;
+}
+
+/* The gated learned solve: Utoprim_2d from the previous-step primitives in
+ * prim, except for a zone that has no valid start from them (code 2) or whose
+ * first Newton step |dW/W| + |dv2/v2| exceeds tau: that zone solves from the
+ * seed (W0, vsq0), and where the seeded solve fails it takes the plain result.
+ * Counts: the first plain iteration counts once for a seeded zone; a failed
+ * seeded attempt adds its iterations to the plain ones. */
+int orc_utoprim_2d_gated(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8],
+                         double gam, double W0, double vsq0, int vsq_from_W, double tau, int *n_iter, int *k_conv,
+                         int *seeded)
+{
+    double gc[NDIM][NDIM], gn[NDIM][NDIM], pp[8], pt[8], errx1 = 0.;
+    int n0 = 0, k0 = 0, n1 = 0, k1 = 0, r0, r1, i, need;
+    memcpy(gc, gcov, sizeof(gc));
+    memcpy(gn, gcon, sizeof(gn));
+    for (i = 0; i < 8; i++) pp[i] = pt[i] = prim[i];
+    r0 = Utoprim_2d_first(U, gc, gn, gdet, pp, gam, &n0, &k0, &errx1);
+    *seeded = 0;
+    need = (r0 == 2 && n0 == 0) || (r0 != -100 && n0 > 0 && errx1 > tau);
+    if (need) {
+        r1 = orc_utoprim_2d_seeded(U, gcov, gcon, gdet, pt, gam, W0, vsq0, vsq_from_W, &n1, &k1);
+        if (r1 == 0) {
+            const int first = (n0 > 0) ? 1 : 0;
+            for (i = 0; i < 8; i++) prim[i] = pt[i];
+            *n_iter = first + n1;
+            *k_conv = first + k1;
+            *seeded = 1;
+            return 0;
+        }
+        if (n0 == 0) {
+            for (i = 0; i < 8; i++) prim[i] = pp[i];
+            *n_iter = n1;
+            *k_conv = n1;
+            return r0;
+        }
+    }
+    for (i = 0; i < 8; i++) prim[i] = pp[i];
+    *n_iter = n1 + n0;
+    *k_conv = n1 + k0;
+    return r0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -134,6 +134,11 @@ int  orc_utoprim_2d_seeded(const double U[8], const double *gcov, const double *
 int  orc_utoprim_2d_learned(const double U[8], const double *gcov, const double *gcon,
                             double gdet, double prim[8], double gam, double W0, double vsq0,
                             int vsq_from_W, int *n_iter, int *k_conv, int *fell_back);
+/* Gated learned solve: the seed only where the first Newton step from prim
+ * exceeds tau or prim gives no start; otherwise Utoprim_2d bit for bit. */
+int  orc_utoprim_2d_gated(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8],
+                          double gam, double W0, double vsq0, int vsq_from_W, double tau, int *n_iter,
+                          int *k_conv, int *seeded);
 /* Tabulated EOS: rho[nr], T[nt] increasing; e, p [nr][nt] (code units);
  * gam is the Gamma-law fallback outside the table. */
 typedef struct orc_eos {

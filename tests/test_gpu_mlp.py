@@ -274,14 +274,31 @@ def test_learned_step_matches_plain_step():
     lt = m.train(fd, td, vd, batch=8192, steps=400, lr0=3e-3, lr1=2e-4, trace=True)
     # FP64 numpy on the same data and schedule reaches ~1.4e-3 from ~1.0
     assert lt[-20:].mean() < 0.01 * lt[:5].mean(), (lt[:5], lt[-20:])
+    sl0 = (slice(None), slice(2, 34), slice(2, 34), slice(None))
     plain = hc.Harm(P)
     learned = hc.Harm(P)
-    for s in (plain, learned):
+    closed = hc.Harm(P)
+    gated = hc.Harm(P)
+    for s in (plain, learned, closed, gated):
         s.init()
         s.set_prims(p0)
     learned.set_guess(1, m)
+    # the gated seed with a threshold no first step reaches: the plain step, bit for bit
+    closed.set_guess(2, m, gate=1e300)
+    # a low threshold: zones whose previous-step start is off by more than 1e-7 take the seed
+    gated.set_guess(2, m, gate=1e-7)
     plain.step_ch(10)
     learned.step_ch(10)
+    closed.step_ch(10)
+    gated.step_ch(10)
+    assert np.array_equal(closed.prims(), plain.prims())
+    assert closed.counters() == plain.counters()
+    pg = gated.prims()
+    for v in range(8):
+        x, y = plain.prims()[sl0][v], pg[sl0][v]
+        assert np.abs(x - y).max() <= 1e-10 * np.abs(x).max(), v
+    closed.close()
+    gated.close()
     pa, pb = plain.prims(), learned.prims()
     sl = (slice(None), slice(2, 34), slice(2, 34), slice(None))
     for v in range(8):

@@ -134,3 +134,55 @@ def test_learned_batch_on_stress_matches_oracle_bitwise():
           f"iters plain {it0.float().mean().item():.3f}, learned {it.mean():.3f}")
     assert np.median(rel) <= 1e-14 and np.quantile(rel, 0.99) <= 1e-10
     m.close()
+
+
+@pytest.mark.parametrize("tau", [1e-2, 1e-12])
+def test_gated_batch_on_stress_matches_oracle_bitwise(tau):
+    """The gated learned inversion (guess_mode 2): each zone starts from the
+    1%-perturbed guess; only zones whose first Newton step exceeds tau take the
+    network's seed (a wave runs the network only when one of its zones asks).
+    Codes, iteration counts and primitives equal the oracle's gated solve fed the
+    same seeds; zones the gate leaves alone equal the plain solve bit for bit."""
+    torch = _torch()
+    spec = hc.stress_spec(0)
+    tr = hc.stress_batch(N, hc.stress_spec(0, seed=(777, 888, 999, 1111)))
+    ptr = tr["guess"].clone()
+    rtr, _, _ = hc.utoprim2d_batch(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], ptr, spec.gam)
+    feat, targ, valid = hc.mlp_pairs(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], ptr, rtr, spec.gam)
+    m = hc.Mlp(seed=5)
+    m.fit_norm(feat, targ, valid)
+    m.set_loss_weights(1.0, 0.0)
+    m.train(feat, targ, valid, batch=8192, steps=300, lr0=3e-3, lr1=2e-4)
+    b = hc.stress_batch(N, spec)
+    _, W0, v0 = m.seed(b["U"], b["gcov"], b["gcon"], b["gdet"])
+    prim = b["guess"].clone()
+    ret, it, kc = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], prim, spec.gam, mlp=m, gate=tau)
+    p_plain = b["guess"].clone()
+    ret0, it0, kc0 = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], p_plain, spec.gam)
+    torch.cuda.synchronize()
+    h = _host(b)
+    W0, v0 = W0.cpu().numpy(), v0.cpu().numpy()
+    ret, it, kc, prim = ret.cpu().numpy(), it.cpu().numpy(), kc.cpu().numpy(), prim.cpu().numpy()
+    ret0, it0, kc0, p_plain = ret0.cpu().numpy(), it0.cpu().numpy(), kc0.cpu().numpy(), p_plain.cpu().numpy()
+    L = olib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.orc_utoprim_2d_gated.argtypes = [dp, dp, dp, ctypes.c_double, dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_double, ip, ip, ip]
+    rng = np.random.default_rng(4)
+    nseed = 0
+    for z in rng.choice(N, 1500, replace=False):
+        U, gcz, gnz, g = _zone(h, z)
+        p = np.ascontiguousarray(h["guess"][:, z].copy())
+        ni, kcz, sd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        r = L.orc_utoprim_2d_gated(_ptr(U), _ptr(gcz), _ptr(gnz), g, _ptr(p), spec.gam, float(W0[z]), float(v0[z]), 1,
+                                   tau, ctypes.byref(ni), ctypes.byref(kcz), ctypes.byref(sd))
+        assert np.int8(r) == ret[z] and ni.value == it[z] and kcz.value == kc[z], z
+        assert np.array_equal(p, prim[:, z]), z
+        nseed += sd.value
+        if not sd.value and ni.value == it0[z]:
+            assert ret[z] == ret0[z] and np.array_equal(prim[:, z], p_plain[:, z]), z
+    print(f"tau {tau:g}: {nseed} of 1500 sampled zones seeded; iters plain {it0.mean():.3f}, gated {it.mean():.3f}")
+    if tau < 1e-10:
+        assert nseed > 1400  # a 1% start never takes a first step below 1e-12
+    m.close()
