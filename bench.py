@@ -56,6 +56,27 @@ ALG_BYTES = {
 }
 
 
+# One wave instruction per CU per cycle: 256 CUs at 2.4 GHz (FP64 FMA is full rate).
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
+PROFILED_KERNEL = {"flux1": "k_flux_strip<1, false, 0>", "flux2": "k_flux_strip<2, false, 0>",
+                   "flux3": "k_flux<3, true, false, 0, true>", "update": "k_update<true, 0, false, true>"}
+
+
+def measured_valu(kernel: str, n1: int, n2: int, n3: int):
+    """SQ_INSTS_VALU per launch of `kernel` (parity build) from the committed
+    rocprofv3 counter summary of this grid, or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_counters_{n1}x{n2}x{n3}.txt")
+    name = PROFILED_KERNEL.get(kernel)
+    if not os.path.exists(path) or name is None:
+        return None
+    for line in open(path):
+        if line.startswith("# fast build"):
+            break
+        if line.startswith(name):
+            return float(line[len(name):].split()[0])
+    return None
+
+
 def measured_traffic(kernel: str, n1: int, n2: int, n3: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary of this grid (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
@@ -557,6 +578,15 @@ def main() -> None:
     g.close()
     if roof is not None:
         roof["traffic"] = measured_traffic(dom, n1, n2, n3)
+        # the stencil kernels are FP64-instruction bound: their VALU issue rate
+        # against one wave instruction per CU per cycle, with the instruction
+        # count per launch from the committed SQ_INSTS_VALU summary
+        valu = measured_valu(dom, n1, n2, n3)
+        if valu is not None:
+            rate = valu / (per[dom] * 1e-3)
+            roof["valu"] = {"instr_per_launch": valu, "achieved_wave_instr_per_s": rate,
+                            "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
+                            "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
 
     fast = None
     if not args.no_fast and os.path.exists(hc.LIB_FAST_PATH):

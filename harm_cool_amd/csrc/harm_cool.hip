@@ -768,6 +768,68 @@ __global__ void __launch_bounds__(TPB) k_div(DGrid g, DivArgs a, double *__restr
     for (int k = 0; k < NPR; k++) S[k * na + sidx] = d[k];
 }
 
+// Zones that the gated update hands to k_seedfix: the zone and its conserved
+// vector (k_seedfix repeats the plain start and its first iteration, which
+// give the same bits as in the update).
+struct SeedList {
+    int *count;
+    int *zone;
+    double *U;  // [8][cap]
+    int cap;
+};
+
+// Puts a zone on the seed list; false when the list is full.
+__device__ __forceinline__ bool seed_defer(const SeedList &sl, int z, const double *U)
+{
+    const int slot = atomicAdd(sl.count, 1);
+    if (slot >= sl.cap) return false;
+    const size_t cap = (size_t)sl.cap;
+    sl.zone[slot] = z;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) sl.U[k * cap + slot] = U[k];
+    return true;
+}
+
+// utoprim_2d (utoprim_2d.c:126-362) with the gate of the learned seed after
+// its first Newton iteration: a zone with no start from the guess (code 2), or
+// whose first step |dW/W| + |dv2/v2| exceeds tau, goes to the seed list and
+// returns SEED_DEFERRED (k_seedfix finishes it); every other zone, and a zone
+// that finds the list full, runs utoprim_2d's operations to the end.
+constexpr int SEED_DEFERRED = 1000;
+__device__ int utoprim_2d_gate(const double *U, const Geom &geom, double gam, double *prim, double tau,
+                               const SeedList &sl, int z, int &n_iter, int &k_conv)
+{
+    n_iter = 0;
+    k_conv = 0;
+#pragma unroll
+    for (int i = B1; i <= B3; i++) prim[i] = div_r(U[i], geom.g, geom.rg);
+    if (U[0] <= 0.) return -100;
+    double U_tmp[NPR];
+    u2p_transform(U, geom, U_tmp);
+    U2P s;
+    s.gam = gam;
+    s.rgam = 1. / gam;
+    s.K_atm = 0.;
+    double Bcon[4], Qcon[4], ncon[4], QdotB;
+    u2p_scalars(s, U_tmp, geom, Bcon, Qcon, ncon, QdotB);
+    double W_last;
+    if (!guess_W(s, geom, prim, W_last)) return seed_defer(sl, z, U) ? SEED_DEFERRED : 2;
+    W_last = raise_W(s, W_last);
+    Gnr2 st;
+    gnr_start(st, fabs(W_last), x1_of_x0(s, W_last));
+    gnr_step(s, st);
+    if (st.keep && fabs(st.errx) > tau && seed_defer(sl, z, U)) return SEED_DEFERRED;
+    while (st.keep) gnr_step(s, st);
+    const int retval = gnr_end(st, n_iter, k_conv);
+    double pt[5];
+    const int ret = finish_2d(s, Bcon, Qcon, ncon, QdotB, retval, st.x0, st.x1, pt);
+    if (ret == 0) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) prim[i] = pt[i];
+    }
+    return ret;
+}
+
 // Optional parts of k_update: the learned Newton seed and the capture of
 // training pairs (no counterpart in the reference).
 struct UpdateExtra {
@@ -777,10 +839,28 @@ struct UpdateExtra {
     uint8_t *cap_valid; // [cap]
     size_t cap, cap_base;
     int stride, phase;
-    double gate_tau;  // > 0: the network only for zones whose first Newton step exceeds it
+    double gate_tau;  // UPD_GATED: zones whose first Newton step exceeds it go to the seed list
+    SeedList seeds;
 };
 
-enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2 };
+enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2, UPD_GATED = 3 };
+
+// Row (i - is) * nj + (j - js) of launch block b over an ni x nj box of phi rows:
+// XCD-contiguous logical blocks (xcd_block's order, G a multiple of 8) walked
+// in UPD_TI x UPD_TJ tiles of (i, j), j fastest inside a tile; the last tile
+// row and column may be partial.  (Measured at 288x256x256 against 4x8, 4x32,
+// 16x8, 2x64 and XCD order alone: 8x16 took the least time, by <1%.)
+constexpr unsigned UPD_TI = 8, UPD_TJ = 16;
+__device__ __forceinline__ unsigned upd_tiled_row(unsigned b, unsigned G, unsigned ni, unsigned nj)
+{
+    const unsigned lb = (b & 7u) * (G >> 3) + (b >> 3);
+    const unsigned ti = lb / (UPD_TI * nj), r = lb - ti * UPD_TI * nj;
+    const unsigned th = min(UPD_TI, ni - ti * UPD_TI);
+    const unsigned tj = r / (th * UPD_TJ), r2 = r - tj * th * UPD_TJ;
+    const unsigned tw = min(UPD_TJ, nj - tj * UPD_TJ);
+    const unsigned ii = ti * UPD_TI + r2 / tw, jj = tj * UPD_TJ + r2 % tw;
+    return ii * nj + jj;
+}
 
 // advance() zone loop (step_ch.c:237-306) + fixup() (fixup.c:51-60), one active
 // zone per thread: conserved update from pi with sources at pb, Noble 2D
@@ -790,17 +870,29 @@ enum { UPD_PLAIN = 0, UPD_CAPTURE = 1, UPD_LEARNED = 2 };
 // seeds the Newton solve from the network (wave-level BF16 MFMA forward) and
 // falls back to the pi guess where the seeded solve fails.
 // FUSED forms the flux divergence in place (flux_div) instead of reading S.
+// The wave-uniform UPD_GATED update is held to the plain one's 4 waves per SIMD
+// (its seed-list store would otherwise take it past 128 VGPRs); every other
+// instance keeps the compiler's choice.
 template <bool UNI, int MODE, bool TAB, bool FUSED>
-__global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb,
-                                                double *__restrict__ pf, int8_t *__restrict__ pflag,
-                                                const double *__restrict__ S, DivArgs dv,
-                                                const double *__restrict__ tstate, double dtfac, int pi_is_pb,
-                                                FontList fl, unsigned long long *__restrict__ cpart, UpdateExtra ex)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu((UNI && MODE == UPD_GATED) ? 4 : 1, 8)))
+k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb, double *__restrict__ pf,
+         int8_t *__restrict__ pflag, const double *__restrict__ S, DivArgs dv, const double *__restrict__ tstate,
+         double dtfac, int pi_is_pb, FontList fl, unsigned long long *__restrict__ cpart, UpdateExtra ex)
 {
     if (halted(g)) return;
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t na = (size_t)ni * nj * g.N3;
-    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if constexpr (FUSED) {
+        // One block is one phi row (N3 == TPB): the rows go to the XCDs as
+        // contiguous ranges of UPD_TI x UPD_TJ (i, j) tiles, j fastest, so the
+        // rows that share face fluxes run on one XCD at about the same time
+        // and find them in its L2 (288x256x256: 357 instead of 485 HBM bytes
+        // per zone; the kernel's time is set by its instruction count).
+        const unsigned G = gridDim.x;
+        if (g.N3 == TPB && (G & 7u) == 0 && G == (unsigned)ni * (unsigned)nj)
+            idx = (size_t)upd_tiled_row(blockIdx.x, G, (unsigned)ni, (unsigned)nj) * TPB + threadIdx.x;
+    }
     const bool active = idx < na;
     int c_call = 0, c_iter = 0, c_kconv = 0, c_f2d = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
     int i = 0, j = 0, h = 0;
@@ -864,6 +956,7 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
 #pragma unroll
     for (int k = 0; k < NPR; k++) pr[k] = piz[k];
     int niter = 0, kconv = 0, ret = 0;
+    bool seed_deferred = false;  // UPD_GATED: k_seedfix finishes this zone
     if constexpr (TAB) {
         // advance() with COOL_EOS (step_ch.c:265-268): Utoprim_5d, no Font fix
         if (active) {
@@ -873,34 +966,17 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
             ret = utoprim_5d(U, geom, e5, pr, niter);
             kconv = niter;
         }
-    } else if constexpr (MODE == UPD_LEARNED) {
-      if (ex.gate_tau > 0.) {
-        // gated: the plain start and its first Newton iteration; the wave runs
-        // the network only if one of its zones asks for a seed
-        Gnr2 st;
-        bool need = false;
-        int ph1 = 0;
-        if (active) ph1 = u2p_gate_first(U, geom, g.gam, pr, ex.gate_tau, st, need);
-        double W0 = 0., vsq0 = 0.;
-        if (__any(need)) {
-            double f[MLP_NIN];
-            const bool ok = need && mlp_features_U(U, geom, f);
-            float x[MLP_NIN];
-            if (ok)
-                mlp_standardize(ex.mlp, f, x);
-            else
-#pragma unroll
-                for (int k = 0; k < MLP_NIN; k++) x[k] = 0.f;
-            float y0, y1;
-            mlp_forward_wave(ex.mlp, x, y0, y1);
-            if (ok) mlp_unstandardize(ex.mlp, y0, y1, mlp_wref(f), W0, vsq0);
-        }
+    } else if constexpr (MODE == UPD_GATED) {
+        // the plain start and its first Newton iteration; a zone that asks for
+        // the network's seed goes to the seed list (k_seedfix), the others
+        // resume the plain solve (the network is not in this kernel, which keeps
+        // the plain update's register budget)
         if (active) {
-            int seeded;
-            ret = u2p_gate_finish(U, geom, g.gam, pr, ph1, st, need, W0, vsq0, ex.mlp.vsq_from_W, niter, kconv,
-                                  seeded);
+            ret = utoprim_2d_gate(U, geom, g.gam, pr, ex.gate_tau, ex.seeds, (int)z, niter, kconv);
+            seed_deferred = (ret == SEED_DEFERRED);
         }
-      } else {
+    } else if constexpr (MODE == UPD_LEARNED) {
+      {
         double f[MLP_NIN];
         const bool ok = active && mlp_features_U(U, geom, f);
         float x[MLP_NIN];
@@ -941,7 +1017,9 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
             }
         }
     }
-    if (active) {
+    if (active && seed_deferred) {
+        c_call = 1;  // its iterations are counted by k_seedfix
+    } else if (active) {
         c_call = 1;
         c_iter = niter;
         c_kconv = kconv;
@@ -984,6 +1062,102 @@ __global__ void __launch_bounds__(TPB) k_update(DGrid g, const double *__restric
     }
     const int v[8] = {c_call, c_iter, c_kconv, c_f2d, c_gam, c_flr, c_cap, fatal};
     block_counts<8>(v, cpart);
+}
+
+
+// The zones the gated update put on the seed list: the network's seed for
+// them (64 listed zones per wavefront, so every lane of the MFMA pass is a
+// zone that asked), the seeded solve, where it fails the plain solve resumed
+// from its first iteration; then, as in k_update, the Font-fix list or
+// fixup1zone.  pi holds the zones' guess primitives (unchanged by the update
+// for a listed zone).  Grid-stride over the list with a grid-uniform number of
+// rounds, so whole wavefronts reach the network pass; the count is reset by
+// k_bound.
+constexpr int SEEDFIX_BLOCKS = 256;
+__global__ void __launch_bounds__(TPB) k_seedfix(DGrid g, const double *__restrict__ pi, double *__restrict__ pf,
+                                                 int8_t *__restrict__ pflag, SeedList sl, FontList fl, MlpDev mlp,
+                                                 unsigned long long *__restrict__ cnt)
+{
+    if (halted(g)) return;
+    const int n = min(*sl.count, sl.cap);
+    int c_iter = 0, c_kconv = 0, c_f2d = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
+    const int stride = gridDim.x * TPB;
+    const int nround = (n + stride - 1) / stride;
+    const size_t cap = (size_t)sl.cap, nz = g.nz;
+    for (int r = 0; r < nround; r++) {
+        const int e = r * stride + blockIdx.x * TPB + threadIdx.x;
+        const bool act = e < n;
+        size_t z = 0, col = 0;
+        Geom geom;
+        double U[NPR], pr[NPR];
+        if (act) {
+            z = (size_t)sl.zone[e];
+            col = z / g.N3;
+            load_geom(g.geom[CENT], g.ncol, col, geom);
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                U[k] = sl.U[k * cap + e];
+                pr[k] = pi[k * nz + z];
+            }
+        }
+        double f[MLP_NIN];
+        const bool ok = act && mlp_features_U(U, geom, f);
+        float x[MLP_NIN];
+        if (ok)
+            mlp_standardize(mlp, f, x);
+        else
+#pragma unroll
+            for (int k = 0; k < MLP_NIN; k++) x[k] = 0.f;
+        float y0, y1;
+        mlp_forward_wave(mlp, x, y0, y1);
+        if (!act) continue;
+        double W0 = 0., vsq0 = 0.;
+        if (ok) mlp_unstandardize(mlp, y0, y1, mlp_wref(f), W0, vsq0);
+        // the plain start and its first iteration again (as in the update: same bits)
+        Gnr2 st;
+        bool need;
+        const int ph1 = u2p_gate_first(U, geom, g.gam, pr, 0., st, need);
+        int niter, kconv, seeded;
+        const int ret = u2p_gate_finish(U, geom, g.gam, pr, ph1, st, true, W0, vsq0, mlp.vsq_from_W, niter, kconv,
+                                        seeded);
+        c_iter += niter;
+        c_kconv += kconv;
+        int8_t flag = (int8_t)ret;
+        bool deferred = false;
+        if (flag) {
+            c_f2d++;
+            if (g.font_fix) {
+                const int slot = atomicAdd(fl.count, 1);
+                if (slot < fl.cap) {
+                    fl.zone[slot] = (int)z;
+#pragma unroll
+                    for (int k = 0; k < NPR; k++) fl.U[(size_t)k * fl.cap + slot] = U[k];
+                    deferred = true;
+                } else {
+                    atomicOr(fl.overflow, 1);
+                }
+            }
+        }
+        if (!deferred) {
+            const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
+            if (fx & FIX_FLOOR) c_flr++;
+            if (fx & FIX_GAMMA_CAP) c_cap++;
+            if (fx & FIX_GAMMA_FAIL) {
+                flag = -33;
+                c_gam++;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
+        pflag[z] = flag;
+    }
+    add_counter(cnt, HARM_NEWTON_ITERS, c_iter);
+    add_counter(cnt, HARM_NEWTON_KCONV, c_kconv);
+    add_counter(cnt, HARM_FAIL_2D, c_f2d);
+    add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
+    add_counter(cnt, HARM_FLOOR_ZONES, c_flr);
+    add_counter(cnt, HARM_GAMMA_CAP, c_cap);
+    add_counter(cnt, HARM_FATAL, fatal);
 }
 
 // Font fix (step_ch.c:277-300) + fixup1zone for the zones k_update deferred.
@@ -1100,11 +1274,13 @@ __device__ __forceinline__ size_t bound_nthreads(const DGrid &g)
 }
 
 __global__ void __launch_bounds__(TPB) k_bound(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag,
-                                               int *__restrict__ font_count, unsigned long long *__restrict__ cnt)
+                                               int *__restrict__ font_count, int *__restrict__ seed_count,
+                                               unsigned long long *__restrict__ cnt)
 {
     if (halted(g)) return;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     if (idx == 0 && font_count) *font_count = 0;
+    if (idx == 0 && seed_count) *seed_count = 0;
     int fatal = 0;
     if (idx < bound_nthreads(g)) {
         const int h = (int)(idx % g.N3);
@@ -2478,6 +2654,7 @@ struct harm_ctx {
     unsigned fix_nwords;
     int *fix_err;
     FontList font;
+    SeedList seeds;  // guess mode 2; allocated by harm_set_guess
     bool uniform;  // N3 % 64 == 0: a wave's lanes share one (i,j) column
     int halted;    // error code of the step that stopped the last batch (11, 14), else 0
     int eos_kind;  // 0 Gamma-law, 1 tabulated (the table's device arrays belong to the caller)
@@ -2663,7 +2840,7 @@ int bound(harm_ctx *c, double *p)
     const DGrid &g = c->g;
     const size_t nb = ((size_t)4 * g.SN2 + (size_t)g.N1 * (4 + 2 * g.polefix)) * g.N3;
     LAUNCH(HARM_K_BOUND, hipLaunchKernelGGL(k_bound, dim3(nblocks(nb)), dim3(TPB), 0, c->stream, g, p, c->pflag,
-                                            c->font.count, c->cnt));
+                                            c->font.count, c->seeds.count, c->cnt));
     return 0;
 }
 
@@ -2692,10 +2869,14 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     int mode = UPD_PLAIN;
     if (c->eos_kind == 1) {
         // the tabulated-EOS path has its own (5D) solver: no learned seed, no capture
-    } else if ((c->guess_mode == 1 || c->guess_mode == 2) && c->mlp) {
+    } else if (c->guess_mode == 1 && c->mlp) {
         mode = UPD_LEARNED;
         ex.mlp = harm_mlp_view(c->mlp);
-        ex.gate_tau = (c->guess_mode == 2) ? c->gate_tau : 0.;
+    } else if (c->guess_mode == 2 && c->mlp && c->seeds.count) {
+        mode = UPD_GATED;
+        ex.mlp = harm_mlp_view(c->mlp);
+        ex.gate_tau = c->gate_tau;
+        ex.seeds = c->seeds;
     } else if (c->cap_feat && c->cap_count < c->cap) {
         mode = UPD_CAPTURE;
         ex.cap_feat = c->cap_feat;
@@ -2721,6 +2902,7 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL((k_update<U_, M_, T_, F_>), dim3(nblocks(na)), dim3(TPB), 0, c->stream, \
                                              g, pi, pb, pf, c->pflag, c->S, dv, c->tstate, dtfac, same, c->font,   \
                                              c->cpart, ex))
+#define GATED_LAUNCH(U_, F_) UPD_LAUNCH(U_, UPD_GATED, false, F_)
     if (c->eos_kind == 1) {
         if (fused)
             UPD_LAUNCH(true, UPD_PLAIN, true, true);
@@ -2735,6 +2917,10 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
             UPD_LAUNCH(true, UPD_PLAIN, false, false);
         else if (mode == UPD_CAPTURE)
             UPD_LAUNCH(true, UPD_CAPTURE, false, false);
+        else if (mode == UPD_GATED && fused)
+            GATED_LAUNCH(true, true);
+        else if (mode == UPD_GATED)
+            GATED_LAUNCH(true, false);
         else if (fused)
             UPD_LAUNCH(true, UPD_LEARNED, false, true);
         else
@@ -2744,10 +2930,13 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
             UPD_LAUNCH(false, UPD_PLAIN, false, false);
         else if (mode == UPD_CAPTURE)
             UPD_LAUNCH(false, UPD_CAPTURE, false, false);
+        else if (mode == UPD_GATED)
+            GATED_LAUNCH(false, false);
         else
             UPD_LAUNCH(false, UPD_LEARNED, false, false);
     }
 #undef UPD_LAUNCH
+#undef GATED_LAUNCH
     {
         CntMap map = {8, {HARM_C2P_CALLS, HARM_NEWTON_ITERS, HARM_NEWTON_KCONV, HARM_FAIL_2D, HARM_FIXUP_GAMMA,
                           HARM_FLOOR_ZONES, HARM_GAMMA_CAP, HARM_FATAL}};
@@ -2755,6 +2944,9 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
                            c->cnt, g.halt);
         HIPCHK(hipGetLastError());
     }
+    if (mode == UPD_GATED)
+        LAUNCH(HARM_K_MLP, hipLaunchKernelGGL(k_seedfix, dim3(SEEDFIX_BLOCKS), dim3(TPB), 0, c->stream, g, pi, pf,
+                                              c->pflag, c->seeds, c->font, ex.mlp, c->cnt));
     if (g.font_fix)
         LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(FONTFIX_BLOCKS), dim3(TPB), 0, c->stream, g, pf, c->pflag,
                                                   c->font, c->cnt));
@@ -2966,7 +3158,8 @@ void ctx_free(harm_ctx *c)
                     (void *)c->e2, (void *)c->e3, (void *)c->S, (void *)c->pflag, (void *)c->pflag2, (void *)c->d_conn,
                     (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->dtpart, (void *)c->cpart, (void *)c->cnt,
                     (void *)c->part, (void *)c->fix_bits, (void *)c->fix_pcount, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
-                    (void *)c->d_cblk, (void *)c->d_fblk})
+                    (void *)c->d_cblk, (void *)c->d_fblk, (void *)c->seeds.count, (void *)c->seeds.zone,
+                    (void *)c->seeds.U})
         if (q) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
         if (c->d_geom[l]) (void)hipFree(c->d_geom[l]);
@@ -3633,6 +3826,19 @@ int harm_set_guess(harm_ctx *c, int mode, const harm_mlp *m)
     if (mode != 0) {
         if (!m) return set_err("guess modes 1 and 2 need a harm_mlp handle", 4);
         if (harm_mlp_device(m) != c->device) return set_err("network and context are on different devices", 4);
+    }
+    if (mode == 2 && !c->seeds.count) {
+        // the seed list of the gated update: one slot per 16 active zones (at
+        // least 65536); a zone that finds it full resumes the plain solve
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const size_t na = (size_t)c->hg.N1 * c->hg.N2 * c->hg.N3;
+        const size_t cap = std::max<size_t>(65536, na / 16);
+        SeedList &sl = c->seeds;
+        sl.cap = (int)cap;
+        int rc;
+        if ((rc = dalloc(&sl.count, 1)) || (rc = dalloc(&sl.zone, cap)) || (rc = dalloc(&sl.U, 8 * cap))) return rc;
+        HIPCHK(hipMemsetAsync(sl.count, 0, sizeof(int), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     c->guess_mode = mode;
     c->mlp = (mode != 0) ? m : nullptr;

@@ -975,6 +975,7 @@ HD int u2p_gate_finish(const double *U, const Geom &geom, double gam, double *pr
     k_conv = 0;
     seeded = 0;
     if (ph1 == -100) return -100;
+    if (ph1 == 2 && !need) return 2;  // no start and no seed: utoprim_2d's code 2
     const int n_first = (ph1 == 0) ? 1 : 0;
     int n1 = 0, k1 = 0;
     if (need) {

@@ -28,6 +28,9 @@ CASES = {
     "3d_uniform_vanleer_llf": lambda: torus_params(16, 16, 128, lim=1, hllf=0.0, laxf=1.0),
     # N3 = 192: wave-uniform but not dividing the 256-thread block (per-face phi kernel)
     "3d_uniform_n3_192": lambda: torus_params(18, 16, 192),
+    # N3 = 256: one block per phi row, the fused update walks the rows in XCD-contiguous
+    # 8x16 (i, j) tiles (here partial in j)
+    "3d_uniform_n3_256": lambda: torus_params(16, 12, 256),
     # PPM reconstruction (lim 3): six zones per face, MC state next to the radial edges
     "2d_ppm": lambda: torus_params(64, 64, 1, lim=3),
     "3d_ppm_llf": lambda: torus_params(24, 16, 8, lim=3, hllf=0.0, laxf=1.0),
@@ -88,7 +91,8 @@ def test_initial_data_and_fluxes_bitwise(case):
 @pytest.mark.parametrize("case,nsteps", [("2d_128", 10), ("3d_32x32x16", 10), ("2d_llf_vanleer", 10),
                                          ("3d_minmod_gamma43", 10), ("3d_uniform_32x32x64", 6),
                                          ("3d_uniform_vanleer_llf", 6), ("3d_uniform_n3_192", 4),
-                                         ("2d_ppm", 10), ("3d_ppm_llf", 10), ("3d_uniform_ppm", 6)])
+                                         ("2d_ppm", 10), ("3d_ppm_llf", 10), ("3d_uniform_ppm", 6),
+                                         ("3d_uniform_n3_256", 4)])
 def test_steps_match_oracle(case, nsteps):
     g, o = _pair(CASES[case]())
     g.init()
