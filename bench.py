@@ -450,6 +450,15 @@ def cache_lookups(hc, torch, tab: dict, n: int = 1 << 22) -> dict:
     res["solve_T"] = {"ms": round(el * 1e3, 3), "solves_per_s": n / el, "converged": int(conv.sum().item()),
                       "max_rel_T_err_converged": (float(((T - y).abs() / y)[conv].max().item())
                                                   if bool(conv.any()) else None)}
+    if bool(conv.any()):
+        # the solve's own criterion is the residual in e (tolf 1e-6); where e hardly
+        # changes with T (low T at high density) a small residual leaves T loose
+        e_at, _, _ = cache.lookup(x, T, None, zi, hc.INTERP_SPLINE, [1, 0, 0, 0, 0, 0], False)
+        r_e = ((e_at[0] - e).abs() / e.abs())[conv]
+        r_T = ((T - y).abs() / y)[conv]
+        q = torch.tensor([0.5, 0.999], device=r_T.device, dtype=r_T.dtype)
+        res["solve_T"]["max_rel_e_residual_converged"] = float(r_e.max().item())
+        res["solve_T"]["rel_T_err_converged_q50_q999"] = [float(v) for v in torch.quantile(r_T[:1 << 20], q).tolist()]
     cache.close()
     return res
 
