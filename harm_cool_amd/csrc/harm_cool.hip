@@ -309,20 +309,51 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
         __syncthreads();
         const unsigned rs = t - (unsigned)h;
         const unsigned tm1 = rs + (unsigned)((h + g.N3 - 1) % g.N3), tp1 = rs + (unsigned)((h + 1) % g.N3);
-        if (act) {
+        if constexpr (LIM == LIM_PPM) {
+            // each zone's MC slope, then its parabola from its two neighbours'
+            // values and slopes; the face takes zone h-1's right value and zone
+            // h's left value (the right values go through sp once every thread
+            // has read the primitives)
+            if (act) {
 #pragma unroll
-            for (int k = 0; k < NPR; k++) {
-                pm[k] = sp[k][tm1];
-                dq[k] = slope_lim(lim, pm[k], pc[k], sp[k][tp1]);
-                sdq[k][t] = dq[k];
+                for (int k = 0; k < NPR; k++) {
+                    dq[k] = slope_lim(LIM_MC, sp[k][tm1], pc[k], sp[k][tp1]);
+                    sdq[k][t] = dq[k];
+                }
             }
-        }
-        __syncthreads();
-        if (act) {
+            __syncthreads();
+            double qr[NPR];
+            if (act) {
 #pragma unroll
-            for (int k = 0; k < NPR; k++) {
-                p_l[k] = pm[k] + 0.5 * sdq[k][tm1];
-                p_r[k] = pc[k] - 0.5 * dq[k];
+                for (int k = 0; k < NPR; k++)
+                    ppm_para(sp[k][tm1], pc[k], sp[k][tp1], sdq[k][tm1], dq[k], sdq[k][tp1], p_r[k], qr[k]);
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < NPR; k++) sp[k][t] = qr[k];
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < NPR; k++) p_l[k] = sp[k][tm1];
+            }
+        } else {
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < NPR; k++) {
+                    pm[k] = sp[k][tm1];
+                    dq[k] = slope_lim(lim, pm[k], pc[k], sp[k][tp1]);
+                    sdq[k][t] = dq[k];
+                }
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < NPR; k++) {
+                    p_l[k] = pm[k] + 0.5 * sdq[k][tm1];
+                    p_r[k] = pc[k] - 0.5 * dq[k];
+                }
             }
         }
     } else if (LIM == LIM_PPM && act && !zero_face) {
@@ -447,6 +478,50 @@ k_flux(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *_
 constexpr int STRIP = 4;                // faces per block (8 and 16 measured 3% and 11% slower)
 constexpr int STRIP_TPB = 64 * STRIP;
 
+// The face of a strip kernel (DIR 1 or 2, face index f along DIR, other index
+// o, phi zone h): step_ch.c:429-475 from the reconstructed states, fix_flux
+// (coord.c:264-324) and the CFL limit.
+template <int DIR, bool TAB>
+__device__ __forceinline__ void strip_face(const DGrid &g, double *__restrict__ F, const double *p_l,
+                                           const double *p_r, int f, int o, int h, int fullstep, double &dtij,
+                                           int &fatal)
+{
+    const size_t nz = g.nz;
+    const int i = (DIR == 1) ? f : o, j = (DIR == 1) ? o : f;
+    const size_t z = zidx(g, i, j, h);
+    if (DIR == 2 && (j == g.js || j == g.je + 1)) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
+        return;
+    }
+    Geom geom;
+    load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<true>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
+    double Fz[NPR], ctop;
+    if constexpr (TAB)
+        face_flux(p_l, p_r, geom, DIR, EosTable{g.tab, g.ref_derivs}, g.hllf, g.laxf, Fz, ctop, fatal);
+    else
+        face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
+    if (DIR == 1) {
+        // coord.c:310-320: no inflow of mass through the radial boundaries
+        if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
+        if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
+    }
+    // coord.c:290-306: antisymmetric B2 flux in the first polar ghost face
+    const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+        if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
+    if (DIR != 2) {
+        if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
+        if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+    }
+    if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+        const double dxd = (DIR == 1) ? g.dx1 : g.dx2;
+        const double d = g.cour * dxd / ctop;
+        if (d < 1.e9) dtij = d;
+    }
+}
+
 template <int DIR, bool TAB, int LIM>
 __global__ void __launch_bounds__(STRIP_TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart,
@@ -527,47 +602,124 @@ k_flux_strip(DGrid g, const double *__restrict__ pb, double *__restrict__ F, dou
     }
     __syncthreads();
     // phase 4: face f0 + w between rows w+1 and w+2
-    const int f = f0 + w;
     double dtij = 1.e300;
     int fatal = 0;
-    if (blk_act && f <= fe) {
+    if (blk_act && f0 + w <= fe) {
         double p_r[NPR];
 #pragma unroll
         for (int k = 0; k < NPR; k++) p_r[k] = B[w + 4][k][lane];
-        const int i = (DIR == 1) ? f : o, j = (DIR == 1) ? o : f;
-        const size_t z = zidx(g, i, j, h);
-        if (DIR == 2 && (j == g.js || j == g.je + 1)) {
+        strip_face<DIR, TAB>(g, F, p_l, p_r, f0 + w, o, h, fullstep, dtij, fatal);
+    }
+    if (fullstep) block_min<STRIP_TPB>(dtij, dtpart);
+    add_counter(cnt, HARM_FATAL, fatal);
+}
+
+// The X1 / X2 fluxes of the wave-uniform path with PPM (lim 3), in the strip
+// layout of k_flux_strip (wave w computes face f0 + w for 64 phi zones of one
+// column): the zone rows, their MC slopes and their parabolas are formed once
+// per block and shared through LDS.  Row r = 0..STRIP+4 holds zone f0-3+r along
+// DIR; face f0+w takes the right value of row w+2 and the left value of row
+// w+3, and a zone whose five-zone stencil leaves the two ghost layers keeps its
+// MC state, as in k_flux<..., LIM_PPM>.  The eight variables pass through LDS
+// in two halves of four (37 KB per block, four blocks per CU).
+constexpr int PPM_ROWS = STRIP + 5, PPM_HALF = NPR / 2;
+
+template <int DIR, bool TAB>
+__global__ void __launch_bounds__(STRIP_TPB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_flux_strip_ppm(DGrid g, const double *__restrict__ pb, double *__restrict__ F, double *__restrict__ dtpart,
+                 int fullstep, unsigned long long *__restrict__ cnt)
+{
+    static_assert(DIR == 1 || DIR == 2, "strips run along X1 or X2");
+    if (halted(g)) return;
+    const int fs = (DIR == 1) ? g.is : g.js, fe = (DIR == 1) ? g.ie + 1 : g.je + 1;
+    const int os = (DIR == 1) ? g.js - 1 : g.is - 1, oe = (DIR == 1) ? g.je + 1 : g.ie + 1;
+    const int lo = (DIR == 1) ? g.is : g.js, hi = (DIR == 1) ? g.ie : g.je;
+    const unsigned nstrip = (unsigned)(fe - fs + STRIP) / STRIP, no = (unsigned)(oe - os + 1);
+    const unsigned nseg = (unsigned)g.N3 / 64;
+    const unsigned b = xcd_block();
+    const bool blk_act = b < nstrip * no * nseg;
+    const unsigned s = b % nstrip, rest = b / nstrip;
+    const int o = os + (int)(rest % no);
+    const unsigned seg = rest / no;
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int h = (int)seg * 64 + lane;
+    const int f0 = fs + (int)s * STRIP;
+    const int sn = (DIR == 1) ? g.SN1 : g.SN2;
+    const size_t nz = g.nz;
+    auto loaded = [&](int r) -> bool {
+        const int zd = f0 - 3 + r;
+        return blk_act && zd >= 0 && zd < sn;
+    };
+    auto zone = [&](int r) -> size_t {
+        const int zd = f0 - 3 + r;
+        return (DIR == 1) ? zidx(g, zd, o, h) : zidx(g, o, zd, h);
+    };
+    __shared__ double V[PPM_ROWS][PPM_HALF][64];
+    __shared__ double S[PPM_ROWS][PPM_HALF][64];  // MC slopes, then the left values
+    // rows of wave w: values w, w+4 (and 8: wave 0); slopes w+1, w+5 (w <= 2);
+    // parabolas w+2 (and 6: wave 0)
+    const int rv2 = (w == 0) ? PPM_ROWS - 1 : -1, rs2 = (w <= 2) ? w + 5 : -1, rp2 = (w == 0) ? 6 : -1;
+    double p_l[NPR], p_r[NPR];
 #pragma unroll
-            for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
-        } else {
-            Geom geom;
-            load_geom_block<FACE_DERIVED>(g.fblk[DIR - 1] + uni_col<true>((size_t)i * g.SN2 + j) * FACE_STRIDE, geom);
-            double Fz[NPR], ctop;
-            if constexpr (TAB)
-                face_flux(p_l, p_r, geom, DIR, EosTable{g.tab, g.ref_derivs}, g.hllf, g.laxf, Fz, ctop, fatal);
-            else
-                face_flux(p_l, p_r, geom, DIR, g.gam, g.hllf, g.laxf, Fz, ctop, fatal);
-            if (DIR == 1) {
-                // coord.c:310-320: no inflow of mass through the radial boundaries
-                if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
-                if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
-            }
-            // coord.c:290-306: antisymmetric B2 flux in the first polar ghost face
-            const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+    for (int half = 0; half < 2; half++) {
+        const int k0 = half * PPM_HALF;
+        // phase 1: the zone rows
+        for (int rr : {w, w + 4, rv2}) {
+            if (rr < 0 || !loaded(rr)) continue;
+            const size_t zz = zone(rr);
 #pragma unroll
-            for (int k = 0; k < NPR; k++)
-                if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
-            if (DIR != 2) {
-                if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
-                if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
-            }
-            if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
-                const double dxd = (DIR == 1) ? g.dx1 : g.dx2;
-                const double d = g.cour * dxd / ctop;
-                if (d < 1.e9) dtij = d;
+            for (int q = 0; q < PPM_HALF; q++) V[rr][q][lane] = pb[(k0 + q) * nz + zz];
+        }
+        __syncthreads();
+        // phase 2: MC slopes of rows 1..STRIP+3
+        for (int rr : {w + 1, rs2}) {
+            if (rr < 0 || !loaded(rr - 1) || !loaded(rr + 1)) continue;
+#pragma unroll
+            for (int q = 0; q < PPM_HALF; q++)
+                S[rr][q][lane] = slope_lim(LIM_MC, V[rr - 1][q][lane], V[rr][q][lane], V[rr + 1][q][lane]);
+        }
+        __syncthreads();
+        // phase 3: the parabolas of rows 2..6 (the MC state for a zone next to
+        // the radial edges); the right value of row w+2 is face f0+w's left state
+        double lv[2][PPM_HALF];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+#pragma unroll
+            for (int q = 0; q < PPM_HALF; q++) lv[a][q] = 0.;
+            const int rr = (a == 0) ? w + 2 : rp2;
+            if (rr < 0 || !loaded(rr - 1) || !loaded(rr + 1)) continue;
+            const int m = f0 - 3 + rr;
+            const bool ppm = m >= lo && m <= hi;
+#pragma unroll
+            for (int q = 0; q < PPM_HALF; q++) {
+                const double y = V[rr][q][lane], dq = S[rr][q][lane];
+                double l, r;
+                if (ppm) {
+                    ppm_para(V[rr - 1][q][lane], y, V[rr + 1][q][lane], S[rr - 1][q][lane], dq, S[rr + 1][q][lane],
+                             l, r);
+                } else {
+                    l = y - 0.5 * dq;
+                    r = y + 0.5 * dq;
+                }
+                lv[a][q] = l;
+                if (a == 0) p_l[k0 + q] = r;
             }
         }
+        __syncthreads();
+        // phase 4: the left values through LDS; face f0+w takes row w+3's
+#pragma unroll
+        for (int q = 0; q < PPM_HALF; q++) {
+            S[w + 2][q][lane] = lv[0][q];
+            if (rp2 >= 0) S[rp2][q][lane] = lv[1][q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PPM_HALF; q++) p_r[k0 + q] = S[w + 3][q][lane];
+        __syncthreads();
     }
+    double dtij = 1.e300;
+    int fatal = 0;
+    if (blk_act && f0 + w <= fe) strip_face<DIR, TAB>(g, F, p_l, p_r, f0 + w, o, h, fullstep, dtij, fatal);
     if (fullstep) block_min<STRIP_TPB>(dtij, dtpart);
     add_counter(cnt, HARM_FATAL, fatal);
 }
@@ -817,9 +969,12 @@ __device__ int utoprim_2d_gate(const double *U, const Geom &geom, double gam, do
     W_last = raise_W(s, W_last);
     Gnr2 st;
     gnr_start(st, fabs(W_last), x1_of_x0(s, W_last));
-    gnr_step(s, st);
-    if (st.keep && fabs(st.errx) > tau && seed_defer(sl, z, U)) return SEED_DEFERRED;
-    while (st.keep) gnr_step(s, st);
+    while (st.keep) {
+        gnr_step(s, st);
+        // the gate, after the first iteration (inside the loop, so that the loop
+        // is the plain solve's and is not peeled)
+        if (st.n_iter == 1 && st.keep && fabs(st.errx) > tau && seed_defer(sl, z, U)) return SEED_DEFERRED;
+    }
     const int retval = gnr_end(st, n_iter, k_conv);
     double pt[5];
     const int ret = finish_2d(s, Bcon, Qcon, ncon, QdotB, retval, st.x0, st.x1, pt);
@@ -2747,8 +2902,28 @@ void flux_kernel_any(harm_ctx *c, dim3 &grid, const double *pb, double *F, int f
 {
     const bool tab = c->eos_kind == 1, mc = c->g.lim == LIM_MC;
     if (c->g.lim == LIM_PPM) {
-        // six zones per face, one face per thread
-        if (c->uniform) {
+        // wave-uniform grids share the zones' slopes and parabolas through LDS
+        // (strips along X1 / X2, whole rows along phi where they divide the
+        // block); otherwise six zones per face, one face per thread
+        if (c->uniform && DIR != 3) {
+            const DGrid &g = c->g;
+            const size_t nf = (DIR == 1) ? (size_t)(g.ie + 2 - g.is) : (size_t)(g.je + 2 - g.js);
+            const size_t no = (DIR == 1) ? (size_t)(g.je + 3 - g.js) : (size_t)(g.ie + 3 - g.is);
+            const size_t nb = (nf + STRIP - 1) / STRIP * no * (size_t)(g.N3 / 64);
+            grid = dim3((unsigned)((nb + 7) / 8 * 8));
+            constexpr int D = (DIR == 3) ? 1 : DIR;
+            if (tab)
+                hipLaunchKernelGGL((k_flux_strip_ppm<D, true>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb, F,
+                                   c->dtpart, fullstep, c->cnt);
+            else
+                hipLaunchKernelGGL((k_flux_strip_ppm<D, false>), grid, dim3(STRIP_TPB), 0, c->stream, c->g, pb, F,
+                                   c->dtpart, fullstep, c->cnt);
+        } else if (c->uniform && DIR == 3 && TPB % c->g.N3 == 0) {
+            if (tab)
+                flux_kernel<DIR, true, true, LIM_PPM, DIR == 3>(c, grid, pb, F, fullstep);
+            else
+                flux_kernel<DIR, true, false, LIM_PPM, DIR == 3>(c, grid, pb, F, fullstep);
+        } else if (c->uniform) {
             if (tab)
                 flux_kernel<DIR, true, true, LIM_PPM>(c, grid, pb, F, fullstep);
             else

@@ -63,3 +63,54 @@ def test_perturbed_torus_two_steps():
     assert g.counters()["fatal"] == 0
     assert np.all(np.isfinite(g.prims()))
     g.close()
+
+
+def test_learned_seed_on_the_full_torus():
+    """The learned seed at 288x256x256: pairs captured from the plain inversion of
+    the first steps, a short training on the GPU, then the same steps from the
+    same state with the network on every zone, gated, and plain.  The seeded
+    Newton solves converge to the plain roots (field-normalized 1e-10, also the
+    gated run, where only zones whose first Newton step exceeds 1e-3 are seeded),
+    and no inversion fails or takes a fatal path."""
+    import torch
+
+    P = config2()
+    g = hc.Harm(P)
+    g.init()
+    cap = 1 << 21
+    feat = torch.zeros((8, cap), dtype=torch.float64, device="cuda:0")
+    targ = torch.zeros((2, cap), dtype=torch.float64, device="cuda:0")
+    valid = torch.zeros(cap, dtype=torch.uint8, device="cuda:0")
+    g.set_capture(feat, targ, valid, 37)
+    g.step_ch(4)
+    n = g.capture_count()
+    g.set_capture(None)
+    assert n > 1_000_000
+    m = hc.Mlp(seed=9)
+    fd, td, vd = feat[:, :n].contiguous(), targ[:, :n].contiguous(), valid[:n].contiguous()
+    del feat, targ, valid
+    m.fit_norm(fd, td, vd)
+    m.set_loss_weights(1.0, 0.0)
+    lt = m.train(fd, td, vd, batch=65536, steps=300, lr0=2e-3, lr1=2e-4, trace=True)
+    assert lt[-20:].mean() < 0.1 * lt[:5].mean()
+    p0, f0, t0 = g.prims(), g.pflag(), g.time()
+    sl = (slice(None), slice(2, P.n1 + 2), slice(2, P.n2 + 2), slice(None))
+    runs = {}
+    for key, mode, gate in (("plain", 0, None), ("always", 1, None), ("gated", 2, 1e-3)):
+        g.set_prims(p0, f0)
+        g.set_time(*t0)
+        g.set_guess(mode, m if mode else None, gate=gate)
+        g.reset_counters()
+        g.step_ch(3)
+        runs[key] = (g.prims()[sl], g.counters())
+        g.set_guess(0)
+    pp, cp = runs["plain"]
+    for key in ("always", "gated"):
+        pk, ck = runs[key]
+        for v in range(8):
+            assert np.abs(pk[v] - pp[v]).max() <= 1e-10 * np.abs(pp[v]).max(), (key, v)
+        assert ck["fail_2d"] == cp["fail_2d"] and ck["fatal"] == 0 and ck["c2p_calls"] == cp["c2p_calls"]
+    print("iterations/zone: " + ", ".join(f"{k} {c['newton_iters'] / c['c2p_calls']:.3f}" for k, (_, c) in runs.items()),
+          "; gated run bitwise equal to plain:", bool(np.array_equal(runs["gated"][0], pp)))
+    g.close()
+    m.close()

@@ -35,6 +35,8 @@ CASES = {
     "2d_ppm": lambda: torus_params(64, 64, 1, lim=3),
     "3d_ppm_llf": lambda: torus_params(24, 16, 8, lim=3, hllf=0.0, laxf=1.0),
     "3d_uniform_ppm": lambda: torus_params(32, 32, 64, lim=3),
+    # PPM along phi with rows that do not divide the block (one face per thread)
+    "3d_ppm_n3_192": lambda: torus_params(12, 16, 192, lim=3),
 }
 
 
@@ -92,7 +94,7 @@ def test_initial_data_and_fluxes_bitwise(case):
                                          ("3d_minmod_gamma43", 10), ("3d_uniform_32x32x64", 6),
                                          ("3d_uniform_vanleer_llf", 6), ("3d_uniform_n3_192", 4),
                                          ("2d_ppm", 10), ("3d_ppm_llf", 10), ("3d_uniform_ppm", 6),
-                                         ("3d_uniform_n3_256", 4)])
+                                         ("3d_uniform_n3_256", 4), ("3d_ppm_n3_192", 4)])
 def test_steps_match_oracle(case, nsteps):
     g, o = _pair(CASES[case]())
     g.init()
