@@ -285,8 +285,9 @@ def test_learned_step_matches_plain_step():
     learned.set_guess(1, m)
     # the gated seed with a threshold no first step reaches: the plain step, bit for bit
     closed.set_guess(2, m, gate=1e300)
-    # a low threshold: zones whose previous-step start is off by more than 1e-7 take the seed
-    gated.set_guess(2, m, gate=1e-7)
+    # a low threshold: zones whose previous-step start is off by more than 1e-9 go
+    # through the seed list (k_seedfix)
+    gated.set_guess(2, m, gate=1e-9)
     plain.step_ch(10)
     learned.step_ch(10)
     closed.step_ch(10)
@@ -297,6 +298,10 @@ def test_learned_step_matches_plain_step():
     for v in range(8):
         x, y = plain.prims()[sl0][v], pg[sl0][v]
         assert np.abs(x - y).max() <= 1e-10 * np.abs(x).max(), v
+    cg = gated.counters()
+    # some zones were seeded (their Newton counts differ from the plain solve's)
+    assert cg["newton_iters"] != plain.counters()["newton_iters"]
+    assert cg["c2p_calls"] == plain.counters()["c2p_calls"] and cg["fatal"] == 0
     closed.close()
     gated.close()
     pa, pb = plain.prims(), learned.prims()

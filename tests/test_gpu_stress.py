@@ -186,3 +186,42 @@ def test_gated_batch_on_stress_matches_oracle_bitwise(tau):
     if tau < 1e-10:
         assert nseed > 1400  # a 1% start never takes a first step below 1e-12
     m.close()
+
+
+def test_learned_seed_gain_on_the_stress_distribution():
+    """What the trained network gains over the method without it, on the stress
+    distribution: trained on solver-generated pairs of a disjoint batch, its
+    seed takes >= 15% fewer Newton iterations per zone than a cold start and
+    fewer than a start 1% off the root (measured at 2^26 zones with 3000
+    training steps: 7.51 against 10.39 and 8.21); the converged primitives agree
+    with the plain solve's to 1e-14 (median) and 1e-10 (99%), no zone fails."""
+    torch = _torch()
+    n = 1 << 18
+    tr = hc.stress_batch(n, hc.stress_spec(0, seed=(777, 888, 999, 1111)))
+    ptr = tr["guess"].clone()
+    gam = hc.stress_spec(0).gam
+    rtr, _, _ = hc.utoprim2d_batch(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], ptr, gam)
+    feat, targ, valid = hc.mlp_pairs(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], ptr, rtr, gam)
+    del tr, ptr
+    m = hc.Mlp(seed=13)
+    m.fit_norm(feat, targ, valid)
+    m.set_loss_weights(1.0, 0.0)
+    m.train(feat, targ, valid, batch=65536, steps=1500, lr0=2e-3, lr1=5e-5)
+    its = {}
+    for kind, kname in ((0, "pert_1pct"), (1, "cold")):
+        b = hc.stress_batch(n, hc.stress_spec(kind))
+        p0 = b["guess"].clone()
+        r0, it0, _ = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], p0, gam)
+        p1 = b["guess"].clone()
+        r1, it1, _ = hc.utoprim2d_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], p1, gam, mlp=m)
+        torch.cuda.synchronize()
+        assert int((r0 != 0).sum()) == 0 and int((r1 != 0).sum()) == 0, kname
+        its[kname] = (it0.double().mean().item(), it1.double().mean().item())
+        rel = ((p1[:2] - p0[:2]).abs() / p0[:2].abs()).max(dim=0).values
+        q = torch.quantile(rel, torch.tensor([0.5, 0.99], device=rel.device, dtype=rel.dtype)).tolist()
+        assert q[0] <= 1e-14 and q[1] <= 1e-10, (kname, q)
+        del b, p0, p1
+    print("iterations/zone (plain, learned):", its)
+    assert its["cold"][1] <= 0.85 * its["cold"][0]
+    assert its["pert_1pct"][1] < its["pert_1pct"][0]
+    m.close()
