@@ -473,9 +473,11 @@ def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_create.restype = c_void_p
     L.harm_create.argtypes = [POINTER(HarmParams), c_int]
     L.harm_destroy.argtypes = [c_void_p]
-    for name in ("harm_init_torus", "harm_reset_counters", "harm_debug_bound", "harm_debug_fixup_utoprim"):
+    for name in ("harm_init_torus", "harm_reset_counters", "harm_debug_bound", "harm_debug_fixup_utoprim",
+                 "harm_debug_fixup"):
         getattr(L, name).argtypes = [c_void_p]
     L.harm_step.argtypes = [c_void_p, c_int]
+    L.harm_debug_advance.argtypes = [c_void_p, c_int, _dp]
     L.harm_get_prims.argtypes = [c_void_p, _dp, POINTER(c_int8)]
     L.harm_set_prims.argtypes = [c_void_p, _dp, POINTER(c_int8)]
     L.harm_get_time.argtypes = [c_void_p, _dp, _dp]
@@ -780,6 +782,21 @@ class Harm:
 
     def debug_fixup_utoprim(self) -> None:
         _check(self.L.harm_debug_fixup_utoprim(self.h), "harm_debug_fixup_utoprim", self.L)
+
+    def debug_advance(self, fullstep: int = 0, ph: np.ndarray | None = None) -> np.ndarray:
+        """advance() then fixup() of one half of step_ch (step_ch.c:101-103,
+        112-114): fullstep 0 returns the predictor's ph (the state unchanged);
+        fullstep 1 takes ph and advances the state by dt."""
+        if fullstep:
+            ph = np.ascontiguousarray(ph, dtype=np.float64)
+            assert ph.shape == self.shape
+        else:
+            ph = np.zeros(self.shape)
+        _check(self.L.harm_debug_advance(self.h, fullstep, _ptr(ph)), "harm_debug_advance", self.L)
+        return ph
+
+    def debug_fixup(self) -> None:
+        _check(self.L.harm_debug_fixup(self.h), "harm_debug_fixup", self.L)
 
     def restart_write(self, path: str, **driver) -> None:
         """restart_write() (restart.c:73-184); ``driver`` sets the header fields

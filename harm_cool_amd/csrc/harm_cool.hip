@@ -1375,26 +1375,40 @@ __global__ void __launch_bounds__(TPB) k_fontfix(DGrid g, double *__restrict__ p
     add_counter(cnt, HARM_FATAL, fatal);
 }
 
-// fixup() alone over all active zones (used by the initial-data path, init.c:311,420).
-__global__ void __launch_bounds__(TPB) k_fixup(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag)
+// fixup() alone over all active zones (fixup.c:51-60): the initial-data path
+// (init.c:311,420) with cnt null, the per-stage test entry with counters.
+__global__ void __launch_bounds__(TPB) k_fixup(DGrid g, double *__restrict__ p, int8_t *__restrict__ pflag,
+                                               unsigned long long *__restrict__ cnt)
 {
     const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (idx >= (size_t)ni * nj * g.N3) return;
-    const unsigned rest = (unsigned)idx / (unsigned)g.N3;
-    const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
-    const unsigned ri = rest / (unsigned)nj;
-    const int j = g.js + (int)(rest - ri * (unsigned)nj);
-    const int i = g.is + (int)ri;
-    const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
-    Geom geom;
-    load_geom(g.geom[CENT], g.ncol, col, geom);
-    double pr[NPR];
-    for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
-    int fatal = 0;
-    const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
-    for (int k = 0; k < NPR; k++) p[k * nz + z] = pr[k];
-    if (fx & FIX_GAMMA_FAIL) pflag[z] = -33;
+    int c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0;
+    if (idx < (size_t)ni * nj * g.N3) {
+        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+        const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+        const unsigned ri = rest / (unsigned)nj;
+        const int j = g.js + (int)(rest - ri * (unsigned)nj);
+        const int i = g.is + (int)ri;
+        const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
+        Geom geom;
+        load_geom(g.geom[CENT], g.ncol, col, geom);
+        double pr[NPR];
+        for (int k = 0; k < NPR; k++) pr[k] = p[k * nz + z];
+        const int fx = fixup1zone(pr, geom, g.floors[col], g.floors[g.ncol + col], g.gam, fatal);
+        for (int k = 0; k < NPR; k++) p[k * nz + z] = pr[k];
+        if (fx & FIX_GAMMA_FAIL) {
+            pflag[z] = -33;
+            c_gam++;
+        }
+        if (fx & FIX_FLOOR) c_flr++;
+        if (fx & FIX_GAMMA_CAP) c_cap++;
+    }
+    if (cnt) {
+        add_counter(cnt, HARM_FIXUP_GAMMA, c_gam);
+        add_counter(cnt, HARM_FLOOR_ZONES, c_flr);
+        add_counter(cnt, HARM_GAMMA_CAP, c_cap);
+        add_counter(cnt, HARM_FATAL, fatal);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3031,9 +3045,10 @@ int fixup_utoprim(harm_ctx *c, double *p)
     return 0;
 }
 
-// advance() (step_ch.c:153-318) followed by fixup(), bound_prim(),
-// fixup_utoprim(), bound_prim() (step_ch.c:103-117).
-int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int fullstep)
+// advance() (step_ch.c:153-318) with fixup() folded into the update and the
+// Font-fix pass (step_ch.c:101-103, 112-114).  The Font-fix and seed lists it
+// fills are emptied by the next bound().
+int advance(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int fullstep)
 {
     const DGrid &g = c->g;
     int rc;
@@ -3125,6 +3140,15 @@ int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int
     if (g.font_fix)
         LAUNCH(HARM_K_FONTFIX, hipLaunchKernelGGL(k_fontfix, dim3(FONTFIX_BLOCKS), dim3(TPB), 0, c->stream, g, pf, c->pflag,
                                                   c->font, c->cnt));
+    return 0;
+}
+
+// advance() and fixup(), then bound_prim(), fixup_utoprim(), bound_prim()
+// (step_ch.c:103-117).
+int half_step(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int fullstep)
+{
+    int rc;
+    if ((rc = advance(c, pi, pb, pf, dtfac, fullstep))) return rc;
     if ((rc = bound(c, pf))) return rc;
     if ((rc = fixup_utoprim(c, pf))) return rc;
     if ((rc = bound(c, pf))) return rc;
@@ -3471,7 +3495,7 @@ int harm_init_torus(harm_ctx *c)
     hipLaunchKernelGGL(k_init_expand, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, d_tor, rhomax,
                        c->P.pert_amp, seed);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag);
+    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag, nullptr);
     if ((rc = bound(c, c->p))) return rc;
     const size_t nA = (size_t)(g.ie + 2 - g.is) * (g.je + 2 - g.js) * g.N3;
     HIPCHK(hipMemsetAsync(c->e3, 0, c->nz * sizeof(double), c->stream));
@@ -3488,7 +3512,7 @@ int harm_init_torus(harm_ctx *c)
     const double beta_act = (c->P.gam - 1.) * umax / (0.5 * bsq_max);  // init.c:391
     const double norm = sqrt(beta_act / c->P.beta);
     hipLaunchKernelGGL(k_init_B, dim3(nb), dim3(TPB), 0, c->stream, g, c->p, c->e3, norm, 1, c->part);
-    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag);
+    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag, nullptr);
     HIPCHK(hipGetLastError());
     if ((rc = bound(c, c->p))) return rc;
     double ts[8] = {0., c->P.dt0, 0., 0., 0., 0., 0., 0.};
@@ -3797,6 +3821,34 @@ int harm_debug_fixup_utoprim(harm_ctx *c)
     int rc = fixup_utoprim(c, c->p);
     if (rc) return rc;
     if ((rc = bound(c, c->p))) return rc;
+    return check_flags(c);
+}
+
+int harm_debug_advance(harm_ctx *c, int fullstep, double *ph)
+{
+    if (!ph) return set_err("harm_debug_advance: ph is null", 2);
+    int rc;
+    const size_t bytes = 8 * c->nz * sizeof(double);
+    c->flux_valid = false;
+    if (fullstep) {
+        HIPCHK(hipMemcpyAsync(c->ph, ph, bytes, hipMemcpyHostToDevice, c->stream));
+        if ((rc = advance(c, c->p, c->ph, c->p, 1.0, 1))) return rc;
+    } else {
+        if ((rc = advance(c, c->p, c->p, c->ph, 0.5, 0))) return rc;
+        HIPCHK(hipMemcpyAsync(ph, c->ph, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    // the step's next bound() would empty the Font-fix and seed lists
+    HIPCHK(hipMemsetAsync(c->font.count, 0, sizeof(int), c->stream));
+    if (c->seeds.count) HIPCHK(hipMemsetAsync(c->seeds.count, 0, sizeof(int), c->stream));
+    return check_flags(c);
+}
+
+int harm_debug_fixup(harm_ctx *c)
+{
+    const DGrid &g = c->g;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    hipLaunchKernelGGL(k_fixup, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->p, c->pflag, c->cnt);
+    HIPCHK(hipGetLastError());
     return check_flags(c);
 }
 

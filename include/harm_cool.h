@@ -193,6 +193,19 @@ int harm_debug_fluxes(harm_ctx *c, int fullstep, double *F1, double *F2, double 
 int harm_debug_bound(harm_ctx *c);
 /* fixup_utoprim (fixup.c:432-538) then bound_prim on the current state and flags. */
 int harm_debug_fixup_utoprim(harm_ctx *c);
+/* advance() then fixup() of one half of step_ch (step_ch.c:101-103, 112-114;
+ * advance step_ch.c:153-318, fixup fixup.c:51-60), without the bound_prim and
+ * fixup_utoprim that follow them.  fullstep 0: the predictor, state -> ph with
+ * dt/2, the state unchanged and ph copied out (its active zones; the ghosts hold
+ * what the half-step buffer held before, as in the reference until the
+ * bound_prim that follows).  fullstep 1: the corrector, from
+ * the state and the given ph -> the state with dt.  ph is 8 x (N1+4) x (N2+4) x N3
+ * doubles in the canonical layout, ghosts included; flags are those advance()
+ * leaves for the active zones (read them with harm_get_state). */
+int harm_debug_advance(harm_ctx *c, int fullstep, double *ph);
+/* fixup() alone on the current state (fixup.c:51-60, fixup1zone fixup.c:142-337):
+ * floors, the drift-frame velocity blend, the gamma cap; flag -33 where gamma fails. */
+int harm_debug_fixup(harm_ctx *c);
 /* The device pow used by the Font-fix solvers (correctly rounded), on host arrays;
  * for tests against the host libm pow the reference calls. */
 int harm_debug_pow(const double *x, const double *y, double *out, size_t n);

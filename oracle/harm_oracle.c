@@ -2833,6 +2833,56 @@ int orc_debug_fixup_utoprim(orc_ctx *o)
     return 0;
 }
 
+/* One primitive array between the oracle's zone-major layout and the canonical
+ * one (8 x (N1+4) x (N2+4) x N3); to_canon selects the direction. */
+static void canon_copy(orc_ctx *o, double *arr, double *canon, int to_canon)
+{
+    const size_t nh = o->N3, nj = o->SN2, ni = o->SN1;
+    for (size_t i = 0; i < ni; i++)
+        for (size_t j = 0; j < nj; j++)
+            for (size_t h = 0; h < nh; h++) {
+                double *z = PV(o, arr, (int)h, (int)i, (int)j);
+                for (int k = 0; k < NPR; k++) {
+                    double *c = &canon[(((size_t)k * ni + i) * nj + j) * nh + h];
+                    if (to_canon)
+                        *c = z[k];
+                    else
+                        z[k] = *c;
+                }
+            }
+}
+
+/* advance() then fixup(), the first two calls of each half of step_ch
+ * (step_ch.c:101-103 and 112-114), without the bound_prim / fixup_utoprim that
+ * follow them.  fullstep 0: the predictor p -> ph with dt/2 (p unchanged; ph is
+ * returned in the canonical layout).  fullstep 1: the corrector from p and the
+ * given ph (canonical layout, ghosts included) -> p with dt. */
+int orc_debug_advance(orc_ctx *o, int fullstep, double *ph)
+{
+    g_step_eos = o->eos;
+    g_step_refd = o->eos_refd;
+    if (fullstep) {
+        canon_copy(o, o->ph, ph, 0);
+        advance(o, o->p, o->ph, o->dt, o->p, 1);
+        fixup(o, o->p, o->cnt);
+    } else {
+        advance(o, o->p, o->p, 0.5 * o->dt, o->ph, 0);
+        fixup(o, o->ph, o->cnt);
+        canon_copy(o, o->ph, ph, 1);
+    }
+    o->flux_valid = 0;
+    g_step_eos = NULL;
+    return 0;
+}
+
+/* fixup() alone on the current state (fixup.c:51-60): floors, the drift-frame
+ * velocity blend and the gamma cap per active zone; flag -33 where gamma fails. */
+int orc_debug_fixup(orc_ctx *o)
+{
+    fixup(o, o->p, o->cnt);
+    return 0;
+}
+
 /* bound_prim on the current state (bounds.c:54-331). */
 int orc_debug_bound(orc_ctx *o)
 {

@@ -80,6 +80,12 @@ int      orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, doub
 int      orc_debug_fixup_utoprim(orc_ctx *o);
 /* bound_prim() on the current state. */
 int      orc_debug_bound(orc_ctx *o);
+/* advance() then fixup() of one half of step_ch: fullstep 0 = predictor
+ * p -> ph (dt/2), ph returned; fullstep 1 = corrector from p and the given ph
+ * -> p (dt).  ph in the canonical layout, ghosts included. */
+int      orc_debug_advance(orc_ctx *o, int fullstep, double *ph);
+/* fixup() alone on the current state. */
+int      orc_debug_fixup(orc_ctx *o);
 /* out[0] = mass (sum rho gdet dV), out[1] = sum U[UU] dV, out[2] = mdot_in,
  * out[3] = mdot_out (diag.c:477-580, diag.c:59-230). */
 void     orc_diag(const orc_ctx *o, double *out);

@@ -117,6 +117,8 @@ def _bind(L):
         L.orc_debug_fluxes.argtypes = [c_void_p, c_int, _dp, _dp, _dp, _dp]
         L.orc_debug_fixup_utoprim.argtypes = [c_void_p]
         L.orc_debug_bound.argtypes = [c_void_p]
+        L.orc_debug_advance.argtypes = [c_void_p, c_int, _dp]
+        L.orc_debug_fixup.argtypes = [c_void_p]
         L.orc_divb_max.restype = c_double
         L.orc_divb_max.argtypes = [c_void_p]
         L.orc_diag.argtypes = [c_void_p, _dp]
@@ -282,6 +284,18 @@ class Oracle:
 
     def debug_bound(self) -> None:
         self.L.orc_debug_bound(self.h)
+
+    def debug_advance(self, fullstep: int = 0, ph: np.ndarray | None = None) -> np.ndarray:
+        if fullstep:
+            ph = np.ascontiguousarray(ph, dtype=np.float64).copy()
+            assert ph.shape == self.shape
+        else:
+            ph = np.zeros(self.shape)
+        self.L.orc_debug_advance(self.h, fullstep, _ptr(ph))
+        return ph
+
+    def debug_fixup(self) -> None:
+        self.L.orc_debug_fixup(self.h)
 
     def divb_max(self) -> float:
         return self.L.orc_divb_max(self.h)

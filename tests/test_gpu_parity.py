@@ -187,3 +187,105 @@ def test_bound_prim_bitwise_3d_pole_and_inflow():
         side.debug_bound()
     assert np.array_equal(g.prims(), o.prims())
     assert np.array_equal(g.pflag(), o.pflag())
+
+
+def _same_up_to_pow(a, b, label, font_zones):
+    """Bitwise, or within TOL (field-normalized) where the stage ran Font-fix
+    solves (the device pow against glibc's in the last bit)."""
+    if font_zones == 0:
+        assert np.array_equal(a, b), label
+        return
+    for k in range(8):
+        err = np.abs(a[k] - b[k]).max() / (np.abs(b[k]).max() + 1e-300)
+        assert err <= TOL, f"{label}: variable {k} field-normalized error {err:.3e}"
+
+
+STAGE_KEYS = ("c2p_calls", "newton_iters", "newton_kconv", "fail_2d", "fail_font1", "fail_font2",
+              "fixup_gamma", "floor_zones", "gamma_cap", "fatal")
+
+
+def _delta(after, before):
+    return {k: after[k] - before[k] for k in STAGE_KEYS}
+
+
+@pytest.mark.parametrize("case", ["2d_128", "3d_32x32x16", "3d_uniform_32x32x64", "3d_ppm_llf"])
+def test_advance_stages_bitwise(case):
+    """advance() + fixup() of each half of step_ch alone (step_ch.c:101-103 and
+    112-114): from the same evolved state, the predictor's ph, the flags it
+    leaves and its counters; then the corrector from that state and the oracle's
+    ph after its bound_prim / fixup_utoprim tail.  The predictor must leave the
+    state itself untouched."""
+    P = CASES[case]()
+    g, o = _pair(P)
+    o.init_torus()
+    for _ in range(3):
+        o.step()
+    p, f = o.prims(), o.pflag()
+    t, dt = o.time()
+    for side in (g, o):
+        side.set_prims(p, f)
+        side.set_time(t, dt)
+    c0g, c0o = g.counters(), o.counters()
+    phg = g.debug_advance(0)
+    pho = o.debug_advance(0)
+    dg, do = _delta(g.counters(), c0g), _delta(o.counters(), c0o)
+    assert dg == do
+    assert do["c2p_calls"] == P.n1 * P.n2 * P.n3 and do["fatal"] == 0
+    # advance() writes the active zones of ph; its ghosts hold whatever each side
+    # left there before (the next bound_prim sets them)
+    act = (slice(None), slice(2, P.n1 + 2), slice(2, P.n2 + 2), slice(None))
+    _same_up_to_pow(phg[act], pho[act], f"{case} predictor", do["fail_2d"])
+    assert np.array_equal(g.pflag(), o.pflag())
+    assert np.array_equal(g.prims(), p) and np.array_equal(o.prims(), p)
+    # the predictor's output after the step's tail, as the corrector's midpoint
+    o.set_prims(pho, o.pflag())
+    o.debug_bound()
+    o.debug_fixup_utoprim()
+    ph_mid, f_mid = o.prims(), o.pflag()
+    for side in (g, o):
+        side.set_prims(p, f_mid)
+    c0g, c0o = g.counters(), o.counters()
+    g.debug_advance(1, ph_mid)
+    o.debug_advance(1, ph_mid)
+    dg, do = _delta(g.counters(), c0g), _delta(o.counters(), c0o)
+    assert dg == do
+    _same_up_to_pow(g.prims(), o.prims(), f"{case} corrector", do["fail_2d"])
+    assert np.array_equal(g.pflag(), o.pflag())
+    assert not np.array_equal(o.prims(), p)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["2d_128", "3d_32x32x16"])
+def test_fixup_floors_and_gamma_cap_bitwise(case):
+    """fixup() alone (fixup.c:51-60, fixup1zone fixup.c:142-337) on an evolved
+    state pushed under the density and energy floors (with and without the
+    magnetized drift-frame blend) and past the Lorentz-factor limit in random
+    active zones: primitives, flags and the floor / cap counts bitwise."""
+    P = CASES[case]()
+    g, o = _pair(P)
+    o.init_torus()
+    for _ in range(2):
+        o.step()
+    p = o.prims()
+    rng = np.random.default_rng(23)
+    act = (slice(2, P.n1 + 2), slice(2, P.n2 + 2), slice(None))
+    p2 = p.copy()
+    shp = p2[0][act].shape
+    for var, frac, fac in ((0, 0.05, 1e-6), (1, 0.05, 1e-8)):
+        m = rng.random(shp) < frac
+        p2[var][act][m] *= fac
+    m = rng.random(shp) < 0.03
+    for var in (2, 3, 4):
+        p2[var][act][m] *= 300.0
+    f = o.pflag()
+    for side in (g, o):
+        side.set_prims(p2, f)
+    c0g, c0o = g.counters(), o.counters()
+    g.debug_fixup()
+    o.debug_fixup()
+    dg, do = _delta(g.counters(), c0g), _delta(o.counters(), c0o)
+    assert dg == do
+    assert do["floor_zones"] > 0 and do["gamma_cap"] > 0
+    assert np.array_equal(g.prims(), o.prims())
+    assert np.array_equal(g.pflag(), o.pflag())
+    g.close()
