@@ -57,6 +57,26 @@ class EosTable(ctypes.Structure):
                 ("p", c_void_p), ("gam", c_double)]
 
 
+class CeosUnits(ctypes.Structure):
+    """``harm_ceos_units``: the code units and the cache EOS's ranges (cgs);
+    zero ranges take the caches' axis ends."""
+
+    _fields_ = [(n, c_double) for n in ("rho_unit", "u_unit", "p_unit", "rho_min", "rho_max", "T_min", "T_max")]
+
+
+def reference_units() -> dict:
+    """RHO_UNIT, U_UNIT, P_UNIT of the reference's COOL_EOS build
+    (cooleos.c:80-135: M_BLH = 3 Msun, M_UNIT = 1.5e-5 Msun, USCALE = 1), in
+    the reference's order of operations."""
+    GNEWT, MSUN, CL = 6.6742867e-8, 1.9892e33, 2.99792458e10
+    M_BLH = 3. * MSUN
+    L_UNIT = GNEWT * M_BLH / (CL * CL)
+    M_UNIT = 1.5e-5 * MSUN
+    RHO_UNIT = M_UNIT / (L_UNIT * L_UNIT * L_UNIT)
+    U_UNIT = RHO_UNIT * CL * CL * 1.0
+    return dict(rho_unit=RHO_UNIT, u_unit=U_UNIT, p_unit=U_UNIT)
+
+
 class RestartHdr(ctypes.Structure):
     """Binary image of ``harm_restart_hdr``: the restart header in file order
     (restart.c:105-138)."""
@@ -544,6 +564,10 @@ def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_utoprim5d_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_size_t, c_double, POINTER(EosTable), c_void_p]
     L.harm_set_eos.argtypes = [c_void_p, POINTER(EosTable), c_int]
+    L.harm_set_eos_cache.argtypes = [c_void_p, POINTER(c_void_p), c_int, POINTER(CeosUnits)]
+    L.harm_get_last_T.argtypes = [c_void_p, _dp]
+    L.harm_set_last_T.argtypes = [c_void_p, _dp]
+    L.harm_get_eos_solves.argtypes = [c_void_p, POINTER(c_int64)]
     L.harm_restart_write.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
     L.harm_restart_read.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
     L.harm_restart_peek.argtypes = [c_char_p, POINTER(RestartHdr)]
@@ -839,6 +863,40 @@ class Harm:
         _check(self.L.harm_set_eos(self.h, ctypes.byref(tab.desc) if tab is not None else None, int(ref_derivs)),
                "harm_set_eos", self.L)
         self._eos_ref = tab
+
+    def set_eos_cache(self, caches, units: "dict | None" = None) -> None:
+        """The reference's COOL_EOS pressure on cooling-table caches for the
+        step (harm_set_eos_cache): one Cache for every radius or one per radial
+        index (N1+4); units: rho_unit, u_unit, p_unit (default the reference's)
+        and rho_min, rho_max, T_min, T_max (default the caches' axis ends).
+        None: back to the Gamma-law EOS."""
+        if caches is None:
+            _check(self.L.harm_set_eos_cache(self.h, None, 0, None), "harm_set_eos_cache", self.L)
+            self._ceos_ref = None
+            return
+        caches = list(caches)
+        u = dict(reference_units())
+        u.update(units or {})
+        cu = CeosUnits(**u)
+        arr = (c_void_p * len(caches))(*[c.h for c in caches])
+        _check(self.L.harm_set_eos_cache(self.h, arr, len(caches), ctypes.byref(cu)), "harm_set_eos_cache", self.L)
+        self._ceos_ref = caches
+
+    def last_T(self) -> np.ndarray:
+        """The zones' last temperatures ((N1+4) x (N2+4) x N3; 0 = unset)."""
+        T = np.zeros(self.shape[1:])
+        _check(self.L.harm_get_last_T(self.h, _ptr(T)), "harm_get_last_T", self.L)
+        return T
+
+    def set_last_T(self, T: np.ndarray) -> None:
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        assert T.shape == self.shape[1:]
+        _check(self.L.harm_set_last_T(self.h, _ptr(T)), "harm_set_last_T", self.L)
+
+    def eos_solves(self) -> int:
+        n = c_int64()
+        _check(self.L.harm_get_eos_solves(self.h, ctypes.byref(n)), "harm_get_eos_solves", self.L)
+        return n.value
 
     def set_guess(self, mode: int, mlp: "Mlp | None" = None, gate: "float | None" = None) -> None:
         """Newton start of the inversion: 0 previous-step primitives (the

@@ -267,6 +267,21 @@ harm_cache *harm_cache_open(const char *path)
     return harm_cache_create(&d, vx.data(), vy.data(), vz.data(), values.data(), counted.data());
 }
 
+}  // extern "C"
+
+// The device view of a cache and its header, for the step's cache EOS.
+int harm_cache_view_of(const harm_cache *c, harm::CacheView &v, harm_cache_desc &d, int &device)
+{
+    if (!c) return harm_host::set_error("null cache", 1);
+    v = harm::CacheView{c->desc.x,     c->desc.y,  c->desc.z,  c->desc.d,  c->desc.xtype, c->desc.ytype, c->desc.ztype,
+                        c->whole,      c->vx,      c->vy,      c->vz,      c->values,     c->counted};
+    d = c->desc;
+    device = c->device;
+    return 0;
+}
+
+extern "C" {
+
 int harm_cache_info(const harm_cache *c, harm_cache_desc *desc, int *whole)
 {
     if (!c) return harm_host::set_error("null cache", 1);

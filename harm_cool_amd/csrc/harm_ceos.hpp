@@ -1,0 +1,181 @@
+// harm_ceos.hpp -- the step's COOL_EOS pressure on the cooling-table cache
+// (cooleos.c:314-444, 667-741): p(rho0, u) is found by a temperature Newton
+// solve on the cache of the zone's radial index, started from the zone's last
+// temperature, which every converged solve replaces (the reference's LastT,
+// selected with the cache by InitCache_CoolEOS, cooleos.c:770-807).
+//
+// Every pressure call of a face or a zone threads one temperature through the
+// calls in the reference's order.  A call repeated with the same (rho0, u)
+// returns the same bits and leaves the temperature as it was: after a
+// converged solve the start already meets the tolerance (the solve returns it
+// without a step), after a failed one the temperature was not replaced, and
+// the range branches do not depend on it.  So the last call's result is kept
+// and a repeat costs nothing; this is what makes the finite-difference
+// derivatives cheap, whose step rho0 hh / RHO_UNIT (u hh / U_UNIT) is far below
+// one ulp of rho0 (u) at the reference's units, so both their pressure calls
+// repeat the one before and the derivatives are exactly 0 (or NaN at u = 0,
+// as in the reference).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "harm_cache.hpp"
+
+namespace harm {
+
+// cooleos.c:72, 86-95
+constexpr double CEOS_HH = 1.0E-6;
+constexpr double CEOS_KBOL = 1.3e-16;
+constexpr double CEOS_CL = 2.99792458e10;
+constexpr double CEOS_MAV = (1.67262163783e-24 + 1.67492729e-24 + 2. * 9.10956e-28 + 6.6465e-24) / 5.;
+constexpr int CEOS_NTRIAL = 300;     // cooleos.c:395
+constexpr double CEOS_TOL = 1.0E-6;  // cooleos.c:395, 406, 409
+
+// One run's cache EOS: the grid every radial index's cache shares, the values
+// of each (n caches: 1 shared by every radius, or one per radial index of the
+// grid, ghosts included), the units and the ranges of cooleos.c:826-930.
+struct CeosParams {
+    CacheView view;                // axes, sizes, types; view.values is unused
+    const double *const *values;   // device array of n value pointers ([1][X][Y][D] each)
+    int n;
+    double rho_unit, u_unit, p_unit, gam;
+    double rho_min, rho_max;       // rhoMin, rhoMax (g/cm^3)
+    double T_lo, T_hi;             // TMin (1 + hh) (also the floor of a first guess), TMax (1 - hh)
+};
+
+// The state a face or a zone carries through its pressure calls.
+struct CeosZone {
+    double T;  // the zone's last temperature (0: never set)
+    unsigned long long kr = 0, ku = 0;
+    double p = 0.;
+    bool has = false;
+    int nsolve = 0;  // solves run (repeats excluded), for the counters
+};
+
+__device__ __forceinline__ const double *ceos_values(const CeosParams &P, int i)
+{
+    return P.values[P.n == 1 ? 0 : i];
+}
+
+struct CeosOut {
+    double p, T;
+    int solved;
+};
+
+// pressure_rho0_u_CoolEOS with the zone's last temperature T (cooleos.c:314-444;
+// MAXLOS 1: one solve, no random restart): the pressure and the new last T.
+// One out-of-line copy serves every call site of a kernel.
+__device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, const double *vals, double T, double rho0, double u)
+{
+    CeosOut o{0., T, 0};
+    const double rho = rho0 * P->rho_unit;
+    const double e = u * P->u_unit;
+    if (o.T == 0.0) {
+        o.T = (P->gam - 1.) * u / rho0 / CEOS_KBOL * CEOS_MAV * CEOS_CL * CEOS_CL;
+        if (o.T < P->T_lo) o.T = P->T_lo;
+    }
+    if (rho < P->rho_min || rho > P->rho_max) {
+        o.p = (P->gam - 1.) * u;
+        return o;
+    }
+    if (e < 0.0) return o;
+    o.solved = 1;
+    CacheView c = P->view;
+    c.values = vals;
+    double Tn, berr;
+    cache_solve_T(c, rho, e, 0, o.T, P->T_lo, P->T_hi, CEOS_NTRIAL, CEOS_TOL, Tn, berr);
+    if (berr > CEOS_TOL) {
+        o.p = (P->gam - 1.) * u;
+        return o;
+    }
+    double v[2];
+    // the converged solve read e at (rho, Tn), so this window is inside the cache
+    cache_lookup(c, rho, Tn, 0.0, 0, CACHE_SPLINE, 2ull, v, nullptr, 1);
+    o.T = Tn;
+    o.p = v[1] / P->p_unit;
+    return o;
+}
+
+// The tabulated EOS policy of the step's physics templates (face_flux, vchar,
+// source, primtoflux) and, through Eos5Cache, of the 5D inversion.
+struct EosCache {
+    static constexpr bool TAB = true;
+    const CeosParams *P;
+    const double *vals;  // the zone's radial index's cache
+    CeosZone *z;
+
+    __device__ double pres(double rho0, double u) const
+    {
+        const unsigned long long kr = (unsigned long long)__double_as_longlong(rho0);
+        const unsigned long long ku = (unsigned long long)__double_as_longlong(u);
+        if (z->has && kr == z->kr && ku == z->ku) return z->p;
+        const CeosOut o = ceos_solve(P, vals, z->T, rho0, u);
+        z->T = o.T;
+        z->nsolve += o.solved;
+        z->has = true;
+        z->kr = kr;
+        z->ku = ku;
+        z->p = o.p;
+        return o.p;
+    }
+    // dpdrho_calc_CoolEOS (cooleos.c:704-741)
+    __device__ double dpdrho(double rho0, double u) const
+    {
+        const double h = rho0 * CEOS_HH / P->rho_unit;
+        const double rhoP = rho0 + h;
+        double rhoM = rho0 - h;
+        if (rhoM > 0) {
+            const double pP = pres(rhoP, u);
+            const double pM = pres(rhoM, u);
+            return (pP - pM) / (2.0 * h);
+        }
+        rhoM = rho0;
+        const double pP = pres(rhoP, u);
+        const double pM = pres(rhoM, u);
+        return (pP - pM) / (h);
+    }
+    // dpdu_calc_CoolEOS (cooleos.c:667-701)
+    __device__ double dpdu(double rho0, double u) const
+    {
+        const double h = u * CEOS_HH / P->u_unit;
+        const double uP = u + h;
+        double uM = u - h;
+        if (uM > 0) {
+            const double pP = pres(rho0, uP);
+            const double pM = pres(rho0, uM);
+            return (pP - pM) / (2.0 * h);
+        }
+        uM = u;
+        const double pP = pres(rho0, uP);
+        const double pM = pres(rho0, uM);
+        return (pP - pM) / (h);
+    }
+    // vchar's COOL_EOS sound speed (phys.c:365-377), after its pressure call
+    __device__ double cs2(double rho, double u, double p) const
+    {
+        const double e = u / rho;
+        const double enthalpy = 1.0 + e + p / rho;
+        const double dpdr = dpdrho(rho, u);
+        const double dpde = dpdu(rho, u) * rho;
+        return (dpdr + p / rho / rho * dpde) / enthalpy;
+    }
+};
+
+// The 5D inversion's view of the cache EOS (utoprim_5d.c:388-389, 454-481):
+// the pressure, then dp/drho and dp/du, each by its calls above.
+struct Eos5Cache {
+    EosCache e;
+    double gam;
+    __device__ static constexpr bool tabulated() { return true; }
+    __device__ double p(double rho, double u) const { return e.pres(rho, u); }
+    __device__ void p_derivs(double rho, double u, double &p, double &dpdrho, double &dpdu) const
+    {
+        p = e.pres(rho, u);
+        dpdrho = e.dpdrho(rho, u);
+        dpdu = e.dpdu(rho, u);
+    }
+};
+
+}  // namespace harm

@@ -28,6 +28,9 @@
 #include "harm_phys.hpp"
 #include "harm_mlp.hpp"
 #include "harm_eos5d.hpp"
+#include "harm_ceos.hpp"
+
+int harm_cache_view_of(const harm_cache *c, harm::CacheView &v, harm_cache_desc &d, int &device);
 
 using namespace harm;
 
@@ -1219,6 +1222,183 @@ k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb, 
     block_counts<8>(v, cpart);
 }
 
+
+// ---------------------------------------------------------------------------
+// The step with the cache EOS (the reference's COOL_EOS build): every face and
+// every zone update threads its zone's last temperature through its pressure
+// calls (harm_ceos.hpp).  lastT is zone-indexed like the primitives.
+// ---------------------------------------------------------------------------
+
+// One face of fluxcalc (step_ch.c:331-504) with InitCache_CoolEOS before it
+// (step_ch.c:439-441), slopes from four zones along DIR (MC / van Leer /
+// minmod), fix_flux (coord.c:264-324) and the CFL limit, as k_flux.  chain:
+// the zones' last temperatures may be unset, and the loop over j runs in
+// order, so a zone's j-1 neighbour has its value when it is read
+// (cooleos.c:784-799).
+template <int DIR>
+__device__ inline void ceos_face(const DGrid &g, const CeosParams &ce, double *__restrict__ lastT,
+                                 const double *__restrict__ pb, double *__restrict__ F, int i, int j, int h,
+                                 int fullstep, bool chain, double &dtij, int &fatal, int &nsolve)
+{
+    const size_t nz = g.nz;
+    const size_t z = zidx(g, i, j, h);
+    if (DIR == 2 && (j == g.js || j == g.je + 1)) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
+        return;
+    }
+    size_t zm2, zm1, zp1;
+    if (DIR == 1) {
+        const size_t s = (size_t)g.SN2 * g.N3;
+        zm2 = z - 2 * s;
+        zm1 = z - s;
+        zp1 = z + s;
+    } else if (DIR == 2) {
+        const size_t s = (size_t)g.N3;
+        zm2 = z - 2 * s;
+        zm1 = z - s;
+        zp1 = z + s;
+    } else {
+        const size_t base = z - h;
+        zm2 = base + (h + 2 * g.N3 - 2) % g.N3;
+        zm1 = base + (h + g.N3 - 1) % g.N3;
+        zp1 = base + (h + 1) % g.N3;
+    }
+    double p_l[NPR], p_r[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
+        p_l[k] = b + 0.5 * slope_lim(g.lim, a, b, c);
+        p_r[k] = c - 0.5 * slope_lim(g.lim, b, c, d);
+    }
+    Geom geom;
+    const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
+    load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
+    CeosZone zs;
+    zs.T = lastT[z];
+    if (chain && zs.T == 0.0 && j > g.js) zs.T = lastT[z - g.N3];
+    const EosCache eos{&ce, ceos_values(ce, i), &zs};
+    double Fz[NPR], ctop;
+    face_flux(p_l, p_r, geom, DIR, eos, g.hllf, g.laxf, Fz, ctop, fatal);
+    lastT[z] = zs.T;
+    nsolve += zs.nsolve;
+    if (DIR == 1) {
+        if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
+        if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
+    }
+    const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+        if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
+    if (DIR != 2) {
+        if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
+        if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+    }
+    if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+        const double dxd = (DIR == 1) ? g.dx1 : (DIR == 2) ? g.dx2 : g.dx3;
+        const double d = g.cour * dxd / ctop;
+        if (d < 1.e9 && d < dtij) dtij = d;
+    }
+}
+
+// One face per thread, or (chain) one (h, i) column of faces per thread in j
+// order.  The face ranges are k_flux's.
+template <int DIR>
+__global__ void __launch_bounds__(TPB) k_flux_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
+                                                   const double *__restrict__ pb, double *__restrict__ F,
+                                                   double *__restrict__ dtpart, int fullstep, int chain,
+                                                   unsigned long long *__restrict__ cnt,
+                                                   unsigned long long *__restrict__ solves)
+{
+    if (halted(g)) return;
+    const int i0 = (DIR == 1) ? g.is : g.is - 1;
+    const int j0 = (DIR == 2) ? g.js : g.js - 1;
+    const int ni = g.ie + 1 - i0 + 1, nj = g.je + 1 - j0 + 1;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    const size_t n = chain ? (size_t)ni * g.N3 : (size_t)ni * nj * g.N3;
+    double dtij = 1.e300;
+    int fatal = 0, nsolve = 0;
+    if (idx < n) {
+        const int h = (int)(idx % (size_t)g.N3);
+        const size_t rest = idx / (size_t)g.N3;
+        const int i = chain ? i0 + (int)rest : i0 + (int)(rest / (size_t)nj);
+        const int ja = chain ? j0 : j0 + (int)(rest % (size_t)nj), jb = chain ? g.je + 1 : ja;
+        for (int j = ja; j <= jb; j++)
+            ceos_face<DIR>(g, ce, lastT, pb, F, i, j, h, fullstep, chain != 0, dtij, fatal, nsolve);
+    }
+    if (fullstep) block_min(dtij, dtpart);
+    add_counter(cnt, HARM_FATAL, fatal);
+    add_counter(solves, 0, nsolve);
+}
+
+// advance()'s zone loop with COOL_EOS (step_ch.c:237-306): InitCache_CoolEOS,
+// the source at pb, primtoU at pi, the update with the flux divergence S,
+// Utoprim_5d seeded with pi (no Font fix), then fixup1zone (fixup.c:51-60).
+__global__ void __launch_bounds__(TPB) k_update_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
+                                                     const double *__restrict__ pi, const double *__restrict__ pb,
+                                                     double *__restrict__ pf, int8_t *__restrict__ pflag,
+                                                     const double *__restrict__ S,
+                                                     const double *__restrict__ tstate, double dtfac, int pi_is_pb,
+                                                     unsigned long long *__restrict__ cpart,
+                                                     unsigned long long *__restrict__ solves)
+{
+    if (halted(g)) return;
+    const int ni = g.ie - g.is + 1, nj = g.je - g.js + 1;
+    const size_t na = (size_t)ni * nj * g.N3;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    int c_call = 0, c_iter = 0, c_kconv = 0, c_f2d = 0, c_gam = 0, c_flr = 0, c_cap = 0, fatal = 0, nsolve = 0;
+    if (idx < na) {
+        const unsigned rest = (unsigned)idx / (unsigned)g.N3;
+        const int h = (int)((unsigned)idx - rest * (unsigned)g.N3);
+        const unsigned ri = rest / (unsigned)nj;
+        const int j = g.js + (int)(rest - ri * (unsigned)nj);
+        const int i = g.is + (int)ri;
+        const size_t z = zidx(g, i, j, h), nz = g.nz, col = (size_t)i * g.SN2 + j;
+        const double Dt = dtfac * tstate[1];
+        Geom geom;
+        load_geom(g.geom[CENT], g.ncol, col, geom);
+        CeosZone zs;
+        zs.T = lastT[z];
+        const EosCache eos{&ce, ceos_values(ce, i), &zs};
+        double pbz[NPR], piz[NPR], dU[NPR], U[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pbz[k] = pb[k * nz + z];
+        State q;
+        source(pbz, geom, g.conn, g.ncol, col, eos, dU, q, fatal);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) piz[k] = pi_is_pb ? pbz[k] : pi[k * nz + z];
+        if (!pi_is_pb) get_state(piz, geom, q, fatal);
+        primtoflux(piz, q, 0, geom, eos, U);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U[k] += Dt * (S[k * na + idx] + dU[k]);
+        double pr[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pr[k] = piz[k];
+        int niter = 0;
+        const Eos5Cache e5{eos, g.gam};
+        const int ret = utoprim_5d(U, geom, e5, pr, niter);
+        lastT[z] = zs.T;
+        nsolve = zs.nsolve;
+        c_call = 1;
+        c_iter = niter;
+        c_kconv = niter;
+        int8_t flag = (int8_t)ret;
+        if (flag) c_f2d = 1;
+        const int fx = fixup1zone(pr, geom, ldc(g.floors, col), ldc(g.floors, g.ncol + col), g.gam, fatal);
+        if (fx & FIX_FLOOR) c_flr = 1;
+        if (fx & FIX_GAMMA_CAP) c_cap = 1;
+        if (fx & FIX_GAMMA_FAIL) {
+            flag = -33;
+            c_gam = 1;
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k * nz + z] = pr[k];
+        pflag[z] = flag;
+    }
+    const int v[8] = {c_call, c_iter, c_kconv, c_f2d, c_gam, c_flr, c_cap, fatal};
+    block_counts<8>(v, cpart);
+    add_counter(solves, 0, nsolve);
+}
 
 // The zones the gated update put on the seed list: the network's seed for
 // them (64 listed zones per wavefront, so every lane of the MFMA pass is a
@@ -2826,7 +3006,15 @@ struct harm_ctx {
     SeedList seeds;  // guess mode 2; allocated by harm_set_guess
     bool uniform;  // N3 % 64 == 0: a wave's lanes share one (i,j) column
     int halted;    // error code of the step that stopped the last batch (11, 14), else 0
-    int eos_kind;  // 0 Gamma-law, 1 tabulated (the table's device arrays belong to the caller)
+    int eos_kind;  // 0 Gamma-law, 1 tabulated (the table's device arrays belong to the caller), 2 cache
+    // the cache EOS (eos_kind 2): parameters, the zones' last temperatures, the
+    // caches' value pointers, the solve count and, per direction, whether the
+    // next flux sweep may meet unset temperatures (then it runs in j order)
+    CeosParams ceos;
+    double *lastT;
+    const double **ceos_vals;
+    unsigned long long *ceos_solves;
+    bool ceos_chain[3];
     // learned Newton seed and training-pair capture
     int guess_mode;
     double gate_tau = 0.1;  // guess mode 2
@@ -2995,7 +3183,24 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
     const int j0 = (dir == 2) ? g.js : g.js - 1;
     const size_t total = (size_t)(g.ie + 2 - i0) * (g.je + 2 - j0) * g.N3;
     dim3 grid(nblocks8(total));  // a variant may launch another grid and report it here
-    if (dir == 1)
+    if (c->eos_kind == 2) {
+        if (g.lim < 0 || g.lim > 2)
+            return set_err("the cache EOS step reconstructs with MC, van Leer or minmod (lim 0-2)", 4);
+        const int chain = c->ceos_chain[dir - 1] ? 1 : 0;
+        const size_t n = chain ? (size_t)(g.ie + 2 - i0) * g.N3 : total;
+        grid = dim3(nblocks(n));
+#define CEOS_FLUX(D_, K_)                                                                                          \
+    LAUNCH(K_, hipLaunchKernelGGL(k_flux_ceos<D_>, grid, dim3(TPB), 0, c->stream, g, c->ceos, c->lastT, pb, F,      \
+                                  c->dtpart, fullstep, chain, c->cnt, c->ceos_solves))
+        if (dir == 1)
+            CEOS_FLUX(1, HARM_K_FLUX1);
+        else if (dir == 2)
+            CEOS_FLUX(2, HARM_K_FLUX2);
+        else
+            CEOS_FLUX(3, HARM_K_FLUX3);
+#undef CEOS_FLUX
+        c->ceos_chain[dir - 1] = false;
+    } else if (dir == 1)
         LAUNCH(HARM_K_FLUX1, flux_kernel_any<1>(c, grid, pb, F, fullstep));
     else if (dir == 2)
         LAUNCH(HARM_K_FLUX2, flux_kernel_any<2>(c, grid, pb, F, fullstep));
@@ -3048,8 +3253,30 @@ int fixup_utoprim(harm_ctx *c, double *p)
 // advance() (step_ch.c:153-318) with fixup() folded into the update and the
 // Font-fix pass (step_ch.c:101-103, 112-114).  The Font-fix and seed lists it
 // fills are emptied by the next bound().
+// advance() with the cache EOS: the fluxes, their divergence, then the zone
+// update (k_update_ceos) with its fixup.
+int advance_ceos(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int fullstep)
+{
+    const DGrid &g = c->g;
+    int rc;
+    const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
+    const DivArgs dv = {c->F1, c->F2, c->F3, c->e1, c->e2, c->e3};
+    if ((rc = fluxes_and_emf(c, pb, fullstep, true))) return rc;
+    LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, dv, c->S));
+    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL(k_update_ceos, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->ceos,
+                                             c->lastT, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac,
+                                             (pi == pb) ? 1 : 0, c->cpart, c->ceos_solves));
+    CntMap map = {8, {HARM_C2P_CALLS, HARM_NEWTON_ITERS, HARM_NEWTON_KCONV, HARM_FAIL_2D, HARM_FIXUP_GAMMA,
+                      HARM_FLOOR_ZONES, HARM_GAMMA_CAP, HARM_FATAL}};
+    hipLaunchKernelGGL(k_sum_partials, dim3(RED_BLOCKS, 8), dim3(256), 0, c->stream, c->cpart, (size_t)nblocks(na), map,
+                       c->cnt, g.halt);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int advance(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, int fullstep)
 {
+    if (c->eos_kind == 2) return advance_ceos(c, pi, pb, pf, dtfac, fullstep);
     const DGrid &g = c->g;
     int rc;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
@@ -3358,7 +3585,7 @@ void ctx_free(harm_ctx *c)
                     (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->dtpart, (void *)c->cpart, (void *)c->cnt,
                     (void *)c->part, (void *)c->fix_bits, (void *)c->fix_pcount, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk, (void *)c->seeds.count, (void *)c->seeds.zone,
-                    (void *)c->seeds.U})
+                    (void *)c->seeds.U, (void *)c->lastT, (void *)c->ceos_vals, (void *)c->ceos_solves})
         if (q) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
         if (c->d_geom[l]) (void)hipFree(c->d_geom[l]);
@@ -3620,6 +3847,9 @@ int diag_volume(harm_ctx *c, double *sums, double &divbmax, double &key)
             key = q[6];
         }
     }
+    // the energy needs the EOS; the cache EOS's pressure would move the zones'
+    // last temperatures (the reference's diag() does), so it is not formed here
+    if (c->eos_kind == 2) sums[2] = NAN;
     return 0;
 }
 
@@ -3656,6 +3886,7 @@ int harm_diag(harm_ctx *c, double *out)
 int harm_diag_log(harm_ctx *c, harm_diag_rec *r)
 {
     if (!c || !r) return set_err("null argument");
+    if (c->eos_kind == 2) return set_err("not available with the cache EOS (it would move the zones' last temperatures)", 4);
     const DGrid &g = c->g;
     double sums[5], dv, key;
     int rc;
@@ -3885,6 +4116,113 @@ int harm_set_eos(harm_ctx *c, const harm_eos_table *tab, int ref_derivs)
     return 0;
 }
 
+int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, const harm_ceos_units *u)
+{
+    if (!c) return set_err("null context");
+    if (!caches) {
+        if (c->eos_kind == 2) c->eos_kind = 0;
+        return 0;
+    }
+    if (!u) return set_err("harm_set_eos_cache: null units", 1);
+    if (n != 1 && n != c->g.SN1) return set_err("harm_set_eos_cache: one cache, or one per radial index (N1+4)", 4);
+    if (c->g.lim < 0 || c->g.lim > 2)
+        return set_err("harm_set_eos_cache: the cache EOS step reconstructs with MC, van Leer or minmod (lim 0-2)", 4);
+    if (c->tr.on) return set_err("harm_set_eos_cache: not with tracers (their records take the EOS temperature)", 4);
+    if (!(u->rho_unit > 0.) || !(u->u_unit > 0.) || !(u->p_unit > 0.))
+        return set_err("harm_set_eos_cache: units must be positive", 4);
+    CacheView v0{};
+    harm_cache_desc d0;
+    std::vector<const double *> vals((size_t)n);
+    std::vector<double> ax0, ay0;
+    for (int k = 0; k < n; k++) {
+        CacheView v;
+        harm_cache_desc d;
+        int dev, rc;
+        if ((rc = harm_cache_view_of(caches[k], v, d, dev))) return rc;
+        if (dev != c->device) return set_err("harm_set_eos_cache: a cache lives on another device", 4);
+        if (!v.whole) return set_err("harm_set_eos_cache: every cache must be whole (all points computed)", 4);
+        if (v.D < 2) return set_err("harm_set_eos_cache: a cache needs components e and p", 4);
+        if (v.xtype == AXIS_ROLLING || v.ytype == AXIS_ROLLING)
+            return set_err("harm_set_eos_cache: rolling caches are not supported", 4);
+        std::vector<double> ax((size_t)v.X), ay((size_t)v.Y);
+        HIPCHK(hipMemcpy(ax.data(), v.vx, ax.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ay.data(), v.vy, ay.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (k == 0) {
+            v0 = v;
+            d0 = d;
+            ax0 = ax;
+            ay0 = ay;
+        } else if (v.X != v0.X || v.Y != v0.Y || v.D != v0.D || ax != ax0 || ay != ay0) {
+            return set_err("harm_set_eos_cache: the caches must share one (rho, T) grid", 4);
+        }
+        // z plane 0 of each cache: [X][Y][D]
+        vals[(size_t)k] = v.values;
+    }
+    CeosParams P{};
+    P.view = v0;
+    P.view.whole = 1;
+    P.n = n;
+    P.rho_unit = u->rho_unit;
+    P.u_unit = u->u_unit;
+    P.p_unit = u->p_unit;
+    P.gam = c->P.gam;
+    P.rho_min = u->rho_min != 0. ? u->rho_min : d0.xmin;
+    P.rho_max = u->rho_max != 0. ? u->rho_max : d0.xmax;
+    const double Tmin = u->T_min != 0. ? u->T_min : d0.ymin;
+    const double Tmax = u->T_max != 0. ? u->T_max : d0.ymax;
+    P.T_lo = Tmin * (1.0 + CEOS_HH);  // cooleos.c:384-385
+    P.T_hi = Tmax * (1.0 - CEOS_HH);
+    if (!(P.rho_min < P.rho_max) || !(P.T_lo < P.T_hi))
+        return set_err("harm_set_eos_cache: empty density or temperature range", 4);
+    int rc;
+    if (c->ceos_vals) (void)hipFree((void *)c->ceos_vals);
+    c->ceos_vals = nullptr;
+    if ((rc = dalloc(&c->ceos_vals, (size_t)n))) return rc;
+    HIPCHK(hipMemcpy((void *)c->ceos_vals, vals.data(), (size_t)n * sizeof(double *), hipMemcpyHostToDevice));
+    P.values = c->ceos_vals;
+    if (!c->lastT && (rc = dalloc(&c->lastT, c->nz))) return rc;
+    if (!c->ceos_solves && (rc = dalloc(&c->ceos_solves, 1))) return rc;
+    HIPCHK(hipMemset(c->lastT, 0, c->nz * sizeof(double)));  // calloc'ed by SetupCache_CoolEOS, cooleos.c:834
+    HIPCHK(hipMemset(c->ceos_solves, 0, sizeof(unsigned long long)));
+    HIPCHK(hipDeviceSynchronize());
+    c->ceos = P;
+    for (bool &b : c->ceos_chain) b = true;
+    c->eos_kind = 2;
+    return 0;
+}
+
+int harm_get_last_T(harm_ctx *c, double *T)
+{
+    if (!c || !T) return set_err("null argument");
+    if (!c->lastT) return set_err("harm_get_last_T: no cache EOS was set", 4);
+    HIPCHK(hipMemcpyAsync(T, c->lastT, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int harm_set_last_T(harm_ctx *c, const double *T)
+{
+    if (!c || !T) return set_err("null argument");
+    if (!c->lastT) return set_err("harm_set_last_T: no cache EOS was set", 4);
+    HIPCHK(hipMemcpyAsync(c->lastT, T, c->nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // unset temperatures may now sit anywhere: the next sweep of each direction runs in j order
+    for (bool &b : c->ceos_chain) b = true;
+    return 0;
+}
+
+int harm_get_eos_solves(harm_ctx *c, int64_t *n)
+{
+    if (!c || !n) return set_err("null argument");
+    unsigned long long v = 0;
+    if (c->ceos_solves) {
+        HIPCHK(hipMemcpyAsync(&v, c->ceos_solves, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    *n = (int64_t)v;
+    return 0;
+}
+
 namespace {
 
 // tracer_name (tracers.c:100-107) under the run directory's tracers/
@@ -3968,6 +4306,10 @@ int harm_restart_write(harm_ctx *c, const char *path, const harm_restart_hdr *hd
         hipLaunchKernelGGL(k_zone_temp, dim3(nblocks(c->nz)), dim3(TPB), 0, c->stream, c->g.tab, c->p, c->S, c->nz);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(temp.data(), c->S, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    } else if (c->eos_kind == 2) {
+        // restart_write_CoolEOS (cooleos.c:1000-1014): the zones' last temperatures
+        temp.resize(c->nz);
+        HIPCHK(hipMemcpyAsync(temp.data(), c->lastT, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     h.t = ts[0];
@@ -4007,7 +4349,15 @@ int harm_restart_read(harm_ctx *c, const char *path, harm_restart_hdr *hdr)
     std::vector<double> prim(8 * c->nz);
     harm_host::TracerBlock tb;
     int has_tb = 0;
-    if ((rc = harm_host::restart_read_file(path, h, prim.data(), nullptr, nullptr, &tb, &has_tb))) return rc;
+    std::vector<double> temp(c->eos_kind == 2 ? c->nz : 0);
+    int has_temp = 0;
+    if ((rc = harm_host::restart_read_file(path, h, prim.data(), temp.empty() ? nullptr : temp.data(),
+                                           temp.empty() ? nullptr : &has_temp, &tb, &has_tb)))
+        return rc;
+    if (has_temp) {
+        // restart_read_CoolEOS (cooleos.c:1016-1030)
+        if ((rc = harm_set_last_T(c, temp.data()))) return rc;
+    }
     // cour, lim, tf, defcon come from the file (restart.c:335-367)
     c->P.cour = c->g.cour = h.cour;
     c->P.lim = c->g.lim = h.lim;
@@ -4344,6 +4694,7 @@ int Utoprim_5d(double U[8], double gcov[4][4], double gcon[4][4], double gdet, d
 int harm_tracers_init(harm_ctx *c, double rho_unit, double t_unit, int64_t *n_live)
 {
     if (!c) return set_err("null context");
+    if (c->eos_kind == 2) return set_err("not available with the cache EOS (it would move the zones' last temperatures)", 4);
     if (c->P.n3 == 2) return set_err("tracers: N3 == 2 is not supported", 3);
     HIPCHK(hipStreamSynchronize(c->stream));
     tracers_free(c);
@@ -4566,6 +4917,7 @@ int dump_fill(harm_ctx *c, double *dcols, size_t cap, size_t z0, size_t n, doubl
 int harm_dump_columns(harm_ctx *c, size_t z0, size_t n, double *cols)
 {
     if (!c || (n && !cols)) return set_err("null argument");
+    if (c->eos_kind == 2) return set_err("not available with the cache EOS (it would move the zones' last temperatures)", 4);
     const DGrid &g = c->g;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
     if (z0 > na || n > na - z0) return set_err("harm_dump_columns: zone range outside the grid", 3);
@@ -4582,6 +4934,7 @@ int harm_dump_columns(harm_ctx *c, size_t z0, size_t n, double *cols)
 int harm_dump_columns_noalloc(harm_ctx *c, size_t z0, size_t n, double *cols)
 {
     if (!c || (n && !cols)) return set_err("null argument");
+    if (c->eos_kind == 2) return set_err("not available with the cache EOS (it would move the zones' last temperatures)", 4);
     const DGrid &g = c->g;
     const size_t na = (size_t)(g.ie - g.is + 1) * (g.je - g.js + 1) * g.N3;
     if (z0 > na || n > na - z0) return set_err("harm_dump_columns_noalloc: zone range outside the grid", 3);
@@ -4601,6 +4954,7 @@ int harm_dump_columns_noalloc(harm_ctx *c, size_t z0, size_t n, double *cols)
 int harm_dump_write(harm_ctx *c, const char *path, const harm_restart_hdr *hdr)
 {
     if (!c || !path) return set_err("null argument");
+    if (c->eos_kind == 2) return set_err("not available with the cache EOS (it would move the zones' last temperatures)", 4);
     harm_restart_hdr h;
     if (hdr) {
         h = *hdr;

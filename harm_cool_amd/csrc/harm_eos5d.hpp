@@ -83,12 +83,17 @@ __device__ inline int eos_table_p(const EosTab &t, double rho, double u, double 
 struct Eos5 {
     const EosTab *tab;
     double gam;
+    __device__ bool tabulated() const { return tab != nullptr; }
     __device__ double p(double rho, double u) const
     {
         if (!tab) return (gam - 1.) * u;
         double pp, a, b;
         eos_table_p(*tab, rho, u, pp, a, b);
         return pp;
+    }
+    __device__ void p_derivs(double rho, double u, double &p, double &dpdrho, double &dpdu) const
+    {
+        eos_table_p(*tab, rho, u, p, dpdrho, dpdu);
     }
 };
 
@@ -135,7 +140,8 @@ __device__ __forceinline__ void bcon_calc_g5(const double *pr, const double *uco
 }
 
 // primtoU_g (u2p_util.c:84-145) with the solver's EOS.
-__device__ inline void primtoU_g5(const Eos5 &eos, const double *pr, const Geom &g, double *U)
+template <class E5>
+__device__ inline void primtoU_g5(const E5 &eos, const double *pr, const Geom &g, double *U)
 {
     double ucon[4], ucov[4], bcon[4], bcov[4], ncov[4];
     const double alpha = 1.0 / sqrt(-g.gcon[0][0]);
@@ -184,7 +190,8 @@ __device__ __forceinline__ int gamma_calc_g5(const double *pr, const Geom &g, do
 }
 
 // dudp_calc_g (utoprim_5d.c:409-529) with its helpers (utoprim_5d.c:531-640).
-__device__ inline void dudp_calc_g5(const Eos5 &eos, const double *pr, const Geom &g, double (&Am)[N5][N5])
+template <class E5>
+__device__ inline void dudp_calc_g5(const E5 &eos, const double *pr, const Geom &g, double (&Am)[N5][N5])
 {
     double dutdui[4], dbtdui[4], dbsqdui[4], dbiduj[4][4], dgdvi[4], duidvj[4][4];
     double dudduu[4][4], dbdiduj[4][4], tempA[N5][N5];
@@ -239,9 +246,9 @@ __device__ inline void dudp_calc_g5(const Eos5 &eos, const double *pr, const Geo
             for (int l = 0; l < 4; l++) dbdiduj[j][k] += g.gcov[j][l] * dbiduj[l][k];
         }
     double eta;
-    if (eos.tab) {
+    if (eos.tabulated()) {
         double p, dpdrho, dpdu;
-        eos_table_p(*eos.tab, pr[RHO], pr[UU], p, dpdrho, dpdu);
+        eos.p_derivs(pr[RHO], pr[UU], p, dpdrho, dpdu);
         eta = pr[RHO] + pr[UU] + p + bsq;
         Am[RHO][RHO] = ucon[0];
 #pragma unroll
@@ -404,7 +411,8 @@ subst:
 }
 
 // func_5d (utoprim_5d.c:250-359), NORMMETHOD 1.
-__device__ inline void func_5d(const Eos5 &eos, const double *U_target, const Geom &g, const double *x,
+template <class E5>
+__device__ inline void func_5d(const E5 &eos, const double *U_target, const Geom &g, const double *x,
                                double (&dx)[N5], double &f, double &df)
 {
     double pr[NPR], U_curr[NPR], Am[N5][N5];
@@ -444,7 +452,8 @@ __device__ inline void func_5d(const Eos5 &eos, const double *U_target, const Ge
 
 // Utoprim_5d (utoprim_5d.c:179-239) with general_newton_raphson
 // (utoprim_5d.c:890-983) and calc_errx_5d (utoprim_5d.c:378-400).
-__device__ inline int utoprim_5d(const double *U, const Geom &g, const Eos5 &eos, double *prim, int &n_iter_out)
+template <class E5>
+__device__ inline int utoprim_5d(const double *U, const Geom &g, const E5 &eos, double *prim, int &n_iter_out)
 {
 #pragma unroll
     for (int k = B1; k <= B3; k++) prim[k] = U[k] / g.g;
@@ -466,7 +475,7 @@ __device__ inline int utoprim_5d(const double *U, const Geom &g, const Eos5 &eos
 #pragma unroll
             for (int jd = 1; jd < 4; jd++) vsq += g.gcov[id][jd] * x[U1 + id - 1] * x[U1 + jd - 1];
         const double gsqtmp = 1. + vsq;
-        if (eos.tab)
+        if (eos.tabulated())
             W = gsqtmp * (x[UU] + x[RHO] + eos.p(x[RHO], x[UU]));
         else
             W = gsqtmp * (eos.gam * x[UU] + x[RHO]);

@@ -524,6 +524,42 @@ int harm_cache_lookup(const harm_cache *c, int mode, const int *dimmask, const d
 int harm_cache_solve_T(const harm_cache *c, const double *rho, const double *e, const int *zindex, double *T,
                        double Tlo, double Thi, int ntrial, double tolf, double *berr, int *ret, size_t n, void *stream);
 
+/* The step's COOL_EOS pressure on the cooling-table cache, as the reference's
+ * COOL_EOS build computes it (replaces pressure_rho0_u_CoolEOS,
+ * dpdrho_calc_CoolEOS and dpdu_calc_CoolEOS, cooleos.c:314-444 and 667-741,
+ * with the LastT array and per-radius caches of InitCache_CoolEOS and
+ * SetupCache_CoolEOS, cooleos.c:770-807, 826-930).  Every pressure the step
+ * takes -- the face states of fluxcalc, the source and primtoU of the update,
+ * the 5D inversion's residual and Jacobian, vchar's sound speed -- is a
+ * temperature Newton solve (mnewtc<1>, 300 trials, 1e-6) on the cache of the
+ * zone's radial index for e = u U_UNIT at rho = rho0 RHO_UNIT, started from the
+ * zone's last temperature, which every converged solve replaces; then
+ * p = P(rho, T) / P_UNIT from the same cache (component 0 e, 1 p, cgs, z plane
+ * 0).  Outside [rho_min, rho_max] and where the solve does not converge the
+ * pressure is (gam - 1) u; for e < 0 it is 0; a zone whose last temperature is
+ * unset starts from (gam - 1) u / rho0 / k * m_av c^2 (at least
+ * T_min (1 + 1e-6)), or from its j-1 neighbour's in the reference's loop order.
+ * The step inverts with Utoprim_5d and has no Font fix, as with harm_set_eos.
+ * caches: n = 1 (every radius) or N1+4 (one per radial index, ghosts included),
+ * whole constant-grid caches sharing one (rho, T) grid, on the context's
+ * device; they must outlive their use.  Zero ranges take the cache's axis ends
+ * (xmin, xmax, ymin, ymax).  Every zone's last temperature starts unset.
+ * Reconstruction MC, van Leer or minmod (lim 0-2).  caches NULL returns to the
+ * Gamma-law EOS.  The ASCII dump, the tracers and the ener.out record are not
+ * available with this EOS (they would change the zones' last temperatures). */
+typedef struct harm_ceos_units {
+    double rho_unit, u_unit, p_unit;         /* RHO_UNIT, U_UNIT, P_UNIT (cooleos.c:114-135) */
+    double rho_min, rho_max, T_min, T_max;   /* rhoMin, rhoMax, TMin, TMax (cgs) */
+} harm_ceos_units;
+int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, const harm_ceos_units *u);
+/* The zones' last temperatures (restart_write_CoolEOS / restart_read_CoolEOS,
+ * cooleos.c:1000-1030), (N1+4) x (N2+4) x N3 doubles, 0 = unset. */
+int harm_get_last_T(harm_ctx *c, double *T);
+int harm_set_last_T(harm_ctx *c, const double *T);
+/* Temperature solves the cache EOS ran since it was set (repeated calls with
+ * the same arguments excluded: they return the same bits without solving). */
+int harm_get_eos_solves(harm_ctx *c, int64_t *n);
+
 /* dump() ASCII file (dump.c:1710-1990; the HDFDUMP=0 build): a header line
  * (t, N1, N2 and the theta origin/spacing in 2D or N3 and the phi ones in 3D,
  * then tf, nstep, a, gam, cour, DTd, DTl, DTi, DTr, dump/image/rdump counters,

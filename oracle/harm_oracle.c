@@ -137,6 +137,10 @@ struct orc_ctx {
     double tr_Rin, tr_Rout, tr_thmin, tr_thmax, tr_rho_unit, tr_t_unit;
     const orc_eos *eos; /* tabulated EOS of the step (NULL: Gamma-law) */
     int eos_refd;
+    /* the cooling-table cache EOS of the step (cooleos.c), when ceos_on */
+    int ceos_on;
+    orc_ceos ceos;
+    double *lastT; /* the zones' last temperatures (cooleos.c:67-69), [zi] */
 };
 
 /* The EOS of the step being computed: set from the context on entry to
@@ -145,10 +149,119 @@ struct orc_ctx {
 static const orc_eos *g_step_eos = NULL;
 static int g_step_refd = 0;
 
+/* Planted mistakes for the tests that show a pin catches them: built with
+ * -DORC_MUTANT=n, site MUT(n) takes its wrong branch (tests/oracle_lib.py
+ * build_mutant).  The default build has none. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(n) (ORC_MUTANT == (n))
+
+/* The cache EOS of the step being computed (the context, or NULL), and per
+ * OpenMP thread the zone InitCache_CoolEOS selected: its last temperature and
+ * its radial index's cache values. */
+static const orc_ctx *g_step_ceos = NULL;
+static inline size_t zi(const orc_ctx *o, int h, int i, int j);
+static _Thread_local double *t_lastT = NULL;
+static _Thread_local const double *t_cvals = NULL;
+
+/* cooleos.c:72, 86-95 */
+#define CEOS_HH 1.0E-6
+static const double CEOS_KBOL = 1.3e-16;
+static const double CEOS_CL = 2.99792458e10;
+static const double CEOS_MAV = (1.67262163783e-24 + 1.67492729e-24 + 2. * 9.10956e-28 + 6.6465e-24) / 5.;
+
+/* InitCache_CoolEOS (cooleos.c:770-807): the zone's last temperature, or, when
+ * it was never set, its j-1 neighbour's (the loops run j upwards), and the
+ * cache of radial index i. */
+static void ceos_init(const orc_ctx *o, int h, int i, int j)
+{
+    double *L = o->lastT;
+    t_lastT = &L[zi(o, h, i, j)];
+    if (*t_lastT == 0.0 && !MUT(16)) {
+        if (j > o->jstart && (*t_lastT = L[zi(o, h, i, j - 1)]) != 0.0) {
+        }
+    }
+    const orc_ceos *c = &o->ceos;
+    t_cvals = c->values + (size_t)(c->n == 1 ? 0 : i) * c->X * c->Y * c->D;
+}
+
+/* pressure_rho0_u_CoolEOS (cooleos.c:314-444), MAXLOS 1 (one solve). */
+static double ceos_pres(double rho0, double u)
+{
+    const orc_ceos *c = &g_step_ceos->ceos;
+    static const int types[3] = {0, 0, 2};
+    const double zv = 0.0;
+    double rho = rho0 * c->rho_unit;
+    double e = u * c->u_unit;
+    if (*t_lastT == 0.0) {
+        const double corrfact = 1.0;
+        *t_lastT = (c->gam - 1.) * u / rho0 / CEOS_KBOL * CEOS_MAV * CEOS_CL * CEOS_CL * corrfact;
+        if (*t_lastT < c->T_min * (1.0 + CEOS_HH)) *t_lastT = c->T_min * (1.0 + CEOS_HH);
+    }
+    if (rho < c->rho_min) return (c->gam - 1.) * u;
+    if (rho > c->rho_max) return (c->gam - 1.) * u;
+    if (e < 0.0) return 0.0;
+    double minx = c->T_min * (1.0 + CEOS_HH), maxx = c->T_max * (1.0 - CEOS_HH);
+    double x = *t_lastT, berr;
+    orc_cache_solve_T(c->X, c->Y, 1, c->D, types, 1, c->vx, c->vy, &zv, t_cvals, NULL, rho, e, 0, x, minx, maxx, 300,
+                      1.0E-6, &x, &berr);
+    if (berr > 1.0E-6) return (c->gam - 1.) * u;
+    double val[64];
+    int mask[64] = {0};
+    mask[1] = 1;
+    if (MUT(18)) mask[0] = 1;
+    orc_cache_get(c->X, c->Y, 1, c->D, types, 1, c->vx, c->vy, &zv, t_cvals, NULL, rho, x, 0.0, 0, 1, mask, val, NULL,
+                  1);
+    *t_lastT = x;
+    return val[MUT(18) ? 0 : 1] / c->p_unit;
+}
+
+/* dpdu_calc_CoolEOS (cooleos.c:667-701) */
+static double ceos_dpdu(double rho0, double u)
+{
+    double h = u * CEOS_HH / g_step_ceos->ceos.u_unit;
+    double uP = u + h;
+    double uM = u - h;
+    double ret;
+    if (uM > 0) {
+        double pP = ceos_pres(rho0, uP);
+        double pM = ceos_pres(rho0, uM);
+        ret = (pP - pM) / (2.0 * h);
+    } else {
+        uM = u;
+        double pP = ceos_pres(rho0, uP);
+        double pM = ceos_pres(rho0, uM);
+        ret = (pP - pM) / (h);
+    }
+    return ret;
+}
+
+/* dpdrho_calc_CoolEOS (cooleos.c:704-741) */
+static double ceos_dpdrho(double rho0, double u)
+{
+    double h = rho0 * CEOS_HH / g_step_ceos->ceos.rho_unit;
+    double rhoP = rho0 + h;
+    double rhoM = rho0 - h;
+    double ret;
+    if (rhoM > 0) {
+        double pP = ceos_pres(rhoP, u);
+        double pM = ceos_pres(rhoM, u);
+        ret = (pP - pM) / (2.0 * h);
+    } else {
+        rhoM = rho0;
+        double pP = ceos_pres(rhoP, u);
+        double pM = ceos_pres(rhoM, u);
+        ret = (pP - pM) / (h);
+    }
+    return ret;
+}
+
 /* mhd_calc_CoolEOS's pressure (phys.c:122-141) or the Gamma-law one. */
 static double step_pres(double gam, double r, double u)
 {
     double p;
+    if (g_step_ceos) return ceos_pres(r, u);
     if (!g_step_eos) return (gam - 1.) * u;
     orc_eos_table_p(g_step_eos, r, u, &p, NULL, NULL, NULL);
     return p;
@@ -162,13 +275,6 @@ static inline size_t zi(const orc_ctx *o, int h, int i, int j)
     return ((size_t)h * o->SN1 + (size_t)i) * o->SN2 + (size_t)j;
 }
 #define PV(o, arr, h, i, j) (&(arr)[zi(o, h, i, j) * NPR])
-/* Planted mistakes for the tests that show a pin catches them: built with
- * -DORC_MUTANT=n, site MUT(n) takes its wrong branch (tests/oracle_lib.py
- * build_mutant).  The default build has none. */
-#ifndef ORC_MUTANT
-#define ORC_MUTANT 0
-#endif
-#define MUT(n) (ORC_MUTANT == (n))
 
 static inline int hwrap(const orc_ctx *o, int h)
 {
@@ -660,18 +766,27 @@ static int vchar(const double *pr, const state_t *q, const geom_t *geom, int js,
     bsq = dot(q->bcon, q->bcov);
     rho = pr[RHO];
     u = pr[UU];
-    if (g_step_eos) {
+    if (g_step_eos || g_step_ceos) {
         /* phys.c:359-378, 398-406 */
         double p = step_pres(gam, rho, u), pp, dpdrho, dpdu, e, enthalpy, dpde;
         EF = rho + u + p;
         EE = bsq + EF;
         va2 = bsq / EE;
+        if (g_step_ceos) {
+            e = u / rho;
+            enthalpy = 1.0 + e + p / rho;
+            dpdrho = ceos_dpdrho(rho, u);
+            dpde = ceos_dpdu(rho, u) * rho;
+            cs2 = (dpdrho + p / rho / rho * dpde) / enthalpy;
+            goto have_cs2;
+        }
         orc_eos_table_p(g_step_eos, rho, u, &pp, &dpdrho, &dpdu, NULL);
         if (g_step_refd) dpdrho = dpdu = 0.;
         e = u / rho;
         enthalpy = 1.0 + e + p / rho;
         dpde = dpdu * rho;
         cs2 = (dpdrho + p / rho / rho * dpde) / enthalpy;
+    have_cs2:
         cms2 = cs2 + va2 - cs2 * va2;
         if (cms2 < SMALL) cms2 = SMALL;
         if (cms2 > 1.) cms2 = 1.;
@@ -1861,6 +1976,7 @@ static double fluxcalc(orc_ctx *o, const double *pr, double *F, int dir, int ful
                 if (ppm_zone(o, dir, i, j)) ppm_face(o, pr, h, i, j, hdel, idel, jdel, p_r, NULL);
             }
             get_geometry(o, i, j, face, &geom);
+            if (g_step_ceos) ceos_init(o, h, MUT(17) ? i - idel : i, j); /* step_ch.c:439-441 */
             face_flux(p_l, p_r, &geom, dir, o->P.gam, o->P.hllf, o->P.laxf, Fz, &ctop, &fatal);
             if (fullstep && i >= o->istart && i <= o->istop && j >= o->jstart && j <= o->jstop) {
                 double dtij = o->P.cour * o->dx[dir] / ctop;
@@ -1963,6 +2079,9 @@ static void flux_ct(orc_ctx *o)
 #undef EM
 }
 
+static int utoprim_5d_s(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8],
+                        double gam, const orc_eos *eos, int ceos, int *n_iter);
+
 /* step_ch.c:153-318 */
 static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf, int fullstep)
 {
@@ -1995,6 +2114,7 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
             double U[NPR], dU[NPR];
             int k, niter, kconv, ret;
             get_geometry(o, i, j, CENT, &geom);
+            if (g_step_ceos) ceos_init(o, h, i, j); /* step_ch.c:241-243 */
             source(PV(o, pb, h, i, j), &geom, CONN(o, i, j), o->P.gam, dU, &lc[ORC_FATAL]);
             get_state(PV(o, pi, h, i, j), &geom, &q, &lc[ORC_FATAL]);
             primtoflux(PV(o, pi, h, i, j), &q, 0, &geom, o->P.gam, U);
@@ -2010,10 +2130,10 @@ static double advance(orc_ctx *o, double *pi, double *pb, double Dt, double *pf,
                                   (PV(o, o->F2, h, i, j + 1)[k] - PV(o, o->F2, h, i, j)[k]) / o->dx[TH] + dU[k]);
             }
             double *pz = PV(o, pf, h, i, j);
-            if (g_step_eos) {
+            if (g_step_eos || g_step_ceos) {
                 /* step_ch.c:265-268 with COOL_EOS: Utoprim_5d, no Font fix */
-                ret = orc_utoprim_5d(U, &geom.gcov[0][0], &geom.gcon[0][0], geom.g, pz, o->P.gam, g_step_eos,
-                                     &niter);
+                ret = utoprim_5d_s(U, &geom.gcov[0][0], &geom.gcon[0][0], geom.g, pz, o->P.gam, g_step_eos,
+                                   g_step_ceos != NULL, &niter);
                 kconv = niter;
                 lc[ORC_C2P_CALLS]++;
                 lc[ORC_NEWTON_ITERS] += niter;
@@ -2336,10 +2456,56 @@ void orc_set_eos(orc_ctx *o, const orc_eos *eos, int ref_derivs)
     o->eos_refd = ref_derivs ? 1 : 0;
 }
 
+int orc_set_eos_cache(orc_ctx *o, const orc_ceos *c)
+{
+    const size_t nz = (size_t)o->N3 * o->SN1 * o->SN2;
+    if (!c) {
+        o->ceos_on = 0;
+        return 0;
+    }
+    if (c->n != 1 && c->n != o->SN1) return 2;
+    if (!o->lastT && !(o->lastT = (double *)malloc(nz * sizeof(double)))) return 3;
+    memset(o->lastT, 0, nz * sizeof(double)); /* calloc'ed by SetupCache_CoolEOS, cooleos.c:834 */
+    o->ceos = *c;
+    o->ceos_on = 1;
+    return 0;
+}
+
+int orc_get_lastT(const orc_ctx *o, double *T)
+{
+    if (!o->lastT) return 2;
+    const size_t nh = o->N3, nj = o->SN2, ni = o->SN1;
+    for (size_t i = 0; i < ni; i++)
+        for (size_t j = 0; j < nj; j++)
+            for (size_t h = 0; h < nh; h++) T[(i * nj + j) * nh + h] = o->lastT[zi(o, (int)h, (int)i, (int)j)];
+    return 0;
+}
+
+int orc_set_lastT(orc_ctx *o, const double *T)
+{
+    if (!o->lastT) return 2;
+    const size_t nh = o->N3, nj = o->SN2, ni = o->SN1;
+    for (size_t i = 0; i < ni; i++)
+        for (size_t j = 0; j < nj; j++)
+            for (size_t h = 0; h < nh; h++) o->lastT[zi(o, (int)h, (int)i, (int)j)] = T[(i * nj + j) * nh + h];
+    return 0;
+}
+
+double orc_ceos_pressure(orc_ctx *o, int h, int i, int j, double rho0, double u)
+{
+    if (!o->ceos_on) return NAN;
+    g_step_ceos = o;
+    ceos_init(o, h, i, j);
+    const double p = ceos_pres(rho0, u);
+    g_step_ceos = NULL;
+    return p;
+}
+
 int orc_step(orc_ctx *o)
 {
     double ndt;
     g_step_eos = o->eos;
+    g_step_ceos = o->ceos_on ? o : NULL;
     g_step_refd = o->eos_refd;
     int64_t fatal0 = o->cnt[ORC_FATAL];
     ndt = advance(o, o->p, o->p, 0.5 * o->dt, o->ph, 0);
@@ -2356,8 +2522,12 @@ int orc_step(orc_ctx *o)
      * the clock update; here the step's state is kept and 14 returned. */
     if (o->cnt[ORC_FATAL] != fatal0) {
         g_step_eos = NULL;
+        g_step_ceos = NULL;
         return 14;
     }
+    /* diag_mass (diag.c:477-580) takes no pressure: the cache EOS's zone
+     * selection ends with the advance */
+    if (!MUT(19)) g_step_ceos = NULL;
     {
         double out[4];
         orc_diag(o, out);
@@ -2366,6 +2536,7 @@ int orc_step(orc_ctx *o)
     }
     if (o->tr_on) tracers_update(o, o->dt); /* step_ch.c:121-123 */
     g_step_eos = NULL;
+    g_step_ceos = NULL;
     if (o->dt < 1.e-9) return 11;           /* step_ch.c:126-129: exit before t += dt */
     o->t += o->dt;
     if (ndt > SAFE * o->dt) ndt = SAFE * o->dt;
@@ -2801,6 +2972,7 @@ int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F
 {
     int64_t cnt[ORC_NCOUNTERS];
     g_step_eos = o->eos;
+    g_step_ceos = o->ceos_on ? o : NULL;
     g_step_refd = o->eos_refd;
     memset(cnt, 0, sizeof(cnt));
     ndt[0] = fluxcalc(o, o->p, o->F1, 1, fullstep, cnt);
@@ -2821,6 +2993,7 @@ int orc_debug_fluxes(orc_ctx *o, int fullstep, double *F1, double *F2, double *F
                         dst[d][(((size_t)k * ni + i) * nj + j) * nh + h] = PV(o, src[d], (int)h, (int)i, (int)j)[k];
     }
     g_step_eos = NULL;
+    g_step_ceos = NULL;
     return 0;
 }
 
@@ -2860,6 +3033,7 @@ static void canon_copy(orc_ctx *o, double *arr, double *canon, int to_canon)
 int orc_debug_advance(orc_ctx *o, int fullstep, double *ph)
 {
     g_step_eos = o->eos;
+    g_step_ceos = o->ceos_on ? o : NULL;
     g_step_refd = o->eos_refd;
     if (fullstep) {
         canon_copy(o, o->ph, ph, 0);
@@ -2872,6 +3046,7 @@ int orc_debug_advance(orc_ctx *o, int fullstep, double *ph)
     }
     o->flux_valid = 0;
     g_step_eos = NULL;
+    g_step_ceos = NULL;
     return 0;
 }
 
@@ -2950,7 +3125,7 @@ void orc_destroy(orc_ctx *o)
     free(o->p); free(o->ph); free(o->dq); free(o->F1); free(o->F2); free(o->F3);
     free(o->e1); free(o->e2); free(o->e3); free(o->pflag);
     free(o->gcov); free(o->gcon); free(o->gdet); free(o->conn); free(o->coords); free(o->Katm);
-    free(o->tr_X); free(o->tr_state); free(o->tr_ev);
+    free(o->tr_X); free(o->tr_state); free(o->tr_ev); free(o->lastT);
     free(o);
 }
 
@@ -3379,7 +3554,14 @@ typedef struct {
     double U_target[NPR];
     double gcov[NDIM][NDIM], gcon[NDIM][NDIM], gdet, gam;
     const orc_eos *eos;
+    int ceos; /* the step's cache EOS (the zone InitCache_CoolEOS selected) */
 } u5_t;
+
+static double eos5_ps(const u5_t *s, double rho, double u)
+{
+    if (s->ceos) return ceos_pres(rho, u);
+    return eos5_p(s->eos, s->gam, rho, u);
+}
 
 /* primtoU_g (u2p_util.c:84-145) with the EOS of the solver. */
 static void primtoU_g5(const u5_t *s, const double prim[NPR], double U[NPR])
@@ -3403,7 +3585,7 @@ static void primtoU_g5(const u5_t *s, const double prim[NPR], double U[NPR])
     for (i = 0; i < 4; i++) bsq += bcov[i] * bcon[i];
     rho0 = prim[RHO];
     u = prim[UU];
-    p = eos5_p(s->eos, s->gam, rho0, u);
+    p = eos5_ps(s, rho0, u);
     w = rho0 + u + p;
     U[RHO] = ucon[0] * rho0;
     for (i = 0; i < 4; i++) {
@@ -3473,9 +3655,16 @@ static void dudp_calc_g5(const u5_t *s, const double *pr, double Am[N5][N5])
     for (j = 0; j < NDIM; j++)
         for (k = 1; k < NDIM; k++)
             for (l = 0; l < NDIM; l++) dbdiduj[j][k] += gc[j][l] * dbiduj[l][k];
-    if (s->eos) {
+    if (s->eos || s->ceos) {
         double p, dpdrho, dpdu;
-        orc_eos_table_p(s->eos, pr[RHO], pr[UU], &p, &dpdrho, &dpdu, NULL);
+        if (s->ceos) {
+            /* utoprim_5d.c:454-481: eta's pressure, then dp/drho, then dp/du */
+            p = ceos_pres(pr[RHO], pr[UU]);
+            dpdrho = ceos_dpdrho(pr[RHO], pr[UU]);
+            dpdu = ceos_dpdu(pr[RHO], pr[UU]);
+        } else {
+            orc_eos_table_p(s->eos, pr[RHO], pr[UU], &p, &dpdrho, &dpdu, NULL);
+        }
         eta = pr[RHO] + pr[UU] + p + bsq;
         Am[RHO][RHO] = ucon[0];
         for (j = 0; j < NDIM; j++) Am[UU + j][RHO] = (1.0 + dpdrho) * ucon[0] * ucov[j] + (j == 0 ? 1. : 0.) * dpdrho;
@@ -3628,8 +3817,8 @@ static void calc_errx_5d(const u5_t *s, double *W, double *vsq, const double x[]
         for (jd = 1; jd < 4; jd++) *vsq += s->gcov[id][jd] * x[U1 + id - 1] * x[U1 + jd - 1];
     gsqtmp = 1. + *vsq;
     *vsq /= gsqtmp;
-    if (s->eos)
-        *W = gsqtmp * (x[UU] + x[RHO] + eos5_p(s->eos, s->gam, x[RHO], x[UU]));
+    if (s->eos || s->ceos)
+        *W = gsqtmp * (x[UU] + x[RHO] + eos5_ps(s, x[RHO], x[UU]));
     else
         *W = gsqtmp * (s->gam * x[UU] + x[RHO]);
 }
@@ -3661,8 +3850,8 @@ static int gnr_5d(const u5_t *s, double x[N5], int *n_iter_out)
 
 /* Utoprim_5d (utoprim_5d.c:179-239): U is HARM's conserved vector (x gdet),
  * prim the guess in and the result out (B always, the rest only on success). */
-int orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8], double gam,
-                   const orc_eos *eos, int *n_iter)
+static int utoprim_5d_s(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8],
+                        double gam, const orc_eos *eos, int ceos, int *n_iter)
 {
     u5_t s;
     double prim_tmp[NPR], gamma_tmp;
@@ -3672,6 +3861,7 @@ int orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, do
     s.gdet = gdet;
     s.gam = gam;
     s.eos = eos;
+    s.ceos = ceos;
     for (k = 0; k < NPR; k++) s.U_target[k] = U[k];
     for (k = B1; k <= B3; k++) prim[k] = U[k] / gdet;
     for (k = 0; k < NPR; k++) prim_tmp[k] = prim[k];
@@ -3681,6 +3871,12 @@ int orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, do
     if (gamma_calc_g5(prim_tmp, s.gcov, &gamma_tmp) == 1) return 4;
     for (k = 0; k < NPR; k++) prim[k] = prim_tmp[k];
     return 0;
+}
+
+int orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, double gdet, double prim[8], double gam,
+                   const orc_eos *eos, int *n_iter)
+{
+    return utoprim_5d_s(U, gcov, gcon, gdet, prim, gam, eos, 0, n_iter);
 }
 
 /* primtoU_g with the solver's EOS (for round trips and tests). */
@@ -3693,6 +3889,7 @@ void orc_primtoU_eos(const double prim[8], const double *gcov, const double *gco
     s.gdet = gdet;
     s.gam = gam;
     s.eos = eos;
+    s.ceos = 0;
     primtoU_g5(&s, prim, U);
 }
 
@@ -3707,6 +3904,7 @@ void orc_dudp_5d(const double prim[8], const double *gcov, const double *gcon, d
     s.gdet = gdet;
     s.gam = gam;
     s.eos = eos;
+    s.ceos = 0;
     dudp_calc_g5(&s, prim, A);
     memcpy(Am, A, sizeof(A));
 }

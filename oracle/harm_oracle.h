@@ -158,6 +158,36 @@ int  orc_eos_table_p(const orc_eos *t, double rho, double u, double *p, double *
                      double *dpdu, double *T);
 /* Step with the tabulated EOS (the reference's COOL_EOS build); NULL: Gamma-law. */
 void orc_set_eos(orc_ctx *o, const orc_eos *eos, int ref_derivs);
+/* The step's COOL_EOS pressure on the cooling-table cache (cooleos.c:314-444,
+ * 667-741, 770-807, 826-930): a temperature Newton solve per call on the cache
+ * of the zone's radial index, from the zone's last temperature.  The caches
+ * share the (rho, T) grid vx[X] x vy[Y] (cgs, z = 1 plane); values holds n of
+ * them, [n][X][Y][D] (component 0 e, 1 p, cgs), n = 1 (every radius) or N1+4
+ * (one per radial index, ghosts included); rho_min..T_max are rhoMin, rhoMax,
+ * TMin, TMax.  The arrays must outlive the context's use of them. */
+typedef struct orc_ceos {
+    int X, Y, D, n;
+    const double *vx, *vy, *values;
+    double rho_unit, u_unit, p_unit, gam;
+    double rho_min, rho_max, T_min, T_max;
+} orc_ceos;
+/* Step with the cache EOS (NULL: back to orc_set_eos's EOS); the zones' last
+ * temperatures start at 0 (never set). */
+int  orc_set_eos_cache(orc_ctx *o, const orc_ceos *c);
+/* The zones' last temperatures, canonical zone layout (N1+4) x (N2+4) x N3. */
+int  orc_get_lastT(const orc_ctx *o, double *T);
+int  orc_set_lastT(orc_ctx *o, const double *T);
+/* pressure_rho0_u_CoolEOS for one call on zone (h, i, j) (array indices),
+ * with InitCache_CoolEOS's selection; the zone's last T is updated. */
+double orc_ceos_pressure(orc_ctx *o, int h, int i, int j, double rho0, double u);
+/* The cache's lookup and temperature solve (cache_oracle.c). */
+int  orc_cache_get(int X, int Y, int Z, int D, const int *types, int whole, const double *vx, const double *vy,
+                   const double *vz, const double *values, const uint8_t *counted, double x, double y, double z,
+                   int zindex, int spline, const int *dimmask, double *out, double *der, size_t ostride);
+int  orc_cache_solve_T(int X, int Y, int Z, int D, const int *types, int whole, const double *vx, const double *vy,
+                       const double *vz, const double *values, const uint8_t *counted, double rho, double e,
+                       int zindex, double T0, double lo, double hi, int ntrial, double tolf, double *T,
+                       double *berr);
 /* Utoprim_5d (utoprim_5d.c:179-239); eos NULL = Gamma-law. */
 int  orc_utoprim_5d(const double U[8], const double *gcov, const double *gcon, double gdet,
                     double prim[8], double gam, const orc_eos *eos, int *n_iter);

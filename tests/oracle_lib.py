@@ -46,6 +46,26 @@ def orc_eos(tab: dict, gam: float) -> OrcEos:
     return d
 
 
+class OrcCeos(ctypes.Structure):
+    _fields_ = [("X", c_int), ("Y", c_int), ("D", c_int), ("n", c_int), ("vx", _dp), ("vy", _dp), ("values", _dp)] + [
+        (k, c_double) for k in ("rho_unit", "u_unit", "p_unit", "gam", "rho_min", "rho_max", "T_min", "T_max")]
+
+
+def orc_ceos(vx, vy, values, gam: float, rho_unit: float, u_unit: float, p_unit: float, rho_min=None, rho_max=None,
+             T_min=None, T_max=None) -> OrcCeos:
+    """Oracle view of host caches sharing the grid vx x vy: values [n, X, Y, D]
+    (cgs); ranges default to the axis ends.  Keeps the arrays alive."""
+    vx, vy = np.ascontiguousarray(vx, dtype=np.float64), np.ascontiguousarray(vy, dtype=np.float64)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    n, X, Y, D = values.shape
+    d = OrcCeos(X=X, Y=Y, D=D, n=n, vx=_ptr(vx), vy=_ptr(vy), values=_ptr(values), rho_unit=rho_unit, u_unit=u_unit,
+                p_unit=p_unit, gam=gam, rho_min=vx[0] if rho_min is None else rho_min,
+                rho_max=vx[-1] if rho_max is None else rho_max, T_min=vy[0] if T_min is None else T_min,
+                T_max=vy[-1] if T_max is None else T_max)
+    d._keep = (vx, vy, values)
+    return d
+
+
 _lib = None
 
 
@@ -156,6 +176,11 @@ def _bind(L):
         L.orc_primtoU_eos.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
         L.orc_dudp_5d.argtypes = [_dp, _dp, _dp, c_double, c_double, POINTER(OrcEos), _dp]
         L.orc_set_eos.argtypes = [c_void_p, POINTER(OrcEos), c_int]
+        L.orc_set_eos_cache.argtypes = [c_void_p, POINTER(OrcCeos)]
+        L.orc_get_lastT.argtypes = [c_void_p, _dp]
+        L.orc_set_lastT.argtypes = [c_void_p, _dp]
+        L.orc_ceos_pressure.argtypes = [c_void_p, c_int, c_int, c_int, c_double, c_double]
+        L.orc_ceos_pressure.restype = c_double
         L.orc_restart_write.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp]
         L.orc_restart_read.argtypes = [ctypes.c_char_p, POINTER(OrcRestartHdr), _dp, _dp, c_int]
         L.orc_dump_write.argtypes = [c_void_p, ctypes.c_char_p, POINTER(OrcRestartHdr)]
@@ -296,6 +321,23 @@ class Oracle:
 
     def debug_fixup(self) -> None:
         self.L.orc_debug_fixup(self.h)
+
+    def set_eos_cache(self, ceos: "OrcCeos | None") -> None:
+        rc = self.L.orc_set_eos_cache(self.h, ctypes.byref(ceos) if ceos is not None else None)
+        assert rc == 0, rc
+        self._ceos = ceos
+
+    def last_T(self) -> np.ndarray:
+        T = np.zeros(self.shape[1:])
+        assert self.L.orc_get_lastT(self.h, _ptr(T)) == 0
+        return T
+
+    def set_last_T(self, T: np.ndarray) -> None:
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        assert self.L.orc_set_lastT(self.h, _ptr(T)) == 0
+
+    def ceos_pressure(self, h: int, i: int, j: int, rho0: float, u: float) -> float:
+        return self.L.orc_ceos_pressure(self.h, h, i, j, rho0, u)
 
     def divb_max(self) -> float:
         return self.L.orc_divb_max(self.h)
