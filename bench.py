@@ -313,6 +313,62 @@ def tabulated_torus(hc, host: dict, steps: int, warmup: int) -> dict:
             "frac_zones_in_table_rho_range": float(inside.mean()), "frac_zones_on_table": float(on.mean())}
 
 
+def cache_eos_torus(hc, tab: dict, steps: int, warmup: int) -> dict:
+    """The torus on the reference's own COOL_EOS pressure path: every pressure a
+    temperature solve on the cooling-table cache (the generator's 200 x 200
+    table as one cache for every radius) from the zone's last T; 192x192x128,
+    rho_unit 1e11 g/cm^3 and U_UNIT = P_UNIT = rho_unit c^2 as the tabulated
+    section.  The first step (in the warm-up) runs the first sweep of each
+    direction in j order."""
+    import numpy as np
+
+    from harm_cool_amd.params import torus_params
+
+    ax_rho, ax_T = tab["_axes"]
+    hd = hc.c3_hdfac()
+    desc = hc.cache_desc(ax_rho.size, ax_rho[0], ax_rho[-1], ax_T.size, ax_T[0], ax_T[-1], 1, hd, hd, d=6,
+                         ztype=hc.AXIS_CUSTOM)
+    values = tab["_vals"].T.reshape(1, ax_rho.size, ax_T.size, 6)
+    cache = hc.Cache.create(desc, ax_rho, ax_T, np.array([hd]), values, np.ones(values.shape[:3], np.uint8))
+    rho_unit = 1e11
+    u_unit = rho_unit * 2.99792458e10 * 2.99792458e10
+    P = torus_params(192, 192, 128)
+    g = hc.Harm(P, device=0)
+    g.set_eos_cache([cache], {"rho_unit": rho_unit, "u_unit": u_unit, "p_unit": u_unit})
+    g.init()
+    t0 = time.perf_counter()
+    g.step_ch(1)
+    g.time()
+    first = time.perf_counter() - t0
+    g.step_ch(max(warmup - 1, 0))
+    g.reset_counters()
+    s0 = g.eos_solves()
+    g.time()
+    t0 = time.perf_counter()
+    g.step_ch(steps)
+    g.time()
+    el = time.perf_counter() - t0
+    c = g.counters()
+    solves = g.eos_solves() - s0
+    pr = g.prims()
+    T = g.last_T()
+    g.close()
+    cache.close()
+    nz = 192 * 192 * 128
+    rho = pr[0][2:194, 2:194] * rho_unit
+    inside = (rho >= ax_rho[0]) & (rho <= ax_rho[-1])
+    Ta = T[2:194, 2:194]
+    solved = inside & (Ta > ax_T[0]) & (Ta < ax_T[-1])
+    return {"workload": "FM torus 192x192x128, the reference's COOL_EOS pressure: mnewtc<1> T solve per call on "
+                        "the cooling-table cache (Akima 4x4, the generator's 200x200 table as one cache), "
+                        "per-zone last T, Utoprim_5d",
+            "steps": steps, "zone_cycles_per_s": nz * steps / el, "ms_per_step": el / steps * 1e3,
+            "first_step_ms": first * 1e3, "n_iter_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1),
+            "fail_5d": c["fail_2d"], "T_solves_per_zone_cycle": solves / (nz * steps),
+            "frac_zones_in_cache_rho_range": float(inside.mean()),
+            "frac_zones_with_T_in_cache": float(solved.mean())}
+
+
 def fast_mode(hc, n: tuple, steps: int, warmup: int, dev_steps: int = 10) -> dict:
     """The same step built with FMA contraction (libharmcool_fast.so), reported
     beside the parity build: its step time, per-kernel times, and its deviation
@@ -604,9 +660,10 @@ def main() -> None:
     eos_tab = None
     if not (args.no_eos_table and args.no_tabulated):
         eos_tab = eos_table(hc)
-    tabulated = None
+    tabulated = cache_eos = None
     if not args.no_tabulated:
         tabulated = tabulated_torus(hc, eos_tab["_table"], args.steps, args.warmup)
+        cache_eos = cache_eos_torus(hc, eos_tab, args.steps, args.warmup)
     if not args.no_eos_table:
         eos_tab["cache_lookup"] = cache_lookups(hc, torch, eos_tab)
     else:
@@ -644,7 +701,7 @@ def main() -> None:
         "newton_iters_per_zone": c["newton_iters"] / max(c["c2p_calls"], 1),
         "newton_kconv_per_zone": c["newton_kconv"] / max(c["c2p_calls"], 1),
         "newton_stopping_rule": "reference rule for both starts: |dW/W|+|dv2/v2| <= 1e-10, then 2 extra iterations",
-        "learned_seed_torus": learned_rep, "stress_batch": st, "tabulated_eos_torus": tabulated,
+        "learned_seed_torus": learned_rep, "stress_batch": st, "tabulated_eos_torus": tabulated, "cache_eos_torus": cache_eos,
         "roofline": roof, "kernels": kernels, "fast_mode_fma": fast, "tracers": tracers, "eos_table": eos_tab,
         "cpu_baseline": cpu,
     }

@@ -32,12 +32,20 @@ enum { AXIS_LOG = 0, AXIS_LINEAR, AXIS_CUSTOM, AXIS_NONUNIFORM, AXIS_ROLLING };
 
 // 1-based index `left` of the bracket [x[left-1], x[left]] of xval in the
 // increasing x[0..n): the first i in 2..n-1 with xval < x[i-1] gives i-1,
-// none gives n-1 (akima.c:234-298).
+// none gives n-1 (akima.c:234-298).  The reference scans i upwards; as x
+// increases, xval < x[i-1] holds from that first i on, so bisection finds the
+// same i in log2(n) reads (a NaN never compares less: n-1 either way).
 HC_HD int cache_bracket(int n, const double *x, double xval)
 {
-    for (int i = 2; i <= n - 1; i++)
-        if (xval < x[i - 1]) return i - 1;
-    return n - 1;
+    int lo = 2, hi = n;  // the first i in [lo, hi) with xval < x[i-1], hi = none
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (xval < x[mid - 1])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return (lo <= n - 1) ? lo - 1 : n - 1;
 }
 
 // Window [lo, hi] of grid indices a lookup reads along one axis
