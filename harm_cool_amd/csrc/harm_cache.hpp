@@ -145,6 +145,77 @@ HC_HD bool cache_akima_der(int m, const double *xs, const double *ys, double t, 
     return true;
 }
 
+// cache_akima with the window size M (3 or 4) and the number of output points
+// NT fixed at compile time: the same operations in the same order, the
+// interval search unrolled (it advances i one interval at a time, as the
+// loop it replaces does), so every array stays in registers.
+template <int M, int NT>
+HC_HD bool cache_akima_m(const double *xs, const double *ys, const double *ts, double *out)
+{
+    double S[M + 3];
+#pragma unroll
+    for (int k = 0; k < M - 1; k++) {
+        const double dx = xs[k + 1] - xs[k];
+        if (dx < 1e-12) return false;
+        S[k + 2] = (ys[k + 1] - ys[k]) / dx;
+    }
+    {
+        const double x0 = xs[0], x1 = xs[1], y0 = ys[0];
+        const double xl1 = x0 + x1 - xs[2];
+        const double xl0 = xl1 + x0 - x1;
+        const double yl1 = (x0 - xl1) * (S[3] - 2.0 * S[2]) + y0;
+        const double g1 = (y0 - yl1) / (x0 - xl1);
+        const double yl0 = (xl1 - xl0) * (S[2] - 2.0 * g1) + yl1;
+        S[0] = (yl1 - yl0) / (xl1 - xl0);
+        S[1] = g1;
+    }
+    {
+        const double xa = xs[M - 2], xb = xs[M - 1], yb = ys[M - 1];
+        const double xr2 = xb + xa - xs[M - 3];
+        const double xr3 = xr2 + xb - xa;
+        const double yr2 = (2.0 * S[M] - S[M - 1]) * (xr2 - xb) + yb;
+        S[M + 1] = (yr2 - yb) / (xr2 - xb);
+        const double yr3 = (2.0 * S[M + 1] - S[M]) * (xr3 - xr2) + yr2;
+        S[M + 2] = (yr3 - yr2) / (xr3 - xr2);
+    }
+    double sl[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const double a = S[k], b = S[k + 1], c = S[k + 2], d = S[k + 3];
+        double d0 = d - c, d1 = b - a;
+        if (d0 < 0.0) d0 *= -1.0;
+        if (d1 < 0.0) d1 *= -1.0;
+        sl[k] = (d0 + d1) < 1e-9 ? 0.5 * (b + c) : (d0 * b + d1 * c) / (d0 + d1);
+    }
+    int i = -1;
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const double t = ts[q];
+#pragma unroll
+        for (int s = 0; s <= M - 2; s++)
+            if (i == s - 1 && t > xs[s]) i = s;
+        if (i < 0) i = 0;
+        double xi = xs[0], xi1 = xs[1], yi = ys[0], yi1 = ys[1], g0 = sl[0], g1 = sl[1];
+#pragma unroll
+        for (int s = 1; s <= M - 2; s++)
+            if (i == s) {
+                xi = xs[s];
+                xi1 = xs[s + 1];
+                yi = ys[s];
+                yi1 = ys[s + 1];
+                g0 = sl[s];
+                g1 = sl[s + 1];
+            }
+        const double d0 = xi1 - xi;
+        const double t2 = (yi1 - yi) / d0;
+        const double p1 = (3.0 * t2 - 2.0 * g0 - g1) / d0;
+        const double p2 = (g0 + g1 - 2.0 * t2) / (d0 * d0);
+        const double t1 = t - xi;
+        out[q] = yi + g0 * t1 + p1 * t1 * t1 + p2 * t1 * t1 * t1;
+    }
+    return true;
+}
+
 struct CacheView {
     int X, Y, Z, D;
     int xtype, ytype, ztype;
@@ -244,6 +315,93 @@ HC_HD int cache_lookup(const CacheView &c, double x, double y, double z, int zin
     return CACHE_OK;
 }
 
+// The spline value (and with DER the derivative along y, cache_akima_der's
+// centred difference) of component k over an NX x NY window at (x0, y0) of z
+// plane zp: cache_lookup's operations for one component.
+template <int NX, int NY, bool DER>
+HC_HD bool cache_spline_win(const CacheView &c, int x0, int y0, int zp, int k, double x, double y, double &v,
+                            double &dv)
+{
+    const size_t XY = (size_t)c.X * c.Y;
+    const double *xs = c.vx + x0, *ys = c.vy + y0;
+    double xt[NX], xd[NX];
+    bool ok = true;
+#pragma unroll
+    for (int b = 0; b < NX; b++) {
+        const double *row = c.values + ((size_t)zp * XY + (size_t)(x0 + b) * c.Y + y0) * c.D + k;
+        double yv[NY];
+#pragma unroll
+        for (int q = 0; q < NY; q++) yv[q] = row[(size_t)q * c.D];
+        if (DER) {
+            const double h = y * 1.0E-6;
+            const double ts[3] = {y - h, y, y + h};
+            double o3[3];
+            ok &= cache_akima_m<NY, 3>(ys, yv, ts, o3);
+            xt[b] = o3[1];
+            xd[b] = (o3[2] - o3[0]) / (2.0 * h);
+        } else {
+            ok &= cache_akima_m<NY, 1>(ys, yv, &y, &xt[b]);
+        }
+    }
+    ok &= cache_akima_m<NX, 1>(xs, xt, &x, &v);
+    if (DER) ok &= cache_akima_m<NX, 1>(xs, xd, &x, &dv);
+    return ok;
+}
+
+// cache_lookup for one component k of a z plane (zindex >= 0) in spline mode:
+// the same status and bits, with the window's size fixed per call.
+HC_HD int cache_lookup(const CacheView &c, double x, double y, double z, int zindex, int mode, uint64_t mask,
+                       double *out, double *dery, size_t ostride);
+
+template <bool DER>
+HC_HD int cache_lookup1(const CacheView &c, double x, double y, int zindex, int k, double &v, double &dv)
+{
+    if (zindex == -1) {
+        // interpolation in z (at z = 0): the general lookup
+        double o[64], d[64];
+        const int st = cache_lookup(c, x, y, 0.0, -1, CACHE_SPLINE, 1ull << k, o, DER ? d : nullptr, 1);
+        if (st == CACHE_OK) {
+            v = o[k];
+            if (DER) dv = d[k];
+        }
+        return st;
+    }
+    int rolling = 0;
+    int s = cache_axis_check(c.xtype, c.X, c.vx, x, true);
+    if (s == CACHE_OUT_OF_RANGE) return s;
+    rolling |= s == CACHE_NEEDS_RECENTRE;
+    s = cache_axis_check(c.ytype, c.Y, c.vy, y, true);
+    if (s == CACHE_OUT_OF_RANGE) return s;
+    rolling |= s == CACHE_NEEDS_RECENTRE;
+    s = cache_axis_check(c.ztype, c.Z, c.vz, 0.0, false);
+    rolling |= s == CACHE_NEEDS_RECENTRE;
+    if (rolling) return CACHE_NEEDS_RECENTRE;
+    if (zindex < 0 || zindex >= c.Z) return CACHE_BAD_ZINDEX;
+    int x0, x1, y0, y1;
+    cache_window(c.X, c.vx, x, CACHE_SPLINE, x0, x1);
+    cache_window(c.Y, c.vy, y, CACHE_SPLINE, y0, y1);
+    if (!c.whole) {
+        const long XY = (long)c.X * c.Y;
+        for (int ix = x0; ix <= x1; ix++)
+            for (int iy = y0; iy <= y1; iy++) {
+                const uint8_t f = c.counted[zindex * XY + (long)ix * c.Y + iy];
+                if (f == 0) return CACHE_MISSING;
+                if (f != 1) return CACHE_OUT_OF_RANGE;
+            }
+    }
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+    bool ok;
+    if (nx == 4 && ny == 4)
+        ok = cache_spline_win<4, 4, DER>(c, x0, y0, zindex, k, x, y, v, dv);
+    else if (nx == 4)
+        ok = cache_spline_win<4, 3, DER>(c, x0, y0, zindex, k, x, y, v, dv);
+    else if (ny == 4)
+        ok = cache_spline_win<3, 4, DER>(c, x0, y0, zindex, k, x, y, v, dv);
+    else
+        ok = cache_spline_win<3, 3, DER>(c, x0, y0, zindex, k, x, y, v, dv);
+    return ok ? CACHE_OK : CACHE_OUT_OF_RANGE;
+}
+
 // Temperature at which the cache's energy density (component 0) equals e, by
 // the bounded Newton iteration the reference's COOL_EOS pressure and cooling
 // routines run (mnewt.h:104-205 mnewtc<1> with cooleos.c:228-262/296-306 as
@@ -261,7 +419,7 @@ HC_HD int cache_solve_T(const CacheView &c, double rho, double e, int zindex, do
     int ret = 0;
     for (int k = 1; k <= ntrial; k++) {
         double val = 0.0, der = 0.0;
-        if (cache_lookup(c, rho, x, 0.0, zindex, CACHE_SPLINE, 1ull, &val, &der, 0) != CACHE_OK) {
+        if (cache_lookup1<true>(c, rho, x, zindex, 0, val, der) != CACHE_OK) {
             T = x;
             berr = besterr;
             return -1;

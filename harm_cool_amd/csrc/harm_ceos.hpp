@@ -90,11 +90,11 @@ __device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, const double *va
         o.p = (P->gam - 1.) * u;
         return o;
     }
-    double v[2];
+    double pv, unused;
     // the converged solve read e at (rho, Tn), so this window is inside the cache
-    cache_lookup(c, rho, Tn, 0.0, 0, CACHE_SPLINE, 2ull, v, nullptr, 1);
+    cache_lookup1<false>(c, rho, Tn, 0, 1, pv, unused);
     o.T = Tn;
-    o.p = v[1] / P->p_unit;
+    o.p = pv / P->p_unit;
     return o;
 }
 
