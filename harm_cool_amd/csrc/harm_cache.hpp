@@ -216,6 +216,78 @@ HC_HD bool cache_akima_m(const double *xs, const double *ys, const double *ts, d
     return true;
 }
 
+// The per-interval numbers cache_akima_m's evaluation uses, for the M-1
+// intervals of a window: co[4 i ..] = (y_i, g_i, p1_i, p2_i), formed by the
+// same operations.  False as cache_akima_m when two abscissae are too close.
+template <int M>
+HC_HD bool cache_akima_coefs(const double *xs, const double *ys, double *co)
+{
+    double S[M + 3];
+#pragma unroll
+    for (int k = 0; k < M - 1; k++) {
+        const double dx = xs[k + 1] - xs[k];
+        if (dx < 1e-12) return false;
+        S[k + 2] = (ys[k + 1] - ys[k]) / dx;
+    }
+    {
+        const double x0 = xs[0], x1 = xs[1], y0 = ys[0];
+        const double xl1 = x0 + x1 - xs[2];
+        const double xl0 = xl1 + x0 - x1;
+        const double yl1 = (x0 - xl1) * (S[3] - 2.0 * S[2]) + y0;
+        const double g1 = (y0 - yl1) / (x0 - xl1);
+        const double yl0 = (xl1 - xl0) * (S[2] - 2.0 * g1) + yl1;
+        S[0] = (yl1 - yl0) / (xl1 - xl0);
+        S[1] = g1;
+    }
+    {
+        const double xa = xs[M - 2], xb = xs[M - 1], yb = ys[M - 1];
+        const double xr2 = xb + xa - xs[M - 3];
+        const double xr3 = xr2 + xb - xa;
+        const double yr2 = (2.0 * S[M] - S[M - 1]) * (xr2 - xb) + yb;
+        S[M + 1] = (yr2 - yb) / (xr2 - xb);
+        const double yr3 = (2.0 * S[M + 1] - S[M]) * (xr3 - xr2) + yr2;
+        S[M + 2] = (yr3 - yr2) / (xr3 - xr2);
+    }
+    double sl[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const double a = S[k], b = S[k + 1], c = S[k + 2], d = S[k + 3];
+        double d0 = d - c, d1 = b - a;
+        if (d0 < 0.0) d0 *= -1.0;
+        if (d1 < 0.0) d1 *= -1.0;
+        sl[k] = (d0 + d1) < 1e-9 ? 0.5 * (b + c) : (d0 * b + d1 * c) / (d0 + d1);
+    }
+#pragma unroll
+    for (int i = 0; i < M - 1; i++) {
+        const double d0 = xs[i + 1] - xs[i];
+        const double t2 = (ys[i + 1] - ys[i]) / d0;
+        co[4 * i + 0] = ys[i];
+        co[4 * i + 1] = sl[i];
+        co[4 * i + 2] = (3.0 * t2 - 2.0 * sl[i] - sl[i + 1]) / d0;
+        co[4 * i + 3] = (sl[i] + sl[i + 1] - 2.0 * t2) / (d0 * d0);
+    }
+    return true;
+}
+
+// cache_akima_m's output points from those numbers: the same interval search
+// and the same cubic.
+template <int M, int NT>
+HC_HD void cache_akima_eval(const double *xs, const double *co, const double *ts, double *out)
+{
+    int i = -1;
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const double t = ts[q];
+#pragma unroll
+        for (int s = 0; s <= M - 2; s++)
+            if (i == s - 1 && t > xs[s]) i = s;
+        if (i < 0) i = 0;
+        const double *ci = co + 4 * i;
+        const double t1 = t - xs[i];
+        out[q] = ci[0] + ci[1] * t1 + ci[2] * t1 * t1 + ci[3] * t1 * t1 * t1;
+    }
+}
+
 struct CacheView {
     int X, Y, Z, D;
     int xtype, ytype, ztype;
@@ -348,13 +420,61 @@ HC_HD bool cache_spline_win(const CacheView &c, int x0, int y0, int zp, int k, d
     return ok;
 }
 
+// Coefficient table of the splines along y of one component of one z plane:
+// for x row ix and bracket left = 1..Y-1 (the y window of a query in that
+// bracket, cache_window) the window's intervals' numbers (cache_akima_coefs),
+// [X][Y][3][4] doubles (left 0 unused; a 3-point window fills 2 intervals).
+constexpr int CACHE_YCO = 12;
+
+HC_HD void cache_ycoef_one(const CacheView &c, int zp, int k, int ix, int left, double *co)
+{
+    const int lo = left - 2 < 0 ? 0 : left - 2;
+    const int hi = left + 1 >= c.Y ? c.Y - 1 : left + 1;
+    const int m = hi - lo + 1;
+    const double *row = c.values + ((size_t)zp * c.X * c.Y + (size_t)ix * c.Y + lo) * c.D + k;
+    double yv[4];
+    for (int q = 0; q < m; q++) yv[q] = row[(size_t)q * c.D];
+    if (m == 4)
+        cache_akima_coefs<4>(c.vy + lo, yv, co);
+    else
+        cache_akima_coefs<3>(c.vy + lo, yv, co);
+}
+
+// cache_spline_win with the y splines read from a coefficient table.
+template <int NX, int NY, bool DER>
+HC_HD void cache_spline_win_co(const CacheView &c, const double *yco, int x0, int y0, int left, double x, double y,
+                               double &v, double &dv, bool &ok)
+{
+    const double *xs = c.vx + x0, *ys = c.vy + y0;
+    double xt[NX], xd[NX];
+#pragma unroll
+    for (int b = 0; b < NX; b++) {
+        const double *co = yco + ((size_t)(x0 + b) * c.Y + left) * CACHE_YCO;
+        if (DER) {
+            const double h = y * 1.0E-6;
+            const double ts[3] = {y - h, y, y + h};
+            double o3[3];
+            cache_akima_eval<NY, 3>(ys, co, ts, o3);
+            xt[b] = o3[1];
+            xd[b] = (o3[2] - o3[0]) / (2.0 * h);
+        } else {
+            cache_akima_eval<NY, 1>(ys, co, &y, &xt[b]);
+        }
+    }
+    ok &= cache_akima_m<NX, 1>(xs, xt, &x, &v);
+    if (DER) ok &= cache_akima_m<NX, 1>(xs, xd, &x, &dv);
+}
+
 // cache_lookup for one component k of a z plane (zindex >= 0) in spline mode:
 // the same status and bits, with the window's size fixed per call.
 HC_HD int cache_lookup(const CacheView &c, double x, double y, double z, int zindex, int mode, uint64_t mask,
                        double *out, double *dery, size_t ostride);
 
+// With yco (the coefficient table of component k, plane zindex, of a cache
+// whose axes were checked when it was built) the y splines are read from it.
 template <bool DER>
-HC_HD int cache_lookup1(const CacheView &c, double x, double y, int zindex, int k, double &v, double &dv)
+HC_HD int cache_lookup1(const CacheView &c, double x, double y, int zindex, int k, double &v, double &dv,
+                        const double *yco = nullptr)
 {
     if (zindex == -1) {
         // interpolation in z (at z = 0): the general lookup
@@ -390,7 +510,19 @@ HC_HD int cache_lookup1(const CacheView &c, double x, double y, int zindex, int 
             }
     }
     const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
-    bool ok;
+    bool ok = true;
+    if (yco) {
+        const int left = cache_bracket(c.Y, c.vy, y);
+        if (nx == 4 && ny == 4)
+            cache_spline_win_co<4, 4, DER>(c, yco, x0, y0, left, x, y, v, dv, ok);
+        else if (nx == 4)
+            cache_spline_win_co<4, 3, DER>(c, yco, x0, y0, left, x, y, v, dv, ok);
+        else if (ny == 4)
+            cache_spline_win_co<3, 4, DER>(c, yco, x0, y0, left, x, y, v, dv, ok);
+        else
+            cache_spline_win_co<3, 3, DER>(c, yco, x0, y0, left, x, y, v, dv, ok);
+        return ok ? CACHE_OK : CACHE_OUT_OF_RANGE;
+    }
     if (nx == 4 && ny == 4)
         ok = cache_spline_win<4, 4, DER>(c, x0, y0, zindex, k, x, y, v, dv);
     else if (nx == 4)
@@ -413,13 +545,13 @@ HC_HD int cache_lookup1(const CacheView &c, double x, double y, int zindex, int 
 // converged, T the iterate of least error) or -1 when a lookup fails (the
 // reference reads values it did not get).
 HC_HD int cache_solve_T(const CacheView &c, double rho, double e, int zindex, double T0, double lo, double hi,
-                        int ntrial, double tolf, double &T, double &berr)
+                        int ntrial, double tolf, double &T, double &berr, const double *yco = nullptr)
 {
     double x = T0, best = T0, besterr = 1.0E100;
     int ret = 0;
     for (int k = 1; k <= ntrial; k++) {
         double val = 0.0, der = 0.0;
-        if (cache_lookup1<true>(c, rho, x, zindex, 0, val, der) != CACHE_OK) {
+        if (cache_lookup1<true>(c, rho, x, zindex, 0, val, der, yco) != CACHE_OK) {
             T = x;
             berr = besterr;
             return -1;

@@ -39,6 +39,7 @@ constexpr double CEOS_TOL = 1.0E-6;  // cooleos.c:395, 406, 409
 struct CeosParams {
     CacheView view;                // axes, sizes, types; view.values is unused
     const double *const *values;   // device array of n value pointers ([1][X][Y][D] each)
+    const double *const *yco;      // n coefficient tables of e and p along T ([2][X][Y][12] each)
     int n;
     double rho_unit, u_unit, p_unit, gam;
     double rho_min, rho_max;       // rhoMin, rhoMax (g/cm^3)
@@ -54,10 +55,7 @@ struct CeosZone {
     int nsolve = 0;  // solves run (repeats excluded), for the counters
 };
 
-__device__ __forceinline__ const double *ceos_values(const CeosParams &P, int i)
-{
-    return P.values[P.n == 1 ? 0 : i];
-}
+__device__ __forceinline__ int ceos_radius(const CeosParams &P, int i) { return P.n == 1 ? 0 : i; }
 
 struct CeosOut {
     double p, T;
@@ -67,7 +65,7 @@ struct CeosOut {
 // pressure_rho0_u_CoolEOS with the zone's last temperature T (cooleos.c:314-444;
 // MAXLOS 1: one solve, no random restart): the pressure and the new last T.
 // One out-of-line copy serves every call site of a kernel.
-__device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, const double *vals, double T, double rho0, double u)
+__device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, int r, double T, double rho0, double u)
 {
     CeosOut o{0., T, 0};
     const double rho = rho0 * P->rho_unit;
@@ -83,16 +81,17 @@ __device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, const double *va
     if (e < 0.0) return o;
     o.solved = 1;
     CacheView c = P->view;
-    c.values = vals;
+    c.values = P->values[r];
+    const double *yco = P->yco[r];
     double Tn, berr;
-    cache_solve_T(c, rho, e, 0, o.T, P->T_lo, P->T_hi, CEOS_NTRIAL, CEOS_TOL, Tn, berr);
+    cache_solve_T(c, rho, e, 0, o.T, P->T_lo, P->T_hi, CEOS_NTRIAL, CEOS_TOL, Tn, berr, yco);
     if (berr > CEOS_TOL) {
         o.p = (P->gam - 1.) * u;
         return o;
     }
     double pv, unused;
     // the converged solve read e at (rho, Tn), so this window is inside the cache
-    cache_lookup1<false>(c, rho, Tn, 0, 1, pv, unused);
+    cache_lookup1<false>(c, rho, Tn, 0, 1, pv, unused, yco + (size_t)c.X * c.Y * CACHE_YCO);
     o.T = Tn;
     o.p = pv / P->p_unit;
     return o;
@@ -103,7 +102,7 @@ __device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, const double *va
 struct EosCache {
     static constexpr bool TAB = true;
     const CeosParams *P;
-    const double *vals;  // the zone's radial index's cache
+    int r;  // the zone's radial index's cache
     CeosZone *z;
 
     __device__ double pres(double rho0, double u) const
@@ -111,7 +110,7 @@ struct EosCache {
         const unsigned long long kr = (unsigned long long)__double_as_longlong(rho0);
         const unsigned long long ku = (unsigned long long)__double_as_longlong(u);
         if (z->has && kr == z->kr && ku == z->ku) return z->p;
-        const CeosOut o = ceos_solve(P, vals, z->T, rho0, u);
+        const CeosOut o = ceos_solve(P, r, z->T, rho0, u);
         z->T = o.T;
         z->nsolve += o.solved;
         z->has = true;

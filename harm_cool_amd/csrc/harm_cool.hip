@@ -1229,6 +1229,27 @@ k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb, 
 // calls (harm_ceos.hpp).  lastT is zone-indexed like the primitives.
 // ---------------------------------------------------------------------------
 
+#ifndef CEOS_WPE
+#define CEOS_WPE 2
+#endif
+
+// The coefficient tables of the splines along T of e and p of cache blockIdx.y
+// (cache_ycoef_one): one (component, x row, bracket) per thread.
+__global__ void __launch_bounds__(TPB) k_ceos_ycoef(CacheView v, const double *const *values,
+                                                   double *const *yco)
+{
+    const size_t per = (size_t)v.X * v.Y;
+    const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (idx >= 2 * per) return;
+    const int comp = (int)(idx / per);
+    const size_t r = idx - (size_t)comp * per;
+    const int ix = (int)(r / (size_t)v.Y), left = (int)(r % (size_t)v.Y);
+    if (left < 1) return;
+    CacheView c = v;
+    c.values = values[blockIdx.y];
+    cache_ycoef_one(c, 0, comp, ix, left, yco[blockIdx.y] + idx * CACHE_YCO);
+}
+
 // One face of fluxcalc (step_ch.c:331-504) with InitCache_CoolEOS before it
 // (step_ch.c:439-441), slopes from four zones along DIR (MC / van Leer /
 // minmod), fix_flux (coord.c:264-324) and the CFL limit, as k_flux.  chain:
@@ -1277,7 +1298,7 @@ __device__ inline void ceos_face(const DGrid &g, const CeosParams &ce, double *_
     CeosZone zs;
     zs.T = lastT[z];
     if (chain && zs.T == 0.0 && j > g.js) zs.T = lastT[z - g.N3];
-    const EosCache eos{&ce, ceos_values(ce, i), &zs};
+    const EosCache eos{&ce, ceos_radius(ce, i), &zs};
     double Fz[NPR], ctop;
     face_flux(p_l, p_r, geom, DIR, eos, g.hllf, g.laxf, Fz, ctop, fatal);
     lastT[z] = zs.T;
@@ -1304,7 +1325,7 @@ __device__ inline void ceos_face(const DGrid &g, const CeosParams &ce, double *_
 // One face per thread, or (chain) one (h, i) column of faces per thread in j
 // order.  The face ranges are k_flux's.
 template <int DIR>
-__global__ void __launch_bounds__(TPB) k_flux_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_WPE, 8))) k_flux_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
                                                    const double *__restrict__ pb, double *__restrict__ F,
                                                    double *__restrict__ dtpart, int fullstep, int chain,
                                                    unsigned long long *__restrict__ cnt,
@@ -1334,7 +1355,7 @@ __global__ void __launch_bounds__(TPB) k_flux_ceos(DGrid g, CeosParams ce, doubl
 // advance()'s zone loop with COOL_EOS (step_ch.c:237-306): InitCache_CoolEOS,
 // the source at pb, primtoU at pi, the update with the flux divergence S,
 // Utoprim_5d seeded with pi (no Font fix), then fixup1zone (fixup.c:51-60).
-__global__ void __launch_bounds__(TPB) k_update_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_WPE, 8))) k_update_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
                                                      const double *__restrict__ pi, const double *__restrict__ pb,
                                                      double *__restrict__ pf, int8_t *__restrict__ pflag,
                                                      const double *__restrict__ S,
@@ -1359,7 +1380,7 @@ __global__ void __launch_bounds__(TPB) k_update_ceos(DGrid g, CeosParams ce, dou
         load_geom(g.geom[CENT], g.ncol, col, geom);
         CeosZone zs;
         zs.T = lastT[z];
-        const EosCache eos{&ce, ceos_values(ce, i), &zs};
+        const EosCache eos{&ce, ceos_radius(ce, i), &zs};
         double pbz[NPR], piz[NPR], dU[NPR], U[NPR];
 #pragma unroll
         for (int k = 0; k < NPR; k++) pbz[k] = pb[k * nz + z];
@@ -3013,6 +3034,8 @@ struct harm_ctx {
     CeosParams ceos;
     double *lastT;
     const double **ceos_vals;
+    std::vector<double *> ceos_ycoef;  // per cache: the splines along T of e and p
+    double **ceos_yco;                 // device array of ceos_ycoef
     unsigned long long *ceos_solves;
     bool ceos_chain[3];
     // learned Newton seed and training-pair capture
@@ -3585,8 +3608,10 @@ void ctx_free(harm_ctx *c)
                     (void *)c->d_floors, (void *)c->d_katm, (void *)c->tstate, (void *)c->ndt_slot, (void *)c->dtpart, (void *)c->cpart, (void *)c->cnt,
                     (void *)c->part, (void *)c->fix_bits, (void *)c->fix_pcount, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk, (void *)c->seeds.count, (void *)c->seeds.zone,
-                    (void *)c->seeds.U, (void *)c->lastT, (void *)c->ceos_vals, (void *)c->ceos_solves})
+                    (void *)c->seeds.U, (void *)c->lastT, (void *)c->ceos_vals, (void *)c->ceos_solves,
+                    (void *)c->ceos_yco})
         if (q) (void)hipFree(q);
+    for (double *q : c->ceos_ycoef) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
         if (c->d_geom[l]) (void)hipFree(c->d_geom[l]);
     for (auto &e : c->ev_pool) {
@@ -4180,6 +4205,21 @@ int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, cons
     if ((rc = dalloc(&c->ceos_vals, (size_t)n))) return rc;
     HIPCHK(hipMemcpy((void *)c->ceos_vals, vals.data(), (size_t)n * sizeof(double *), hipMemcpyHostToDevice));
     P.values = c->ceos_vals;
+    // the splines along T of e and p, formed once per cache
+    const size_t nco = 2 * (size_t)v0.X * v0.Y * CACHE_YCO;
+    for (double *q : c->ceos_ycoef) (void)hipFree(q);
+    c->ceos_ycoef.assign((size_t)n, nullptr);
+    for (int k = 0; k < n; k++)
+        if ((rc = dalloc(&c->ceos_ycoef[(size_t)k], nco))) return rc;
+    if (c->ceos_yco) (void)hipFree((void *)c->ceos_yco);
+    c->ceos_yco = nullptr;
+    if ((rc = dalloc(&c->ceos_yco, (size_t)n))) return rc;
+    HIPCHK(hipMemcpy((void *)c->ceos_yco, c->ceos_ycoef.data(), (size_t)n * sizeof(double *), hipMemcpyHostToDevice));
+    P.yco = c->ceos_yco;
+    hipLaunchKernelGGL(k_ceos_ycoef, dim3(nblocks(2 * (size_t)v0.X * v0.Y), (unsigned)n), dim3(TPB), 0, c->stream, v0,
+                       P.values, c->ceos_yco);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (!c->lastT && (rc = dalloc(&c->lastT, c->nz))) return rc;
     if (!c->ceos_solves && (rc = dalloc(&c->ceos_solves, 1))) return rc;
     HIPCHK(hipMemset(c->lastT, 0, c->nz * sizeof(double)));  // calloc'ed by SetupCache_CoolEOS, cooleos.c:834
