@@ -62,6 +62,94 @@ struct CeosOut {
     int solved;
 };
 
+// The y bracket of t (cache_bracket's) found by walking from b: down while
+// the bracket below also holds t, then up to the first one that does.  The
+// axis increases, so this is the bisection's index; a start near t takes one
+// or two reads.
+__device__ __forceinline__ int ceos_walk(int n, const double *v, double t, int b)
+{
+    while (b > 1 && t < v[b - 1]) b--;
+    while (b < n - 1 && !(t < v[b])) b++;
+    return b;
+}
+
+template <bool DER>
+__device__ __forceinline__ bool ceos_win(const CacheView &c, const double *yco, int x0, int nx, int yb, double rho,
+                                         double t, double &v, double &dv)
+{
+    const int y0 = yb - 2 < 0 ? 0 : yb - 2;
+    const int y1 = yb + 1 >= c.Y ? c.Y - 1 : yb + 1;
+    const int ny = y1 - y0 + 1;
+    bool ok = true;
+    if (nx == 4 && ny == 4)
+        cache_spline_win_co<4, 4, DER>(c, yco, x0, y0, yb, rho, t, v, dv, ok);
+    else if (nx == 4)
+        cache_spline_win_co<4, 3, DER>(c, yco, x0, y0, yb, rho, t, v, dv, ok);
+    else if (ny == 4)
+        cache_spline_win_co<3, 4, DER>(c, yco, x0, y0, yb, rho, t, v, dv, ok);
+    else
+        cache_spline_win_co<3, 3, DER>(c, yco, x0, y0, yb, rho, t, v, dv, ok);
+    return ok;
+}
+
+// cache_solve_T on component 0 of plane 0 of a whole, non-rolling cache with
+// its coefficient tables (yco: e, then p), and at convergence p = P(rho, T):
+// every trial's lookup as cache_lookup1 makes it, with the density's window
+// formed once and the temperature's bracket walked from the trial before.
+__device__ inline int ceos_solve_Tp(const CacheView &c, const double *yco, double rho, double e, double T0, double lo,
+                                    double hi, double &T, double &berr, double &pv)
+{
+    double x = T0, best = T0, besterr = 1.0E100;
+    int ret = 0;
+    if (rho < c.vx[0] || rho > c.vx[c.X - 1]) {
+        T = x;
+        berr = besterr;
+        return -1;
+    }
+    int x0, x1;
+    cache_window(c.X, c.vx, rho, CACHE_SPLINE, x0, x1);
+    const int nx = x1 - x0 + 1;
+    int yb = cache_bracket(c.Y, c.vy, x);
+    for (int k = 1; k <= CEOS_NTRIAL; k++) {
+        double val = 0.0, der = 0.0;
+        bool ok = !(x < c.vy[0] || x > c.vy[c.Y - 1]);
+        if (ok) {
+            yb = ceos_walk(c.Y, c.vy, x, yb);
+            ok = ceos_win<true>(c, yco, x0, nx, yb, rho, x, val, der);
+        }
+        if (!ok) {
+            T = x;
+            berr = besterr;
+            return -1;
+        }
+        const double F = (e - val) / e;
+        const double D = -der / e;
+        double bet = -F;
+        const double errf = 0.0 + fabs(bet);
+        if (errf <= CEOS_TOL) {
+            T = x;
+            berr = errf;
+            double unused;
+            ceos_win<false>(c, yco + (size_t)c.X * c.Y * CACHE_YCO, x0, nx, yb, rho, x, pv, unused);
+            return 1;
+        }
+        if (errf < besterr) {
+            if (k > 1) ret = 2;
+            besterr = errf;
+            best = x;
+        }
+        if (fabs(D) > 0.0) bet = bet / D;
+        const double nv = x + bet;
+        if (nv >= lo && nv <= hi && nv != x)
+            x = nv;
+        else
+            break;
+    }
+    T = best;
+    berr = besterr;
+    return ret;
+}
+
 // pressure_rho0_u_CoolEOS with the zone's last temperature T (cooleos.c:314-444;
 // MAXLOS 1: one solve, no random restart): the pressure and the new last T.
 // One out-of-line copy serves every call site of a kernel.
@@ -83,15 +171,13 @@ __device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, int r, double T,
     CacheView c = P->view;
     c.values = P->values[r];
     const double *yco = P->yco[r];
-    double Tn, berr;
-    cache_solve_T(c, rho, e, 0, o.T, P->T_lo, P->T_hi, CEOS_NTRIAL, CEOS_TOL, Tn, berr, yco);
+    double Tn, berr, pv = 0.;
+    ceos_solve_Tp(c, yco, rho, e, o.T, P->T_lo, P->T_hi, Tn, berr, pv);
     if (berr > CEOS_TOL) {
         o.p = (P->gam - 1.) * u;
         return o;
     }
-    double pv, unused;
-    // the converged solve read e at (rho, Tn), so this window is inside the cache
-    cache_lookup1<false>(c, rho, Tn, 0, 1, pv, unused, yco + (size_t)c.X * c.Y * CACHE_YCO);
+    // the P lookup shares the converged trial's windows (cooleos.c:432)
     o.T = Tn;
     o.p = pv / P->p_unit;
     return o;
