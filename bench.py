@@ -369,6 +369,38 @@ def cache_eos_torus(hc, tab: dict, steps: int, warmup: int) -> dict:
             "frac_zones_with_T_in_cache": float(solved.mean())}
 
 
+def cache_eos_cpu(tab: dict, threads: int, n3: int = 8, nsteps: int = 2) -> dict:
+    """The oracle's step with the same cache EOS on a slab of the cache-EOS
+    torus (192x192 x n3 phi planes; the torus is axisymmetric up to the seeded
+    perturbation, so the work per plane is the full grid's), one warm-up step
+    (the first, from unset temperatures) and nsteps timed."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from harm_cool_amd.params import torus_params
+    from oracle_lib import Oracle, lib, orc_ceos
+
+    ax_rho, ax_T = tab["_axes"]
+    values = np.ascontiguousarray(tab["_vals"].T.reshape(1, ax_rho.size, ax_T.size, 6))
+    rho_unit = 1e11
+    u_unit = rho_unit * 2.99792458e10 * 2.99792458e10
+    lib().orc_set_threads(threads)
+    P = torus_params(192, 192, n3)
+    o = Oracle(P)
+    o.set_eos_cache(orc_ceos(ax_rho, ax_T, values, P.gam, rho_unit, u_unit, u_unit))
+    o.init_torus()
+    o.step()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        o.step()
+    el = time.perf_counter() - t0
+    del o
+    value = 192 * 192 * n3 * nsteps / el
+    return {"value": value, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"oracle on a 192x192x{n3} slab of the same torus and cache, {nsteps} steps after 1 warm-up "
+                      f"step, {el:.1f} s"}
+
+
 def fast_mode(hc, n: tuple, steps: int, warmup: int, dev_steps: int = 10) -> dict:
     """The same step built with FMA contraction (libharmcool_fast.so), reported
     beside the parity build: its step time, per-kernel times, and its deviation
@@ -557,6 +589,10 @@ def main() -> None:
     ap.add_argument("--no-fast", action="store_true", help="skip the FMA-contracted (fast mode) section")
     ap.add_argument("--lim", choices=("mc", "ppm"), default="mc",
                     help="reconstruction of the headline torus (the headline workload is MC)")
+    ap.add_argument("--cache-eos-only", action="store_true",
+                    help="only the cache-EOS torus section (the generator's table, then the torus), one JSON line")
+    ap.add_argument("--table-points", type=str, default="200x200",
+                    help="cooling-table grid of the tabulated and cache-EOS sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the whole domain lives on one MI355X; --gpus {args.gpus} is not "
@@ -568,6 +604,13 @@ def main() -> None:
 
     import numpy as np
     import torch
+
+    tnr, tnt = (int(x) for x in args.table_points.split("x"))
+    if args.cache_eos_only:
+        tab = eos_table(hc, tnr, tnt)
+        print(json.dumps({"eos_table_s": tab["seconds"],
+                          "cache_eos_torus": cache_eos_torus(hc, tab, args.steps, args.warmup)}), flush=True)
+        return
 
     n1, n2, n3 = (int(x) for x in args.n.split("x"))
     nz = n1 * n2 * n3
@@ -659,7 +702,7 @@ def main() -> None:
 
     eos_tab = None
     if not (args.no_eos_table and args.no_tabulated):
-        eos_tab = eos_table(hc)
+        eos_tab = eos_table(hc, tnr, tnt)
     tabulated = cache_eos = None
     if not args.no_tabulated:
         tabulated = tabulated_torus(hc, eos_tab["_table"], args.steps, args.warmup)
@@ -686,6 +729,8 @@ def main() -> None:
     if eos_tab is not None:
         if not args.no_cpu:
             eos_tab["cpu_reference"] = eos_table_cpu(hc, eos_tab)
+            if cache_eos is not None:
+                cache_eos["cpu_baseline"] = cache_eos_cpu(eos_tab, min(16, os.cpu_count() or 1))
         for k in ("_rho", "_T", "_vals", "_axes", "_table"):
             eos_tab.pop(k)
 
