@@ -1232,6 +1232,9 @@ k_update(DGrid g, const double *__restrict__ pi, const double *__restrict__ pb, 
 #ifndef CEOS_WPE
 #define CEOS_WPE 2
 #endif
+#ifndef CEOS_EOS_WPE
+#define CEOS_EOS_WPE 2
+#endif
 
 // The coefficient tables of the splines along T of e and p of cache blockIdx.y
 // (cache_ycoef_one): one (component, x row, bracket) per thread.
@@ -1250,25 +1253,10 @@ __global__ void __launch_bounds__(TPB) k_ceos_ycoef(CacheView v, const double *c
     cache_ycoef_one(c, 0, comp, ix, left, yco[blockIdx.y] + idx * CACHE_YCO);
 }
 
-// One face of fluxcalc (step_ch.c:331-504) with InitCache_CoolEOS before it
-// (step_ch.c:439-441), slopes from four zones along DIR (MC / van Leer /
-// minmod), fix_flux (coord.c:264-324) and the CFL limit, as k_flux.  chain:
-// the zones' last temperatures may be unset, and the loop over j runs in
-// order, so a zone's j-1 neighbour has its value when it is read
-// (cooleos.c:784-799).
+// The stencil of a face along DIR: zones m-2, m-1 (left of the face), m, m+1.
 template <int DIR>
-__device__ inline void ceos_face(const DGrid &g, const CeosParams &ce, double *__restrict__ lastT,
-                                 const double *__restrict__ pb, double *__restrict__ F, int i, int j, int h,
-                                 int fullstep, bool chain, double &dtij, int &fatal, int &nsolve)
+__device__ __forceinline__ void ceos_stencil(const DGrid &g, size_t z, int h, size_t &zm2, size_t &zm1, size_t &zp1)
 {
-    const size_t nz = g.nz;
-    const size_t z = zidx(g, i, j, h);
-    if (DIR == 2 && (j == g.js || j == g.je + 1)) {
-#pragma unroll
-        for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
-        return;
-    }
-    size_t zm2, zm1, zp1;
     if (DIR == 1) {
         const size_t s = (size_t)g.SN2 * g.N3;
         zm2 = z - 2 * s;
@@ -1285,71 +1273,145 @@ __device__ inline void ceos_face(const DGrid &g, const CeosParams &ce, double *_
         zm1 = base + (h + g.N3 - 1) % g.N3;
         zp1 = base + (h + 1) % g.N3;
     }
-    double p_l[NPR], p_r[NPR];
-#pragma unroll
-    for (int k = 0; k < NPR; k++) {
-        const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
-        p_l[k] = b + 0.5 * slope_lim(g.lim, a, b, c);
-        p_r[k] = c - 0.5 * slope_lim(g.lim, b, c, d);
-    }
-    Geom geom;
-    const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
-    load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
-    CeosZone zs;
-    zs.T = lastT[z];
-    if (chain && zs.T == 0.0 && j > g.js) zs.T = lastT[z - g.N3];
-    const EosCache eos{&ce, ceos_radius(ce, i), &zs};
-    double Fz[NPR], ctop;
-    face_flux(p_l, p_r, geom, DIR, eos, g.hllf, g.laxf, Fz, ctop, fatal);
-    lastT[z] = zs.T;
-    nsolve += zs.nsolve;
-    if (DIR == 1) {
-        if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
-        if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
-    }
-    const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
-#pragma unroll
-    for (int k = 0; k < NPR; k++)
-        if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
-    if (DIR != 2) {
-        if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
-        if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
-    }
-    if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
-        const double dxd = (DIR == 1) ? g.dx1 : (DIR == 2) ? g.dx2 : g.dx3;
-        const double d = g.cour * dxd / ctop;
-        if (d < 1.e9 && d < dtij) dtij = d;
-    }
 }
 
-// One face per thread, or (chain) one (h, i) column of faces per thread in j
-// order.  The face ranges are k_flux's.
-template <int DIR>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_WPE, 8))) k_flux_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
-                                                   const double *__restrict__ pb, double *__restrict__ F,
-                                                   double *__restrict__ dtpart, int fullstep, int chain,
-                                                   unsigned long long *__restrict__ cnt,
-                                                   unsigned long long *__restrict__ solves)
+// The face range of a sweep along DIR (k_flux's) and, with chain, one (h, i)
+// column per thread walked in j order; calls f(i, j, h) per face.
+template <int DIR, class Fn>
+__device__ __forceinline__ void ceos_faces(const DGrid &g, int chain, Fn &&f)
 {
-    if (halted(g)) return;
     const int i0 = (DIR == 1) ? g.is : g.is - 1;
     const int j0 = (DIR == 2) ? g.js : g.js - 1;
     const int ni = g.ie + 1 - i0 + 1, nj = g.je + 1 - j0 + 1;
     const size_t idx = (size_t)blockIdx.x * TPB + threadIdx.x;
     const size_t n = chain ? (size_t)ni * g.N3 : (size_t)ni * nj * g.N3;
-    double dtij = 1.e300;
-    int fatal = 0, nsolve = 0;
-    if (idx < n) {
-        const int h = (int)(idx % (size_t)g.N3);
-        const size_t rest = idx / (size_t)g.N3;
-        const int i = chain ? i0 + (int)rest : i0 + (int)(rest / (size_t)nj);
-        const int ja = chain ? j0 : j0 + (int)(rest % (size_t)nj), jb = chain ? g.je + 1 : ja;
-        for (int j = ja; j <= jb; j++)
-            ceos_face<DIR>(g, ce, lastT, pb, F, i, j, h, fullstep, chain != 0, dtij, fatal, nsolve);
+    if (idx >= n) return;
+    const int h = (int)(idx % (size_t)g.N3);
+    const size_t rest = idx / (size_t)g.N3;
+    const int i = chain ? i0 + (int)rest : i0 + (int)(rest / (size_t)nj);
+    const int ja = chain ? j0 : j0 + (int)(rest % (size_t)nj), jb = chain ? g.je + 1 : ja;
+    for (int j = ja; j <= jb; j++) f(i, j, h);
+}
+
+// The pressures of fluxcalc's face (h, i, j) with InitCache_CoolEOS before it
+// (step_ch.c:439-441): its left state's pressure and vchar sound speed (with
+// dp/drho, dp/du by their pressure calls), then its right state's, in the
+// order primtoflux and vchar take them (step_ch.c:450-456), on the zone's last
+// T; out: p_l, cs2_l, p_r, cs2_r at the face's zone.  Only density and energy
+// of the two states are reconstructed (the pressure sees nothing else).  chain:
+// the zones' last temperatures may be unset, and the loop over j runs in
+// order, so a zone's j-1 neighbour has its value when it is read
+// (cooleos.c:784-799).
+template <int DIR>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_EOS_WPE, 8)))
+k_face_eos(DGrid g, CeosParams ce, double *__restrict__ lastT, const double *__restrict__ pb,
+           double *__restrict__ fe, int chain, unsigned long long *__restrict__ solves)
+{
+    if (halted(g)) return;
+    int nsolve = 0;
+    ceos_faces<DIR>(g, chain, [&](int i, int j, int h) {
+        if (DIR == 2 && (j == g.js || j == g.je + 1)) return;  // pole faces: no InitCache, no pressure
+        const size_t nz = g.nz, z = zidx(g, i, j, h);
+        size_t zm2, zm1, zp1;
+        ceos_stencil<DIR>(g, z, h, zm2, zm1, zp1);
+        double rl, ul, rr, ur;
+        {
+            const double a = pb[RHO * nz + zm2], b = pb[RHO * nz + zm1], c = pb[RHO * nz + z],
+                         d = pb[RHO * nz + zp1];
+            rl = b + 0.5 * slope_lim(g.lim, a, b, c);
+            rr = c - 0.5 * slope_lim(g.lim, b, c, d);
+        }
+        {
+            const double a = pb[UU * nz + zm2], b = pb[UU * nz + zm1], c = pb[UU * nz + z],
+                         d = pb[UU * nz + zp1];
+            ul = b + 0.5 * slope_lim(g.lim, a, b, c);
+            ur = c - 0.5 * slope_lim(g.lim, b, c, d);
+        }
+        CeosZone zs;
+        zs.T = lastT[z];
+        if (chain && zs.T == 0.0 && j > g.js) zs.T = lastT[z - g.N3];
+        const EosCache eos{&ce, ceos_radius(ce, i), &zs};
+        const double pl = eos.pres(rl, ul);
+        const double cl = eos.cs2(rl, ul, pl);
+        const double pr = eos.pres(rr, ur);
+        const double cr = eos.cs2(rr, ur, pr);
+        lastT[z] = zs.T;
+        nsolve += zs.nsolve;
+        fe[0 * nz + z] = pl;
+        fe[1 * nz + z] = cl;
+        fe[2 * nz + z] = pr;
+        fe[3 * nz + z] = cr;
+    });
+    add_counter(solves, 0, nsolve);
+}
+
+// The EOS policy of a face whose two states' pressures and sound speeds
+// k_face_eos formed: a state is told apart by its density and energy bits.
+struct EosGiven {
+    static constexpr bool TAB = true;
+    double rl, ul, pl, cl, pr, cr;
+    __device__ bool left(double rho, double u) const
+    {
+        return __double_as_longlong(rho) == __double_as_longlong(rl) && __double_as_longlong(u) == __double_as_longlong(ul);
     }
+    __device__ double pres(double rho, double u) const { return left(rho, u) ? pl : pr; }
+    __device__ double cs2(double rho, double u, double) const { return left(rho, u) ? cl : cr; }
+};
+
+// fluxcalc (step_ch.c:331-504) with the face pressures k_face_eos formed:
+// slopes from four zones along DIR (MC / van Leer / minmod), fix_flux
+// (coord.c:264-324) and the CFL limit, as k_flux.  One face per thread.
+template <int DIR>
+__global__ void __launch_bounds__(TPB) k_flux_ceos(DGrid g, const double *__restrict__ fe,
+                                                   const double *__restrict__ pb, double *__restrict__ F,
+                                                   double *__restrict__ dtpart, int fullstep,
+                                                   unsigned long long *__restrict__ cnt)
+{
+    if (halted(g)) return;
+    double dtij = 1.e300;
+    int fatal = 0;
+    ceos_faces<DIR>(g, 0, [&](int i, int j, int h) {
+        const size_t nz = g.nz, z = zidx(g, i, j, h);
+        if (DIR == 2 && (j == g.js || j == g.je + 1)) {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) F[k * nz + z] = 0.;
+            return;
+        }
+        size_t zm2, zm1, zp1;
+        ceos_stencil<DIR>(g, z, h, zm2, zm1, zp1);
+        double p_l[NPR], p_r[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            const double a = pb[k * nz + zm2], b = pb[k * nz + zm1], c = pb[k * nz + z], d = pb[k * nz + zp1];
+            p_l[k] = b + 0.5 * slope_lim(g.lim, a, b, c);
+            p_r[k] = c - 0.5 * slope_lim(g.lim, b, c, d);
+        }
+        Geom geom;
+        const int loc = (DIR == 1) ? FACE1 : (DIR == 2) ? FACE2 : FACE3;
+        load_geom(g.geom[loc], g.ncol, (size_t)i * g.SN2 + j, geom);
+        const EosGiven eos{p_l[RHO], p_l[UU], fe[0 * nz + z], fe[1 * nz + z], fe[2 * nz + z], fe[3 * nz + z]};
+        double Fz[NPR], ctop;
+        face_flux(p_l, p_r, geom, DIR, eos, g.hllf, g.laxf, Fz, ctop, fatal);
+        if (DIR == 1) {
+            if (i == g.is && Fz[RHO] > 0) Fz[RHO] = 0;
+            if (i == g.ie + 1 && Fz[RHO] < 0) Fz[RHO] = 0;
+        }
+        const bool skip_b2 = (DIR != 2) && (j == g.js - 1 || j == g.je + 1);
+#pragma unroll
+        for (int k = 0; k < NPR; k++)
+            if (!(k == B2 && skip_b2)) F[k * nz + z] = Fz[k];
+        if (DIR != 2) {
+            if (j == g.js) F[B2 * nz + zidx(g, i, g.js - 1, h)] = -Fz[B2];
+            if (j == g.je) F[B2 * nz + zidx(g, i, g.je + 1, h)] = -Fz[B2];
+        }
+        if (fullstep && i >= g.is && i <= g.ie && j >= g.js && j <= g.je) {
+            const double dxd = (DIR == 1) ? g.dx1 : (DIR == 2) ? g.dx2 : g.dx3;
+            const double d = g.cour * dxd / ctop;
+            if (d < 1.e9 && d < dtij) dtij = d;
+        }
+    });
     if (fullstep) block_min(dtij, dtpart);
     add_counter(cnt, HARM_FATAL, fatal);
-    add_counter(solves, 0, nsolve);
 }
 
 // advance()'s zone loop with COOL_EOS (step_ch.c:237-306): InitCache_CoolEOS,
@@ -3036,6 +3098,7 @@ struct harm_ctx {
     const double **ceos_vals;
     std::vector<double *> ceos_ycoef;  // per cache: the splines along T of e and p
     double **ceos_yco;                 // device array of ceos_ycoef
+    double *ceos_fe;                   // [4][nz]: each face's p_l, cs2_l, p_r, cs2_r (k_face_eos)
     unsigned long long *ceos_solves;
     bool ceos_chain[3];
     // learned Newton seed and training-pair capture
@@ -3210,11 +3273,14 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
         if (g.lim < 0 || g.lim > 2)
             return set_err("the cache EOS step reconstructs with MC, van Leer or minmod (lim 0-2)", 4);
         const int chain = c->ceos_chain[dir - 1] ? 1 : 0;
-        const size_t n = chain ? (size_t)(g.ie + 2 - i0) * g.N3 : total;
-        grid = dim3(nblocks(n));
+        const size_t nc = (size_t)(g.ie + 2 - i0) * g.N3;
+        const dim3 ge(nblocks(chain ? nc : total));
+        grid = dim3(nblocks(total));
 #define CEOS_FLUX(D_, K_)                                                                                          \
-    LAUNCH(K_, hipLaunchKernelGGL(k_flux_ceos<D_>, grid, dim3(TPB), 0, c->stream, g, c->ceos, c->lastT, pb, F,      \
-                                  c->dtpart, fullstep, chain, c->cnt, c->ceos_solves))
+    LAUNCH(K_, hipLaunchKernelGGL(k_face_eos<D_>, ge, dim3(TPB), 0, c->stream, g, c->ceos, c->lastT, pb, c->ceos_fe, \
+                                  chain, c->ceos_solves);                                                          \
+           hipLaunchKernelGGL(k_flux_ceos<D_>, grid, dim3(TPB), 0, c->stream, g, c->ceos_fe, pb, F, c->dtpart,     \
+                              fullstep, c->cnt))
         if (dir == 1)
             CEOS_FLUX(1, HARM_K_FLUX1);
         else if (dir == 2)
@@ -3609,7 +3675,7 @@ void ctx_free(harm_ctx *c)
                     (void *)c->part, (void *)c->fix_bits, (void *)c->fix_pcount, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk, (void *)c->seeds.count, (void *)c->seeds.zone,
                     (void *)c->seeds.U, (void *)c->lastT, (void *)c->ceos_vals, (void *)c->ceos_solves,
-                    (void *)c->ceos_yco})
+                    (void *)c->ceos_yco, (void *)c->ceos_fe})
         if (q) (void)hipFree(q);
     for (double *q : c->ceos_ycoef) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
@@ -4221,6 +4287,7 @@ int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, cons
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!c->lastT && (rc = dalloc(&c->lastT, c->nz))) return rc;
+    if (!c->ceos_fe && (rc = dalloc(&c->ceos_fe, 4 * c->nz))) return rc;
     if (!c->ceos_solves && (rc = dalloc(&c->ceos_solves, 1))) return rc;
     HIPCHK(hipMemset(c->lastT, 0, c->nz * sizeof(double)));  // calloc'ed by SetupCache_CoolEOS, cooleos.c:834
     HIPCHK(hipMemset(c->ceos_solves, 0, sizeof(unsigned long long)));
