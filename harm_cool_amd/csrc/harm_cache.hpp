@@ -447,17 +447,49 @@ HC_HD void cache_spline_win_co(const CacheView &c, const double *yco, int x0, in
 {
     const double *xs = c.vx + x0, *ys = c.vy + y0;
     double xt[NX], xd[NX];
+    if (DER) {
+        // cache_akima_eval's intervals of the three points, shared by every row
+        // (the window's abscissae are the same); they almost always coincide, and
+        // then each row reads one interval's numbers
+        const double h = y * 1.0E-6;
+        const double ts[3] = {y - h, y, y + h};
+        int iq[3], i = -1;
 #pragma unroll
-    for (int b = 0; b < NX; b++) {
-        const double *co = yco + ((size_t)(x0 + b) * c.Y + left) * CACHE_YCO;
-        if (DER) {
-            const double h = y * 1.0E-6;
-            const double ts[3] = {y - h, y, y + h};
-            double o3[3];
-            cache_akima_eval<NY, 3>(ys, co, ts, o3);
-            xt[b] = o3[1];
-            xd[b] = (o3[2] - o3[0]) / (2.0 * h);
+        for (int q = 0; q < 3; q++) {
+#pragma unroll
+            for (int s = 0; s <= NY - 2; s++)
+                if (i == s - 1 && ts[q] > ys[s]) i = s;
+            if (i < 0) i = 0;
+            iq[q] = i;
+        }
+        if (iq[0] == iq[2]) {
+            const int i0 = iq[0];
+            const double t1[3] = {ts[0] - ys[i0], ts[1] - ys[i0], ts[2] - ys[i0]};
+#pragma unroll
+            for (int b = 0; b < NX; b++) {
+                const double *ci = yco + ((size_t)(x0 + b) * c.Y + left) * CACHE_YCO + 4 * i0;
+                const double a0 = ci[0], a1 = ci[1], a2 = ci[2], a3 = ci[3];
+                double o3[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    o3[q] = a0 + a1 * t1[q] + a2 * t1[q] * t1[q] + a3 * t1[q] * t1[q] * t1[q];
+                xt[b] = o3[1];
+                xd[b] = (o3[2] - o3[0]) / (2.0 * h);
+            }
         } else {
+#pragma unroll
+            for (int b = 0; b < NX; b++) {
+                const double *co = yco + ((size_t)(x0 + b) * c.Y + left) * CACHE_YCO;
+                double o3[3];
+                cache_akima_eval<NY, 3>(ys, co, ts, o3);
+                xt[b] = o3[1];
+                xd[b] = (o3[2] - o3[0]) / (2.0 * h);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NX; b++) {
+            const double *co = yco + ((size_t)(x0 + b) * c.Y + left) * CACHE_YCO;
             cache_akima_eval<NY, 1>(ys, co, &y, &xt[b]);
         }
     }
