@@ -185,7 +185,12 @@ __device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, int r, double T,
 
 // The tabulated EOS policy of the step's physics templates (face_flux, vchar,
 // source, primtoflux) and, through Eos5Cache, of the 5D inversion.
-struct EosCache {
+// V: the finite-difference steps rho0 hh / RHO_UNIT and u hh / U_UNIT are below
+// half an ulp of every rho0 and u (both units at least hh 2^55), so rho0 +- h
+// is rho0 and u +- h is u bit for bit: each derivative's two pressure calls
+// repeat the call before it, whose result the derivative then reuses.
+template <bool V>
+struct EosCacheT {
     static constexpr bool TAB = true;
     const CeosParams *P;
     int r;  // the zone's radial index's cache
@@ -211,6 +216,10 @@ struct EosCache {
         const double h = rho0 * CEOS_HH / P->rho_unit;
         const double rhoP = rho0 + h;
         double rhoM = rho0 - h;
+        if (V) {
+            const double p = pres(rho0, u);
+            return (rhoM > 0) ? (p - p) / (2.0 * h) : (p - p) / (h);
+        }
         if (rhoM > 0) {
             const double pP = pres(rhoP, u);
             const double pM = pres(rhoM, u);
@@ -227,6 +236,10 @@ struct EosCache {
         const double h = u * CEOS_HH / P->u_unit;
         const double uP = u + h;
         double uM = u - h;
+        if (V) {
+            const double p = pres(rho0, u);
+            return (uM > 0) ? (p - p) / (2.0 * h) : (p - p) / (h);
+        }
         if (uM > 0) {
             const double pP = pres(rho0, uP);
             const double pM = pres(rho0, uM);
@@ -250,8 +263,9 @@ struct EosCache {
 
 // The 5D inversion's view of the cache EOS (utoprim_5d.c:388-389, 454-481):
 // the pressure, then dp/drho and dp/du, each by its calls above.
-struct Eos5Cache {
-    EosCache e;
+template <bool V>
+struct Eos5CacheT {
+    EosCacheT<V> e;
     double gam;
     __device__ static constexpr bool tabulated() { return true; }
     __device__ double p(double rho, double u) const { return e.pres(rho, u); }
@@ -262,5 +276,10 @@ struct Eos5Cache {
         dpdu = e.dpdu(rho, u);
     }
 };
+
+constexpr bool ceos_vanishing_steps(double rho_unit, double u_unit)
+{
+    return CEOS_HH / rho_unit <= 0x1p-55 && CEOS_HH / u_unit <= 0x1p-55;
+}
 
 }  // namespace harm

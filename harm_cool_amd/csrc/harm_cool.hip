@@ -1302,7 +1302,7 @@ __device__ __forceinline__ void ceos_faces(const DGrid &g, int chain, Fn &&f)
 // the zones' last temperatures may be unset, and the loop over j runs in
 // order, so a zone's j-1 neighbour has its value when it is read
 // (cooleos.c:784-799).
-template <int DIR>
+template <int DIR, bool V>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_EOS_WPE, 8)))
 k_face_eos(DGrid g, CeosParams ce, double *__restrict__ lastT, const double *__restrict__ pb,
            double *__restrict__ fe, int chain, unsigned long long *__restrict__ solves)
@@ -1330,7 +1330,7 @@ k_face_eos(DGrid g, CeosParams ce, double *__restrict__ lastT, const double *__r
         CeosZone zs;
         zs.T = lastT[z];
         if (chain && zs.T == 0.0 && j > g.js) zs.T = lastT[z - g.N3];
-        const EosCache eos{&ce, ceos_radius(ce, i), &zs};
+        const EosCacheT<V> eos{&ce, ceos_radius(ce, i), &zs};
         const double pl = eos.pres(rl, ul);
         const double cl = eos.cs2(rl, ul, pl);
         const double pr = eos.pres(rr, ur);
@@ -1417,6 +1417,7 @@ __global__ void __launch_bounds__(TPB) k_flux_ceos(DGrid g, const double *__rest
 // advance()'s zone loop with COOL_EOS (step_ch.c:237-306): InitCache_CoolEOS,
 // the source at pb, primtoU at pi, the update with the flux divergence S,
 // Utoprim_5d seeded with pi (no Font fix), then fixup1zone (fixup.c:51-60).
+template <bool V>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_WPE, 8))) k_update_ceos(DGrid g, CeosParams ce, double *__restrict__ lastT,
                                                      const double *__restrict__ pi, const double *__restrict__ pb,
                                                      double *__restrict__ pf, int8_t *__restrict__ pflag,
@@ -1442,7 +1443,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_W
         load_geom(g.geom[CENT], g.ncol, col, geom);
         CeosZone zs;
         zs.T = lastT[z];
-        const EosCache eos{&ce, ceos_radius(ce, i), &zs};
+        const EosCacheT<V> eos{&ce, ceos_radius(ce, i), &zs};
         double pbz[NPR], piz[NPR], dU[NPR], U[NPR];
 #pragma unroll
         for (int k = 0; k < NPR; k++) pbz[k] = pb[k * nz + z];
@@ -1458,7 +1459,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(CEOS_W
 #pragma unroll
         for (int k = 0; k < NPR; k++) pr[k] = piz[k];
         int niter = 0;
-        const Eos5Cache e5{eos, g.gam};
+        const Eos5CacheT<V> e5{eos, g.gam};
         const int ret = utoprim_5d(U, geom, e5, pr, niter);
         lastT[z] = zs.T;
         nsolve = zs.nsolve;
@@ -3099,6 +3100,7 @@ struct harm_ctx {
     std::vector<double *> ceos_ycoef;  // per cache: the splines along T of e and p
     double **ceos_yco;                 // device array of ceos_ycoef
     double *ceos_fe;                   // [4][nz]: each face's p_l, cs2_l, p_r, cs2_r (k_face_eos)
+    bool ceos_vanish;                  // the units make dp/drho, dp/du repeat calls (ceos_vanishing_steps)
     unsigned long long *ceos_solves;
     bool ceos_chain[3];
     // learned Newton seed and training-pair capture
@@ -3277,8 +3279,10 @@ int flux_launch(harm_ctx *c, int dir, const double *pb, double *F, int fullstep)
         const dim3 ge(nblocks(chain ? nc : total));
         grid = dim3(nblocks(total));
 #define CEOS_FLUX(D_, K_)                                                                                          \
-    LAUNCH(K_, hipLaunchKernelGGL(k_face_eos<D_>, ge, dim3(TPB), 0, c->stream, g, c->ceos, c->lastT, pb, c->ceos_fe, \
-                                  chain, c->ceos_solves);                                                          \
+    LAUNCH(K_, if (c->ceos_vanish) hipLaunchKernelGGL((k_face_eos<D_, true>), ge, dim3(TPB), 0, c->stream, g, c->ceos, \
+                                                      c->lastT, pb, c->ceos_fe, chain, c->ceos_solves);            \
+               else hipLaunchKernelGGL((k_face_eos<D_, false>), ge, dim3(TPB), 0, c->stream, g, c->ceos, c->lastT, pb, \
+                                       c->ceos_fe, chain, c->ceos_solves);                                         \
            hipLaunchKernelGGL(k_flux_ceos<D_>, grid, dim3(TPB), 0, c->stream, g, c->ceos_fe, pb, F, c->dtpart,     \
                               fullstep, c->cnt))
         if (dir == 1)
@@ -3352,9 +3356,15 @@ int advance_ceos(harm_ctx *c, double *pi, double *pb, double *pf, double dtfac, 
     const DivArgs dv = {c->F1, c->F2, c->F3, c->e1, c->e2, c->e3};
     if ((rc = fluxes_and_emf(c, pb, fullstep, true))) return rc;
     LAUNCH(HARM_K_DIV, hipLaunchKernelGGL(k_div, dim3(nblocks8(na)), dim3(TPB), 0, c->stream, g, dv, c->S));
-    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL(k_update_ceos, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g, c->ceos,
-                                             c->lastT, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac,
-                                             (pi == pb) ? 1 : 0, c->cpart, c->ceos_solves));
+#define CEOS_UPD(V_)                                                                                             \
+    LAUNCH(HARM_K_UPDATE, hipLaunchKernelGGL(k_update_ceos<V_>, dim3(nblocks(na)), dim3(TPB), 0, c->stream, g,     \
+                                             c->ceos, c->lastT, pi, pb, pf, c->pflag, c->S, c->tstate, dtfac,      \
+                                             (pi == pb) ? 1 : 0, c->cpart, c->ceos_solves))
+    if (c->ceos_vanish)
+        CEOS_UPD(true);
+    else
+        CEOS_UPD(false);
+#undef CEOS_UPD
     CntMap map = {8, {HARM_C2P_CALLS, HARM_NEWTON_ITERS, HARM_NEWTON_KCONV, HARM_FAIL_2D, HARM_FIXUP_GAMMA,
                       HARM_FLOOR_ZONES, HARM_GAMMA_CAP, HARM_FATAL}};
     hipLaunchKernelGGL(k_sum_partials, dim3(RED_BLOCKS, 8), dim3(256), 0, c->stream, c->cpart, (size_t)nblocks(na), map,
@@ -4293,6 +4303,7 @@ int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, cons
     HIPCHK(hipMemset(c->ceos_solves, 0, sizeof(unsigned long long)));
     HIPCHK(hipDeviceSynchronize());
     c->ceos = P;
+    c->ceos_vanish = ceos_vanishing_steps(P.rho_unit, P.u_unit);
     for (bool &b : c->ceos_chain) b = true;
     c->eos_kind = 2;
     return 0;

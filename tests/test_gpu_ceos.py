@@ -31,15 +31,17 @@ def _dev_caches(vx, vy, vals):
     return out
 
 
-def _pair(P, vals_of, ranges=None):
+def _pair(P, vals_of, ranges=None, units=None):
     vx, vy, vals = vals_of(P)
     ranges = ranges or {}
+    un = dict(U)
+    un.update(units or {})
     olib().orc_set_threads(16)
     o = Oracle(P)
-    o.set_eos_cache(orc_ceos(vx, vy, vals, P.gam, U["rho_unit"], U["u_unit"], U["p_unit"], **ranges))
+    o.set_eos_cache(orc_ceos(vx, vy, vals, P.gam, un["rho_unit"], un["u_unit"], un["p_unit"], **ranges))
     g = hc.Harm(P)
     caches = _dev_caches(vx, vy, vals)
-    g.set_eos_cache(caches, ranges)
+    g.set_eos_cache(caches, dict(ranges, **un))
     return g, o, caches
 
 
@@ -51,8 +53,15 @@ def _same(g, o, label):
     assert g.time() == o.time(), label
 
 
+# density unit 1e3 g/cm^3: the finite-difference steps rho0 1e-6 / RHO_UNIT and u 1e-6 /
+# U_UNIT are resolved, so dp/drho and dp/du take real solves at rho0 +- h, u +- h (a
+# nonzero sound speed in vchar and the 5D Jacobian)
+SMALL_UNITS = {"rho_unit": 1e3, "u_unit": 1e3 * cc.CL * cc.CL, "p_unit": 1e3 * cc.CL * cc.CL}
+
 CASES = {
     "2d_shared": (lambda: torus_params(48, 48, 1), lambda P: cc.gas_rad(), None),
+    "2d_resolved_derivative_steps": (lambda: torus_params(32, 32, 1), lambda P: cc.gas_rad(rho=(1e-5, 1e5)),
+                                     None, SMALL_UNITS),
     "3d_per_radius": (lambda: torus_params(16, 12, 8), lambda P: cc.gas_rad(n=P.n1 + 4), None),
     "2d_vanleer_llf_rho_min": (lambda: torus_params(32, 32, 1, lim=1, hllf=0.0, laxf=1.0),
                                lambda P: cc.gas_rad(), {"rho_min": 1e6}),
@@ -62,9 +71,9 @@ CASES = {
 
 @pytest.mark.parametrize("case", list(CASES))
 def test_steps_match_oracle(case):
-    mk, vals_of, ranges = CASES[case]
+    mk, vals_of, ranges, *units = CASES[case]
     P = mk()
-    g, o, caches = _pair(P, vals_of, ranges)
+    g, o, caches = _pair(P, vals_of, ranges, units[0] if units else None)
     o.init_torus()
     g.init()
     assert np.array_equal(g.prims(), o.prims())
