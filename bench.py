@@ -571,6 +571,14 @@ def eos_table_cpu(hc, tab: dict, sample: int = 24) -> dict:
             "ms_per_point": el / sample * 1e3, "points_per_s": sample / el, "max_rel_diff_gpu_vs_reference": rel}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what: str) -> None:
+    """One line on stderr as each section of the run ends."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {what}", file=sys.stderr, flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -623,6 +631,7 @@ def main() -> None:
         # steps 0..capture_steps-1 double as the warm-up of the timed run
         m_torus, learned_rep = learned_torus(g, hc, torch, nz, args.steps, args.capture_steps, args.train_steps,
                                              args.train_batch)
+        progress("capture and training of the learned seed")
     g.step_ch(args.warmup)
     p_start, f_start = g.prims(), g.pflag()
     t_start = g.time()
@@ -633,6 +642,7 @@ def main() -> None:
     el = time.perf_counter() - t0
     c = g.counters()
     value = nz * args.steps / el
+    progress(f"headline torus: {el / args.steps * 1e3:.3f} ms/step")
     if m_torus is not None:
         p_plain = g.prims()
         sl = (slice(None), slice(2, n1 + 2), slice(2, n2 + 2), slice(None))
@@ -663,6 +673,7 @@ def main() -> None:
                 "max_field_normalized_diff_vs_plain": max(dev), "max_rel_diff_rho_vs_plain": rel_rho,
                 "bitwise_equal_to_plain": bool(max(dev) == 0.0)}
         del p_plain, rho
+        progress("learned-seed torus runs")
         g.set_prims(p_start, f_start)  # leave the state as the plain run had it for the timing pass below
         g.set_time(*t_start)
     del p_start, f_start
@@ -696,17 +707,21 @@ def main() -> None:
                             "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
                             "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
 
+    progress("per-kernel timing and tracers")
     fast = None
     if not args.no_fast and os.path.exists(hc.LIB_FAST_PATH):
         fast = fast_mode(hc, (n1, n2, n3), args.steps, args.warmup)
+        progress("fast mode")
 
     eos_tab = None
     if not (args.no_eos_table and args.no_tabulated):
         eos_tab = eos_table(hc, tnr, tnt)
+        progress("cooling table")
     tabulated = cache_eos = None
     if not args.no_tabulated:
         tabulated = tabulated_torus(hc, eos_tab["_table"], args.steps, args.warmup)
         cache_eos = cache_eos_torus(hc, eos_tab, args.steps, args.warmup)
+        progress("tabulated and cache-EOS tori")
     if not args.no_eos_table:
         eos_tab["cache_lookup"] = cache_lookups(hc, torch, eos_tab)
     else:
@@ -715,12 +730,14 @@ def main() -> None:
     st = None
     if not args.no_learned and args.stress_n > 0:
         st = stress(hc, torch, args.stress_n, m_torus, args.train_steps, args.train_batch)
+        progress("inversion stress batch")
     if m_torus is not None:
         m_torus.close()
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(min(16, os.cpu_count() or 1))
+        progress("CPU baseline")
         if st is not None:
             cpu["stress_vs_oracle"] = stress_oracle_check(st["_host"], st["_sample"], 13. / 9.)
     if st is not None:
