@@ -517,6 +517,7 @@ def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.harm_fermi_batch.argtypes = [c_void_p] * 6 + [c_size_t, c_void_p]
     L.harm_eos_species_batch.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p]
     L.harm_eos_point_batch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
+    L.harm_eos_point_batch_thread.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]
     L.harm_dump_write.argtypes = [c_void_p, c_char_p, POINTER(RestartHdr)]
     L.harm_dump_columns.argtypes = [c_void_p, c_size_t, c_size_t, c_void_p]
     L.harm_dump_columns_noalloc.argtypes = [c_void_p, c_size_t, c_size_t, c_void_p]

@@ -121,6 +121,7 @@ const NseRules &nse_rules()
             N.lx40[i] = (double)l40x[i];
             N.lw40[i] = (double)l40w[i];
         }
+        R.wave = 0;
     });
     return N;
 }
@@ -129,6 +130,7 @@ const NseRules &nse_rules()
 // (kernels read them in place rather than copying a by-value argument to
 // their stack for the noinline routines that take them by reference).
 __constant__ NseRules c_rules;
+__constant__ NseRules c_rules_wave;  // the same rules for the wave-per-point kernel
 
 int upload_rules()
 {
@@ -141,6 +143,9 @@ int upload_rules()
     if (!done[dev]) {
         const NseRules &N = nse_rules();
         FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules), &N, sizeof(NseRules)));
+        NseRules W = N;
+        W.f.wave = 1;
+        FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules_wave), &W, sizeof(NseRules)));
         done[dev] = true;
     }
     return 0;
@@ -201,6 +206,23 @@ __global__ void __launch_bounds__(64) k_abundance(const double *__restrict__ rho
     for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = v[k];
 }
 
+// One cooling-table point per wavefront: every lane runs the point's chain
+// (the same values, so the same control flow) and the wavefront shares each
+// dfermi's quadrature nodes (dfermi_wave), which shortens the chain of the
+// slowest points -- the ones whose Newton solves run long -- and gives the
+// launch a wavefront per point to spread over the chip.
+__global__ void __launch_bounds__(64) k_abundance_wave(const double *__restrict__ rho,
+                                                       const double *__restrict__ T,
+                                                       const double *__restrict__ hdfac, double *__restrict__ out,
+                                                       size_t n)
+{
+    const size_t i = blockIdx.x;
+    double v[6];
+    harm::abundance(rho[i], T[i], hdfac[i], c_rules_wave, v[0], v[1], v[2], v[3], v[4], v[5]);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = v[k];
+}
+
 unsigned nb(size_t n) { return (unsigned)((n + TPB_F - 1) / TPB_F); }
 
 }  // namespace
@@ -237,6 +259,21 @@ int harm_fermi_batch(const double *dk, const double *eta, const double *theta, d
 
 int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
                          void *stream)
+{
+    if (!n) return 0;
+    if (!rho || !T || !hdfac || !out) return harm_host::set_error("null argument", 1);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = upload_rules();
+    if (rc) return rc;
+    if (n > 0x7fffffffu) return harm_host::set_error("harm_eos_point_batch: at most 2^31-1 points per call", 3);
+    hipLaunchKernelGGL(k_abundance_wave, dim3((unsigned)n), dim3(64), 0, s, rho, T, hdfac, out, n);
+    FCHK(hipGetLastError());
+    FCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int harm_eos_point_batch_thread(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
+                                void *stream)
 {
     if (!n) return 0;
     if (!rho || !T || !hdfac || !out) return harm_host::set_error("null argument", 1);

@@ -36,6 +36,7 @@ constexpr double FD_XME = 0.510998910, FD_XMP = 938.272013, FD_XMN = 939.56536; 
 // the weight function), both in increasing node order.
 struct FermiRules {
     double lx[10], lw[10], gx[20], gw[20];
+    int wave;  // device: a whole wavefront evaluates each dfermi (dfermi_wave)
 };
 
 // fdfunc1 (gauss_integral.f90:354-393)
@@ -122,6 +123,66 @@ HD void fd_lag20(const FermiRules &R, double a, double b, double dk, double eta,
     ddres = ddres * b;
 }
 
+// dfermi's four quadratures with the whole wavefront on one (dk, eta,
+// theta): lanes 0-29 take the Gauss-Legendre node pairs (lane / 10 = which
+// subinterval, lane % 10 = which pair), lanes 30-49 the Gauss-Laguerre nodes;
+// each lane forms its term as the serial loop does, then every lane adds the
+// terms of each rule in the serial loop's order, so the sums are the serial
+// ones bit for bit.  Every lane of the wave must call it with the same values.
+__device__ inline void dfermi_wave(const FermiRules &R, double dk, double eta, double theta, double s1, double s2,
+                                   double s3, double s12, double &fd, double &fdeta, double &fdtheta)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (lane < 30) {
+        const int q = lane / 10, j = lane - 10 * q;
+        const double a = (q == 0) ? 0.0 : (q == 1) ? s1 : s2;
+        const double b = (q == 0) ? s12 : (q == 1) ? s2 : s3;
+        const double center = 0.5 * (a + b), hlfrun = 0.5 * (b - a);
+        const double x1 = center + hlfrun * R.lx[j], x2 = center - hlfrun * R.lx[j];
+        double f1, d1, dd1, f2, d2, dd2;
+        if (q == 0) {
+            fd_func2(x1, dk, eta, theta, f1, d1, dd1);
+            fd_func2(x2, dk, eta, theta, f2, d2, dd2);
+        } else {
+            fd_func1(x1, dk, eta, theta, f1, d1, dd1);
+            fd_func1(x2, dk, eta, theta, f2, d2, dd2);
+        }
+        t0 = (f1 + f2) * R.lw[j];
+        t1 = (d1 + d2) * R.lw[j];
+        t2 = (dd1 + dd2) * R.lw[j];
+    } else if (lane < 50) {
+        const int j = lane - 30;
+        double f, d, dd;
+        fd_func1(s3 + 1.0 * R.gx[j], dk, eta, theta, f, d, dd);
+        t0 = f * R.gw[j];
+        t1 = d * R.gw[j];
+        t2 = dd * R.gw[j];
+    }
+    double r[4], dr[4], ddr[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int base = (q < 3) ? 10 * q : 30, m = (q < 3) ? 10 : 20;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int j = 0; j < m; j++) {
+            a0 = a0 + __shfl(t0, base + j, 64);
+            a1 = a1 + __shfl(t1, base + j, 64);
+            a2 = a2 + __shfl(t2, base + j, 64);
+        }
+        r[q] = a0;
+        dr[q] = a1;
+        ddr[q] = a2;
+    }
+    // fd_leg20's closing products by its half-run, fd_lag20's by b = 1
+    const double h0 = 0.5 * (s12 - 0.0), h1 = 0.5 * (s2 - s1), h2 = 0.5 * (s3 - s2);
+    const double r1 = r[0] * h0, r2 = r[1] * h1, r3 = r[2] * h2, r4 = r[3] * 1.0;
+    const double dr1 = dr[0] * h0, dr2 = dr[1] * h1, dr3 = dr[2] * h2, dr4 = dr[3] * 1.0;
+    const double ddr1 = ddr[0] * h0, ddr2 = ddr[1] * h1, ddr3 = ddr[2] * h2, ddr4 = ddr[3] * 1.0;
+    fd = r1 + r2 + r3 + r4;
+    fdeta = dr1 + dr2 + dr3 + dr4;
+    fdtheta = ddr1 + ddr2 + ddr3 + ddr4;
+}
+
 // dfermi (gauss_integral.f90:241-349): F_k(eta, theta) = int_0^inf x^k
 // sqrt(1 + x theta / 2) / (exp(x - eta) + 1) dx and its eta and theta
 // derivatives, over Aparicio's four subintervals.  The breakpoint fit
@@ -138,6 +199,12 @@ HD_CALL void dfermi(double dk, double eta, double theta, const FermiRules &R, do
     const double x2 = (a2 + b2 * xi + c2 * d2 * xi2) / (1.0 + e2 * xi + c2 * xi2);
     const double x3 = (a3 + b3 * xi + c3 * d3 * xi2) / (1.0 + e3 * xi + c3 * xi2);
     const double s1 = x1 - x2, s2 = x1, s3 = x1 + x3, s12 = sqrt(s1);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (R.wave) {
+        dfermi_wave(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+        return;
+    }
+#endif
     double r1, dr1, ddr1, r2, dr2, ddr2, r3, dr3, ddr3, r4, dr4, ddr4;
     fd_leg20<2>(R, 0.0, s12, dk, eta, theta, r1, dr1, ddr1);
     fd_leg20<1>(R, s1, s2, dk, eta, theta, r2, dr2, ddr2);

@@ -422,10 +422,14 @@ int harm_fermi_rules(double *out /* 100 */);
 /* One cooling-table point per element (abundance.f90:6-243): rho [g cm^-3], T
  * [K] and the disk-height factor HdFac -> out[6][n]: e, p [erg cm^-3], Qnu
  * [erg cm^-2 s^-1], optical depth tau, n_e-, n_e+ [fm^-3].  The chemical
- * equilibrium solve, the neutrino trapping iteration and all rates run in the
- * thread of their point. */
+ * equilibrium solve, the neutrino trapping iteration and all rates of a point
+ * run in one wavefront, which shares each Fermi-Dirac integral's quadrature
+ * nodes (the sums in the serial order). */
 int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
                          void *stream);
+/* The same with one point per thread (every integral serial); the same bits. */
+int harm_eos_point_batch_thread(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
+                                void *stream);
 /* The step's tabulated EOS (harm_eos_table) from this generator: e and p of
  * abundance() at every point of the spec's log-spaced (rho, T) grid at one
  * disk-height factor hdfac [s] (the reference's per-radius cooling tables,

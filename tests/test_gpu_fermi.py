@@ -106,3 +106,25 @@ def test_table_points_match_reference():
           f"GPU {el * 1e3:.1f} ms, reference on one core {el_ref * 1e3:.0f} ms")
     assert np.all(np.isfinite(got))
     assert rel.max() <= 1e-6
+
+
+def test_wave_per_point_equals_thread_per_point():
+    """The wave-per-point generator (each dfermi's nodes across the wavefront,
+    the sums in the serial order) gives the one-thread-per-point kernel's bits
+    at every point of a grid across the table's range, including points whose
+    Newton solves run to their iteration limit."""
+    import time
+    torch = _torch()
+    rho, T = np.meshgrid(np.geomspace(1e6, 1e13, 16), np.geomspace(1e9, 1e12, 16), indexing="ij")
+    rho, T = rho.ravel(), T.ravel()
+    hd = np.full(rho.size, 6.28e-4)
+    n = rho.size
+    ins = [_dev(torch, a) for a in (rho, T, hd)]
+    outs = []
+    for fn in (hc.lib().harm_eos_point_batch, hc.lib().harm_eos_point_batch_thread):
+        out = torch.empty(6 * n, dtype=torch.float64, device="cuda:0")
+        t0 = time.perf_counter()
+        assert fn(*(ctypes_ptr(t) for t in ins), ctypes_ptr(out), n, None) == 0
+        print(f"{fn.__name__}: {(time.perf_counter() - t0) * 1e3:.0f} ms")
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
