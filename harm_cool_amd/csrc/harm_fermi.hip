@@ -130,7 +130,12 @@ const NseRules &nse_rules()
 // (kernels read them in place rather than copying a by-value argument to
 // their stack for the noinline routines that take them by reference).
 __constant__ NseRules c_rules;
-__constant__ NseRules c_rules_wave;  // the same rules for the wave-per-point kernel
+__constant__ NseRules c_rules_group;  // the same rules for the lane-group-per-point kernel
+
+#ifndef GEN_GROUP
+#define GEN_GROUP 8
+#endif
+constexpr int GEN_G = GEN_GROUP;  // lanes per point of k_abundance_group
 
 int upload_rules()
 {
@@ -144,8 +149,8 @@ int upload_rules()
         const NseRules &N = nse_rules();
         FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules), &N, sizeof(NseRules)));
         NseRules W = N;
-        W.f.wave = 1;
-        FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules_wave), &W, sizeof(NseRules)));
+        W.f.wave = GEN_G;
+        FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules_group), &W, sizeof(NseRules)));
         done[dev] = true;
     }
     return 0;
@@ -206,20 +211,21 @@ __global__ void __launch_bounds__(64) k_abundance(const double *__restrict__ rho
     for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = v[k];
 }
 
-// One cooling-table point per wavefront: every lane runs the point's chain
-// (the same values, so the same control flow) and the wavefront shares each
-// dfermi's quadrature nodes (dfermi_wave), which shortens the chain of the
-// slowest points -- the ones whose Newton solves run long -- and gives the
-// launch a wavefront per point to spread over the chip.
-__global__ void __launch_bounds__(64) k_abundance_wave(const double *__restrict__ rho,
-                                                       const double *__restrict__ T,
-                                                       const double *__restrict__ hdfac, double *__restrict__ out,
-                                                       size_t n)
+// One cooling-table point per group of GEN_G lanes (64 / GEN_G points per
+// wavefront): every lane of a group runs its point's chain (the same values,
+// so the same branches) and the group shares each dfermi's quadrature nodes
+// (dfermi_group), which shortens the chain of the slowest points -- the ones
+// whose Newton solves run long -- and spreads the points over the chip.
+__global__ void __launch_bounds__(64) k_abundance_group(const double *__restrict__ rho,
+                                                        const double *__restrict__ T,
+                                                        const double *__restrict__ hdfac, double *__restrict__ out,
+                                                        size_t n)
 {
-    const size_t i = blockIdx.x;
+    const size_t i = (size_t)blockIdx.x * (64 / GEN_G) + threadIdx.x / GEN_G;
+    if (i >= n) return;  // whole groups leave together
     double v[6];
-    harm::abundance(rho[i], T[i], hdfac[i], c_rules_wave, v[0], v[1], v[2], v[3], v[4], v[5]);
-    if (threadIdx.x == 0)
+    harm::abundance(rho[i], T[i], hdfac[i], c_rules_group, v[0], v[1], v[2], v[3], v[4], v[5]);
+    if (threadIdx.x % GEN_G == 0)
         for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = v[k];
 }
 
@@ -265,8 +271,9 @@ int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac
     hipStream_t s = (hipStream_t)stream;
     int rc = upload_rules();
     if (rc) return rc;
-    if (n > 0x7fffffffu) return harm_host::set_error("harm_eos_point_batch: at most 2^31-1 points per call", 3);
-    hipLaunchKernelGGL(k_abundance_wave, dim3((unsigned)n), dim3(64), 0, s, rho, T, hdfac, out, n);
+    const size_t nblk = (n + 64 / GEN_G - 1) / (64 / GEN_G);
+    if (nblk > 0x7fffffffu) return harm_host::set_error("harm_eos_point_batch: too many points for one call", 3);
+    hipLaunchKernelGGL(k_abundance_group, dim3((unsigned)nblk), dim3(64), 0, s, rho, T, hdfac, out, n);
     FCHK(hipGetLastError());
     FCHK(hipStreamSynchronize(s));
     return 0;

@@ -36,7 +36,7 @@ constexpr double FD_XME = 0.510998910, FD_XMP = 938.272013, FD_XMN = 939.56536; 
 // the weight function), both in increasing node order.
 struct FermiRules {
     double lx[10], lw[10], gx[20], gw[20];
-    int wave;  // device: a whole wavefront evaluates each dfermi (dfermi_wave)
+    int wave;  // device: 0, or the lanes (4, 8, 16, 32, 64) that share each dfermi (dfermi_group)
 };
 
 // fdfunc1 (gauss_integral.f90:354-393)
@@ -123,51 +123,62 @@ HD void fd_lag20(const FermiRules &R, double a, double b, double dk, double eta,
     ddres = ddres * b;
 }
 
-// dfermi's four quadratures with the whole wavefront on one (dk, eta,
-// theta): lanes 0-29 take the Gauss-Legendre node pairs (lane / 10 = which
-// subinterval, lane % 10 = which pair), lanes 30-49 the Gauss-Laguerre nodes;
-// each lane forms its term as the serial loop does, then every lane adds the
-// terms of each rule in the serial loop's order, so the sums are the serial
-// ones bit for bit.  Every lane of the wave must call it with the same values.
-__device__ inline void dfermi_wave(const FermiRules &R, double dk, double eta, double theta, double s1, double s2,
-                                   double s3, double s12, double &fd, double &fdeta, double &fdtheta)
+// dfermi's four quadratures with a group of G lanes (G = 4 ... 64, the
+// groups aligned in the wavefront) on one (dk, eta, theta): its 50 integrand
+// evaluations -- e = 0..29 the Gauss-Legendre node pairs of the three finite
+// subintervals (e / 10, e % 10), e = 30..49 the Gauss-Laguerre nodes -- go to
+// lane e % G of the group, each term formed as the serial loop forms it; then
+// every lane of the group adds each rule's terms in the serial loop's order,
+// so the sums are the serial ones bit for bit.  Every lane of the group must
+// call it with the same values.
+template <int G>
+__device__ inline void dfermi_group(const FermiRules &R, double dk, double eta, double theta, double s1, double s2,
+                                    double s3, double s12, double &fd, double &fdeta, double &fdtheta)
 {
-    const int lane = (int)(threadIdx.x & 63u);
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    if (lane < 30) {
-        const int q = lane / 10, j = lane - 10 * q;
-        const double a = (q == 0) ? 0.0 : (q == 1) ? s1 : s2;
-        const double b = (q == 0) ? s12 : (q == 1) ? s2 : s3;
-        const double center = 0.5 * (a + b), hlfrun = 0.5 * (b - a);
-        const double x1 = center + hlfrun * R.lx[j], x2 = center - hlfrun * R.lx[j];
-        double f1, d1, dd1, f2, d2, dd2;
-        if (q == 0) {
-            fd_func2(x1, dk, eta, theta, f1, d1, dd1);
-            fd_func2(x2, dk, eta, theta, f2, d2, dd2);
-        } else {
-            fd_func1(x1, dk, eta, theta, f1, d1, dd1);
-            fd_func1(x2, dk, eta, theta, f2, d2, dd2);
+    constexpr int NE = 50, NS = (NE + G - 1) / G;
+    const int lane = (int)(threadIdx.x & 63u), gl = lane % G, gbase = lane - gl;
+    double t0[NS], t1[NS], t2[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        const int e = gl + k * G;
+        t0[k] = t1[k] = t2[k] = 0.0;
+        if (e < 30) {
+            const int q = e / 10, j = e - 10 * q;
+            const double a = (q == 0) ? 0.0 : (q == 1) ? s1 : s2;
+            const double b = (q == 0) ? s12 : (q == 1) ? s2 : s3;
+            const double center = 0.5 * (a + b), hlfrun = 0.5 * (b - a);
+            const double x1 = center + hlfrun * R.lx[j], x2 = center - hlfrun * R.lx[j];
+            double f1, d1, dd1, f2, d2, dd2;
+            if (q == 0) {
+                fd_func2(x1, dk, eta, theta, f1, d1, dd1);
+                fd_func2(x2, dk, eta, theta, f2, d2, dd2);
+            } else {
+                fd_func1(x1, dk, eta, theta, f1, d1, dd1);
+                fd_func1(x2, dk, eta, theta, f2, d2, dd2);
+            }
+            t0[k] = (f1 + f2) * R.lw[j];
+            t1[k] = (d1 + d2) * R.lw[j];
+            t2[k] = (dd1 + dd2) * R.lw[j];
+        } else if (e < NE) {
+            const int j = e - 30;
+            double f, d, dd;
+            fd_func1(s3 + 1.0 * R.gx[j], dk, eta, theta, f, d, dd);
+            t0[k] = f * R.gw[j];
+            t1[k] = d * R.gw[j];
+            t2[k] = dd * R.gw[j];
         }
-        t0 = (f1 + f2) * R.lw[j];
-        t1 = (d1 + d2) * R.lw[j];
-        t2 = (dd1 + dd2) * R.lw[j];
-    } else if (lane < 50) {
-        const int j = lane - 30;
-        double f, d, dd;
-        fd_func1(s3 + 1.0 * R.gx[j], dk, eta, theta, f, d, dd);
-        t0 = f * R.gw[j];
-        t1 = d * R.gw[j];
-        t2 = dd * R.gw[j];
     }
     double r[4], dr[4], ddr[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int base = (q < 3) ? 10 * q : 30, m = (q < 3) ? 10 : 20;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
         for (int j = 0; j < m; j++) {
-            a0 = a0 + __shfl(t0, base + j, 64);
-            a1 = a1 + __shfl(t1, base + j, 64);
-            a2 = a2 + __shfl(t2, base + j, 64);
+            const int e = base + j, src = gbase + e % G;
+            a0 = a0 + __shfl(t0[e / G], src, 64);
+            a1 = a1 + __shfl(t1[e / G], src, 64);
+            a2 = a2 + __shfl(t2[e / G], src, 64);
         }
         r[q] = a0;
         dr[q] = a1;
@@ -200,8 +211,24 @@ HD_CALL void dfermi(double dk, double eta, double theta, const FermiRules &R, do
     const double x3 = (a3 + b3 * xi + c3 * d3 * xi2) / (1.0 + e3 * xi + c3 * xi2);
     const double s1 = x1 - x2, s2 = x1, s3 = x1 + x3, s12 = sqrt(s1);
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (R.wave) {
-        dfermi_wave(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+    if (R.wave == 64) {
+        dfermi_group<64>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+        return;
+    }
+    if (R.wave == 32) {
+        dfermi_group<32>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+        return;
+    }
+    if (R.wave == 16) {
+        dfermi_group<16>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+        return;
+    }
+    if (R.wave == 8) {
+        dfermi_group<8>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+        return;
+    }
+    if (R.wave == 4) {
+        dfermi_group<4>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
         return;
     }
 #endif

@@ -423,8 +423,8 @@ int harm_fermi_rules(double *out /* 100 */);
  * [K] and the disk-height factor HdFac -> out[6][n]: e, p [erg cm^-3], Qnu
  * [erg cm^-2 s^-1], optical depth tau, n_e-, n_e+ [fm^-3].  The chemical
  * equilibrium solve, the neutrino trapping iteration and all rates of a point
- * run in one wavefront, which shares each Fermi-Dirac integral's quadrature
- * nodes (the sums in the serial order). */
+ * run in one group of lanes, which shares each Fermi-Dirac integral's
+ * quadrature nodes (the sums in the serial order). */
 int harm_eos_point_batch(const double *rho, const double *T, const double *hdfac, double *out, size_t n,
                          void *stream);
 /* The same with one point per thread (every integral serial); the same bits. */
