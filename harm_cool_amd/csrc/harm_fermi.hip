@@ -132,10 +132,7 @@ const NseRules &nse_rules()
 __constant__ NseRules c_rules;
 __constant__ NseRules c_rules_group;  // the same rules for the lane-group-per-point kernel
 
-#ifndef GEN_GROUP
-#define GEN_GROUP 8
-#endif
-constexpr int GEN_G = GEN_GROUP;  // lanes per point of k_abundance_group
+constexpr int GEN_G = harm::FD_GROUP;  // lanes per point of k_abundance_group
 
 int upload_rules()
 {
@@ -149,7 +146,7 @@ int upload_rules()
         const NseRules &N = nse_rules();
         FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules), &N, sizeof(NseRules)));
         NseRules W = N;
-        W.f.wave = GEN_G;
+        W.f.wave = 1;
         FCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rules_group), &W, sizeof(NseRules)));
         done[dev] = true;
     }

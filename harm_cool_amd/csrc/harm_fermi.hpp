@@ -36,7 +36,7 @@ constexpr double FD_XME = 0.510998910, FD_XMP = 938.272013, FD_XMN = 939.56536; 
 // the weight function), both in increasing node order.
 struct FermiRules {
     double lx[10], lw[10], gx[20], gw[20];
-    int wave;  // device: 0, or the lanes (4, 8, 16, 32, 64) that share each dfermi (dfermi_group)
+    int wave;  // device: 1 = the FD_GROUP lanes of a point share each dfermi (dfermi_group)
 };
 
 // fdfunc1 (gauss_integral.f90:354-393)
@@ -122,6 +122,10 @@ HD void fd_lag20(const FermiRules &R, double a, double b, double dk, double eta,
     dres = dres * b;
     ddres = ddres * b;
 }
+
+// Lanes per cooling-table point in the group kernel: 8 measured fastest for a
+// 200x200 table (22 s; 16: 26 s, 4: 30 s, 32: 37 s, 64: 64 s).
+constexpr int FD_GROUP = 8;
 
 // dfermi's four quadratures with a group of G lanes (G = 4 ... 64, the
 // groups aligned in the wavefront) on one (dk, eta, theta): its 50 integrand
@@ -211,24 +215,8 @@ HD_CALL void dfermi(double dk, double eta, double theta, const FermiRules &R, do
     const double x3 = (a3 + b3 * xi + c3 * d3 * xi2) / (1.0 + e3 * xi + c3 * xi2);
     const double s1 = x1 - x2, s2 = x1, s3 = x1 + x3, s12 = sqrt(s1);
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (R.wave == 64) {
-        dfermi_group<64>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
-        return;
-    }
-    if (R.wave == 32) {
-        dfermi_group<32>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
-        return;
-    }
-    if (R.wave == 16) {
-        dfermi_group<16>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
-        return;
-    }
-    if (R.wave == 8) {
-        dfermi_group<8>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
-        return;
-    }
-    if (R.wave == 4) {
-        dfermi_group<4>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
+    if (R.wave) {
+        dfermi_group<FD_GROUP>(R, dk, eta, theta, s1, s2, s3, s12, fd, fdeta, fdtheta);
         return;
     }
 #endif
