@@ -269,21 +269,41 @@ HC_HD bool cache_akima_coefs(const double *xs, const double *ys, double *co)
     return true;
 }
 
+// The first three abscissae of a window (every window has at least three
+// points), read together and handed to the device code that follows as
+// register values (an empty asm takes them): their reads go out at once ahead
+// of the first comparison, and a pick among them stays a select.
+struct CacheAbsc {
+    double a0, a1, a2;
+};
+
+HC_HD CacheAbsc cache_absc(const double *ys)
+{
+    CacheAbsc w{ys[0], ys[1], ys[2]};
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(w.a0), "+v"(w.a1), "+v"(w.a2));
+#endif
+    return w;
+}
+
+HC_HD double cache_absc_at(const CacheAbsc &w, int s) { return s == 0 ? w.a0 : (s == 1 ? w.a1 : w.a2); }
+
 // cache_akima_m's output points from those numbers: the same interval search
-// and the same cubic.
+// and the same cubic.  The search's abscissae are read together, ahead of it.
 template <int M, int NT>
 HC_HD void cache_akima_eval(const double *xs, const double *co, const double *ts, double *out)
 {
+    const CacheAbsc xv = cache_absc(xs);
     int i = -1;
 #pragma unroll
     for (int q = 0; q < NT; q++) {
         const double t = ts[q];
 #pragma unroll
         for (int s = 0; s <= M - 2; s++)
-            if (i == s - 1 && t > xs[s]) i = s;
+            if ((i == s - 1) & (t > cache_absc_at(xv, s))) i = s;
         if (i < 0) i = 0;
         const double *ci = co + 4 * i;
-        const double t1 = t - xs[i];
+        const double t1 = t - cache_absc_at(xv, i);
         out[q] = ci[0] + ci[1] * t1 + ci[2] * t1 * t1 + ci[3] * t1 * t1 * t1;
     }
 }
@@ -453,18 +473,20 @@ HC_HD void cache_spline_win_co(const CacheView &c, const double *yco, int x0, in
         // then each row reads one interval's numbers
         const double h = y * 1.0E-6;
         const double ts[3] = {y - h, y, y + h};
+        const CacheAbsc yv = cache_absc(ys);
         int iq[3], i = -1;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
 #pragma unroll
             for (int s = 0; s <= NY - 2; s++)
-                if (i == s - 1 && ts[q] > ys[s]) i = s;
+                if ((i == s - 1) & (ts[q] > cache_absc_at(yv, s))) i = s;
             if (i < 0) i = 0;
             iq[q] = i;
         }
         if (iq[0] == iq[2]) {
             const int i0 = iq[0];
-            const double t1[3] = {ts[0] - ys[i0], ts[1] - ys[i0], ts[2] - ys[i0]};
+            const double y0v = cache_absc_at(yv, i0);
+            const double t1[3] = {ts[0] - y0v, ts[1] - y0v, ts[2] - y0v};
 #pragma unroll
             for (int b = 0; b < NX; b++) {
                 const double *ci = yco + ((size_t)(x0 + b) * c.Y + left) * CACHE_YCO + 4 * i0;

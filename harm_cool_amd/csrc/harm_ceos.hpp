@@ -33,6 +33,18 @@ constexpr double CEOS_MAV = (1.67262163783e-24 + 1.67492729e-24 + 2. * 9.10956e-
 constexpr int CEOS_NTRIAL = 300;     // cooleos.c:395
 constexpr double CEOS_TOL = 1.0E-6;  // cooleos.c:395, 406, 409
 
+// Where a bracket walk along an axis of positive values starts: the axis's
+// bracket (cache_bracket) of the smallest double of each key, key = the bits
+// of t >> shift (the exponent and the top mantissa bits, so a key spans a
+// fixed fraction of an octave).  The walk finds the same bracket from any
+// start; a start from the table is one or two reads away from it where the
+// bisection takes log2(n) dependent reads.  nk = 0: no table (bisection).
+struct CeosHint {
+    const int *idx;
+    unsigned long long k0;  // key of idx[0]
+    int nk, shift;
+};
+
 // One run's cache EOS: the grid every radial index's cache shares, the values
 // of each (n caches: 1 shared by every radius, or one per radial index of the
 // grid, ghosts included), the units and the ranges of cooleos.c:826-930.
@@ -44,6 +56,7 @@ struct CeosParams {
     double rho_unit, u_unit, p_unit, gam;
     double rho_min, rho_max;       // rhoMin, rhoMax (g/cm^3)
     double T_lo, T_hi;             // TMin (1 + hh) (also the floor of a first guess), TMax (1 - hh)
+    CeosHint hx, hy;               // bracket starts along rho and T
 };
 
 // The state a face or a zone carries through its pressure calls.
@@ -62,15 +75,43 @@ struct CeosOut {
     int solved;
 };
 
-// The y bracket of t (cache_bracket's) found by walking from b: down while
-// the bracket below also holds t, then up to the first one that does.  The
-// axis increases, so this is the bisection's index; a start near t takes one
-// or two reads.
+// The bracket of t (cache_bracket's) found by walking from b in [1, n-1]:
+// down while the bracket below also holds t, else up to the first one that
+// does (after a step down t < v[b] holds, so the up walk would not move).  The
+// axis increases, so this is the bisection's index; both neighbours of the
+// start are read at once, and a start near t needs no more.
+// Reads through a global-memory pointer: the out-of-line solve sees generic
+// pointers, whose loads each wait for every load before them.
+template <class T>
+__device__ __forceinline__ T ceos_ld(const T *p, int i)
+{
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
+__device__ __forceinline__ int ceos_walk_at(int n, const double *v, double t, int b, double below, double at)
+{
+    if (b > 1 && t < below) {
+        b--;
+        while (b > 1 && t < ceos_ld(v, b - 1)) b--;
+    } else if (b < n - 1 && !(t < at)) {
+        b++;
+        while (b < n - 1 && !(t < ceos_ld(v, b))) b++;
+    }
+    return b;
+}
+
 __device__ __forceinline__ int ceos_walk(int n, const double *v, double t, int b)
 {
-    while (b > 1 && t < v[b - 1]) b--;
-    while (b < n - 1 && !(t < v[b])) b++;
-    return b;
+    return ceos_walk_at(n, v, t, b, ceos_ld(v, b - 1), ceos_ld(v, b));
+}
+
+// The hint table's start for t (a table with nk > 0): any t, NaN and values
+// off the axis included, reads an entry inside the table.
+__device__ __forceinline__ int ceos_hint_start(const CeosHint &h, double t)
+{
+    const unsigned long long k = (unsigned long long)__double_as_longlong(t) >> h.shift;
+    const unsigned long long d = k < h.k0 ? 0ull : k - h.k0;
+    return ceos_ld(h.idx, d >= (unsigned long long)h.nk ? h.nk - 1 : (int)d);
 }
 
 template <bool DER>
@@ -96,25 +137,42 @@ __device__ __forceinline__ bool ceos_win(const CacheView &c, const double *yco, 
 // its coefficient tables (yco: e, then p), and at convergence p = P(rho, T):
 // every trial's lookup as cache_lookup1 makes it, with the density's window
 // formed once and the temperature's bracket walked from the trial before.
-__device__ inline int ceos_solve_Tp(const CacheView &c, const double *yco, double rho, double e, double T0, double lo,
-                                    double hi, double &T, double &berr, double &pv)
+__device__ inline int ceos_solve_Tp(const CacheView &c, const CeosHint &hx, const CeosHint &hy, const double *yco,
+                                    double rho, double e, double T0, double lo, double hi, double &T, double &berr,
+                                    double &pv)
 {
     double x = T0, best = T0, besterr = 1.0E100;
     int ret = 0;
-    if (rho < c.vx[0] || rho > c.vx[c.X - 1]) {
-        T = x;
-        berr = besterr;
-        return -1;
+    const double xlo = ceos_ld(c.vx, 0), xhi = ceos_ld(c.vx, c.X - 1);
+    const double ylo = ceos_ld(c.vy, 0), yhi = ceos_ld(c.vy, c.Y - 1);
+    // the brackets of rho and of the start T (cache_bracket's); with hint
+    // tables both walks' first reads go out at once.  A rho off the axis fails
+    // the first trial, with what the reference's lookup returns there (as for a
+    // T off the axis: T0, no error reached, -1).
+    int left, yb;
+    if (hx.nk && hy.nk) {
+        int sx = ceos_hint_start(hx, rho), sy = ceos_hint_start(hy, x);
+        // (each empty asm takes its reads' values, so the reads of a group are
+        // in flight together before any is used)
+        asm volatile("" : "+v"(sx), "+v"(sy));
+        double bx = ceos_ld(c.vx, sx - 1), ax = ceos_ld(c.vx, sx), by = ceos_ld(c.vy, sy - 1), ay = ceos_ld(c.vy, sy);
+        asm volatile("" : "+v"(bx), "+v"(ax), "+v"(by), "+v"(ay));
+        left = ceos_walk_at(c.X, c.vx, rho, sx, bx, ax);
+        yb = ceos_walk_at(c.Y, c.vy, x, sy, by, ay);
+    } else {
+        left = cache_bracket(c.X, c.vx, rho);
+        yb = cache_bracket(c.Y, c.vy, x);
     }
-    int x0, x1;
-    cache_window(c.X, c.vx, rho, CACHE_SPLINE, x0, x1);
+    const bool rho_in = !((rho < xlo) | (rho > xhi));
+    // cache_window's spline window around rho's bracket
+    const int x0 = left - 2 < 0 ? 0 : left - 2;
+    const int x1 = left + 1 >= c.X ? c.X - 1 : left + 1;
     const int nx = x1 - x0 + 1;
-    int yb = cache_bracket(c.Y, c.vy, x);
     for (int k = 1; k <= CEOS_NTRIAL; k++) {
         double val = 0.0, der = 0.0;
-        bool ok = !(x < c.vy[0] || x > c.vy[c.Y - 1]);
+        bool ok = rho_in & !((x < ylo) | (x > yhi));
         if (ok) {
-            yb = ceos_walk(c.Y, c.vy, x, yb);
+            if (k > 1) yb = ceos_walk(c.Y, c.vy, x, yb);  // the first trial's T is T0, bracketed above
             ok = ceos_win<true>(c, yco, x0, nx, yb, rho, x, val, der);
         }
         if (!ok) {
@@ -155,31 +213,34 @@ __device__ inline int ceos_solve_Tp(const CacheView &c, const double *yco, doubl
 // One out-of-line copy serves every call site of a kernel.
 __device__ __noinline__ CeosOut ceos_solve(const CeosParams *P, int r, double T, double rho0, double u)
 {
+    // every parameter read at once, ahead of the first branch
+    const double rho_unit = P->rho_unit, u_unit = P->u_unit, p_unit = P->p_unit, gam = P->gam;
+    const double rho_min = P->rho_min, rho_max = P->rho_max, T_lo = P->T_lo, T_hi = P->T_hi;
+    const CacheView c = P->view;
+    const CeosHint hx = P->hx, hy = P->hy;
+    const double *yco = P->yco[r];
     CeosOut o{0., T, 0};
-    const double rho = rho0 * P->rho_unit;
-    const double e = u * P->u_unit;
+    const double rho = rho0 * rho_unit;
+    const double e = u * u_unit;
     if (o.T == 0.0) {
-        o.T = (P->gam - 1.) * u / rho0 / CEOS_KBOL * CEOS_MAV * CEOS_CL * CEOS_CL;
-        if (o.T < P->T_lo) o.T = P->T_lo;
+        o.T = (gam - 1.) * u / rho0 / CEOS_KBOL * CEOS_MAV * CEOS_CL * CEOS_CL;
+        if (o.T < T_lo) o.T = T_lo;
     }
-    if (rho < P->rho_min || rho > P->rho_max) {
-        o.p = (P->gam - 1.) * u;
+    if (rho < rho_min || rho > rho_max) {
+        o.p = (gam - 1.) * u;
         return o;
     }
     if (e < 0.0) return o;
     o.solved = 1;
-    CacheView c = P->view;
-    c.values = P->values[r];
-    const double *yco = P->yco[r];
     double Tn, berr, pv = 0.;
-    ceos_solve_Tp(c, yco, rho, e, o.T, P->T_lo, P->T_hi, Tn, berr, pv);
+    ceos_solve_Tp(c, hx, hy, yco, rho, e, o.T, T_lo, T_hi, Tn, berr, pv);
     if (berr > CEOS_TOL) {
-        o.p = (P->gam - 1.) * u;
+        o.p = (gam - 1.) * u;
         return o;
     }
     // the P lookup shares the converged trial's windows (cooleos.c:432)
     o.T = Tn;
-    o.p = pv / P->p_unit;
+    o.p = pv / p_unit;
     return o;
 }
 

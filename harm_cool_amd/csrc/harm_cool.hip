@@ -3103,6 +3103,7 @@ struct harm_ctx {
     bool ceos_vanish;                  // the units make dp/drho, dp/du repeat calls (ceos_vanishing_steps)
     unsigned long long *ceos_solves;
     bool ceos_chain[3];
+    int *ceos_hint;                    // the rho and T axes' bracket-start tables (CeosHint)
     // learned Newton seed and training-pair capture
     int guess_mode;
     double gate_tau = 0.1;  // guess mode 2
@@ -3685,7 +3686,7 @@ void ctx_free(harm_ctx *c)
                     (void *)c->part, (void *)c->fix_bits, (void *)c->fix_pcount, (void *)c->fix_err, (void *)c->font.count, (void *)c->font.zone, (void *)c->font.U,
                     (void *)c->d_cblk, (void *)c->d_fblk, (void *)c->seeds.count, (void *)c->seeds.zone,
                     (void *)c->seeds.U, (void *)c->lastT, (void *)c->ceos_vals, (void *)c->ceos_solves,
-                    (void *)c->ceos_yco, (void *)c->ceos_fe})
+                    (void *)c->ceos_yco, (void *)c->ceos_fe, (void *)c->ceos_hint})
         if (q) (void)hipFree(q);
     for (double *q : c->ceos_ycoef) (void)hipFree(q);
     for (int l = 0; l < 5; l++)
@@ -3714,6 +3715,38 @@ int copy_state(harm_ctx *c, double *host_prim, int8_t *host_flag, bool to_host)
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// The bracket-start table of an increasing axis (CeosHint): for each key
+// between those of the axis's ends, the bracket of the smallest double with
+// that key, with the shift that keeps the table within 8192 entries (at the
+// cache grids' spacing a key spans well under one grid interval).  An axis
+// with a value that is not positive and finite gets none (nk = 0).
+CeosHint ceos_hint_table(const std::vector<double> &ax, std::vector<int> &tab)
+{
+    CeosHint h{};
+    tab.clear();
+    for (double a : ax)
+        if (!(a > 0.) || !std::isfinite(a)) return h;
+    auto key = [](double x, int s) {
+        unsigned long long b;
+        std::memcpy(&b, &x, sizeof b);
+        return b >> s;
+    };
+    int s = 40;
+    while (key(ax.back(), s) - key(ax.front(), s) + 1 > 8192) s++;
+    const unsigned long long k0 = key(ax.front(), s), k1 = key(ax.back(), s);
+    tab.resize((size_t)(k1 - k0 + 1));
+    for (size_t k = 0; k < tab.size(); k++) {
+        const unsigned long long b = (k0 + k) << s;
+        double x;
+        std::memcpy(&x, &b, sizeof x);
+        tab[k] = cache_bracket((int)ax.size(), ax.data(), x);
+    }
+    h.k0 = k0;
+    h.nk = (int)tab.size();
+    h.shift = s;
+    return h;
 }
 
 }  // namespace
@@ -4259,6 +4292,9 @@ int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, cons
         // z plane 0 of each cache: [X][Y][D]
         vals[(size_t)k] = v.values;
     }
+    for (const std::vector<double> *a : {&ax0, &ay0})
+        for (size_t k = 1; k < a->size(); k++)
+            if (!((*a)[k - 1] < (*a)[k])) return set_err("harm_set_eos_cache: the cache axes must increase", 4);
     CeosParams P{};
     P.view = v0;
     P.view.whole = 1;
@@ -4292,6 +4328,18 @@ int harm_set_eos_cache(harm_ctx *c, const harm_cache *const *caches, int n, cons
     if ((rc = dalloc(&c->ceos_yco, (size_t)n))) return rc;
     HIPCHK(hipMemcpy((void *)c->ceos_yco, c->ceos_ycoef.data(), (size_t)n * sizeof(double *), hipMemcpyHostToDevice));
     P.yco = c->ceos_yco;
+    std::vector<int> hx, hy;
+    P.hx = ceos_hint_table(ax0, hx);
+    P.hy = ceos_hint_table(ay0, hy);
+    if (c->ceos_hint) (void)hipFree((void *)c->ceos_hint);
+    c->ceos_hint = nullptr;
+    if ((rc = dalloc(&c->ceos_hint, hx.size() + hy.size()))) return rc;
+    if (!hx.empty())
+        HIPCHK(hipMemcpy(c->ceos_hint, hx.data(), hx.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!hy.empty())
+        HIPCHK(hipMemcpy(c->ceos_hint + hx.size(), hy.data(), hy.size() * sizeof(int), hipMemcpyHostToDevice));
+    P.hx.idx = c->ceos_hint;
+    P.hy.idx = c->ceos_hint + hx.size();
     hipLaunchKernelGGL(k_ceos_ycoef, dim3(nblocks(2 * (size_t)v0.X * v0.Y), (unsigned)n), dim3(TPB), 0, c->stream, v0,
                        P.values, c->ceos_yco);
     HIPCHK(hipGetLastError());

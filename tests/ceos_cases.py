@@ -23,12 +23,27 @@ RHO_RANGE = (1e2, 1e15)
 T_RANGE = (1e4, 1e18)
 
 
-def axes(nx=72, ny=72, rho=RHO_RANGE, T=T_RANGE):
-    return np.geomspace(rho[0], rho[1], nx), np.geomspace(T[0], T[1], ny)
+def axes(nx=72, ny=72, rho=RHO_RANGE, T=T_RANGE, irregular=0):
+    """Log-spaced axes; irregular = seed: the interior points moved at random
+    within their log intervals and a run of closely spaced points in each (an
+    increasing axis that no formula indexes)."""
+    vx, vy = np.geomspace(rho[0], rho[1], nx), np.geomspace(T[0], T[1], ny)
+    if irregular:
+        rng = np.random.default_rng(irregular)
+        out = []
+        for v in (vx, vy):
+            lv = np.log(v)
+            d = lv[1] - lv[0]
+            lv[1:-1] += rng.uniform(-0.45, 0.45, lv.size - 2) * d
+            k = lv.size // 3
+            lv[k:k + 6] = lv[k] + np.arange(6) * 0.02 * d
+            out.append(np.exp(np.sort(lv)))
+        vx, vy = out
+    return vx, vy
 
 
-def gas_rad(nx=72, ny=72, n=1, rho=RHO_RANGE, T=T_RANGE, gam_g=5.0 / 3.0):
-    vx, vy = axes(nx, ny, rho, T)
+def gas_rad(nx=72, ny=72, n=1, rho=RHO_RANGE, T=T_RANGE, gam_g=5.0 / 3.0, irregular=0):
+    vx, vy = axes(nx, ny, rho, T, irregular)
     R, TT = np.meshgrid(vx, vy, indexing="ij")
     e = R * KBOL * TT / (MAV * (gam_g - 1.0)) + A_RAD * TT ** 4
     p = R * KBOL * TT / MAV + A_RAD * TT ** 4 / 3.0

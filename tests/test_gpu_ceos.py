@@ -66,6 +66,8 @@ CASES = {
     "2d_vanleer_llf_rho_min": (lambda: torus_params(32, 32, 1, lim=1, hllf=0.0, laxf=1.0),
                                lambda P: cc.gas_rad(), {"rho_min": 1e6}),
     "3d_uniform_64": (lambda: torus_params(12, 8, 64), lambda P: cc.gas_rad(), None),
+    # brackets walked from the axis tables' starts over axes no formula indexes
+    "2d_irregular_axes": (lambda: torus_params(32, 32, 1), lambda P: cc.gas_rad(irregular=7), None),
 }
 
 
@@ -135,3 +137,13 @@ def test_outputs_that_would_move_the_temperatures_are_refused():
     d = g.diag()
     assert np.isnan(d["energy"]) and d["divb_max"] < 1e-12
     g.close()
+
+
+def test_an_axis_that_does_not_increase_is_refused():
+    """The bracket walks take increasing axes (the bisection's index is the
+    walk's only there): a cache with a repeated T point is never made."""
+    vx, vy, vals = cc.gas_rad()
+    vy = vy.copy()
+    vy[10] = vy[9]
+    with pytest.raises(hc.HarmError, match="increasing"):
+        _dev_caches(vx, vy, vals)
