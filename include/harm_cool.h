@@ -545,8 +545,9 @@ int harm_cache_solve_T(const harm_cache *c, const double *rho, const double *e, 
  * T_min (1 + 1e-6)), or from its j-1 neighbour's in the reference's loop order.
  * The step inverts with Utoprim_5d and has no Font fix, as with harm_set_eos.
  * caches: n = 1 (every radius) or N1+4 (one per radial index, ghosts included),
- * whole constant-grid caches sharing one (rho, T) grid, on the context's
- * device; they must outlive their use.  Zero ranges take the cache's axis ends
+ * whole constant-grid caches sharing one (rho, T) grid (increasing axes, as
+ * harm_cache_create requires), on the context's device; they must outlive
+ * their use.  Zero ranges take the cache's axis ends
  * (xmin, xmax, ymin, ymax).  Every zone's last temperature starts unset.
  * Reconstruction MC, van Leer or minmod (lim 0-2).  caches NULL returns to the
  * Gamma-law EOS.  The ASCII dump, the tracers and the ener.out record are not
